@@ -1,0 +1,123 @@
+/*
+ * kifmm3d.h -- C ABI of the MI355X KIFMM3d evaluator (libkifmm3d.so).
+ *
+ * The library is a drop-in for the reference's FMM3d setup/eval pair
+ * (fmm3d/fmm3d.hpp:27-115, fmm3d/knlmat3d.hpp:26-49).  Host C++ builds the
+ * octree, the U/V/W/X lists and the FP64 translation operators; the evaluate
+ * pass runs as hand-written gfx950 HIP kernels on one GPU.
+ *
+ * Conventions
+ *   - Positions and normals are 3 x N column-major (x, y, z interleaved per
+ *     point), as the reference's DblNumMat 3 x N (fmm3d/knlmat3d.hpp:30-32).
+ *   - Densities are [nrhs][N * sdof], values [nrhs][N * tdof], dof fastest
+ *     (fmm3d/fmm3d_eval.cpp:59, :230); nrhs = 1 is exactly the reference's eval.
+ *   - Every call returns 0 on success and a negative KIFMM3D_E* code on
+ *     failure; kifmm3d_last_error() then describes the failure.  The library
+ *     never aborts the process (the reference asserts instead,
+ *     common/commoninc.hpp:59-60).
+ *   - One handle per host thread; handles are independent.  Operators are
+ *     shared read-only between handles through a process-wide cache.
+ */
+#ifndef KIFMM3D_H
+#define KIFMM3D_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct kifmm3d kifmm3d;
+
+/* Status codes */
+#define KIFMM3D_OK 0
+#define KIFMM3D_EINVAL (-1)   /* bad argument (kernel type, np, sizes, options) */
+#define KIFMM3D_EBOX (-2)     /* a point lies outside the root box (let3d.cpp:73, :155) */
+#define KIFMM3D_ESTATE (-3)   /* call out of order (evaluate before setup, ...) */
+#define KIFMM3D_EDEVICE (-4)  /* HIP runtime error or no HIP device */
+#define KIFMM3D_ENOMEM (-5)   /* device or host allocation failed */
+
+/* Evaluation forms */
+#define KIFMM3D_MODE_PLAIN 0  /* FP64 throughout: the drop-in form */
+
+/* Per-stage timer slots filled by kifmm3d_evaluate*() (milliseconds, HIP events) */
+enum {
+  KIFMM3D_STAGE_LOAD = 0,   /* density gather into leaf order           (fmm3d_eval.cpp:50-64)  */
+  KIFMM3D_STAGE_UP = 1,     /* S2M, M2M, UC2UE                          (fmm3d_eval.cpp:69-92)  */
+  KIFMM3D_STAGE_NEAR = 2,   /* U list + direct W/X fallbacks            (fmm3d_eval.cpp:98-111) */
+  KIFMM3D_STAGE_VLIST = 3,  /* FFT, M2L products, inverse FFT           (fmm3d_eval.cpp:115-150)*/
+  KIFMM3D_STAGE_XLIST = 4,  /* S2L                                      (fmm3d_eval.cpp:176-191)*/
+  KIFMM3D_STAGE_DOWN = 5,   /* L2L, DC2DE                               (fmm3d_eval.cpp:195-209)*/
+  KIFMM3D_STAGE_EVAL = 6,   /* L2T + M2T (W list)                       (fmm3d_eval.cpp:154-172, 210-214) */
+  KIFMM3D_STAGE_SAVE = 7,   /* scatter to caller order                  (fmm3d_eval.cpp:221-235)*/
+  KIFMM3D_STAGE_TOTAL = 8,
+  KIFMM3D_NSTAGE = 9
+};
+
+/* Create a handle.  Replaces FMM3d::FMM3d + the knl()/ctr()/rootlvl() setters
+ * (fmm3d/fmm3d.cpp:23-27, fmm3d/tt.cpp:85-90) and reads the three options the
+ * reference's setup requires (-fmm3d_np, -fmm3d_let3d_ptsmax,
+ * -fmm3d_let3d_maxlevel; fmm3d_setup.cpp:30, let3d.cpp:45-48).
+ * kt: 111 (Laplace single layer), 321 (Stokes double layer), 511 (Navier
+ * single layer) -- fmm3d/kernel3d.hpp:29-53.  coefs: {mu} for 321, {mu, nu}
+ * for 511 (kernel3d.cpp:224-226, :318-319); ignored for 111. */
+int kifmm3d_create(kifmm3d** out, int kt, const double* coefs, int ncoefs, int np, int ptsmax,
+                   int maxlevel, const double ctr[3], int rootlvl);
+
+/* Build tree, lists and operators and upload them.  Replaces FMM3d::setup
+ * (fmm3d/fmm3d_setup.cpp:26-63).  srcnor may be NULL when the kernel does not
+ * use normals.  The library copies the inputs; the caller keeps ownership. */
+int kifmm3d_setup(kifmm3d* h, const double* srcpos, const double* srcnor, int nsrc,
+                  const double* trgpos, int ntrg);
+
+/* Evaluate nrhs density vectors.  Replaces FMM3d::eval
+ * (fmm3d/fmm3d_eval.cpp:24-243): trgval is overwritten.  Host pointers; the
+ * call copies srcden to the device and trgval back, and returns when trgval
+ * is complete. */
+int kifmm3d_evaluate(kifmm3d* h, const double* srcden, double* trgval, int nrhs, int mode);
+
+/* Same as kifmm3d_evaluate with device-resident srcden / trgval (HBM
+ * pointers from kifmm3d_device_alloc).  Returns when the work is complete. */
+int kifmm3d_evaluate_device(kifmm3d* h, const double* d_srcden, double* d_trgval, int nrhs, int mode);
+
+/* Stage times (ms) of the last evaluate; n <= KIFMM3D_NSTAGE slots. */
+int kifmm3d_stage_times(const kifmm3d* h, double* ms, int n);
+
+/* Tree and list sizes after setup: out[0..11] = nnode, nlevel, sndecnt,
+ * sextcnt, tndecnt, textcnt, plnUE, plnUC, plnDE, plnDC, sdof, tdof. */
+int kifmm3d_info(const kifmm3d* h, int* out, int n);
+
+/* Copy of a setup product, for verification and for the reference-style
+ * accessors (let3d.hpp:102-136).  what: 0 node table (13 ints per node:
+ * par, chd, path x3, depth, tag, sndeidx, sextbeg, sextnum, tndeidx, textbeg,
+ * textnum), 1..4 U/V/W/X list CSR pointers (nnode+1 ints), 5..8 U/V/W/X list
+ * indices, 9 source leaf permutation (sextcnt ints), 10 target leaf
+ * permutation (textcnt ints).  Returns the element count; buf may be NULL. */
+long kifmm3d_get_tree(const kifmm3d* h, int what, int* buf, long n);
+
+/* Copy of an FP64 operator: what 0 UC2UE (plnUE x plnUC), 1 DC2DE
+ * (plnDE x plnDC), both column-major as the reference stores them
+ * (matmgnt3d.cpp:103, :169).  Returns the element count; buf may be NULL. */
+long kifmm3d_get_operator(const kifmm3d* h, int what, double* buf, long n);
+
+/* Copy of a per-node buffer of the last evaluate (first right-hand side):
+ * what 0 upward equivalent densities (sndecnt x plnUE), 1 upward check values
+ * (sndecnt x plnUC), 2 downward equivalent densities (tndecnt x plnDE),
+ * 3 downward check values (tndecnt x plnDC) -- the reference's _supeden,
+ * _supcval, _tdneden, _tdncval (fmm3d/fmm3d.hpp:61-64).  Returns the element
+ * count; buf may be NULL. */
+long kifmm3d_get_buffer(kifmm3d* h, int what, double* buf, long n);
+
+const char* kifmm3d_last_error(const kifmm3d* h);
+void kifmm3d_destroy(kifmm3d* h);
+
+/* Device memory helpers for callers without a HIP runtime of their own. */
+int kifmm3d_device_alloc(void** ptr, long bytes);
+int kifmm3d_device_free(void* ptr);
+int kifmm3d_memcpy_h2d(void* dst, const void* src, long bytes);
+int kifmm3d_memcpy_d2h(void* dst, const void* src, long bytes);
+int kifmm3d_device_synchronize(void);
+int kifmm3d_device_count(int* n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,788 @@
+// device.hip -- gfx950 kernels of the FP64 evaluate pass.
+//
+// Thread mapping is written for 64-wide wavefronts: blocks of 128/256
+// threads, sources or surface points staged in LDS and read as broadcasts,
+// one target (or surface point, or frequency) per lane.  Kernel closed forms
+// follow fmm3d/kernel3d.cpp (111: :124-138, 301: :160-184, 321: :222-242,
+// 511: :315-337) including the r < 1e-8 exclusion (:21).
+#include <hip/hip_runtime.h>
+
+#include "device.hpp"
+
+namespace kifmm {
+namespace dev {
+
+// ---------------------------------------------------------------------------
+// kernels: accumulate u[q][t] += sum_s K_ts(d, n) f[q*S + s]
+template <int KT>
+struct Knl;
+
+template <>
+struct Knl<111> {
+  static constexpr int S = 1, T = 1;
+  static constexpr bool NRM = false;
+  template <int NR>
+  __device__ static inline void add(double dx, double dy, double dz, const double*, const double* f, double (&u)[NR][T],
+                                    const KC& k)
+  {
+    double r2 = dx * dx + dy * dy + dz * dz;
+    double r = sqrt(r2);
+    if (r < 1e-8) return;
+    double g = k.a / r;
+#pragma unroll
+    for (int q = 0; q < NR; q++) u[q][0] += g * f[q];
+  }
+};
+
+template <>
+struct Knl<301> {
+  static constexpr int S = 4, T = 3;
+  static constexpr bool NRM = false;
+  template <int NR>
+  __device__ static inline void add(double dx, double dy, double dz, const double*, const double* f, double (&u)[NR][T],
+                                    const KC& k)
+  {
+    double r2 = dx * dx + dy * dy + dz * dz;
+    double r = sqrt(r2);
+    if (r < 1e-8) return;
+    double r3 = r2 * r;
+    double G = k.a / r, H = k.a / r3, A = k.b / r3;
+#pragma unroll
+    for (int q = 0; q < NR; q++) {
+      const double* g = f + 4 * q;
+      double df = dx * g[0] + dy * g[1] + dz * g[2];
+      double h = H * df + A * g[3];
+      u[q][0] += G * g[0] + h * dx;
+      u[q][1] += G * g[1] + h * dy;
+      u[q][2] += G * g[2] + h * dz;
+    }
+  }
+};
+
+template <>
+struct Knl<321> {
+  static constexpr int S = 3, T = 3;
+  static constexpr bool NRM = true;
+  template <int NR>
+  __device__ static inline void add(double dx, double dy, double dz, const double* n, const double* f, double (&u)[NR][T],
+                                    const KC& k)
+  {
+    double r2 = dx * dx + dy * dy + dz * dz;
+    double r = sqrt(r2);
+    if (r < 1e-8) return;
+    double rn = dx * n[0] + dy * n[1] + dz * n[2];
+    double r5 = r2 * r2 * r;
+    double c = (k.a / r5) * rn;
+#pragma unroll
+    for (int q = 0; q < NR; q++) {
+      const double* g = f + 3 * q;
+      double w = c * (dx * g[0] + dy * g[1] + dz * g[2]);
+      u[q][0] += w * dx;
+      u[q][1] += w * dy;
+      u[q][2] += w * dz;
+    }
+  }
+};
+
+template <>
+struct Knl<511> {
+  static constexpr int S = 3, T = 3;
+  static constexpr bool NRM = false;
+  template <int NR>
+  __device__ static inline void add(double dx, double dy, double dz, const double*, const double* f, double (&u)[NR][T],
+                                    const KC& k)
+  {
+    double r2 = dx * dx + dy * dy + dz * dz;
+    double r = sqrt(r2);
+    if (r < 1e-8) return;
+    double r3 = r2 * r;
+    double G = k.a / r, H = k.b / r3;
+#pragma unroll
+    for (int q = 0; q < NR; q++) {
+      const double* g = f + 3 * q;
+      double h = H * (dx * g[0] + dy * g[1] + dz * g[2]);
+      u[q][0] += k.c * (G * g[0] + h * dx);
+      u[q][1] += k.c * (G * g[1] + h * dy);
+      u[q][2] += k.c * (G * g[2] + h * dz);
+    }
+  }
+};
+
+// ---------------------------------------------------------------------------
+// permutations
+__global__ void k_gather(int n, int dof, int nr, const int* __restrict__ perm, const double* __restrict__ src,
+                         long src_rs, double* __restrict__ dst, long dst_rs)
+{
+  long tot = (long)n * dof * nr;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < tot; i += (long)gridDim.x * blockDim.x) {
+    long per = (long)n * dof;
+    int r = (int)(i / per);
+    long rem = i - r * per;
+    int k = (int)(rem / dof), d = (int)(rem - (long)k * dof);
+    dst[r * dst_rs + rem] = src[r * src_rs + (long)perm[k] * dof + d];
+  }
+}
+
+__global__ void k_scatter(int n, int dof, int nr, const int* __restrict__ perm, const double* __restrict__ src,
+                          long src_rs, double* __restrict__ dst, long dst_rs)
+{
+  long tot = (long)n * dof * nr;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < tot; i += (long)gridDim.x * blockDim.x) {
+    long per = (long)n * dof;
+    int r = (int)(i / per);
+    long rem = i - r * per;
+    int k = (int)(rem / dof), d = (int)(rem - (long)k * dof);
+    dst[r * dst_rs + (long)perm[k] * dof + d] = src[r * src_rs + rem];
+  }
+}
+
+static int grid_for(long tot, int bs)
+{
+  long g = (tot + bs - 1) / bs;
+  if (g > 65536L * 8) g = 65536L * 8;
+  return (int)(g > 0 ? g : 1);
+}
+
+void gather(hipStream_t s, int n, int dof, int nr, const int* perm, const double* src, long src_rs, double* dst,
+            long dst_rs)
+{
+  long tot = (long)n * dof * nr;
+  if (tot == 0) return;
+  hipLaunchKernelGGL(k_gather, dim3(grid_for(tot, 256)), dim3(256), 0, s, n, dof, nr, perm, src, src_rs, dst, dst_rs);
+}
+
+void scatter(hipStream_t s, int n, int dof, int nr, const int* perm, const double* src, long src_rs, double* dst,
+             long dst_rs)
+{
+  long tot = (long)n * dof * nr;
+  if (tot == 0) return;
+  hipLaunchKernelGGL(k_scatter, dim3(grid_for(tot, 256)), dim3(256), 0, s, n, dof, nr, perm, src, src_rs, dst, dst_rs);
+}
+
+// ---------------------------------------------------------------------------
+// near field (U list and direct W/X)
+constexpr int P2P_BS = 128;
+constexpr int TILE = 64;
+
+template <int KT, int NR>
+__global__ __launch_bounds__(P2P_BS) void k_p2p(const int* __restrict__ tbeg, const int* __restrict__ tnum,
+                                                const int* __restrict__ eptr, const int2* __restrict__ ent, Points src,
+                                                Targets trg, int roff, KC kc)
+{
+  using K = Knl<KT>;
+  constexpr int NPOS = K::NRM ? 6 : 3;
+  constexpr int ND = NR * K::S;
+  __shared__ double s_pos[TILE][NPOS];
+  __shared__ double s_den[TILE][ND];
+  const int c = blockIdx.x, t = threadIdx.x;
+  const int r0 = roff + blockIdx.y * NR;
+  const int tb = tbeg[c], tn = tnum[c];
+  const bool act = t < tn;
+  double x = 0, y = 0, z = 0;
+  if (act) {
+    x = trg.x[tb + t];
+    y = trg.y[tb + t];
+    z = trg.z[tb + t];
+  }
+  double u[NR][K::T];
+#pragma unroll
+  for (int q = 0; q < NR; q++)
+#pragma unroll
+    for (int d = 0; d < K::T; d++) u[q][d] = 0.0;
+  const double* den = src.den + (long)r0 * src.den_rs;
+  for (int e = eptr[c]; e < eptr[c + 1]; e++) {
+    const int2 se = ent[e];
+    for (int b0 = 0; b0 < se.y; b0 += TILE) {
+      const int cnt = min(TILE, se.y - b0);
+      const int j0 = se.x + b0;
+      __syncthreads();
+      for (int i = t; i < cnt; i += P2P_BS) {
+        s_pos[i][0] = src.x[j0 + i];
+        s_pos[i][1] = src.y[j0 + i];
+        s_pos[i][2] = src.z[j0 + i];
+        if (K::NRM) {
+          s_pos[i][NPOS - 3] = src.nx[j0 + i];
+          s_pos[i][NPOS - 2] = src.ny[j0 + i];
+          s_pos[i][NPOS - 1] = src.nz[j0 + i];
+        }
+      }
+      const int per = cnt * K::S;
+      for (int i = t; i < NR * per; i += P2P_BS) {
+        int q = i / per, rem = i - q * per;
+        int jj = rem / K::S, sd = rem - jj * K::S;
+        s_den[jj][q * K::S + sd] = den[(long)q * src.den_rs + (long)j0 * K::S + rem];
+      }
+      __syncthreads();
+      if (act)
+        for (int jj = 0; jj < cnt; jj++)
+          K::template add<NR>(x - s_pos[jj][0], y - s_pos[jj][1], z - s_pos[jj][2], &s_pos[jj][NPOS - 3], s_den[jj], u,
+                              kc);
+    }
+  }
+  if (act) {
+    double* val = trg.val + (long)r0 * trg.val_rs + (long)(tb + t) * K::T;
+#pragma unroll
+    for (int q = 0; q < NR; q++)
+#pragma unroll
+      for (int d = 0; d < K::T; d++) val[(long)q * trg.val_rs + d] += u[q][d];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// points -> check surface (S2M, S2L)
+constexpr int P2S_BS = 256;
+
+template <int KT, int NR>
+__global__ __launch_bounds__(P2S_BS) void k_p2s(const int* __restrict__ out_node, const double4* __restrict__ geom,
+                                                const int* __restrict__ eptr, const int2* __restrict__ ent,
+                                                const double* __restrict__ unit, int npt, Points src,
+                                                double* __restrict__ out, int Rc, int pln, int roff, KC kc)
+{
+  using K = Knl<KT>;
+  constexpr int NPOS = K::NRM ? 6 : 3;
+  constexpr int ND = NR * K::S;
+  __shared__ double s_pos[TILE][NPOS];
+  __shared__ double s_den[TILE][ND];
+  const int it = blockIdx.x, t = threadIdx.x;
+  const int p = blockIdx.y * P2S_BS + t;
+  const int r0 = roff + blockIdx.z * NR;
+  const bool act = p < npt;
+  const double4 g = geom[it];
+  double x = 0, y = 0, z = 0;
+  if (act) {
+    x = g.x + g.w * unit[3 * p];
+    y = g.y + g.w * unit[3 * p + 1];
+    z = g.z + g.w * unit[3 * p + 2];
+  }
+  double u[NR][K::T];
+#pragma unroll
+  for (int q = 0; q < NR; q++)
+#pragma unroll
+    for (int d = 0; d < K::T; d++) u[q][d] = 0.0;
+  const double* den = src.den + (long)r0 * src.den_rs;
+  for (int e = eptr[it]; e < eptr[it + 1]; e++) {
+    const int2 se = ent[e];
+    for (int b0 = 0; b0 < se.y; b0 += TILE) {
+      const int cnt = min(TILE, se.y - b0);
+      const int j0 = se.x + b0;
+      __syncthreads();
+      for (int i = t; i < cnt; i += P2S_BS) {
+        s_pos[i][0] = src.x[j0 + i];
+        s_pos[i][1] = src.y[j0 + i];
+        s_pos[i][2] = src.z[j0 + i];
+        if (K::NRM) {
+          s_pos[i][NPOS - 3] = src.nx[j0 + i];
+          s_pos[i][NPOS - 2] = src.ny[j0 + i];
+          s_pos[i][NPOS - 1] = src.nz[j0 + i];
+        }
+      }
+      const int per = cnt * K::S;
+      for (int i = t; i < NR * per; i += P2S_BS) {
+        int q = i / per, rem = i - q * per;
+        int jj = rem / K::S, sd = rem - jj * K::S;
+        s_den[jj][q * K::S + sd] = den[(long)q * src.den_rs + (long)j0 * K::S + rem];
+      }
+      __syncthreads();
+      if (act)
+        for (int jj = 0; jj < cnt; jj++)
+          K::template add<NR>(x - s_pos[jj][0], y - s_pos[jj][1], z - s_pos[jj][2], &s_pos[jj][NPOS - 3], s_den[jj], u,
+                              kc);
+    }
+  }
+  if (act) {
+    const long node = out_node[it];
+#pragma unroll
+    for (int q = 0; q < NR; q++) {
+      double* o = out + (node * Rc + r0 + q) * pln + (long)p * K::T;
+#pragma unroll
+      for (int d = 0; d < K::T; d++) o[d] += u[q][d];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// equivalent surface -> targets (L2T, M2T), mm kernel
+template <int KT, int NR>
+__global__ __launch_bounds__(P2P_BS) void k_s2p(const int* __restrict__ tbeg, const int* __restrict__ tnum,
+                                                const int* __restrict__ eptr, const int* __restrict__ enode,
+                                                const int* __restrict__ ekind, const double4* __restrict__ egeom,
+                                                const double* __restrict__ unit_de, const double* __restrict__ unit_ue,
+                                                int nsp, const double* __restrict__ dneden,
+                                                const double* __restrict__ upeden, int Rc, int pln, Targets trg,
+                                                int roff, KC kc)
+{
+  using K = Knl<KT>;
+  constexpr int ND = NR * K::S;
+  __shared__ double s_pos[TILE][3];
+  __shared__ double s_den[TILE][ND];
+  const int c = blockIdx.x, t = threadIdx.x;
+  const int r0 = roff + blockIdx.y * NR;
+  const int tb = tbeg[c], tn = tnum[c];
+  const bool act = t < tn;
+  double x = 0, y = 0, z = 0;
+  if (act) {
+    x = trg.x[tb + t];
+    y = trg.y[tb + t];
+    z = trg.z[tb + t];
+  }
+  double u[NR][K::T];
+#pragma unroll
+  for (int q = 0; q < NR; q++)
+#pragma unroll
+    for (int d = 0; d < K::T; d++) u[q][d] = 0.0;
+  for (int e = eptr[c]; e < eptr[c + 1]; e++) {
+    const int kind = ekind[e];
+    const double* unit = kind == 0 ? unit_de : unit_ue;
+    const double* buf = (kind == 0 ? dneden : upeden) + ((long)enode[e] * Rc + r0) * pln;
+    const double4 g = egeom[e];
+    for (int b0 = 0; b0 < nsp; b0 += TILE) {
+      const int cnt = min(TILE, nsp - b0);
+      __syncthreads();
+      for (int i = t; i < cnt; i += P2P_BS) {
+        s_pos[i][0] = g.x + g.w * unit[3 * (b0 + i)];
+        s_pos[i][1] = g.y + g.w * unit[3 * (b0 + i) + 1];
+        s_pos[i][2] = g.z + g.w * unit[3 * (b0 + i) + 2];
+      }
+      const int per = cnt * K::S;
+      for (int i = t; i < NR * per; i += P2P_BS) {
+        int q = i / per, rem = i - q * per;
+        int jj = rem / K::S, sd = rem - jj * K::S;
+        s_den[jj][q * K::S + sd] = buf[(long)q * pln + (long)b0 * K::S + rem];
+      }
+      __syncthreads();
+      if (act)
+        for (int jj = 0; jj < cnt; jj++)
+          K::template add<NR>(x - s_pos[jj][0], y - s_pos[jj][1], z - s_pos[jj][2], nullptr, s_den[jj], u, kc);
+    }
+  }
+  if (act) {
+    double* val = trg.val + (long)r0 * trg.val_rs + (long)(tb + t) * K::T;
+#pragma unroll
+    for (int q = 0; q < NR; q++)
+#pragma unroll
+      for (int d = 0; d < K::T; d++) val[(long)q * trg.val_rs + d] += u[q][d];
+  }
+}
+
+// Split nr right-hand sides into launches of NR in {16, 4, 1}; fn(NR, nblocks, roff).
+template <class F>
+static void rhs_blocks(int nr, F&& fn)
+{
+  int r = 0;
+  while (r < nr) {
+    int left = nr - r;
+    int NR = left >= 16 ? 16 : left >= 4 ? 4 : 1;
+    int nb = left / NR;
+    fn(NR, nb, r);
+    r += nb * NR;
+  }
+}
+
+template <int KT>
+static void p2p_kt(hipStream_t s, int nchunk, const int* tbeg, const int* tnum, const int* eptr, const int2* ent,
+                   Points src, Targets trg, int nr, KC kc)
+{
+  rhs_blocks(nr, [&](int NR, int nb, int roff) {
+    dim3 grid(nchunk, nb);
+    if (NR == 16)
+      hipLaunchKernelGGL((k_p2p<KT, 16>), grid, dim3(P2P_BS), 0, s, tbeg, tnum, eptr, ent, src, trg, roff, kc);
+    else if (NR == 4)
+      hipLaunchKernelGGL((k_p2p<KT, 4>), grid, dim3(P2P_BS), 0, s, tbeg, tnum, eptr, ent, src, trg, roff, kc);
+    else
+      hipLaunchKernelGGL((k_p2p<KT, 1>), grid, dim3(P2P_BS), 0, s, tbeg, tnum, eptr, ent, src, trg, roff, kc);
+  });
+}
+
+void p2p(hipStream_t s, int kt, int nchunk, const int* tbeg, const int* tnum, const int* eptr, const int2* ent,
+         Points src, Targets trg, int nr, KC kc)
+{
+  if (nchunk <= 0) return;
+  switch (kt) {
+  case 111: p2p_kt<111>(s, nchunk, tbeg, tnum, eptr, ent, src, trg, nr, kc); break;
+  case 321: p2p_kt<321>(s, nchunk, tbeg, tnum, eptr, ent, src, trg, nr, kc); break;
+  case 511: p2p_kt<511>(s, nchunk, tbeg, tnum, eptr, ent, src, trg, nr, kc); break;
+  }
+}
+
+template <int KT>
+static void p2s_kt(hipStream_t s, int nitem, const int* out_node, const double4* geom, const int* eptr,
+                   const int2* ent, const double* unit, int npt, Points src, double* out, int Rc, int pln, int nr,
+                   KC kc)
+{
+  rhs_blocks(nr, [&](int NR, int nb, int roff) {
+    dim3 grid(nitem, (npt + P2S_BS - 1) / P2S_BS, nb);
+    if (NR == 16)
+      hipLaunchKernelGGL((k_p2s<KT, 16>), grid, dim3(P2S_BS), 0, s, out_node, geom, eptr, ent, unit, npt, src, out,
+                         Rc, pln, roff, kc);
+    else if (NR == 4)
+      hipLaunchKernelGGL((k_p2s<KT, 4>), grid, dim3(P2S_BS), 0, s, out_node, geom, eptr, ent, unit, npt, src, out,
+                         Rc, pln, roff, kc);
+    else
+      hipLaunchKernelGGL((k_p2s<KT, 1>), grid, dim3(P2S_BS), 0, s, out_node, geom, eptr, ent, unit, npt, src, out,
+                         Rc, pln, roff, kc);
+  });
+}
+
+void p2s(hipStream_t s, int kt, int nitem, const int* out_node, const double4* geom, const int* eptr, const int2* ent,
+         const double* unit, int npt, Points src, double* out, int Rc, int pln, int nr, KC kc)
+{
+  if (nitem <= 0) return;
+  switch (kt) {
+  case 111: p2s_kt<111>(s, nitem, out_node, geom, eptr, ent, unit, npt, src, out, Rc, pln, nr, kc); break;
+  case 321: p2s_kt<321>(s, nitem, out_node, geom, eptr, ent, unit, npt, src, out, Rc, pln, nr, kc); break;
+  case 511: p2s_kt<511>(s, nitem, out_node, geom, eptr, ent, unit, npt, src, out, Rc, pln, nr, kc); break;
+  }
+}
+
+template <int KT>
+static void s2p_kt(hipStream_t s, int nchunk, const int* tbeg, const int* tnum, const int* eptr, const int* enode,
+                   const int* ekind, const double4* egeom, const double* unit_de, const double* unit_ue, int nsp,
+                   const double* dneden, const double* upeden, int Rc, int pln, Targets trg, int nr, KC kc)
+{
+  rhs_blocks(nr, [&](int NR, int nb, int roff) {
+    dim3 grid(nchunk, nb);
+    if (NR == 16)
+      hipLaunchKernelGGL((k_s2p<KT, 16>), grid, dim3(P2P_BS), 0, s, tbeg, tnum, eptr, enode, ekind, egeom, unit_de,
+                         unit_ue, nsp, dneden, upeden, Rc, pln, trg, roff, kc);
+    else if (NR == 4)
+      hipLaunchKernelGGL((k_s2p<KT, 4>), grid, dim3(P2P_BS), 0, s, tbeg, tnum, eptr, enode, ekind, egeom, unit_de,
+                         unit_ue, nsp, dneden, upeden, Rc, pln, trg, roff, kc);
+    else
+      hipLaunchKernelGGL((k_s2p<KT, 1>), grid, dim3(P2P_BS), 0, s, tbeg, tnum, eptr, enode, ekind, egeom, unit_de,
+                         unit_ue, nsp, dneden, upeden, Rc, pln, trg, roff, kc);
+  });
+}
+
+void s2p(hipStream_t s, int kt, int nchunk, const int* tbeg, const int* tnum, const int* eptr, const int* enode,
+         const int* ekind, const double4* egeom, const double* unit_de, const double* unit_ue, int nsp,
+         const double* dneden, const double* upeden, int Rc, int pln, Targets trg, int nr, KC kc)
+{
+  if (nchunk <= 0) return;
+  switch (kt) {
+  case 111:
+    s2p_kt<111>(s, nchunk, tbeg, tnum, eptr, enode, ekind, egeom, unit_de, unit_ue, nsp, dneden, upeden, Rc, pln, trg,
+                nr, kc);
+    break;
+  case 301:
+    s2p_kt<301>(s, nchunk, tbeg, tnum, eptr, enode, ekind, egeom, unit_de, unit_ue, nsp, dneden, upeden, Rc, pln, trg,
+                nr, kc);
+    break;
+  case 511:
+    s2p_kt<511>(s, nchunk, tbeg, tnum, eptr, enode, ekind, egeom, unit_de, unit_ue, nsp, dneden, upeden, Rc, pln, trg,
+                nr, kc);
+    break;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// batched indexed FP64 GEMM: 64 x 64 tile of C per block, K in steps of 16
+__global__ __launch_bounds__(256) void k_gemm(GemmArgs p)
+{
+  const int grp = blockIdx.z;
+  const int n0 = p.gbeg[grp], n1 = p.gbeg[grp + 1];
+  const long cb = (long)n0 * p.Rc + (long)blockIdx.x * 64, ce = (long)n1 * p.Rc;
+  if (cb >= ce) return;
+  const int rb = blockIdx.y * 64;
+  const double* A = p.A + grp * p.a_gstride;
+  const int K = p.nseg * p.Kseg;
+  __shared__ double As[16][65];
+  __shared__ double Bs[16][65];
+  const int t = threadIdx.x, tx = t & 15, ty = t >> 4;
+  double acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) acc[i][j] = 0.0;
+  for (int k0 = 0; k0 < K; k0 += 16) {
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      int e = t + 256 * i, row = e & 63, kk = e >> 6;
+      int gr = rb + row, gk = k0 + kk;
+      As[kk][row] = (gr < p.M && gk < K) ? A[(long)gk * p.M + gr] : 0.0;
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      int e = t + 256 * i, kk = e & 15, cl = e >> 4;
+      int gk = k0 + kk;
+      long j = cb + cl;
+      double v = 0.0;
+      if (j < ce && gk < K) {
+        int n = (int)(j / p.Rc), r = (int)(j - (long)n * p.Rc);
+        int seg = gk / p.Kseg, kin = gk - seg * p.Kseg;
+        int bn = p.bnode[(long)seg * p.ncol + n];
+        if (bn >= 0) {
+          v = p.B[((long)bn * p.Rc + r) * p.Kseg + kin];
+          if (p.bscale) v *= p.bscale[kin];
+        }
+      }
+      Bs[kk][cl] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < 16; kk++) {
+      double a[4], b[4];
+#pragma unroll
+      for (int i = 0; i < 4; i++) a[i] = As[kk][tx + 16 * i];
+#pragma unroll
+      for (int j = 0; j < 4; j++) b[j] = Bs[kk][ty + 16 * j];
+#pragma unroll
+      for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[i][j] += a[i] * b[j];
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    long jj = cb + ty + 16 * j;
+    if (jj >= ce) continue;
+    int n = (int)(jj / p.Rc), r = (int)(jj - (long)n * p.Rc);
+    double* out = p.C + ((long)p.cnode[n] * p.Rc + r) * p.M;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      int row = rb + tx + 16 * i;
+      if (row >= p.M) continue;
+      double v = acc[i][j];
+      if (p.cscale) v *= p.cscale[row];
+      out[row] = p.accumulate ? out[row] + v : v;
+    }
+  }
+}
+
+void gemm(hipStream_t s, const GemmArgs& a)
+{
+  if (a.ngroup <= 0 || a.maxcols <= 0) return;
+  long cols = (long)a.maxcols * a.Rc;
+  dim3 grid((unsigned)((cols + 63) / 64), (a.M + 63) / 64, a.ngroup);
+  hipLaunchKernelGGL(k_gemm, grid, dim3(256), 0, s, a);
+}
+
+// ---------------------------------------------------------------------------
+// FFTs on the (2np)^3 grid in FFTW 2 rfftwnd layout ([z][y][x], x halved).
+// Only the np^3 cube at offset np/2 holds nonzero data (matmgnt3d.cpp:256,
+// :286), so the forward pass sums over np points per dimension and the
+// inverse evaluates only the surface points.
+__device__ inline double2 cmul(double2 a, double2 b) { return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+
+template <int NP>
+__global__ __launch_bounds__(256) void k_fft_fwd(const int* __restrict__ slot_node, int Rc,
+                                                 const double* __restrict__ upeden, int pln, int sm,
+                                                 const double* __restrict__ dscale, const short* __restrict__ c2s,
+                                                 double2* __restrict__ effden, const double2* __restrict__ tw)
+{
+  constexpr int N = 2 * NP, H = NP + 1, C = N * N * H, NC = NP * NP * NP;
+  constexpr int NSP = NP * NP * NP - (NP - 2) * (NP - 2) * (NP - 2);
+  __shared__ double s_den[NSP * 4];
+  __shared__ short s_c2s[NC];
+  __shared__ double2 s_w[N];
+  __shared__ double2 X1[NP * NP * H];
+  __shared__ double2 X2[NP * N * H];
+  const int slot = blockIdx.x, r = blockIdx.y, t = threadIdx.x;
+  const double* den = upeden + ((long)slot_node[slot] * Rc + r) * pln;
+  for (int i = t; i < pln; i += 256) s_den[i] = den[i] * dscale[i % sm];
+  for (int i = t; i < NC; i += 256) s_c2s[i] = c2s[i];
+  if (t < N) s_w[t] = tw[t];
+  __syncthreads();
+  double2* out = effden + ((long)slot * Rc + r) * sm * C;
+  for (int f = 0; f < sm; f++) {
+    for (int o = t; o < NP * NP * H; o += 256) {  // along x: X1[k][j][kx]
+      int kx = o % H, j = (o / H) % NP, k = o / (H * NP);
+      double re = 0, im = 0;
+#pragma unroll
+      for (int i = 0; i < NP; i++) {
+        int si = s_c2s[(i * NP + j) * NP + k];
+        if (si >= 0) {
+          double v = s_den[si * sm + f];
+          double2 w = s_w[((i + NP / 2) * kx) % N];
+          re += v * w.x;
+          im += v * w.y;
+        }
+      }
+      X1[o] = make_double2(re, im);
+    }
+    __syncthreads();
+    for (int o = t; o < NP * N * H; o += 256) {  // along y: X2[k][ky][kx]
+      int kx = o % H, ky = (o / H) % N, k = o / (H * N);
+      double2 acc = make_double2(0, 0);
+#pragma unroll
+      for (int j = 0; j < NP; j++) {
+        double2 v = cmul(X1[(k * NP + j) * H + kx], s_w[((j + NP / 2) * ky) % N]);
+        acc.x += v.x;
+        acc.y += v.y;
+      }
+      X2[o] = acc;
+    }
+    __syncthreads();
+    for (int o = t; o < C; o += 256) {  // along z
+      int kx = o % H, ky = (o / H) % N, kz = o / (H * N);
+      double2 acc = make_double2(0, 0);
+#pragma unroll
+      for (int k = 0; k < NP; k++) {
+        double2 v = cmul(X2[(k * N + ky) * H + kx], s_w[((k + NP / 2) * kz) % N]);
+        acc.x += v.x;
+        acc.y += v.y;
+      }
+      out[(long)f * C + o] = acc;
+    }
+    __syncthreads();
+  }
+}
+
+template <int NP>
+__global__ __launch_bounds__(256) void k_fft_inv(const int* __restrict__ slot_node, int Rc,
+                                                 const double2* __restrict__ effval, int tm,
+                                                 const int* __restrict__ s2c, int nsp, double* __restrict__ dncval,
+                                                 int pln, const double2* __restrict__ tw)
+{
+  constexpr int N = 2 * NP, H = NP + 1, C = N * N * H;
+  __shared__ double2 s_x[C];           // spectrum, later Y2[k][j][kx]
+  __shared__ double2 Y1[NP * N * H];   // [k][ky][kx]
+  __shared__ double2 s_w[N];
+  const int slot = blockIdx.x, r = blockIdx.y, t = threadIdx.x;
+  if (t < N) s_w[t] = tw[t];
+  double* out = dncval + ((long)slot_node[slot] * Rc + r) * pln;
+  const double2* in = effval + ((long)slot * Rc + r) * tm * C;
+  for (int f = 0; f < tm; f++) {
+    __syncthreads();
+    for (int o = t; o < C; o += 256) s_x[o] = in[(long)f * C + o];
+    __syncthreads();
+    for (int o = t; o < NP * N * H; o += 256) {  // along z, only the cube's planes
+      int kx = o % H, ky = (o / H) % N, k = o / (H * N);
+      double2 acc = make_double2(0, 0);
+#pragma unroll
+      for (int kz = 0; kz < N; kz++) {
+        double2 w = s_w[((k + NP / 2) * kz) % N];
+        double2 v = cmul(s_x[(kz * N + ky) * H + kx], make_double2(w.x, -w.y));
+        acc.x += v.x;
+        acc.y += v.y;
+      }
+      Y1[o] = acc;
+    }
+    __syncthreads();
+    for (int o = t; o < NP * NP * H; o += 256) {  // along y, only the cube's rows
+      int kx = o % H, j = (o / H) % NP, k = o / (H * NP);
+      double2 acc = make_double2(0, 0);
+#pragma unroll
+      for (int ky = 0; ky < N; ky++) {
+        double2 w = s_w[((j + NP / 2) * ky) % N];
+        double2 v = cmul(Y1[(k * N + ky) * H + kx], make_double2(w.x, -w.y));
+        acc.x += v.x;
+        acc.y += v.y;
+      }
+      s_x[o] = acc;
+    }
+    __syncthreads();
+    for (int p = t; p < nsp; p += 256) {  // along x (c2r) at the surface points
+      int cc = s2c[p];
+      int i = cc / (NP * NP), j = (cc / NP) % NP, k = cc % NP;
+      const double2* y = &s_x[(k * NP + j) * H];
+      int xi = i + NP / 2;
+      double v = y[0].x + ((xi & 1) ? -y[NP].x : y[NP].x);
+#pragma unroll
+      for (int kx = 1; kx < NP; kx++) {
+        double2 w = s_w[(xi * kx) % N];
+        v += 2.0 * (y[kx].x * w.x + y[kx].y * w.y);
+      }
+      out[p * tm + f] += v;
+    }
+  }
+}
+
+void fft_fwd(hipStream_t s, int np, int nslot, const int* slot_node, int Rc, const double* upeden, int pln, int sm,
+             const double* dscale, const short* cube2surf, double2* effden, const double2* tw)
+{
+  if (nslot <= 0) return;
+  dim3 grid(nslot, Rc);
+  switch (np) {
+  case 4: hipLaunchKernelGGL(k_fft_fwd<4>, grid, dim3(256), 0, s, slot_node, Rc, upeden, pln, sm, dscale, cube2surf, effden, tw); break;
+  case 6: hipLaunchKernelGGL(k_fft_fwd<6>, grid, dim3(256), 0, s, slot_node, Rc, upeden, pln, sm, dscale, cube2surf, effden, tw); break;
+  case 8: hipLaunchKernelGGL(k_fft_fwd<8>, grid, dim3(256), 0, s, slot_node, Rc, upeden, pln, sm, dscale, cube2surf, effden, tw); break;
+  }
+}
+
+void fft_inv(hipStream_t s, int np, int nslot, const int* slot_node, int Rc, const double2* effval, int tm,
+             const int* surf2cube, int nsp, double* dncval, int pln, const double2* tw)
+{
+  if (nslot <= 0) return;
+  dim3 grid(nslot, Rc);
+  switch (np) {
+  case 4: hipLaunchKernelGGL(k_fft_inv<4>, grid, dim3(256), 0, s, slot_node, Rc, effval, tm, surf2cube, nsp, dncval, pln, tw); break;
+  case 6: hipLaunchKernelGGL(k_fft_inv<6>, grid, dim3(256), 0, s, slot_node, Rc, effval, tm, surf2cube, nsp, dncval, pln, tw); break;
+  case 8: hipLaunchKernelGGL(k_fft_inv<8>, grid, dim3(256), 0, s, slot_node, Rc, effval, tm, surf2cube, nsp, dncval, pln, tw); break;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// M2L products in the frequency domain, one frequency per lane:
+// effval[t][r][i][q] = sum_{V pairs} sum_j T_op[j][i][q] effden[s][r][j][q]
+template <int SM, int TM, int RB>
+__global__ __launch_bounds__(256) void k_m2l(int C, int Rc, const int* __restrict__ vptr, const int* __restrict__ vsrc,
+                                             const int* __restrict__ vop, const double2* __restrict__ ops,
+                                             const double2* __restrict__ effden, double2* __restrict__ effval, int roff)
+{
+  const int q = blockIdx.y * 256 + threadIdx.x;
+  const int tr = blockIdx.x;
+  const int r0 = roff + blockIdx.z * RB;
+  if (q >= C) return;
+  double2 acc[RB][TM];
+#pragma unroll
+  for (int rr = 0; rr < RB; rr++)
+#pragma unroll
+    for (int i = 0; i < TM; i++) acc[rr][i] = make_double2(0, 0);
+  for (int e = vptr[tr]; e < vptr[tr + 1]; e++) {
+    const double2* T = ops + (long)vop[e] * SM * TM * C + q;
+    double2 tv[SM][TM];
+#pragma unroll
+    for (int j = 0; j < SM; j++)
+#pragma unroll
+      for (int i = 0; i < TM; i++) tv[j][i] = T[(long)(j * TM + i) * C];
+    const double2* D = effden + ((long)vsrc[e] * Rc + r0) * SM * C + q;
+#pragma unroll
+    for (int rr = 0; rr < RB; rr++) {
+#pragma unroll
+      for (int j = 0; j < SM; j++) {
+        double2 d = D[((long)rr * SM + j) * C];
+#pragma unroll
+        for (int i = 0; i < TM; i++) {
+          acc[rr][i].x += tv[j][i].x * d.x - tv[j][i].y * d.y;
+          acc[rr][i].y += tv[j][i].x * d.y + tv[j][i].y * d.x;
+        }
+      }
+    }
+  }
+  double2* out = effval + ((long)tr * Rc + r0) * TM * C + q;
+#pragma unroll
+  for (int rr = 0; rr < RB; rr++)
+#pragma unroll
+    for (int i = 0; i < TM; i++) out[((long)rr * TM + i) * C] = acc[rr][i];
+}
+
+template <int SM, int TM>
+static void m2l_st(hipStream_t s, int C, int ntrg, int Rc, const int* vptr, const int* vsrc, const int* vop,
+                   const double2* ops, const double2* effden, double2* effval)
+{
+  int r = 0;
+  while (r < Rc) {
+    int left = Rc - r;
+    int RB = left >= 4 ? 4 : 1;
+    int nb = left / RB;
+    dim3 grid(ntrg, (C + 255) / 256, nb);
+    if (RB == 4)
+      hipLaunchKernelGGL((k_m2l<SM, TM, 4>), grid, dim3(256), 0, s, C, Rc, vptr, vsrc, vop, ops, effden, effval, r);
+    else
+      hipLaunchKernelGGL((k_m2l<SM, TM, 1>), grid, dim3(256), 0, s, C, Rc, vptr, vsrc, vop, ops, effden, effval, r);
+    r += nb * RB;
+  }
+}
+
+void m2l(hipStream_t s, int sm, int tm, int C, int ntrg, int Rc, const int* vptr, const int* vsrc, const int* vop,
+         const double2* ops, const double2* effden, double2* effval)
+{
+  if (ntrg <= 0) return;
+  if (sm == 1 && tm == 1) m2l_st<1, 1>(s, C, ntrg, Rc, vptr, vsrc, vop, ops, effden, effval);
+  else if (sm == 4 && tm == 3) m2l_st<4, 3>(s, C, ntrg, Rc, vptr, vsrc, vop, ops, effden, effval);
+  else if (sm == 3 && tm == 3) m2l_st<3, 3>(s, C, ntrg, Rc, vptr, vsrc, vop, ops, effden, effval);
+}
+
+}  // namespace dev
+}  // namespace kifmm

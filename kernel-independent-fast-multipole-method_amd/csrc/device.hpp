@@ -1,0 +1,80 @@
+// device.hpp -- host-callable launchers of the gfx950 kernels in device.hip.
+// All pointers are device pointers; every launch goes to the given stream.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace kifmm {
+namespace dev {
+
+struct KC {  // kernel constants (see kernel_consts in kernels_host.cpp)
+  double a, b, c;
+};
+
+// dst[r*dst_rs + k*dof + d] = src[r*src_rs + perm[k]*dof + d]
+void gather(hipStream_t s, int n, int dof, int nr, const int* perm, const double* src, long src_rs, double* dst,
+            long dst_rs);
+// dst[r*dst_rs + perm[k]*dof + d] = src[r*src_rs + k*dof + d]
+void scatter(hipStream_t s, int n, int dof, int nr, const int* perm, const double* src, long src_rs, double* dst,
+             long dst_rs);
+
+struct Points {  // leaf-ordered source points (SoA) and densities [rhs][n*sdof]
+  const double *x, *y, *z, *nx, *ny, *nz;
+  const double* den;
+  long den_rs;
+};
+struct Targets {  // leaf-ordered targets (SoA) and values [rhs][n*tdof]
+  const double *x, *y, *z;
+  double* val;
+  long val_rs;
+};
+
+// Near field: for every chunk of <= 128 targets, sum the user kernel over the
+// source leaves ent[eptr[c] .. eptr[c+1]) (each ent = {sextbeg, sextnum}).
+void p2p(hipStream_t s, int kt, int nchunk, const int* tbeg, const int* tnum, const int* eptr, const int2* ent,
+         Points src, Targets trg, int nr, KC kc);
+
+// Points to check surface (S2M onto UC, S2L onto DC): out[(node*Rc + r)*pln + p*tdof + t] +=
+// sum over the source leaves of item i of K(ctr_i + rad_i * unit_p <- y) den(y).
+void p2s(hipStream_t s, int kt, int nitem, const int* out_node, const double4* geom, const int* eptr,
+         const int2* ent, const double* unit, int npt, Points src, double* out, int Rc, int pln, int nr, KC kc);
+
+// Equivalent surfaces to targets with the mm kernel (L2T from DE, M2T from UE).
+// Entry e: node enode[e] of buffer ekind[e] (0 = dneden, 1 = upeden), surface
+// ekind 0 -> unit_de, 1 -> unit_ue, placed at geom[e] = (cx, cy, cz, rad).
+void s2p(hipStream_t s, int kt, int nchunk, const int* tbeg, const int* tnum, const int* eptr, const int* enode,
+         const int* ekind, const double4* egeom, const double* unit_de, const double* unit_ue, int nsp,
+         const double* dneden, const double* upeden, int Rc, int pln, Targets trg, int nr, KC kc);
+
+// Batched, indexed FP64 GEMM over (node, rhs) columns:
+// C[:, cnode[n]*Rc + r] (+)= cscale .* sum_seg A_{grp,seg} (bscale .* B[:, bnode[seg*ncol + n]*Rc + r])
+// for the nodes n in [gbeg[grp], gbeg[grp+1]); bnode < 0 contributes nothing.
+struct GemmArgs {
+  int M, Kseg, nseg, Rc, ngroup, maxcols;
+  const double* A;
+  long a_gstride;
+  const double* B;
+  const int* bnode;
+  int ncol;
+  double* C;
+  const int* cnode;
+  const int* gbeg;
+  const double* bscale;
+  const double* cscale;
+  int accumulate;
+};
+void gemm(hipStream_t s, const GemmArgs& a);
+
+// r2c of the scaled upward equivalent densities on the (2np)^3 grid
+// -> effden[slot][Rc][sm][C] (complex).
+void fft_fwd(hipStream_t s, int np, int nslot, const int* slot_node, int Rc, const double* upeden, int pln, int sm,
+             const double* dscale, const short* cube2surf, double2* effden, const double2* tw);
+// M2L products summed over V pairs -> effval[trg][Rc][tm][C] (overwritten).
+void m2l(hipStream_t s, int sm, int tm, int C, int ntrg, int Rc, const int* vptr, const int* vsrc, const int* vop,
+         const double2* ops, const double2* effden, double2* effval);
+// c2r and gather onto the DC surface: dncval[(node*Rc + r)*pln + p*tm + f] += ...
+void fft_inv(hipStream_t s, int np, int nslot, const int* slot_node, int Rc, const double2* effval, int tm,
+             const int* surf2cube, int nsp, double* dncval, int pln, const double2* tw);
+
+}  // namespace dev
+}  // namespace kifmm

@@ -1,0 +1,593 @@
+// engine.cpp -- plan construction, device memory and the evaluate pipeline.
+//
+// Stage order and the W/X direct-versus-expansion decisions follow
+// fmm3d/fmm3d_eval.cpp:24-243; each stage runs as batched launches over
+// whole levels (up/down GEMMs, V-list FFTs) or over all leaves at once
+// (near field, L2T/M2T, S2M, S2L).
+#include "engine.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace kifmm {
+
+#define HIPCHK(call)                                                                                        \
+  do {                                                                                                      \
+    hipError_t e_ = (call);                                                                                 \
+    if (e_ != hipSuccess)                                                                                   \
+      throw Error(e_ == hipErrorOutOfMemory ? KIFMM3D_ENOMEM : KIFMM3D_EDEVICE,                             \
+                  std::string(#call) + ": " + hipGetErrorString(e_));                                       \
+  } while (0)
+
+template <class T>
+void DVec<T>::reset()
+{
+  if (p) (void)hipFree(p);
+  p = nullptr;
+  n = 0;
+}
+template <class T>
+void DVec<T>::alloc(size_t count)
+{
+  reset();
+  if (count == 0) count = 1;
+  HIPCHK(hipMalloc((void**)&p, count * sizeof(T)));
+  n = count;
+}
+template <class T>
+void DVec<T>::upload(const std::vector<T>& v)
+{
+  alloc(v.size());
+  if (!v.empty()) HIPCHK(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+}
+template <class T>
+void DVec<T>::ensure(size_t count)
+{
+  if (count > n || p == nullptr) alloc(count);
+}
+template struct DVec<double>;
+template struct DVec<int>;
+template struct DVec<short>;
+
+namespace {
+
+size_t pool_i(std::vector<int>& I, const std::vector<int>& v)
+{
+  while (I.size() % 4) I.push_back(0);
+  size_t off = I.size();
+  I.insert(I.end(), v.begin(), v.end());
+  return off;
+}
+size_t pool_d(std::vector<double>& D, const std::vector<double>& v)
+{
+  while (D.size() % 4) D.push_back(0.0);
+  size_t off = D.size();
+  D.insert(D.end(), v.begin(), v.end());
+  return off;
+}
+
+}  // namespace
+
+Engine::Engine(int kt, const double* coefs, int ncoefs, int np, int ptsmax, int maxlevel, const double* ctr,
+               int rootlvl)
+{
+  user_ = kernel_info(kt, coefs, ncoefs);
+  mm_ = kernel_info(mm_kernel_type(kt), user_.coefs, 2);
+  kcu_ = kernel_consts(user_);
+  kcm_ = kernel_consts(mm_);
+  if (np != 4 && np != 6 && np != 8) throw Error(KIFMM3D_EINVAL, "np must be 4, 6 or 8");
+  if (ptsmax < 1) throw Error(KIFMM3D_EINVAL, "ptsmax must be positive");
+  if (maxlevel < 1 || maxlevel > 16) throw Error(KIFMM3D_EINVAL, "maxlevel must be in [1, 16]");
+  np_ = np;
+  rootlvl_ = rootlvl;
+  for (int d = 0; d < 3; d++) ctr_[d] = ctr ? ctr[d] : 0.0;
+  tree_.ptsmax = ptsmax;
+  tree_.maxlevel = maxlevel;
+  tree_.rootlvl = rootlvl;
+  for (int d = 0; d < 3; d++) tree_.ctr[d] = ctr_[d];
+}
+
+Engine::~Engine()
+{
+  if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+void Engine::setup(const double* spos, const double* snor, int ns, const double* tpos, int nt)
+{
+  if (ns < 0 || nt < 0 || (ns > 0 && !spos) || (nt > 0 && !tpos)) throw Error(KIFMM3D_EINVAL, "bad point arrays");
+  if (user_.needs_normal && ns > 0 && !snor) throw Error(KIFMM3D_EINVAL, "this kernel needs source normals");
+  setup_done_ = false;
+  host_done_ = false;
+  ops_ = get_operators(mm_, np_);
+  Tree t;
+  t.ptsmax = tree_.ptsmax;
+  t.maxlevel = tree_.maxlevel;
+  t.rootlvl = tree_.rootlvl;
+  for (int d = 0; d < 3; d++) t.ctr[d] = tree_.ctr[d];
+  t.build(spos, ns, tpos, nt);
+  tree_ = std::move(t);
+  ns_ = ns;
+  nt_ = nt;
+  build_plan();
+  host_done_ = true;
+  // device phase
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+    throw Error(KIFMM3D_EDEVICE, "no HIP device is visible (tree, lists and operators are built)");
+  if (!stream_) HIPCHK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+  // leaf-ordered coordinates (fmm3d_setup.cpp:65-123) as SoA
+  std::vector<double> ax(ns), ay(ns), az(ns);
+  for (int k = 0; k < tree_.sextcnt; k++) {
+    const double* p = spos + 3 * (size_t)tree_.sperm[k];
+    ax[k] = p[0]; ay[k] = p[1]; az[k] = p[2];
+  }
+  sx_.upload(ax); sy_.upload(ay); sz_.upload(az);
+  if (user_.needs_normal) {
+    for (int k = 0; k < tree_.sextcnt; k++) {
+      const double* p = snor + 3 * (size_t)tree_.sperm[k];
+      ax[k] = p[0]; ay[k] = p[1]; az[k] = p[2];
+    }
+    snx_.upload(ax); sny_.upload(ay); snz_.upload(az);
+  }
+  std::vector<double> bx(nt), by(nt), bz(nt);
+  for (int k = 0; k < tree_.textcnt; k++) {
+    const double* p = tpos + 3 * (size_t)tree_.tperm[k];
+    bx[k] = p[0]; by[k] = p[1]; bz[k] = p[2];
+  }
+  tx_.upload(bx); ty_.upload(by); tz_.upload(bz);
+  sperm_.upload(tree_.sperm);
+  tperm_.upload(tree_.tperm);
+  upload_static();
+  HIPCHK(hipDeviceSynchronize());
+  setup_done_ = true;
+}
+
+void Engine::upload_static()
+{
+  const Operators& O = *ops_;
+  for (int t = 0; t < 4; t++) unit_[t].upload(O.unit[t]);
+  uc2ue_.upload(O.uc2ue);
+  ue2uc_.upload(O.ue2uc);
+  dc2de_.upload(O.dc2de);
+  de2dc_.upload(O.de2dc);
+  m2l_.upload(O.m2l);
+  c2s_.upload(O.cube2surf);
+  std::vector<int> s2c(O.nsp[SURF_DC], 0);
+  for (int c = 0; c < np_ * np_ * np_; c++)
+    if (O.cube2surf[c] >= 0) s2c[O.cube2surf[c]] = c;
+  s2c_.upload(s2c);
+  std::vector<double> tw(2 * O.n);
+  for (int m = 0; m < O.n; m++) {
+    tw[2 * m] = std::cos(2.0 * M_PI * m / O.n);
+    tw[2 * m + 1] = -std::sin(2.0 * M_PI * m / O.n);
+  }
+  tw_.upload(tw);
+  I_.upload(plan_.I);
+  D_.upload(plan_.D);
+}
+
+void Engine::build_plan()
+{
+  const Tree& T = tree_;
+  const Operators& O = *ops_;
+  Plan P;
+  const int sdof = user_.sdof, tdof = user_.tdof, sm = mm_.sdof;
+  const int pUE = O.pln[SURF_UE], pUC = O.pln[SURF_UC], pDE = O.pln[SURF_DE], pDC = O.pln[SURF_DC];
+  auto geom = [&](int g, std::vector<double>& D) {
+    double c[3];
+    T.center(g, c);
+    D.push_back(c[0]); D.push_back(c[1]); D.push_back(c[2]); D.push_back(T.radius(g));
+  };
+  auto scales = [&](int l, int sign, int len) {
+    std::vector<double> v(len);
+    for (int i = 0; i < len; i++) v[i] = std::pow(2.0, sign * l * mm_.deg[i % sm]);
+    return v;
+  };
+  std::vector<std::vector<int>> by_depth(std::max(T.nlevel, 1));
+  for (int g = 0; g < T.nnode; g++) by_depth[T.depth[g]].push_back(g);
+  for (int g = 0; g < T.nnode; g++) {
+    if ((T.tag[g] & NODE_SRC) && T.depth[g] < 2) P.nlow_s = std::max(P.nlow_s, T.sndeidx[g] + 1);
+    if ((T.tag[g] & NODE_TRG) && T.depth[g] < 2) P.nlow_t = std::max(P.nlow_t, T.tndeidx[g] + 1);
+  }
+  auto list = [&](int l, int g) {
+    return std::make_pair(T.lidx[l].begin() + T.lptr[l][g], T.lidx[l].begin() + T.lptr[l][g + 1]);
+  };
+
+  // S2M items: source leaves at depth >= 2 (fmm3d_eval.cpp:72-77)
+  {
+    std::vector<int> out, eptr(1, 0), ent;
+    std::vector<double> gm;
+    for (int g = 0; g < T.nnode; g++)
+      if ((T.tag[g] & NODE_SRC) && T.terminal(g) && T.depth[g] >= 2) {
+        out.push_back(T.sndeidx[g]);
+        geom(g, gm);
+        ent.push_back(T.sextbeg[g]);
+        ent.push_back(T.sextnum[g]);
+        eptr.push_back((int)ent.size() / 2);
+      }
+    P.s2m.n = (int)out.size();
+    P.s2m.out = pool_i(P.I, out);
+    P.s2m.eptr = pool_i(P.I, eptr);
+    P.s2m.ent = pool_i(P.I, ent);
+    P.s2m.geom = pool_d(P.D, gm);
+  }
+  // up pass, deepest level first (fmm3d_eval.cpp:69-92)
+  for (int L = T.nlevel - 1; L >= 2; L--) {
+    std::vector<int> all, inner;
+    for (int g : by_depth[L])
+      if (T.tag[g] & NODE_SRC) {
+        all.push_back(g);
+        if (!T.terminal(g)) inner.push_back(g);
+      }
+    if (!inner.empty()) {
+      GemmOp op;
+      op.M = pUC; op.Kseg = pUE; op.nseg = 8; op.ngroup = 1;
+      op.ncol = op.maxcols = (int)inner.size();
+      op.a_which = 1; op.b_buf = 1; op.c_buf = 0;
+      std::vector<int> bnode(8 * inner.size()), cnode(inner.size());
+      for (size_t n = 0; n < inner.size(); n++) {
+        int g = inner[n];
+        cnode[n] = T.sndeidx[g];
+        for (int c = 0; c < 8; c++) {
+          int ch = T.chd[g] + c;
+          bnode[c * inner.size() + n] = (T.tag[ch] & NODE_SRC) ? T.sndeidx[ch] : -1;
+        }
+      }
+      op.bnode = pool_i(P.I, bnode);
+      op.cnode = pool_i(P.I, cnode);
+      op.gbeg = pool_i(P.I, {0, (int)inner.size()});
+      op.bscale = (long)pool_d(P.D, scales(L + 1 + rootlvl_, +1, pUE));  // child level (fmm3d_eval.cpp:84)
+      P.up.push_back(op);
+    }
+    if (!all.empty()) {
+      GemmOp op;
+      op.M = pUE; op.Kseg = pUC; op.nseg = 1; op.ngroup = 1;
+      op.ncol = op.maxcols = (int)all.size();
+      op.a_which = 0; op.b_buf = 0; op.c_buf = 1;
+      std::vector<int> idx(all.size());
+      for (size_t n = 0; n < all.size(); n++) idx[n] = T.sndeidx[all[n]];
+      op.bnode = op.cnode = pool_i(P.I, idx);
+      op.gbeg = pool_i(P.I, {0, (int)all.size()});
+      op.cscale = (long)pool_d(P.D, scales(L + rootlvl_, -1, pUE));
+      P.up.push_back(op);
+    }
+  }
+  // near field, S2L items and L2T/M2T chunks, per target leaf
+  {
+    std::vector<int> tbeg, tnum, eptr(1, 0), ent;
+    std::vector<int> ftbeg, ftnum, feptr(1, 0), fnode, fkind;
+    std::vector<double> fgeom;
+    std::vector<int> xout, xeptr(1, 0), xent;
+    std::vector<double> xgeom;
+    std::vector<int> dent, sent, sknd;
+    std::vector<double> sgeom;
+    for (int g = 0; g < T.nnode; g++) {
+      if (!(T.tag[g] & NODE_TRG)) continue;
+      const bool leaf = T.terminal(g);
+      const bool xdirect = leaf && T.textnum[g] * tdof < pDC;  // fmm3d_eval.cpp:183
+      auto X = list(LIST_X, g);
+      if (!xdirect && X.first != X.second) {
+        xout.push_back(T.tndeidx[g]);
+        geom(g, xgeom);
+        for (auto it = X.first; it != X.second; ++it) {
+          xent.push_back(T.sextbeg[*it]);
+          xent.push_back(T.sextnum[*it]);
+        }
+        xeptr.push_back((int)xent.size() / 2);
+      }
+      if (!leaf) continue;
+      dent.clear(); sent.clear(); sknd.clear(); sgeom.clear();
+      long npair = 0, fpair = 0;
+      auto U = list(LIST_U, g);
+      for (auto it = U.first; it != U.second; ++it) {
+        dent.push_back(T.sextbeg[*it]); dent.push_back(T.sextnum[*it]);
+        npair += T.sextnum[*it];
+      }
+      auto W = list(LIST_W, g);
+      for (auto it = W.first; it != W.second; ++it) {
+        int v = *it;
+        if (T.terminal(v) && T.sextnum[v] * sdof < pUE) {  // fmm3d_eval.cpp:161
+          dent.push_back(T.sextbeg[v]); dent.push_back(T.sextnum[v]);
+          npair += T.sextnum[v];
+        } else {
+          sent.push_back(T.sndeidx[v]); sknd.push_back(1); geom(v, sgeom);
+          fpair += O.nsp[SURF_UE];
+        }
+      }
+      if (xdirect)
+        for (auto it = X.first; it != X.second; ++it) {
+          dent.push_back(T.sextbeg[*it]); dent.push_back(T.sextnum[*it]);
+          npair += T.sextnum[*it];
+        }
+      if (T.depth[g] >= 2) {  // L2T from DE (dneden is zero above depth 2)
+        sent.push_back(T.tndeidx[g]); sknd.push_back(0); geom(g, sgeom);
+        fpair += O.nsp[SURF_DE];
+      }
+      P.near_pairs += npair * T.textnum[g];
+      P.far_pairs += fpair * T.textnum[g];
+      for (int off = 0; off < T.textnum[g]; off += 128) {
+        int n = std::min(128, T.textnum[g] - off);
+        tbeg.push_back(T.textbeg[g] + off); tnum.push_back(n);
+        ent.insert(ent.end(), dent.begin(), dent.end());
+        eptr.push_back((int)ent.size() / 2);
+        ftbeg.push_back(T.textbeg[g] + off); ftnum.push_back(n);
+        fnode.insert(fnode.end(), sent.begin(), sent.end());
+        fkind.insert(fkind.end(), sknd.begin(), sknd.end());
+        fgeom.insert(fgeom.end(), sgeom.begin(), sgeom.end());
+        feptr.push_back((int)fnode.size());
+      }
+    }
+    P.near.n = (int)tbeg.size();
+    P.near.tbeg = pool_i(P.I, tbeg); P.near.tnum = pool_i(P.I, tnum);
+    P.near.eptr = pool_i(P.I, eptr); P.near.ent = pool_i(P.I, ent);
+    P.far.n = (int)ftbeg.size();
+    P.far.tbeg = pool_i(P.I, ftbeg); P.far.tnum = pool_i(P.I, ftnum);
+    P.far.eptr = pool_i(P.I, feptr); P.far.enode = pool_i(P.I, fnode);
+    P.far.ekind = pool_i(P.I, fkind); P.far.egeom = pool_d(P.D, fgeom);
+    P.s2l.n = (int)xout.size();
+    P.s2l.out = pool_i(P.I, xout); P.s2l.eptr = pool_i(P.I, xeptr);
+    P.s2l.ent = pool_i(P.I, xent); P.s2l.geom = pool_d(P.D, xgeom);
+  }
+  // V list per level (fmm3d_eval.cpp:115-150)
+  {
+    std::vector<int> slot(T.nnode, -1);
+    for (int L = 2; L < T.nlevel; L++) {
+      std::vector<int> trg, src, vptr(1, 0), vsrc, vop;
+      for (int g : by_depth[L]) {
+        if (!(T.tag[g] & NODE_TRG)) continue;
+        auto V = list(LIST_V, g);
+        if (V.first == V.second) continue;
+        trg.push_back(T.tndeidx[g]);
+        for (auto it = V.first; it != V.second; ++it) {
+          int v = *it;
+          if (slot[v] < 0) {
+            slot[v] = (int)src.size();
+            src.push_back(T.sndeidx[v]);
+          }
+          int d[3], linf = 0;
+          for (int k = 0; k < 3; k++) {
+            d[k] = T.path[3 * v + k] - T.path[3 * g + k];
+            linf = std::max(linf, std::abs(d[k]));
+          }
+          if (linf < 2 || linf > 3) throw Error(KIFMM3D_EINVAL, "internal: V-list offset out of range");
+          vsrc.push_back(slot[v]);
+          vop.push_back((d[0] + 3) * 49 + (d[1] + 3) * 7 + (d[2] + 3));
+        }
+        vptr.push_back((int)vsrc.size());
+      }
+      if (trg.empty()) continue;
+      VLevel vl;
+      vl.depth = L;
+      vl.nsrc = (int)src.size();
+      vl.ntrg = (int)trg.size();
+      vl.npair = (int)vsrc.size();
+      vl.src_node = pool_i(P.I, src);
+      vl.trg_node = pool_i(P.I, trg);
+      vl.vptr = pool_i(P.I, vptr);
+      vl.vsrc = pool_i(P.I, vsrc);
+      vl.vop = pool_i(P.I, vop);
+      vl.dscale = (long)pool_d(P.D, scales(L + rootlvl_, +1, sm));  // plnden2effden (matmgnt3d.cpp:226-235)
+      P.max_vsrc = std::max(P.max_vsrc, vl.nsrc);
+      P.max_vtrg = std::max(P.max_vtrg, vl.ntrg);
+      P.v_pairs += vl.npair;
+      P.vlev.push_back(vl);
+    }
+  }
+  // down pass, top level first (fmm3d_eval.cpp:195-209)
+  for (int L = 2; L < T.nlevel; L++) {
+    std::vector<int> all;
+    for (int g : by_depth[L])
+      if (T.tag[g] & NODE_TRG) all.push_back(g);
+    if (all.empty()) continue;
+    if (L >= 3) {
+      GemmOp op;
+      op.M = pDC; op.Kseg = pDE; op.nseg = 1; op.ngroup = 8;
+      op.a_which = 3; op.a_gstride = (long)pDC * pDE; op.b_buf = 3; op.c_buf = 2; op.accumulate = 1;
+      std::vector<int> cnode, bnode, gbeg(9, 0);
+      for (int c = 0; c < 8; c++) {
+        gbeg[c] = (int)cnode.size();
+        for (int g : all) {
+          int pg = T.par[g];
+          int ci = (T.path[3 * g] - 2 * T.path[3 * pg]) * 4 + (T.path[3 * g + 1] - 2 * T.path[3 * pg + 1]) * 2 +
+                   (T.path[3 * g + 2] - 2 * T.path[3 * pg + 2]);
+          if (ci != c) continue;
+          cnode.push_back(T.tndeidx[g]);
+          bnode.push_back(T.tndeidx[pg]);
+        }
+        op.maxcols = std::max(op.maxcols, (int)cnode.size() - gbeg[c]);
+      }
+      gbeg[8] = (int)cnode.size();
+      op.ncol = (int)cnode.size();
+      op.cnode = pool_i(P.I, cnode);
+      op.bnode = pool_i(P.I, bnode);
+      op.gbeg = pool_i(P.I, gbeg);
+      op.bscale = (long)pool_d(P.D, scales(L - 1 + rootlvl_, +1, pDE));  // parent level (fmm3d_eval.cpp:203)
+      P.down.push_back(op);
+    }
+    GemmOp op;
+    op.M = pDE; op.Kseg = pDC; op.nseg = 1; op.ngroup = 1;
+    op.ncol = op.maxcols = (int)all.size();
+    op.a_which = 2; op.b_buf = 2; op.c_buf = 3;
+    std::vector<int> idx(all.size());
+    for (size_t n = 0; n < all.size(); n++) idx[n] = T.tndeidx[all[n]];
+    op.bnode = op.cnode = pool_i(P.I, idx);
+    op.gbeg = pool_i(P.I, {0, (int)all.size()});
+    op.cscale = (long)pool_d(P.D, scales(L + rootlvl_, -1, pDE));
+    P.down.push_back(op);
+  }
+  (void)pDC;
+  plan_ = std::move(P);
+}
+
+int Engine::choose_rhs_chunk(int nrhs)
+{
+  const Operators& O = *ops_;
+  const Tree& T = tree_;
+  double per = 8.0 * ((double)T.sextcnt * user_.sdof + (double)T.textcnt * user_.tdof +
+                      (double)T.sndecnt * (O.pln[SURF_UC] + O.pln[SURF_UE]) +
+                      (double)T.tndecnt * (O.pln[SURF_DC] + O.pln[SURF_DE])) +
+               16.0 * O.C * ((double)plan_.max_vsrc * mm_.sdof + (double)plan_.max_vtrg * mm_.tdof);
+  size_t fr = 0, tot = 0;
+  HIPCHK(hipMemGetInfo(&fr, &tot));
+  double budget = 0.6 * (double)fr + 8.0 * (double)(sden_.n + tval_.n + upcval_.n + upeden_.n + dncval_.n + dneden_.n) +
+                  8.0 * (double)(effden_.n + effval_.n);
+  int rc = (int)std::max(1.0, std::floor(budget / std::max(per, 1.0)));
+  return std::min(rc, nrhs);
+}
+
+void Engine::ensure_work(int rc)
+{
+  const Operators& O = *ops_;
+  const Tree& T = tree_;
+  sden_.ensure((size_t)rc * T.sextcnt * user_.sdof);
+  tval_.ensure((size_t)rc * T.textcnt * user_.tdof);
+  upcval_.ensure((size_t)rc * T.sndecnt * O.pln[SURF_UC]);
+  upeden_.ensure((size_t)rc * T.sndecnt * O.pln[SURF_UE]);
+  dncval_.ensure((size_t)rc * T.tndecnt * O.pln[SURF_DC]);
+  dneden_.ensure((size_t)rc * T.tndecnt * O.pln[SURF_DE]);
+  effden_.ensure((size_t)2 * rc * plan_.max_vsrc * mm_.sdof * O.C);
+  effval_.ensure((size_t)2 * rc * plan_.max_vtrg * mm_.tdof * O.C);
+}
+
+void Engine::evaluate_device(const double* d_srcden, double* d_trgval, int nrhs, int mode)
+{
+  if (!setup_done_) throw Error(KIFMM3D_ESTATE, "evaluate called before setup");
+  if (nrhs < 1) throw Error(KIFMM3D_EINVAL, "nrhs must be >= 1");
+  if (mode != KIFMM3D_MODE_PLAIN) throw Error(KIFMM3D_EINVAL, "unsupported evaluation mode");
+  if ((ns_ > 0 && !d_srcden) || (nt_ > 0 && !d_trgval)) throw Error(KIFMM3D_EINVAL, "null density or value pointer");
+  for (double& m : stage_ms) m = 0.0;
+  if (nt_ == 0) return;
+  int rc = choose_rhs_chunk(nrhs);
+  ensure_work(rc);
+  hipEvent_t ev[9];
+  for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+  try {
+    for (int r0 = 0; r0 < nrhs; r0 += rc) {
+      int n = std::min(rc, nrhs - r0);
+      run_chunk(d_srcden + (size_t)r0 * ns_ * user_.sdof, d_trgval + (size_t)r0 * nt_ * user_.tdof, n, ev);
+      HIPCHK(hipEventSynchronize(ev[8]));
+      for (int s = 0; s < 8; s++) {
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, ev[s], ev[s + 1]));
+        stage_ms[s] += ms;
+      }
+      last_rc_ = n;
+    }
+  } catch (...) {
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    throw;
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  double tot = 0;
+  for (int s = 0; s < 8; s++) tot += stage_ms[s];
+  stage_ms[KIFMM3D_STAGE_TOTAL] = tot;
+}
+
+void Engine::evaluate_host(const double* srcden, double* trgval, int nrhs, int mode)
+{
+  if (!setup_done_) throw Error(KIFMM3D_ESTATE, "evaluate called before setup");
+  if (nrhs < 1) throw Error(KIFMM3D_EINVAL, "nrhs must be >= 1");
+  size_t ns = (size_t)nrhs * ns_ * user_.sdof, nt = (size_t)nrhs * nt_ * user_.tdof;
+  hsrc_.ensure(ns);
+  htrg_.ensure(nt);
+  if (ns) HIPCHK(hipMemcpy(hsrc_.p, srcden, ns * sizeof(double), hipMemcpyHostToDevice));
+  evaluate_device(hsrc_.p, htrg_.p, nrhs, mode);
+  if (nt) HIPCHK(hipMemcpy(trgval, htrg_.p, nt * sizeof(double), hipMemcpyDeviceToHost));
+}
+
+void Engine::run_chunk(const double* srcden, double* trgval, int rc, hipEvent_t* ev)
+{
+  const Operators& O = *ops_;
+  const Tree& T = tree_;
+  const Plan& P = plan_;
+  hipStream_t s = stream_;
+  const int* I = I_.p;
+  const double* D = D_.p;
+  const int sdof = user_.sdof, tdof = user_.tdof, sm = mm_.sdof, tm = mm_.tdof;
+  const int pUE = O.pln[SURF_UE], pUC = O.pln[SURF_UC], pDE = O.pln[SURF_DE], pDC = O.pln[SURF_DC];
+  const dev::KC kcu{kcu_.a, kcu_.b, kcu_.c}, kcm{kcm_.a, kcm_.b, kcm_.c};
+  double* bufs[4] = {upcval_.p, upeden_.p, dncval_.p, dneden_.p};
+  const double* opsA[4] = {uc2ue_.p, ue2uc_.p, dc2de_.p, de2dc_.p};
+  const long srs = (long)T.sextcnt * sdof, trs = (long)T.textcnt * tdof;
+  dev::Points src{sx_.p, sy_.p, sz_.p, snx_.p, sny_.p, snz_.p, sden_.p, srs};
+  dev::Targets trg{tx_.p, ty_.p, tz_.p, tval_.p, trs};
+
+  auto run_gemm = [&](const GemmOp& op) {
+    dev::GemmArgs a;
+    a.M = op.M; a.Kseg = op.Kseg; a.nseg = op.nseg; a.Rc = rc; a.ngroup = op.ngroup; a.maxcols = op.maxcols;
+    a.A = opsA[op.a_which]; a.a_gstride = op.a_gstride;
+    a.B = bufs[op.b_buf]; a.bnode = I + op.bnode; a.ncol = op.ncol;
+    a.C = bufs[op.c_buf]; a.cnode = I + op.cnode; a.gbeg = I + op.gbeg;
+    a.bscale = op.bscale >= 0 ? D + op.bscale : nullptr;
+    a.cscale = op.cscale >= 0 ? D + op.cscale : nullptr;
+    a.accumulate = op.accumulate;
+    dev::gemm(s, a);
+  };
+
+  HIPCHK(hipEventRecord(ev[0], s));
+  // zero the accumulated buffers (fmm3d_eval.cpp:33-42)
+  HIPCHK(hipMemsetAsync(tval_.p, 0, sizeof(double) * (size_t)rc * trs, s));
+  HIPCHK(hipMemsetAsync(upcval_.p, 0, sizeof(double) * (size_t)rc * T.sndecnt * pUC, s));
+  HIPCHK(hipMemsetAsync(dncval_.p, 0, sizeof(double) * (size_t)rc * T.tndecnt * pDC, s));
+  if (P.nlow_s) HIPCHK(hipMemsetAsync(upeden_.p, 0, sizeof(double) * (size_t)rc * P.nlow_s * pUE, s));
+  if (P.nlow_t) HIPCHK(hipMemsetAsync(dneden_.p, 0, sizeof(double) * (size_t)rc * P.nlow_t * pDE, s));
+  // load (fmm3d_eval.cpp:50-64)
+  dev::gather(s, T.sextcnt, sdof, rc, sperm_.p, srcden, (long)ns_ * sdof, sden_.p, srs);
+  HIPCHK(hipEventRecord(ev[1], s));
+  // up: S2M, then M2M + UC2UE level by level
+  dev::p2s(s, user_.kt, P.s2m.n, I + P.s2m.out, (const double4*)(D + P.s2m.geom), I + P.s2m.eptr,
+           (const int2*)(I + P.s2m.ent), unit_[SURF_UC].p, O.nsp[SURF_UC], src, upcval_.p, rc, pUC, rc, kcu);
+  for (const GemmOp& op : P.up) run_gemm(op);
+  HIPCHK(hipEventRecord(ev[2], s));
+  // U list and direct W/X
+  dev::p2p(s, user_.kt, P.near.n, I + P.near.tbeg, I + P.near.tnum, I + P.near.eptr, (const int2*)(I + P.near.ent),
+           src, trg, rc, kcu);
+  HIPCHK(hipEventRecord(ev[3], s));
+  // V list
+  for (const VLevel& vl : P.vlev) {
+    dev::fft_fwd(s, np_, vl.nsrc, I + vl.src_node, rc, upeden_.p, pUE, sm, D + vl.dscale, c2s_.p,
+                 (double2*)effden_.p, (const double2*)tw_.p);
+    dev::m2l(s, sm, tm, O.C, vl.ntrg, rc, I + vl.vptr, I + vl.vsrc, I + vl.vop, (const double2*)m2l_.p,
+             (const double2*)effden_.p, (double2*)effval_.p);
+    dev::fft_inv(s, np_, vl.ntrg, I + vl.trg_node, rc, (const double2*)effval_.p, tm, s2c_.p, O.nsp[SURF_DC],
+                 dncval_.p, pDC, (const double2*)tw_.p);
+  }
+  HIPCHK(hipEventRecord(ev[4], s));
+  // X list S2L
+  dev::p2s(s, user_.kt, P.s2l.n, I + P.s2l.out, (const double4*)(D + P.s2l.geom), I + P.s2l.eptr,
+           (const int2*)(I + P.s2l.ent), unit_[SURF_DC].p, O.nsp[SURF_DC], src, dncval_.p, rc, pDC, rc, kcu);
+  HIPCHK(hipEventRecord(ev[5], s));
+  // down: L2L + DC2DE level by level
+  for (const GemmOp& op : P.down) run_gemm(op);
+  HIPCHK(hipEventRecord(ev[6], s));
+  // L2T + M2T
+  dev::s2p(s, mm_.kt, P.far.n, I + P.far.tbeg, I + P.far.tnum, I + P.far.eptr, I + P.far.enode, I + P.far.ekind,
+           (const double4*)(D + P.far.egeom), unit_[SURF_DE].p, unit_[SURF_UE].p, O.nsp[SURF_DE], dneden_.p,
+           upeden_.p, rc, pDE, trg, rc, kcm);
+  HIPCHK(hipEventRecord(ev[7], s));
+  // save (fmm3d_eval.cpp:221-235)
+  dev::scatter(s, T.textcnt, tdof, rc, tperm_.p, tval_.p, trs, trgval, (long)nt_ * tdof);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(ev[8], s));
+}
+
+long Engine::get_buffer(int what, double* buf, long n)
+{
+  if (!setup_done_ || last_rc_ == 0) return KIFMM3D_ESTATE;
+  const Operators& O = *ops_;
+  const DVec<double>* src[4] = {&upeden_, &upcval_, &dneden_, &dncval_};
+  const int pln[4] = {O.pln[SURF_UE], O.pln[SURF_UC], O.pln[SURF_DE], O.pln[SURF_DC]};
+  const int cnt[4] = {tree_.sndecnt, tree_.sndecnt, tree_.tndecnt, tree_.tndecnt};
+  if (what < 0 || what > 3) return KIFMM3D_EINVAL;
+  long tot = (long)cnt[what] * pln[what];
+  if (!buf) return tot;
+  if (n < tot) return KIFMM3D_EINVAL;
+  HIPCHK(hipStreamSynchronize(stream_));
+  for (int k = 0; k < cnt[what]; k++)
+    HIPCHK(hipMemcpy(buf + (size_t)k * pln[what], src[what]->p + (size_t)k * last_rc_ * pln[what],
+                     sizeof(double) * pln[what], hipMemcpyDeviceToHost));
+  return tot;
+}
+
+}  // namespace kifmm

@@ -1,0 +1,128 @@
+// engine.hpp -- one evaluator instance: tree, operators, device plan and the
+// evaluate pipeline (the counterpart of the reference's FMM3d object).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+#include "device.hpp"
+#include "operators.hpp"
+#include "tree.hpp"
+
+namespace kifmm {
+
+// Owning device array.
+template <class T>
+struct DVec {
+  T* p = nullptr;
+  size_t n = 0;
+  DVec() = default;
+  DVec(const DVec&) = delete;
+  DVec& operator=(const DVec&) = delete;
+  ~DVec() { reset(); }
+  void reset();
+  void alloc(size_t count);                // uninitialised
+  void upload(const std::vector<T>& v);    // alloc + copy
+  void ensure(size_t count);               // grow only
+};
+
+// A GEMM step of the up or down pass (see dev::GemmArgs).
+struct GemmOp {
+  int M = 0, Kseg = 0, nseg = 1, ngroup = 1, maxcols = 0, ncol = 0;
+  int a_which = 0;  // 0 uc2ue, 1 ue2uc, 2 dc2de, 3 de2dc
+  long a_gstride = 0;
+  int b_buf = 0, c_buf = 0;  // 0 upcval, 1 upeden, 2 dncval, 3 dneden
+  size_t bnode = 0, cnode = 0, gbeg = 0;
+  long bscale = -1, cscale = -1;
+  int accumulate = 0;
+};
+
+struct ItemList {  // point -> surface items
+  int n = 0;
+  size_t out = 0, geom = 0, eptr = 0, ent = 0;
+};
+
+struct ChunkList {  // chunks of <= 128 leaf targets
+  int n = 0;
+  size_t tbeg = 0, tnum = 0, eptr = 0;
+  size_t ent = 0;                          // near field: int2 {sextbeg, sextnum}
+  size_t enode = 0, ekind = 0, egeom = 0;  // surfaces: node, kind, (cx, cy, cz, rad)
+};
+
+struct VLevel {
+  int depth = 0, nsrc = 0, ntrg = 0, npair = 0;
+  size_t src_node = 0, trg_node = 0, vptr = 0, vsrc = 0, vop = 0;
+  long dscale = -1;
+};
+
+struct Plan {
+  ItemList s2m, s2l;
+  ChunkList near, far;
+  std::vector<GemmOp> up, down;
+  std::vector<VLevel> vlev;
+  int max_vsrc = 0, max_vtrg = 0;
+  int nlow_s = 0, nlow_t = 0;  // source / target node indices of depth < 2 form these prefixes
+  long near_pairs = 0;         // source-target pairs of the near field (one RHS)
+  long far_pairs = 0;          // surface point - target pairs of L2T + M2T
+  long v_pairs = 0;
+  std::vector<int> I;          // index pool (uploaded once)
+  std::vector<double> D;       // double pool (geometry, scales)
+};
+
+class Engine {
+ public:
+  Engine(int kt, const double* coefs, int ncoefs, int np, int ptsmax, int maxlevel, const double* ctr, int rootlvl);
+  ~Engine();
+
+  void setup(const double* spos, const double* snor, int ns, const double* tpos, int nt);
+  void evaluate_device(const double* d_srcden, double* d_trgval, int nrhs, int mode);
+  void evaluate_host(const double* srcden, double* trgval, int nrhs, int mode);
+
+  const Tree& tree() const { return tree_; }
+  const Operators& ops() const { return *ops_; }
+  const KernelInfo& user() const { return user_; }
+  bool is_setup() const { return setup_done_; }
+  bool host_ready() const { return host_done_; }
+  double stage_ms[KIFMM3D_NSTAGE] = {0};
+  long get_buffer(int what, double* buf, long n);
+  const Plan& plan() const { return plan_; }
+
+ private:
+  void build_plan();
+  void upload_static();
+  int choose_rhs_chunk(int nrhs);
+  void ensure_work(int rc);
+  void run_chunk(const double* srcden, double* trgval, int rc, hipEvent_t* ev);
+
+  KernelInfo user_, mm_;
+  KernelConsts kcu_, kcm_;
+  int np_, rootlvl_;
+  double ctr_[3];
+  std::shared_ptr<const Operators> ops_;
+  Tree tree_;
+  Plan plan_;
+  int ns_ = 0, nt_ = 0;
+  bool setup_done_ = false;
+  bool host_done_ = false;
+  int last_rc_ = 0;
+  hipStream_t stream_ = nullptr;
+
+  // device data
+  DVec<double> sx_, sy_, sz_, snx_, sny_, snz_, tx_, ty_, tz_;
+  DVec<int> sperm_, tperm_, s2c_;
+  DVec<short> c2s_;
+  DVec<double> unit_[4];
+  DVec<double> uc2ue_, ue2uc_, dc2de_, de2dc_, m2l_;
+  DVec<double> tw_;
+  DVec<int> I_;
+  DVec<double> D_;
+  // work buffers (grow with the RHS chunk)
+  DVec<double> sden_, tval_, upcval_, upeden_, dncval_, dneden_, effden_, effval_;
+  DVec<double> hsrc_, htrg_;  // staging for the host-pointer entry point
+};
+
+}  // namespace kifmm

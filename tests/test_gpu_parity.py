@@ -1,0 +1,136 @@
+"""GPU parity: the HIP evaluate pass against the CPU oracle on the same inputs.
+
+The oracle (oracle/kifmm_oracle.c) restates the reference's FMM3d::eval
+(fmm3d/fmm3d_eval.cpp:24-243).  Both sides use the product's pseudoinverses
+(kifmm3d_get_operator -> OracleFMM.set_pinv), so the comparison isolates the
+evaluate pass; the bar is 1e-12 relative L2 in FP64.  A second check against
+the FP64 direct sum (fmm3d/fmm3d_check.cpp) bounds the FMM error itself.
+"""
+import numpy as np
+import pytest
+
+import kifmm3d as K
+import oracle as O
+from problems import COEFS, clusters, densities, relerr, sphere, uniform_cube
+
+pytestmark = pytest.mark.gpu
+
+PARITY = 1e-12                        # plain FP64 GPU vs CPU restatement
+FMM_BAND = {4: 3e-3, 6: 2e-4, 8: 2e-5}  # FMM error vs direct sum, by np
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _need_gpu():
+    if K.device_count() < 1:
+        pytest.fail("no HIP device visible: the GPU tests need an MI355X")
+
+
+def make_fmm(kt, np_, ptsmax, maxlevel, ctr, rootlvl, spos, snor, tpos):
+    f = K.FMM3d("fmm3d_")
+    f.srcpos, f.srcnor, f.trgpos = spos, snor, tpos
+    f.ctr, f.rootlvl = ctr, rootlvl
+    f.knl = K.Kernel3d(kt, COEFS[kt])
+    f.setup({"-fmm3d_np": str(np_), "-fmm3d_let3d_ptsmax": str(ptsmax), "-fmm3d_let3d_maxlevel": str(maxlevel)})
+    return f
+
+
+def make_oracle(f, kt, np_, ptsmax, maxlevel, ctr, rootlvl, spos, snor, tpos):
+    o = O.OracleFMM(kt, COEFS[kt], np_, ptsmax, maxlevel, ctr, rootlvl)
+    o.setup(spos, snor, tpos, pinv=(f.operator(0), f.operator(1)))
+    return o
+
+
+def run_case(name, kt, np_, ptsmax, spos, snor, tpos, ctr, rootlvl, nrhs=1, maxlevel=10, seed=0, ncheck=100):
+    print(f"\n[case] {name} kt={kt} np={np_} ptsmax={ptsmax} ns={len(spos)} nt={len(tpos)} nrhs={nrhs}", flush=True)
+    f = make_fmm(kt, np_, ptsmax, maxlevel, ctr, rootlvl, spos, snor, tpos)
+    sdof, tdof = f.knl.sdof(), f.knl.tdof()
+    den = densities(len(spos), sdof, nrhs, seed)
+    val = f.eval(den, nrhs=nrhs).reshape(nrhs, -1)
+    o = make_oracle(f, kt, np_, ptsmax, maxlevel, ctr, rootlvl, spos, snor, tpos)
+    worst = 0.0
+    for r in range(nrhs):
+        ref = o.eval(den[r])
+        e = relerr(val[r], ref)
+        worst = max(worst, e)
+    rng = np.random.default_rng(seed + 99)
+    idx = rng.choice(len(tpos), min(ncheck, len(tpos)), replace=False)
+    d = O.direct(kt, COEFS[kt], spos, snor, den[0], tpos[idx])
+    ferr = relerr(val[0].reshape(-1, tdof)[idx], d)
+    print(f"  gpu-vs-oracle rel L2 {worst:.3e}   fmm-vs-direct {ferr:.3e}   stages {f.stage_times()}", flush=True)
+    assert np.all(np.isfinite(val))
+    assert worst <= PARITY, f"{name}: GPU differs from the oracle by {worst:.3e}"
+    assert ferr <= FMM_BAND[np_], f"{name}: FMM error {ferr:.3e} above the np={np_} band"
+    return f, o, val
+
+
+def test_laplace_uniform_np4():
+    p = uniform_cube(6000, 1)
+    run_case("laplace-uniform", 111, 4, 40, p, None, p, (0.5, 0.5, 0.5), 1)
+
+
+def test_laplace_clusters_np6_adaptive():
+    p = clusters(8000, 2)
+    f, o, _ = run_case("laplace-clusters", 111, 6, 30, p, None, p, (0.5, 0.5, 0.5), 1)
+    _, w = o.list_csr(2)
+    _, x = o.list_csr(3)
+    assert len(w) > 0 and len(x) > 0, "the clustered case must exercise W and X lists"
+
+
+def test_stokes_dl_sphere_np4():
+    p, n = sphere(6000, 3)
+    run_case("stokes-dl-sphere", 321, 4, 40, p, n, p, (0.0, 0.0, 0.0), 0)
+
+
+def test_stokes_dl_sphere_np6():
+    p, n = sphere(4000, 4, 0.9)
+    run_case("stokes-dl-sphere-np6", 321, 6, 60, p, n, p, (0.0, 0.0, 0.0), 0)
+
+
+def test_navier_clusters_np4():
+    p = clusters(6000, 5)
+    run_case("navier-clusters", 511, 4, 40, p, None, p, (0.5, 0.5, 0.5), 1)
+
+
+def test_navier_np8():
+    p = clusters(3000, 6, k=3)
+    run_case("navier-np8", 511, 8, 80, p, None, p, (0.5, 0.5, 0.5), 1)
+
+
+def test_multi_rhs_laplace():
+    p = uniform_cube(5000, 7)
+    run_case("laplace-5rhs", 111, 6, 50, p, None, p, (0.5, 0.5, 0.5), 1, nrhs=5)
+
+
+def test_multi_rhs_stokes():
+    p, n = sphere(3000, 8)
+    run_case("stokes-3rhs", 321, 4, 40, p, n, p, (0.0, 0.0, 0.0), 0, nrhs=3)
+
+
+def test_separate_targets():
+    s = clusters(4000, 9)
+    t = uniform_cube(3000, 10)
+    run_case("laplace-src-ne-trg", 111, 4, 30, s, None, t, (0.5, 0.5, 0.5), 1)
+
+
+def test_root_leaf_only():
+    p = uniform_cube(50, 11)
+    run_case("root-leaf", 111, 4, 100, p, None, p, (0.5, 0.5, 0.5), 1)
+
+
+def test_stage_buffers_match_oracle():
+    p = clusters(5000, 12)
+    f, o, _ = run_case("buffers", 111, 4, 30, p, None, p, (0.5, 0.5, 0.5), 1)
+    for w in range(4):
+        a, b = f.buffer(w), o.buffer(w)
+        e = relerr(a, b)
+        print(f"  buffer {w}: rel L2 {e:.3e}")
+        assert e <= PARITY
+
+
+def test_repeat_eval_is_deterministic():
+    p = uniform_cube(4000, 13)
+    f = make_fmm(111, 4, 40, 10, (0.5, 0.5, 0.5), 1, p, None, p)
+    den = densities(len(p), 1, 1, 13)
+    a = f.eval(den).copy()
+    b = f.eval(den).copy()
+    assert np.array_equal(a, b)
