@@ -475,7 +475,37 @@ void s2p(hipStream_t s, int kt, int nchunk, const int* tbeg, const int* tnum, co
 }
 
 // ---------------------------------------------------------------------------
-// batched indexed FP64 GEMM: 64 x 64 tile of C per block, K in steps of 16
+// batched indexed FP64 GEMM: 64 x 64 tile of C per block, K in steps of 16.
+//
+// These products feed or apply the truncated pseudoinverses, whose singular
+// values run down to 0.1^(np+1) of the largest: y = P x cancels massively, so
+// the rounding of a plain FP64 dot product depends on the summation order at
+// the 1e-11 (np 6) to 1e-9 (np 8) level of the final potential.  Each output
+// is therefore accumulated with the error-free transformations TwoProd (FMA)
+// and TwoSum (Knuth), i.e. the compensated dot product "Dot2" of Ogita, Rump
+// and Oishi (SIAM J. Sci. Comput. 26, 2005), and rounded once at the end.
+__device__ inline void dot2_acc(double& hi, double& lo, double a, double b)
+{
+#pragma clang fp contract(off)
+  double p = a * b;
+  double e = fma(a, b, -p);
+  double s = hi + p;
+  double bb = s - hi;
+  double t = (hi - (s - bb)) + (p - bb);
+  hi = s;
+  lo += t + e;
+}
+
+// (o + hi + lo) rounded once (up to the last TwoSum)
+__device__ inline double dot2_finish(double o, double hi, double lo)
+{
+#pragma clang fp contract(off)
+  double s = o + hi;
+  double bb = s - o;
+  double t = (o - (s - bb)) + (hi - bb);
+  return s + (t + lo);
+}
+
 __global__ __launch_bounds__(256) void k_gemm(GemmArgs p)
 {
   const int grp = blockIdx.z;
@@ -488,11 +518,11 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs p)
   __shared__ double As[16][65];
   __shared__ double Bs[16][65];
   const int t = threadIdx.x, tx = t & 15, ty = t >> 4;
-  double acc[4][4];
+  double hi[4][4], lo[4][4];
 #pragma unroll
   for (int i = 0; i < 4; i++)
 #pragma unroll
-    for (int j = 0; j < 4; j++) acc[i][j] = 0.0;
+    for (int j = 0; j < 4; j++) hi[i][j] = lo[i][j] = 0.0;
   for (int k0 = 0; k0 < K; k0 += 16) {
 #pragma unroll
     for (int i = 0; i < 4; i++) {
@@ -512,13 +542,13 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs p)
         int bn = p.bnode[(long)seg * p.ncol + n];
         if (bn >= 0) {
           v = p.B[((long)bn * p.Rc + r) * p.Kseg + kin];
-          if (p.bscale) v *= p.bscale[kin];
+          if (p.bscale) v *= p.bscale[kin];  // power of two: exact
         }
       }
       Bs[kk][cl] = v;
     }
     __syncthreads();
-#pragma unroll
+#pragma unroll 4
     for (int kk = 0; kk < 16; kk++) {
       double a[4], b[4];
 #pragma unroll
@@ -528,7 +558,7 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs p)
 #pragma unroll
       for (int i = 0; i < 4; i++)
 #pragma unroll
-        for (int j = 0; j < 4; j++) acc[i][j] += a[i] * b[j];
+        for (int j = 0; j < 4; j++) dot2_acc(hi[i][j], lo[i][j], a[i], b[j]);
     }
     __syncthreads();
   }
@@ -542,9 +572,12 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs p)
     for (int i = 0; i < 4; i++) {
       int row = rb + tx + 16 * i;
       if (row >= p.M) continue;
-      double v = acc[i][j];
-      if (p.cscale) v *= p.cscale[row];
-      out[row] = p.accumulate ? out[row] + v : v;
+      double h = hi[i][j], l = lo[i][j];
+      if (p.cscale) {  // power of two: exact
+        h *= p.cscale[row];
+        l *= p.cscale[row];
+      }
+      out[row] = dot2_finish(p.accumulate ? out[row] : 0.0, h, l);
     }
   }
 }

@@ -67,6 +67,29 @@ static void iv_sort_unique(ivec* v)
 /* kernels: fmm3d/kernel3d.cpp                                          */
 static const double MINDIF = 1e-8; /* kernel3d.cpp:21 */
 
+/* Summation-order switches (bit mask) used to measure the FP64 rounding floor
+ * of the algorithm: the reference's sums run in BLAS/FFTW order, which is not
+ * specified, so reversing them gives an equally valid evaluation.
+ *   1: kernel sums over sources (Kernel3d::kernel + dgemv)
+ *   2: dense matrix-vector products (pinv applies, M2M, L2L)
+ *   4: order of the V-list accumulation
+ *   8: DFT sums */
+static int g_order = 0;
+void orc_set_order(int mask) { g_order = mask; }
+
+/* Accumulation of the dense matrix-vector products (the UC2UE / DC2DE
+ * pseudoinverse applies, M2M and L2L; vecmatop.cpp:97-113).  The
+ * pseudoinverses are ill-conditioned (singular values kept down to
+ * 0.1^(np+1) sigma_0), so the rounding of y += A x in plain FP64 depends on
+ * the summation order at the 1e-11 (np 6) / 1e-9 (np 8) level of the final
+ * potential, and the reference's own result is only defined to that level
+ * (its BLAS fixes the order).  With accurate = 1 (the default) these sums are
+ * accumulated in extended precision (long double) and rounded once, which
+ * makes the result independent of the summation order to 1e-14 (np 6) /
+ * 4e-13 (np 8).  accurate = 0 restores plain FP64 accumulation. */
+static int g_accurate = 1;
+void orc_set_accurate(int on) { g_accurate = on; }
+
 typedef struct { int kt; double c[2]; } oknl;
 
 /* kernel3d.cpp:24-85 (sdof/tdof) */
@@ -180,7 +203,8 @@ static void kapply(const oknl* k, int ns, const double* spos, const double* snor
   double b[12];
   for (int i = 0; i < nt; i++) {
     double acc[3] = {0, 0, 0};
-    for (int j = 0; j < ns; j++) {
+    for (int jj = 0; jj < ns; jj++) {
+      int j = (g_order & 1) ? ns - 1 - jj : jj;
       kblock(k, tpos[3 * i] - spos[3 * j], tpos[3 * i + 1] - spos[3 * j + 1],
              tpos[3 * i + 2] - spos[3 * j + 2], snor ? snor + 3 * j : NULL, b);
       for (int t = 0; t < tdof; t++) {
@@ -263,7 +287,8 @@ int orc_rfft3(int n, int hm, const double* in, double* out)
       for (int y = 0; y < n; y++)
         for (int kx = 0; kx < h; kx++) {
           double re = 0, im = 0;
-          for (int x = 0; x < n; x++) {
+          for (int xx = 0; xx < n; xx++) {
+            int x = (g_order & 8) ? n - 1 - xx : xx;
             double v = in[((size_t)(z * n + y) * n + x) * hm + f];
             int m = (kx * x) % n;
             re += v * c[m]; im -= v * s[m];
@@ -276,7 +301,8 @@ int orc_rfft3(int n, int hm, const double* in, double* out)
       for (int ky = 0; ky < n; ky++)
         for (int kx = 0; kx < h; kx++) {
           double re = 0, im = 0;
-          for (int y = 0; y < n; y++) {
+          for (int yy = 0; yy < n; yy++) {
+            int y = (g_order & 8) ? n - 1 - yy : yy;
             size_t o = (size_t)(z * n + y) * h + kx;
             int m = (ky * y) % n;
             re += a[2 * o] * c[m] + a[2 * o + 1] * s[m];
@@ -290,7 +316,8 @@ int orc_rfft3(int n, int hm, const double* in, double* out)
       for (int ky = 0; ky < n; ky++)
         for (int kx = 0; kx < h; kx++) {
           double re = 0, im = 0;
-          for (int z = 0; z < n; z++) {
+          for (int zz = 0; zz < n; zz++) {
+            int z = (g_order & 8) ? n - 1 - zz : zz;
             size_t o = (size_t)(z * n + ky) * h + kx;
             int m = (kz * z) % n;
             re += b[2 * o] * c[m] + b[2 * o + 1] * s[m];
@@ -321,7 +348,8 @@ int orc_irfft3(int n, int hm, const double* in, double* out)
       for (int ky = 0; ky < n; ky++)
         for (int kx = 0; kx < h; kx++) {
           double re = 0, im = 0;
-          for (int kz = 0; kz < n; kz++) {
+          for (int kk = 0; kk < n; kk++) {
+            int kz = (g_order & 8) ? n - 1 - kk : kk;
             size_t o = ((size_t)(kz * n + ky) * h + kx) * hm + f;
             int m = (kz * z) % n;
             re += in[2 * o] * c[m] - in[2 * o + 1] * s[m];
@@ -335,7 +363,8 @@ int orc_irfft3(int n, int hm, const double* in, double* out)
       for (int y = 0; y < n; y++)
         for (int kx = 0; kx < h; kx++) {
           double re = 0, im = 0;
-          for (int ky = 0; ky < n; ky++) {
+          for (int kk = 0; kk < n; kk++) {
+            int ky = (g_order & 8) ? n - 1 - kk : kk;
             size_t o = (size_t)(z * n + ky) * h + kx;
             int m = (ky * y) % n;
             re += a[2 * o] * c[m] - a[2 * o + 1] * s[m];
@@ -848,11 +877,25 @@ static void locpos(const orc_fmm* h, int tp, const double* c, double r, double* 
 
 static void dgemv_acc(int m, int n, const double* A, const double* x, double* y)
 { /* y += A x, column-major (vecmatop.cpp:97-113) */
-  for (int j = 0; j < n; j++) {
-    double xj = x[j];
-    const double* a = A + (size_t)j * m;
-    for (int i = 0; i < m; i++) y[i] += a[i] * xj;
+  if (!g_accurate) {
+    for (int jj = 0; jj < n; jj++) {
+      int j = (g_order & 2) ? n - 1 - jj : jj;
+      double xj = x[j];
+      const double* a = A + (size_t)j * m;
+      for (int i = 0; i < m; i++) y[i] += a[i] * xj;
+    }
+    return;
   }
+  long double* t = (long double*)malloc(sizeof(long double) * (size_t)m);
+  for (int i = 0; i < m; i++) t[i] = y[i];
+  for (int jj = 0; jj < n; jj++) {
+    int j = (g_order & 2) ? n - 1 - jj : jj;
+    long double xj = x[j];
+    const double* a = A + (size_t)j * m;
+    for (int i = 0; i < m; i++) t[i] += (long double)a[i] * xj;
+  }
+  for (int i = 0; i < m; i++) y[i] = (double)t[i];
+  free(t);
 }
 
 /* UE2UC_dgemv, matmgnt3d.cpp:125-155 (homogeneous branch, R = 1) */
@@ -1068,7 +1111,8 @@ int orc_eval(orc_fmm* h, const double* srcden, double* trgval)
     center(h, g, gc);
     double D = 2.0 * radius(h, g);
     int l = n->depth + h->rootlvl;
-    for (int q = 0; q < n->V.n; q++) {
+    for (int qq = 0; qq < n->V.n; qq++) {
+      int q = (g_order & 4) ? n->V.n - 1 - qq : qq;
       int vi = n->V.a[q];
       double vc[3];
       center(h, vi, vc);
