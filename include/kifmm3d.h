@@ -52,6 +52,22 @@ enum {
   KIFMM3D_NSTAGE = 9
 };
 
+/* Kernel classes timed by kifmm3d_kernel_times() (HIP events around each launch group) */
+enum {
+  KIFMM3D_K_GATHER = 0,     /* density permutation into leaf order       */
+  KIFMM3D_K_S2M = 1,        /* leaf sources -> UC check surface          */
+  KIFMM3D_K_GEMM_UP = 2,    /* M2M and UC2UE batched GEMMs               */
+  KIFMM3D_K_P2P = 3,        /* near field (U list, direct W/X)           */
+  KIFMM3D_K_FFT_FWD = 4,    /* UE densities -> spectra                   */
+  KIFMM3D_K_M2L = 5,        /* frequency-domain M2L products             */
+  KIFMM3D_K_FFT_INV = 6,    /* spectra -> DC check values                */
+  KIFMM3D_K_S2L = 7,        /* X-list sources -> DC check surface        */
+  KIFMM3D_K_GEMM_DOWN = 8,  /* L2L and DC2DE batched GEMMs               */
+  KIFMM3D_K_S2P = 9,        /* L2T and M2T                               */
+  KIFMM3D_K_SCATTER = 10,   /* values back to caller order               */
+  KIFMM3D_NKCLASS = 11
+};
+
 /* Create a handle.  Replaces FMM3d::FMM3d + the knl()/ctr()/rootlvl() setters
  * (fmm3d/fmm3d.cpp:23-27, fmm3d/tt.cpp:85-90) and reads the three options the
  * reference's setup requires (-fmm3d_np, -fmm3d_let3d_ptsmax,
@@ -81,6 +97,10 @@ int kifmm3d_evaluate_device(kifmm3d* h, const double* d_srcden, double* d_trgval
 /* Stage times (ms) of the last evaluate; n <= KIFMM3D_NSTAGE slots. */
 int kifmm3d_stage_times(const kifmm3d* h, double* ms, int n);
 
+/* Per-class kernel time (ms) and launch-group count of the last evaluate;
+ * n <= KIFMM3D_NKCLASS slots; calls may be NULL. */
+int kifmm3d_kernel_times(const kifmm3d* h, double* ms, int* calls, int n);
+
 /* Tree and list sizes after setup: out[0..11] = nnode, nlevel, sndecnt,
  * sextcnt, tndecnt, textcnt, plnUE, plnUC, plnDE, plnDC, sdof, tdof. */
 int kifmm3d_info(const kifmm3d* h, int* out, int n);
@@ -105,6 +125,21 @@ long kifmm3d_get_operator(const kifmm3d* h, int what, double* buf, long n);
  * _supcval, _tdneden, _tdncval (fmm3d/fmm3d.hpp:61-64).  Returns the element
  * count; buf may be NULL. */
 long kifmm3d_get_buffer(kifmm3d* h, int what, double* buf, long n);
+
+/* FP64 direct sum at selected targets: out[r][k*tdof + t] = sum over all
+ * sources of K(trgpos[trgidx[k]] <- src) srcden[r] -- the exact sum of
+ * FMM3d::check (fmm3d/fmm3d_check.cpp:25-74) and Dense3d::eval
+ * (fmm3d/dense3d.cpp:38-58), computed on the GPU.  srcden is [nrhs][Nsrc*sdof]
+ * host memory; trgidx and out are host memory. */
+int kifmm3d_direct(kifmm3d* h, const double* srcden, int nrhs, const int* trgidx, int nchk, double* out);
+/* Same with srcden resident in device memory. */
+int kifmm3d_direct_device(kifmm3d* h, const double* d_srcden, int nrhs, const int* trgidx, int nchk, double* out);
+
+/* Work counts of the plan after setup, per right-hand side: out[0] near-field
+ * source-target pairs, [1] L2T + M2T surface-target pairs, [2] V-list pairs,
+ * [3] S2M pairs, [4] S2L pairs, [5] up-pass GEMM flops, [6] down-pass GEMM
+ * flops, [7] forward FFTs, [8] inverse FFTs, [9] complex spectrum length C. */
+int kifmm3d_plan_stats(const kifmm3d* h, double* out, int n);
 
 const char* kifmm3d_last_error(const kifmm3d* h);
 void kifmm3d_destroy(kifmm3d* h);

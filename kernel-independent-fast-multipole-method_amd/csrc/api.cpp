@@ -85,6 +85,16 @@ int kifmm3d_stage_times(const kifmm3d* h, double* ms, int n)
   return KIFMM3D_OK;
 }
 
+int kifmm3d_kernel_times(const kifmm3d* h, double* ms, int* calls, int n)
+{
+  if (!h || !ms || n < 0) return KIFMM3D_EINVAL;
+  for (int i = 0; i < n && i < KIFMM3D_NKCLASS; i++) {
+    ms[i] = h->eng->kernel_ms[i];
+    if (calls) calls[i] = h->eng->kernel_calls[i];
+  }
+  return KIFMM3D_OK;
+}
+
 int kifmm3d_info(const kifmm3d* h, int* out, int n)
 {
   if (!h || !out) return KIFMM3D_EINVAL;
@@ -150,6 +160,30 @@ long kifmm3d_get_buffer(kifmm3d* h, int what, double* buf, long n)
   long r = KIFMM3D_EINVAL;
   int st = guarded(h, [&] { r = h->eng->get_buffer(what, buf, n); });
   return st == KIFMM3D_OK ? r : st;
+}
+
+int kifmm3d_direct(kifmm3d* h, const double* srcden, int nrhs, const int* trgidx, int nchk, double* out)
+{
+  if (!h) return KIFMM3D_EINVAL;
+  return guarded(h, [&] { h->eng->direct(srcden, false, nrhs, trgidx, nchk, out); });
+}
+
+int kifmm3d_direct_device(kifmm3d* h, const double* d_srcden, int nrhs, const int* trgidx, int nchk, double* out)
+{
+  if (!h) return KIFMM3D_EINVAL;
+  return guarded(h, [&] { h->eng->direct(d_srcden, true, nrhs, trgidx, nchk, out); });
+}
+
+int kifmm3d_plan_stats(const kifmm3d* h, double* out, int n)
+{
+  if (!h || !out) return KIFMM3D_EINVAL;
+  if (!h->eng->host_ready()) return KIFMM3D_ESTATE;
+  const kifmm::Plan& P = h->eng->plan();
+  double v[10] = {(double)P.near_pairs, (double)P.far_pairs, (double)P.v_pairs, (double)P.s2m_pairs,
+                  (double)P.s2l_pairs, P.up_flops, P.down_flops, (double)P.fft_fwd, (double)P.fft_inv,
+                  (double)h->eng->ops().C};
+  for (int i = 0; i < n && i < 10; i++) out[i] = v[i];
+  return KIFMM3D_OK;
 }
 
 const char* kifmm3d_last_error(const kifmm3d* h)

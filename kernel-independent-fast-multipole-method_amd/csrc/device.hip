@@ -228,6 +228,124 @@ __global__ __launch_bounds__(P2P_BS) void k_p2p(const int* __restrict__ tbeg, co
   }
 }
 
+// Split nr right-hand sides into launches of NR in {16, 4, 1}; fn(NR, nblocks, roff).
+template <class F>
+static void rhs_blocks(int nr, F&& fn)
+{
+  int r = 0;
+  while (r < nr) {
+    int left = nr - r;
+    int NR = left >= 16 ? 16 : left >= 4 ? 4 : 1;
+    int nb = left / NR;
+    fn(NR, nb, r);
+    r += nb * NR;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// FP64 direct sum at selected targets (FMM3d::check, fmm3d_check.cpp:48-54):
+// block (chunk of 128 targets, source segment) writes a partial sum; the
+// partials are reduced in segment order, so the result is deterministic.
+template <int KT, int NR>
+__global__ __launch_bounds__(P2P_BS) void k_direct_part(int nchk, const double* __restrict__ cx,
+                                                        const double* __restrict__ cy, const double* __restrict__ cz,
+                                                        int nsrc, int seglen, Points src, double* __restrict__ part,
+                                                        int roff, KC kc)
+{
+  using K = Knl<KT>;
+  constexpr int NPOS = K::NRM ? 6 : 3;
+  constexpr int ND = NR * K::S;
+  __shared__ double s_pos[TILE][NPOS];
+  __shared__ double s_den[TILE][ND];
+  const int t = threadIdx.x, k = blockIdx.x * P2P_BS + t, seg = blockIdx.y;
+  const int r0 = roff + blockIdx.z * NR;
+  const bool act = k < nchk;
+  double x = act ? cx[k] : 0.0, y = act ? cy[k] : 0.0, z = act ? cz[k] : 0.0;
+  double u[NR][K::T];
+#pragma unroll
+  for (int q = 0; q < NR; q++)
+#pragma unroll
+    for (int d = 0; d < K::T; d++) u[q][d] = 0.0;
+  const double* den = src.den + (long)r0 * src.den_rs;
+  const int sb = seg * seglen, se = min(nsrc, sb + seglen);
+  for (int b0 = sb; b0 < se; b0 += TILE) {
+    const int cnt = min(TILE, se - b0);
+    __syncthreads();
+    for (int i = t; i < cnt; i += P2P_BS) {
+      s_pos[i][0] = src.x[b0 + i];
+      s_pos[i][1] = src.y[b0 + i];
+      s_pos[i][2] = src.z[b0 + i];
+      if (K::NRM) {
+        s_pos[i][NPOS - 3] = src.nx[b0 + i];
+        s_pos[i][NPOS - 2] = src.ny[b0 + i];
+        s_pos[i][NPOS - 1] = src.nz[b0 + i];
+      }
+    }
+    const int per = cnt * K::S;
+    for (int i = t; i < NR * per; i += P2P_BS) {
+      int q = i / per, rem = i - q * per;
+      int jj = rem / K::S, sd = rem - jj * K::S;
+      s_den[jj][q * K::S + sd] = den[(long)q * src.den_rs + (long)b0 * K::S + rem];
+    }
+    __syncthreads();
+    if (act)
+      for (int jj = 0; jj < cnt; jj++)
+        K::template add<NR>(x - s_pos[jj][0], y - s_pos[jj][1], z - s_pos[jj][2], &s_pos[jj][NPOS - 3], s_den[jj], u,
+                            kc);
+  }
+  if (act) {
+    const int nseg = gridDim.y;
+#pragma unroll
+    for (int q = 0; q < NR; q++)
+#pragma unroll
+      for (int d = 0; d < K::T; d++) part[(((long)(r0 + q) * nseg + seg) * nchk + k) * K::T + d] = u[q][d];
+  }
+}
+
+__global__ void k_direct_reduce(int n, int nseg, int nr, const double* __restrict__ part, double* __restrict__ out)
+{
+  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;  // over nr * n values
+  if (i >= (long)nr * n) return;
+  int r = (int)(i / n);
+  long k = i - (long)r * n;
+  double s = 0.0;
+  for (int g = 0; g < nseg; g++) s += part[((long)r * nseg + g) * n + k];
+  out[i] = s;
+}
+
+template <int KT>
+static void direct_kt(hipStream_t s, int nchk, const double* cx, const double* cy, const double* cz, int nsrc,
+                      int nseg, int seglen, Points src, double* part, int nr, KC kc)
+{
+  rhs_blocks(nr, [&](int NR, int nb, int roff) {
+    dim3 grid((nchk + P2P_BS - 1) / P2P_BS, nseg, nb);
+    if (NR == 16)
+      hipLaunchKernelGGL((k_direct_part<KT, 16>), grid, dim3(P2P_BS), 0, s, nchk, cx, cy, cz, nsrc, seglen, src,
+                         part, roff, kc);
+    else if (NR == 4)
+      hipLaunchKernelGGL((k_direct_part<KT, 4>), grid, dim3(P2P_BS), 0, s, nchk, cx, cy, cz, nsrc, seglen, src, part,
+                         roff, kc);
+    else
+      hipLaunchKernelGGL((k_direct_part<KT, 1>), grid, dim3(P2P_BS), 0, s, nchk, cx, cy, cz, nsrc, seglen, src, part,
+                         roff, kc);
+  });
+}
+
+void direct(hipStream_t s, int kt, int tdof, int nchk, const double* cx, const double* cy, const double* cz,
+            int nsrc, int nseg, Points src, double* part, double* out, int nr, KC kc)
+{
+  if (nchk <= 0) return;
+  const int seglen = (nsrc + nseg - 1) / nseg;
+  switch (kt) {
+  case 111: direct_kt<111>(s, nchk, cx, cy, cz, nsrc, nseg, seglen, src, part, nr, kc); break;
+  case 321: direct_kt<321>(s, nchk, cx, cy, cz, nsrc, nseg, seglen, src, part, nr, kc); break;
+  case 511: direct_kt<511>(s, nchk, cx, cy, cz, nsrc, nseg, seglen, src, part, nr, kc); break;
+  }
+  long tot = (long)nr * nchk * tdof;
+  hipLaunchKernelGGL(k_direct_reduce, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, nchk * tdof, nseg, nr,
+                     part, out);
+}
+
 // ---------------------------------------------------------------------------
 // points -> check surface (S2M, S2L)
 constexpr int P2S_BS = 256;
@@ -361,20 +479,6 @@ __global__ __launch_bounds__(P2P_BS) void k_s2p(const int* __restrict__ tbeg, co
     for (int q = 0; q < NR; q++)
 #pragma unroll
       for (int d = 0; d < K::T; d++) val[(long)q * trg.val_rs + d] += u[q][d];
-  }
-}
-
-// Split nr right-hand sides into launches of NR in {16, 4, 1}; fn(NR, nblocks, roff).
-template <class F>
-static void rhs_blocks(int nr, F&& fn)
-{
-  int r = 0;
-  while (r < nr) {
-    int left = nr - r;
-    int NR = left >= 16 ? 16 : left >= 4 ? 4 : 1;
-    int nb = left / NR;
-    fn(NR, nb, r);
-    r += nb * NR;
   }
 }
 

@@ -34,6 +34,11 @@ struct Targets {  // leaf-ordered targets (SoA) and values [rhs][n*tdof]
 void p2p(hipStream_t s, int kt, int nchunk, const int* tbeg, const int* tnum, const int* eptr, const int2* ent,
          Points src, Targets trg, int nr, KC kc);
 
+// FP64 direct sum of the user kernel over all nsrc sources at nchk points
+// (cx, cy, cz): out[r][k*tdof + t]; part holds nr * nseg * nchk * tdof doubles.
+void direct(hipStream_t s, int kt, int tdof, int nchk, const double* cx, const double* cy, const double* cz,
+            int nsrc, int nseg, Points src, double* part, double* out, int nr, KC kc);
+
 // Points to check surface (S2M onto UC, S2L onto DC): out[(node*Rc + r)*pln + p*tdof + t] +=
 // sum over the source leaves of item i of K(ctr_i + rad_i * unit_p <- y) den(y).
 void p2s(hipStream_t s, int kt, int nitem, const int* out_node, const double4* geom, const int* eptr,

@@ -7,7 +7,10 @@
 #include "engine.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 
 namespace kifmm {
@@ -90,6 +93,7 @@ Engine::Engine(int kt, const double* coefs, int ncoefs, int np, int ptsmax, int 
 
 Engine::~Engine()
 {
+  for (hipEvent_t e : evpool_) (void)hipEventDestroy(e);
   if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -99,7 +103,15 @@ void Engine::setup(const double* spos, const double* snor, int ns, const double*
   if (user_.needs_normal && ns > 0 && !snor) throw Error(KIFMM3D_EINVAL, "this kernel needs source normals");
   setup_done_ = false;
   host_done_ = false;
+  const bool trace = std::getenv("KIFMM3D_TRACE") != nullptr;
+  auto clk = std::chrono::steady_clock::now();
+  auto lap = [&](const char* what) {
+    auto now = std::chrono::steady_clock::now();
+    if (trace) std::fprintf(stderr, "[kifmm3d] setup %-10s %8.3f s\n", what, std::chrono::duration<double>(now - clk).count());
+    clk = now;
+  };
   ops_ = get_operators(mm_, np_);
+  lap("operators");
   Tree t;
   t.ptsmax = tree_.ptsmax;
   t.maxlevel = tree_.maxlevel;
@@ -109,7 +121,10 @@ void Engine::setup(const double* spos, const double* snor, int ns, const double*
   tree_ = std::move(t);
   ns_ = ns;
   nt_ = nt;
+  tpos_host_.assign(tpos, tpos + 3 * (size_t)nt);
+  lap("tree");
   build_plan();
+  lap("plan");
   host_done_ = true;
   // device phase
   int ndev = 0;
@@ -140,6 +155,7 @@ void Engine::setup(const double* spos, const double* snor, int ns, const double*
   tperm_.upload(tree_.tperm);
   upload_static();
   HIPCHK(hipDeviceSynchronize());
+  lap("upload");
   setup_done_ = true;
 }
 
@@ -206,6 +222,7 @@ void Engine::build_plan()
         ent.push_back(T.sextnum[g]);
         eptr.push_back((int)ent.size() / 2);
       }
+    for (size_t i = 1; i < ent.size(); i += 2) P.s2m_pairs += (long)ent[i] * O.nsp[SURF_UC];
     P.s2m.n = (int)out.size();
     P.s2m.out = pool_i(P.I, out);
     P.s2m.eptr = pool_i(P.I, eptr);
@@ -238,6 +255,7 @@ void Engine::build_plan()
       op.cnode = pool_i(P.I, cnode);
       op.gbeg = pool_i(P.I, {0, (int)inner.size()});
       op.bscale = (long)pool_d(P.D, scales(L + 1 + rootlvl_, +1, pUE));  // child level (fmm3d_eval.cpp:84)
+      P.up_flops += 2.0 * op.M * op.Kseg * op.nseg * op.ncol;
       P.up.push_back(op);
     }
     if (!all.empty()) {
@@ -250,6 +268,7 @@ void Engine::build_plan()
       op.bnode = op.cnode = pool_i(P.I, idx);
       op.gbeg = pool_i(P.I, {0, (int)all.size()});
       op.cscale = (long)pool_d(P.D, scales(L + rootlvl_, -1, pUE));
+      P.up_flops += 2.0 * op.M * op.Kseg * op.ncol;
       P.up.push_back(op);
     }
   }
@@ -273,6 +292,7 @@ void Engine::build_plan()
         for (auto it = X.first; it != X.second; ++it) {
           xent.push_back(T.sextbeg[*it]);
           xent.push_back(T.sextnum[*it]);
+          P.s2l_pairs += (long)T.sextnum[*it] * O.nsp[SURF_DC];
         }
         xeptr.push_back((int)xent.size() / 2);
       }
@@ -371,6 +391,8 @@ void Engine::build_plan()
       P.max_vsrc = std::max(P.max_vsrc, vl.nsrc);
       P.max_vtrg = std::max(P.max_vtrg, vl.ntrg);
       P.v_pairs += vl.npair;
+      P.fft_fwd += (long)vl.nsrc * mm_.sdof;
+      P.fft_inv += (long)vl.ntrg * mm_.tdof;
       P.vlev.push_back(vl);
     }
   }
@@ -403,6 +425,7 @@ void Engine::build_plan()
       op.bnode = pool_i(P.I, bnode);
       op.gbeg = pool_i(P.I, gbeg);
       op.bscale = (long)pool_d(P.D, scales(L - 1 + rootlvl_, +1, pDE));  // parent level (fmm3d_eval.cpp:203)
+      P.down_flops += 2.0 * op.M * op.Kseg * op.ncol;
       P.down.push_back(op);
     }
     GemmOp op;
@@ -414,6 +437,7 @@ void Engine::build_plan()
     op.bnode = op.cnode = pool_i(P.I, idx);
     op.gbeg = pool_i(P.I, {0, (int)all.size()});
     op.cscale = (long)pool_d(P.D, scales(L + rootlvl_, -1, pDE));
+    P.down_flops += 2.0 * op.M * op.Kseg * op.ncol;
     P.down.push_back(op);
   }
   (void)pDC;
@@ -457,6 +481,8 @@ void Engine::evaluate_device(const double* d_srcden, double* d_trgval, int nrhs,
   if (mode != KIFMM3D_MODE_PLAIN) throw Error(KIFMM3D_EINVAL, "unsupported evaluation mode");
   if ((ns_ > 0 && !d_srcden) || (nt_ > 0 && !d_trgval)) throw Error(KIFMM3D_EINVAL, "null density or value pointer");
   for (double& m : stage_ms) m = 0.0;
+  for (double& m : kernel_ms) m = 0.0;
+  for (int& c : kernel_calls) c = 0;
   if (nt_ == 0) return;
   int rc = choose_rhs_chunk(nrhs);
   ensure_work(rc);
@@ -471,6 +497,12 @@ void Engine::evaluate_device(const double* d_srcden, double* d_trgval, int nrhs,
         float ms = 0;
         HIPCHK(hipEventElapsedTime(&ms, ev[s], ev[s + 1]));
         stage_ms[s] += ms;
+      }
+      for (size_t e = 0; e + 1 < nev_used_; e += 2) {
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, evpool_[e], evpool_[e + 1]));
+        kernel_ms[evcls_[e / 2]] += ms;
+        kernel_calls[evcls_[e / 2]] += 1;
       }
       last_rc_ = n;
     }
@@ -525,6 +557,24 @@ void Engine::run_chunk(const double* srcden, double* trgval, int rc, hipEvent_t*
     dev::gemm(s, a);
   };
 
+  // kernel-class timing: events around each launch group on this stream
+  size_t nev = 0;
+  auto timed = [&](int cls, auto&& fn) {
+    if (evpool_.size() < nev + 2) {
+      hipEvent_t a, b;
+      HIPCHK(hipEventCreate(&a));
+      HIPCHK(hipEventCreate(&b));
+      evpool_.push_back(a);
+      evpool_.push_back(b);
+    }
+    HIPCHK(hipEventRecord(evpool_[nev], s));
+    fn();
+    HIPCHK(hipEventRecord(evpool_[nev + 1], s));
+    evcls_.resize(nev / 2 + 1);
+    evcls_[nev / 2] = cls;
+    nev += 2;
+  };
+
   HIPCHK(hipEventRecord(ev[0], s));
   // zero the accumulated buffers (fmm3d_eval.cpp:33-42)
   HIPCHK(hipMemsetAsync(tval_.p, 0, sizeof(double) * (size_t)rc * trs, s));
@@ -533,43 +583,58 @@ void Engine::run_chunk(const double* srcden, double* trgval, int rc, hipEvent_t*
   if (P.nlow_s) HIPCHK(hipMemsetAsync(upeden_.p, 0, sizeof(double) * (size_t)rc * P.nlow_s * pUE, s));
   if (P.nlow_t) HIPCHK(hipMemsetAsync(dneden_.p, 0, sizeof(double) * (size_t)rc * P.nlow_t * pDE, s));
   // load (fmm3d_eval.cpp:50-64)
-  dev::gather(s, T.sextcnt, sdof, rc, sperm_.p, srcden, (long)ns_ * sdof, sden_.p, srs);
+  timed(KIFMM3D_K_GATHER, [&] { dev::gather(s, T.sextcnt, sdof, rc, sperm_.p, srcden, (long)ns_ * sdof, sden_.p, srs); });
   HIPCHK(hipEventRecord(ev[1], s));
   // up: S2M, then M2M + UC2UE level by level
-  dev::p2s(s, user_.kt, P.s2m.n, I + P.s2m.out, (const double4*)(D + P.s2m.geom), I + P.s2m.eptr,
-           (const int2*)(I + P.s2m.ent), unit_[SURF_UC].p, O.nsp[SURF_UC], src, upcval_.p, rc, pUC, rc, kcu);
-  for (const GemmOp& op : P.up) run_gemm(op);
+  timed(KIFMM3D_K_S2M, [&] {
+    dev::p2s(s, user_.kt, P.s2m.n, I + P.s2m.out, (const double4*)(D + P.s2m.geom), I + P.s2m.eptr,
+             (const int2*)(I + P.s2m.ent), unit_[SURF_UC].p, O.nsp[SURF_UC], src, upcval_.p, rc, pUC, rc, kcu);
+  });
+  for (const GemmOp& op : P.up) timed(KIFMM3D_K_GEMM_UP, [&] { run_gemm(op); });
   HIPCHK(hipEventRecord(ev[2], s));
   // U list and direct W/X
-  dev::p2p(s, user_.kt, P.near.n, I + P.near.tbeg, I + P.near.tnum, I + P.near.eptr, (const int2*)(I + P.near.ent),
-           src, trg, rc, kcu);
+  timed(KIFMM3D_K_P2P, [&] {
+    dev::p2p(s, user_.kt, P.near.n, I + P.near.tbeg, I + P.near.tnum, I + P.near.eptr,
+             (const int2*)(I + P.near.ent), src, trg, rc, kcu);
+  });
   HIPCHK(hipEventRecord(ev[3], s));
   // V list
   for (const VLevel& vl : P.vlev) {
-    dev::fft_fwd(s, np_, vl.nsrc, I + vl.src_node, rc, upeden_.p, pUE, sm, D + vl.dscale, c2s_.p,
-                 (double2*)effden_.p, (const double2*)tw_.p);
-    dev::m2l(s, sm, tm, O.C, vl.ntrg, rc, I + vl.vptr, I + vl.vsrc, I + vl.vop, (const double2*)m2l_.p,
-             (const double2*)effden_.p, (double2*)effval_.p);
-    dev::fft_inv(s, np_, vl.ntrg, I + vl.trg_node, rc, (const double2*)effval_.p, tm, s2c_.p, O.nsp[SURF_DC],
-                 dncval_.p, pDC, (const double2*)tw_.p);
+    timed(KIFMM3D_K_FFT_FWD, [&] {
+      dev::fft_fwd(s, np_, vl.nsrc, I + vl.src_node, rc, upeden_.p, pUE, sm, D + vl.dscale, c2s_.p,
+                   (double2*)effden_.p, (const double2*)tw_.p);
+    });
+    timed(KIFMM3D_K_M2L, [&] {
+      dev::m2l(s, sm, tm, O.C, vl.ntrg, rc, I + vl.vptr, I + vl.vsrc, I + vl.vop, (const double2*)m2l_.p,
+               (const double2*)effden_.p, (double2*)effval_.p);
+    });
+    timed(KIFMM3D_K_FFT_INV, [&] {
+      dev::fft_inv(s, np_, vl.ntrg, I + vl.trg_node, rc, (const double2*)effval_.p, tm, s2c_.p, O.nsp[SURF_DC],
+                   dncval_.p, pDC, (const double2*)tw_.p);
+    });
   }
   HIPCHK(hipEventRecord(ev[4], s));
   // X list S2L
-  dev::p2s(s, user_.kt, P.s2l.n, I + P.s2l.out, (const double4*)(D + P.s2l.geom), I + P.s2l.eptr,
-           (const int2*)(I + P.s2l.ent), unit_[SURF_DC].p, O.nsp[SURF_DC], src, dncval_.p, rc, pDC, rc, kcu);
+  timed(KIFMM3D_K_S2L, [&] {
+    dev::p2s(s, user_.kt, P.s2l.n, I + P.s2l.out, (const double4*)(D + P.s2l.geom), I + P.s2l.eptr,
+             (const int2*)(I + P.s2l.ent), unit_[SURF_DC].p, O.nsp[SURF_DC], src, dncval_.p, rc, pDC, rc, kcu);
+  });
   HIPCHK(hipEventRecord(ev[5], s));
   // down: L2L + DC2DE level by level
-  for (const GemmOp& op : P.down) run_gemm(op);
+  for (const GemmOp& op : P.down) timed(KIFMM3D_K_GEMM_DOWN, [&] { run_gemm(op); });
   HIPCHK(hipEventRecord(ev[6], s));
   // L2T + M2T
-  dev::s2p(s, mm_.kt, P.far.n, I + P.far.tbeg, I + P.far.tnum, I + P.far.eptr, I + P.far.enode, I + P.far.ekind,
-           (const double4*)(D + P.far.egeom), unit_[SURF_DE].p, unit_[SURF_UE].p, O.nsp[SURF_DE], dneden_.p,
-           upeden_.p, rc, pDE, trg, rc, kcm);
+  timed(KIFMM3D_K_S2P, [&] {
+    dev::s2p(s, mm_.kt, P.far.n, I + P.far.tbeg, I + P.far.tnum, I + P.far.eptr, I + P.far.enode,
+             I + P.far.ekind, (const double4*)(D + P.far.egeom), unit_[SURF_DE].p, unit_[SURF_UE].p,
+             O.nsp[SURF_DE], dneden_.p, upeden_.p, rc, pDE, trg, rc, kcm);
+  });
   HIPCHK(hipEventRecord(ev[7], s));
   // save (fmm3d_eval.cpp:221-235)
-  dev::scatter(s, T.textcnt, tdof, rc, tperm_.p, tval_.p, trs, trgval, (long)nt_ * tdof);
+  timed(KIFMM3D_K_SCATTER, [&] { dev::scatter(s, T.textcnt, tdof, rc, tperm_.p, tval_.p, trs, trgval, (long)nt_ * tdof); });
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(ev[8], s));
+  nev_used_ = nev;
 }
 
 long Engine::get_buffer(int what, double* buf, long n)
@@ -588,6 +653,42 @@ long Engine::get_buffer(int what, double* buf, long n)
     HIPCHK(hipMemcpy(buf + (size_t)k * pln[what], src[what]->p + (size_t)k * last_rc_ * pln[what],
                      sizeof(double) * pln[what], hipMemcpyDeviceToHost));
   return tot;
+}
+
+void Engine::direct(const double* srcden, bool on_device, int nrhs, const int* idx, int nchk, double* out)
+{
+  if (!setup_done_) throw Error(KIFMM3D_ESTATE, "direct called before setup");
+  if (nrhs < 1 || nchk < 0 || (nchk > 0 && (!idx || !out)) || !srcden) throw Error(KIFMM3D_EINVAL, "bad arguments");
+  if (nchk == 0) return;
+  const int sdof = user_.sdof, tdof = user_.tdof;
+  std::vector<double> c((size_t)3 * nchk);
+  for (int k = 0; k < nchk; k++) {
+    if (idx[k] < 0 || idx[k] >= nt_) throw Error(KIFMM3D_EINVAL, "target index out of range");
+    for (int d = 0; d < 3; d++) c[(size_t)d * nchk + k] = tpos_host_[3 * (size_t)idx[k] + d];
+  }
+  const double* dsrc = srcden;
+  size_t nsd = (size_t)nrhs * ns_ * sdof;
+  if (!on_device) {
+    hsrc_.ensure(nsd);
+    HIPCHK(hipMemcpy(hsrc_.p, srcden, nsd * sizeof(double), hipMemcpyHostToDevice));
+    dsrc = hsrc_.p;
+  }
+  chk_.upload(c);
+  const int nseg = std::max(1, std::min(512, ns_ / 512));
+  const long srs = (long)tree_.sextcnt * sdof;
+  // keep the evaluate work buffers intact: the gathered densities get their own buffer
+  DVec<double> sden;
+  sden.alloc((size_t)nrhs * srs);
+  part_.ensure((size_t)nrhs * nseg * nchk * tdof);
+  dout_.ensure((size_t)nrhs * nchk * tdof);
+  dev::gather(stream_, tree_.sextcnt, sdof, nrhs, sperm_.p, dsrc, (long)ns_ * sdof, sden.p, srs);
+  dev::Points src{sx_.p, sy_.p, sz_.p, snx_.p, sny_.p, snz_.p, sden.p, srs};
+  const dev::KC kcu{kcu_.a, kcu_.b, kcu_.c};
+  dev::direct(stream_, user_.kt, tdof, nchk, chk_.p, chk_.p + nchk, chk_.p + 2 * nchk, ns_, nseg, src, part_.p,
+              dout_.p, nrhs, kcu);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(stream_));
+  HIPCHK(hipMemcpy(out, dout_.p, sizeof(double) * (size_t)nrhs * nchk * tdof, hipMemcpyDeviceToHost));
 }
 
 }  // namespace kifmm

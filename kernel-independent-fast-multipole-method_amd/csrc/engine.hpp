@@ -69,6 +69,11 @@ struct Plan {
   long near_pairs = 0;         // source-target pairs of the near field (one RHS)
   long far_pairs = 0;          // surface point - target pairs of L2T + M2T
   long v_pairs = 0;
+  long s2m_pairs = 0;          // source point - UC point pairs of S2M
+  long s2l_pairs = 0;          // source point - DC point pairs of S2L
+  double up_flops = 0;         // M2M + UC2UE multiply-adds x 2, per right-hand side
+  double down_flops = 0;       // L2L + DC2DE multiply-adds x 2, per right-hand side
+  long fft_fwd = 0, fft_inv = 0;  // transforms per right-hand side (times the dof count)
   std::vector<int> I;          // index pool (uploaded once)
   std::vector<double> D;       // double pool (geometry, scales)
 };
@@ -88,7 +93,12 @@ class Engine {
   bool is_setup() const { return setup_done_; }
   bool host_ready() const { return host_done_; }
   double stage_ms[KIFMM3D_NSTAGE] = {0};
+  double kernel_ms[KIFMM3D_NKCLASS] = {0};
+  int kernel_calls[KIFMM3D_NKCLASS] = {0};
   long get_buffer(int what, double* buf, long n);
+  // FP64 direct sum at the caller-order targets idx[0..nchk) for nrhs density
+  // vectors; out is host memory [nrhs][nchk * tdof].
+  void direct(const double* srcden, bool on_device, int nrhs, const int* idx, int nchk, double* out);
   const Plan& plan() const { return plan_; }
 
  private:
@@ -110,6 +120,9 @@ class Engine {
   bool host_done_ = false;
   int last_rc_ = 0;
   hipStream_t stream_ = nullptr;
+  std::vector<hipEvent_t> evpool_;
+  std::vector<int> evcls_;
+  size_t nev_used_ = 0;
 
   // device data
   DVec<double> sx_, sy_, sz_, snx_, sny_, snz_, tx_, ty_, tz_;
@@ -123,6 +136,8 @@ class Engine {
   // work buffers (grow with the RHS chunk)
   DVec<double> sden_, tval_, upcval_, upeden_, dncval_, dneden_, effden_, effval_;
   DVec<double> hsrc_, htrg_;  // staging for the host-pointer entry point
+  DVec<double> chk_, part_, dout_;  // direct-sum scratch
+  std::vector<double> tpos_host_;  // caller-order target positions (for direct sums)
 };
 
 }  // namespace kifmm

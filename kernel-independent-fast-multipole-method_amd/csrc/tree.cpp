@@ -2,8 +2,11 @@
 #include "tree.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
+#include <cstdlib>
 #include <numeric>
 #include <unordered_map>
 
@@ -60,9 +63,19 @@ void Tree::build(const double* spos, int ns, const double* tpos, int nt)
 {
   if (maxlevel < 1 || maxlevel > 16) throw Error(KIFMM3D_EINVAL, "maxlevel must be in [1, 16]");
   rad = std::pow(2.0, -rootlvl);
+  const bool trace = std::getenv("KIFMM3D_TRACE") != nullptr;
+  auto clk = std::chrono::steady_clock::now();
+  auto lap = [&](const char* what) {
+    auto now = std::chrono::steady_clock::now();
+    if (trace) std::fprintf(stderr, "[kifmm3d]   tree %-10s %8.3f s\n", what, std::chrono::duration<double>(now - clk).count());
+    clk = now;
+  };
   build_sources(spos, ns);
+  lap("sources");
   insert_targets(tpos, nt);
+  lap("targets");
   build_lists();
+  lap("lists");
 }
 
 // Let3d::srcdata, let3d.cpp:56-137

@@ -42,10 +42,14 @@ SYMBOLS = [
     ("kifmm3d_evaluate", ctypes.c_int, [_VP, _D, _D, ctypes.c_int, ctypes.c_int]),
     ("kifmm3d_evaluate_device", ctypes.c_int, [_VP, _VP, _VP, ctypes.c_int, ctypes.c_int]),
     ("kifmm3d_stage_times", ctypes.c_int, [_VP, _D, ctypes.c_int]),
+    ("kifmm3d_kernel_times", ctypes.c_int, [_VP, _D, _I, ctypes.c_int]),
     ("kifmm3d_info", ctypes.c_int, [_VP, _I, ctypes.c_int]),
     ("kifmm3d_get_tree", ctypes.c_long, [_VP, ctypes.c_int, _I, ctypes.c_long]),
     ("kifmm3d_get_operator", ctypes.c_long, [_VP, ctypes.c_int, _D, ctypes.c_long]),
     ("kifmm3d_get_buffer", ctypes.c_long, [_VP, ctypes.c_int, _D, ctypes.c_long]),
+    ("kifmm3d_direct", ctypes.c_int, [_VP, _D, ctypes.c_int, _I, ctypes.c_int, _D]),
+    ("kifmm3d_direct_device", ctypes.c_int, [_VP, _VP, ctypes.c_int, _I, ctypes.c_int, _D]),
+    ("kifmm3d_plan_stats", ctypes.c_int, [_VP, _D, ctypes.c_int]),
     ("kifmm3d_last_error", ctypes.c_char_p, [_VP]),
     ("kifmm3d_destroy", None, [_VP]),
     ("kifmm3d_device_alloc", ctypes.c_int, [ctypes.POINTER(_VP), ctypes.c_long]),
@@ -59,6 +63,7 @@ SYMBOLS = [
 OK, EINVAL, EBOX, ESTATE, EDEVICE, ENOMEM = 0, -1, -2, -3, -4, -5
 MODE_PLAIN = 0
 STAGES = ["load", "up", "near", "vlist", "xlist", "down", "eval", "save", "total"]
+KCLASSES = ["gather", "s2m", "gemm_up", "p2p", "fft_fwd", "m2l", "fft_inv", "s2l", "gemm_down", "s2p", "scatter"]
 
 # kernel ids of fmm3d/kernel3d.hpp:29-53 supported by this evaluator
 KNL_LAP_S_U = 111
@@ -257,11 +262,56 @@ class FMM3d:
         except Exception:
             pass
 
+    def direct(self, srcden, trgidx, nrhs: int = 1):
+        """FP64 direct sum at the targets trgidx (caller order) on the GPU:
+        returns [nrhs, len(trgidx) * tdof]."""
+        idx = np.ascontiguousarray(trgidx, dtype=np.int32)
+        out = np.empty((nrhs, idx.size * self.knl.tdof()))
+        if isinstance(srcden, DeviceBuffer):
+            rc = lib().kifmm3d_direct_device(self.h, srcden.ptr, nrhs, idx.ctypes.data_as(_I), idx.size,
+                                             out.ctypes.data_as(_D))
+        else:
+            sd = _f64(srcden).ravel()
+            if sd.size != nrhs * self._ns * self.knl.sdof():
+                raise KifmmError(EINVAL, "srcden has the wrong size")
+            rc = lib().kifmm3d_direct(self.h, sd.ctypes.data_as(_D), nrhs, idx.ctypes.data_as(_I), idx.size,
+                                      out.ctypes.data_as(_D))
+        self._check(rc)
+        return out
+
+    def check(self, srcden, trgval, numchk: int = 20, seed: int = 0, nrhs: int = 1):
+        """FMM3d::check (fmm3d/fmm3d_check.cpp:25-74): compare trgval with the FP64
+        direct sum at numchk sampled targets.  Sampling is seeded and without
+        replacement.  Returns (||u - u_dir|| / ||u_dir||, ||u - u_dir|| / ||u||);
+        the second is the reference's printed relative error."""
+        rng = np.random.default_rng(seed)
+        idx = np.sort(rng.choice(self._nt, min(numchk, self._nt), replace=False)).astype(np.int32)
+        d = self.direct(srcden, idx, nrhs)
+        tdof = self.knl.tdof()
+        v = np.asarray(trgval).reshape(nrhs, self._nt, tdof)[:, idx, :].reshape(nrhs, -1)
+        en = np.linalg.norm(v - d)
+        return float(en / np.linalg.norm(d)), float(en / np.linalg.norm(v))
+
+    def plan_stats(self) -> dict:
+        keys = ["near_pairs", "far_pairs", "v_pairs", "s2m_pairs", "s2l_pairs", "up_flops", "down_flops",
+                "fft_fwd", "fft_inv", "C"]
+        a = np.zeros(len(keys))
+        self._check(lib().kifmm3d_plan_stats(self.h, a.ctypes.data_as(_D), len(keys)))
+        return dict(zip(keys, a.tolist()))
+
     # -- introspection -----------------------------------------------------
     def stage_times(self) -> dict:
         ms = np.zeros(len(STAGES))
         self._check(lib().kifmm3d_stage_times(self.h, ms.ctypes.data_as(_D), len(STAGES)))
         return dict(zip(STAGES, ms.tolist()))
+
+    def kernel_times(self) -> dict:
+        """{class: (ms, launch groups)} of the last evaluate."""
+        ms = np.zeros(len(KCLASSES))
+        calls = np.zeros(len(KCLASSES), dtype=np.int32)
+        self._check(lib().kifmm3d_kernel_times(self.h, ms.ctypes.data_as(_D), calls.ctypes.data_as(_I),
+                                               len(KCLASSES)))
+        return {k: (float(m), int(c)) for k, m, c in zip(KCLASSES, ms, calls)}
 
     def info(self) -> dict:
         a = np.zeros(12, dtype=np.int32)
