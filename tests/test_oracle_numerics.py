@@ -1,0 +1,149 @@
+"""Pins of the oracle's numerics: surface grids (matmgnt3d.cpp:375-419), the
+FFTW2-layout real FFT (against numpy.fft), the truncated pseudoinverse
+(svdrep.cpp:31-74, vecmatop.cpp:124-159; against numpy.linalg.pinv and the
+Moore-Penrose conditions), FFT-based M2L (matmgnt3d.cpp:223-361) against the
+dense kernel product for all 316 offsets, end-to-end error against the
+direct sum, and the FP64 rounding floor of the algorithm."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import oracle as O
+from problems import COEFS, clusters, densities, relerr, sphere, uniform_cube
+
+
+@pytest.mark.parametrize("np_", [3, 4, 6, 8])
+def test_surface_grid(np_):
+    for R in (1.0, 3.0):
+        p = O.splpos(np_, R)
+        assert p.shape[0] == np_ ** 3 - (np_ - 2) ** 3
+        assert np.allclose(np.abs(p).max(axis=1), R)  # every point on the cube surface
+        assert len(np.unique(np.round(p, 12), axis=0)) == p.shape[0]
+    g = O.regpos(np_, 1.0)
+    assert g.shape[0] == (2 * np_) ** 3
+    step = 2.0 / (np_ - 1)
+    idx = np.round(g / step).astype(int)
+    assert idx.min() == -np_ and idx.max() == np_ - 1  # wrap rule I < np ? I : I - 2np
+    assert np.array_equal(idx[:2 * np_, 0], [i if i < np_ else i - 2 * np_ for i in range(2 * np_)])
+
+
+@pytest.mark.parametrize("n,hm", [(8, 1), (12, 3), (16, 4)])
+def test_rfft3_matches_numpy(n, hm):
+    x = np.random.default_rng(n).standard_normal((n, n, n, hm))
+    X = O.rfft3(x)
+    ref = np.fft.rfftn(x, axes=(0, 1, 2))
+    assert np.abs(X - ref).max() < 1e-12 * np.abs(ref).max()
+    y = O.irfft3(X, n)
+    assert np.abs(y - n ** 3 * x).max() < 1e-11 * n ** 3 * np.abs(x).max()
+
+
+@pytest.mark.parametrize("kt,np_", [(111, 4), (111, 6), (301, 4), (511, 4)])
+def test_pinv_against_lapack_and_moore_penrose(kt, np_):
+    f = O.OracleFMM({301: 321}.get(kt, kt), COEFS[{301: 321}.get(kt, kt)], np_, 50, 5, (0, 0, 0), 0)
+    f.setup(np.zeros((1, 3)), None, np.zeros((1, 3)))
+    kuc, kdc = f.check_matrices()
+    eps = 0.1 ** (np_ + 1)
+    for A in (kuc, kdc):
+        P, r = O.pinv_svdrep(A, eps)
+        s = np.linalg.svd(A, compute_uv=False)
+        assert r == int(np.sum(s >= eps * s[0]))  # kept rank
+        ref = np.linalg.pinv(A, rcond=eps)
+        # two backward-stable SVDs (dgesvd here, dgesdd in numpy) agree to ~eps * cond
+        assert relerr(P, ref) < 1e-14 * s[0] / s[r - 1]
+        # Moore-Penrose conditions (the kept-rank projectors are symmetric and idempotent)
+        assert relerr(P @ A @ P, P) < 1e-8
+        AP = A @ P
+        assert np.abs(AP - AP.T).max() < 1e-8
+        PA = P @ A
+        assert np.abs(PA - PA.T).max() < 1e-8
+
+
+@pytest.mark.parametrize("kt", [111, 321, 511])
+def test_fft_m2l_equals_dense_m2l(kt):
+    """UE2DC through the FFT (scatter, r2c, products, c2r, gather) equals the dense
+    kernel product K(DC <- UE shifted by 2 idx) for every V-list offset."""
+    np_ = 4
+    f = O.OracleFMM(kt, COEFS[kt], np_, 50, 5, (0, 0, 0), 0)
+    f.setup(np.zeros((1, 3)), None, np.zeros((1, 3)))
+    mm = O.kernel_mm(kt)
+    sm, tm = O.kernel_dofs(mm)
+    ue = O.splpos(np_, 1.0)
+    dc = O.splpos(np_, 1.0)
+    rng = np.random.default_rng(kt)
+    den = rng.standard_normal(ue.shape[0] * sm)
+    cnt = 0
+    worst = 0.0
+    for i in range(-3, 4):
+        for j in range(-3, 4):
+            for k in range(-3, 4):
+                if max(abs(i), abs(j), abs(k)) <= 1:
+                    with pytest.raises(ValueError):
+                        f.m2l_fft(0, (i, j, k), den)
+                    continue
+                got = f.m2l_fft(0, (i, j, k), den)
+                src = ue + 2.0 * np.array([i, j, k])
+                ref = O.kernel_matrix(mm, COEFS[kt], src, src, dc) @ den
+                worst = max(worst, relerr(got, ref))
+                cnt += 1
+    assert cnt == 316
+    assert worst < 1e-12, worst
+
+
+BANDS = {4: 1e-2, 6: 2e-4}
+
+
+@pytest.mark.parametrize("kt,np_,gen", [(111, 4, "cube"), (111, 6, "clusters"), (321, 4, "sphere"),
+                                        (511, 4, "clusters")])
+def test_oracle_end_to_end_vs_direct(kt, np_, gen):
+    if gen == "sphere":
+        p, n = sphere(3000, 21)
+        ctr, rl = (0.0, 0.0, 0.0), 0
+    else:
+        p = uniform_cube(4000, 22) if gen == "cube" else clusters(4000, 23)
+        n, ctr, rl = None, (0.5, 0.5, 0.5), 1
+    f = O.OracleFMM(kt, COEFS[kt], np_, 40, 10, ctr, rl)
+    f.setup(p, n, p)
+    den = densities(len(p), O.kernel_dofs(kt)[0], 1, 3)[0]
+    v = f.eval(den)
+    idx = np.random.default_rng(5).choice(len(p), 60, replace=False)
+    d = O.direct(kt, COEFS[kt], p, n, den, p[idx])
+    tdof = O.kernel_dofs(kt)[1]
+    e = relerr(v.reshape(-1, tdof)[idx], d)
+    assert e < BANDS[np_], e
+    # accuracy improves with np (the "low/mid/high accuracy" of options_0:5)
+    if np_ == 4 and kt == 111:
+        g = O.OracleFMM(kt, COEFS[kt], 6, 40, 10, ctr, rl)
+        g.setup(p, n, p)
+        assert relerr(g.eval(den).reshape(-1, tdof)[idx], d) < e / 10
+
+
+def test_rounding_floor_of_the_algorithm():
+    """With accurate accumulation of the dense mat-vecs (the default), reversing
+    every summation order changes the result by < 1e-12; plain FP64 accumulation
+    (the reference's BLAS behaviour) is only defined to ~1e-11 at np 6."""
+    L = O.lib()
+    L.orc_set_order.argtypes = [ctypes.c_int]
+    L.orc_set_accurate.argtypes = [ctypes.c_int]
+    p = clusters(6000, 2)
+    f = O.OracleFMM(111, [1.0], 6, 30, 10, (0.5, 0.5, 0.5), 1)
+    f.setup(p, None, p)
+    den = densities(len(p), 1, 1, 0)[0]
+    try:
+        L.orc_set_order(0)
+        a = f.eval(den)
+        L.orc_set_order(15)
+        b = f.eval(den)
+        assert relerr(b, a) < 1e-12
+        L.orc_set_accurate(0)
+        L.orc_set_order(0)
+        c = f.eval(den)
+        L.orc_set_order(15)
+        d = f.eval(den)
+        floor = relerr(d, c)
+        print(f"plain-FP64 reorder floor at np 6: {floor:.2e}; accurate vs plain {relerr(c, a):.2e}")
+        assert 1e-14 < floor < 1e-9
+        assert relerr(c, a) < 1e-9
+    finally:
+        L.orc_set_order(0)
+        L.orc_set_accurate(1)
