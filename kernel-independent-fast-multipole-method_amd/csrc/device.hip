@@ -894,6 +894,106 @@ __global__ __launch_bounds__(256) void k_m2l(int C, int Rc, const int* __restric
     for (int i = 0; i < TM; i++) out[((long)rr * TM + i) * C] = acc[rr][i];
 }
 
+// M2L grouped by target parent.  One lane per frequency; the lane keeps the
+// spectra of all 8 target children for RB right-hand sides in registers and
+// streams the children of the 26 neighbour parents, so every source spectrum
+// is read once per target parent instead of once per V pair, and each
+// operator value serves RB right-hand sides.  Blocks are ordered parent-
+// fastest within a frequency block and parents are dealt to the 8 XCDs in
+// contiguous ranges, so neighbouring parents share an L2.
+template <int SM, int TM, int RB>
+__global__ __launch_bounds__(256) void k_m2l_grp(int C, int Rc, int npar, const int* __restrict__ tslot,
+                                                 const int* __restrict__ sslot, const int* __restrict__ optab,
+                                                 const double2* __restrict__ ops, const double2* __restrict__ effden,
+                                                 double2* __restrict__ effval, int roff)
+{
+  const int per = (npar + 7) / 8;
+  const int b = blockIdx.x;
+  const int pidx = (b & 7) * per + (b >> 3);
+  const int q = blockIdx.y * 256 + threadIdx.x;
+  const int r0 = roff + blockIdx.z * RB;
+  if (pidx >= npar || q >= C) return;
+  double2 acc[8][RB][TM];
+#pragma unroll
+  for (int c = 0; c < 8; c++)
+#pragma unroll
+    for (int rr = 0; rr < RB; rr++)
+#pragma unroll
+      for (int i = 0; i < TM; i++) acc[c][rr][i] = make_double2(0, 0);
+  const int* ss = sslot + (long)pidx * 26 * 8;
+  for (int n = 0; n < 26; n++) {
+    for (int sc = 0; sc < 8; sc++) {
+      const int sl = ss[n * 8 + sc];
+      if (sl < 0) continue;
+      double2 x[RB][SM];
+      const double2* D = effden + ((long)sl * Rc + r0) * SM * C + q;
+#pragma unroll
+      for (int rr = 0; rr < RB; rr++)
+#pragma unroll
+        for (int j = 0; j < SM; j++) x[rr][j] = D[((long)rr * SM + j) * C];
+      const int* ot = optab + (n * 8 + sc) * 8;
+#pragma unroll
+      for (int c = 0; c < 8; c++) {
+        const int o = ot[c];
+        if (o < 0) continue;
+        const double2* Tp = ops + (long)o * SM * TM * C + q;
+#pragma unroll
+        for (int j = 0; j < SM; j++)
+#pragma unroll
+          for (int i = 0; i < TM; i++) {
+            const double2 t = Tp[(long)(j * TM + i) * C];
+#pragma unroll
+            for (int rr = 0; rr < RB; rr++) {
+              acc[c][rr][i].x += t.x * x[rr][j].x - t.y * x[rr][j].y;
+              acc[c][rr][i].y += t.x * x[rr][j].y + t.y * x[rr][j].x;
+            }
+          }
+      }
+    }
+  }
+  const int* ts = tslot + (long)pidx * 8;
+#pragma unroll
+  for (int c = 0; c < 8; c++) {
+    const int tl = ts[c];
+    if (tl < 0) continue;
+    double2* out = effval + ((long)tl * Rc + r0) * TM * C + q;
+#pragma unroll
+    for (int rr = 0; rr < RB; rr++)
+#pragma unroll
+      for (int i = 0; i < TM; i++) out[((long)rr * TM + i) * C] = acc[c][rr][i];
+  }
+}
+
+template <int SM, int TM, int RBMAX>
+static void m2l_grp_st(hipStream_t s, int C, int npar, int Rc, const int* tslot, const int* sslot, const int* optab,
+                       const double2* ops, const double2* effden, double2* effval)
+{
+  const int per = (npar + 7) / 8;
+  int r = 0;
+  while (r < Rc) {
+    int left = Rc - r;
+    int RB = left >= RBMAX ? RBMAX : 1;
+    int nb = left / RB;
+    dim3 grid(8 * per, (C + 255) / 256, nb);
+    if (RB == RBMAX)
+      hipLaunchKernelGGL((k_m2l_grp<SM, TM, RBMAX>), grid, dim3(256), 0, s, C, Rc, npar, tslot, sslot, optab, ops,
+                         effden, effval, r);
+    else
+      hipLaunchKernelGGL((k_m2l_grp<SM, TM, 1>), grid, dim3(256), 0, s, C, Rc, npar, tslot, sslot, optab, ops,
+                         effden, effval, r);
+    r += nb * RB;
+  }
+}
+
+void m2l_grouped(hipStream_t s, int sm, int tm, int C, int npar, int Rc, const int* tslot, const int* sslot,
+                 const int* optab, const double2* ops, const double2* effden, double2* effval)
+{
+  if (npar <= 0) return;
+  if (sm == 1 && tm == 1) m2l_grp_st<1, 1, 4>(s, C, npar, Rc, tslot, sslot, optab, ops, effden, effval);
+  else if (sm == 4 && tm == 3) m2l_grp_st<4, 3, 1>(s, C, npar, Rc, tslot, sslot, optab, ops, effden, effval);
+  else if (sm == 3 && tm == 3) m2l_grp_st<3, 3, 1>(s, C, npar, Rc, tslot, sslot, optab, ops, effden, effval);
+}
+
 template <int SM, int TM>
 static void m2l_st(hipStream_t s, int C, int ntrg, int Rc, const int* vptr, const int* vsrc, const int* vop,
                    const double2* ops, const double2* effden, double2* effval)

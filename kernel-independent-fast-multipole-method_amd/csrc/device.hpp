@@ -77,6 +77,11 @@ void fft_fwd(hipStream_t s, int np, int nslot, const int* slot_node, int Rc, con
 // M2L products summed over V pairs -> effval[trg][Rc][tm][C] (overwritten).
 void m2l(hipStream_t s, int sm, int tm, int C, int ntrg, int Rc, const int* vptr, const int* vsrc, const int* vop,
          const double2* ops, const double2* effden, double2* effval);
+// M2L grouped by target parent: for parent p, effval[tslot[p][c]] =
+// sum over the 26 neighbour directions n and source children s of
+// T_{optab[n][s][c]} effden[sslot[p][n][s]] (entries < 0 are skipped).
+void m2l_grouped(hipStream_t s, int sm, int tm, int C, int npar, int Rc, const int* tslot, const int* sslot,
+                 const int* optab, const double2* ops, const double2* effden, double2* effval);
 // c2r and gather onto the DC surface: dncval[(node*Rc + r)*pln + p*tm + f] += ...
 void fft_inv(hipStream_t s, int np, int nslot, const int* slot_node, int Rc, const double2* effval, int tm,
              const int* surf2cube, int nsp, double* dncval, int pln, const double2* tw);

@@ -12,6 +12,8 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <string>
+#include <unordered_map>
 
 namespace kifmm {
 
@@ -349,20 +351,52 @@ void Engine::build_plan()
     P.s2l.out = pool_i(P.I, xout); P.s2l.eptr = pool_i(P.I, xeptr);
     P.s2l.ent = pool_i(P.I, xent); P.s2l.geom = pool_d(P.D, xgeom);
   }
-  // V list per level (fmm3d_eval.cpp:115-150)
+  // V list per level (fmm3d_eval.cpp:115-150).  The V list of a node c is the set
+  // of source children s of the 26 neighbours Q of its parent P that are not
+  // adjacent to c (let3d.cpp:228-248), i.e. offset o = 2 (Q - P) + s - c with
+  // |o|_inf >= 2.  The grouped plan stores, per target parent, the spectrum
+  // slots of its 8 children and of the 8 children of each neighbour; the plan
+  // is checked against the lists pair by pair.
   {
-    std::vector<int> slot(T.nnode, -1);
+    const char* m2l_env = std::getenv("KIFMM3D_M2L");
+    P.m2l_pairs = m2l_env && std::string(m2l_env) == "pair";
+    std::vector<int> optab(26 * 64, -1);
+    int dirs[26][3];
+    {
+      int n = 0;
+      for (int dx = -1; dx <= 1; dx++)
+        for (int dy = -1; dy <= 1; dy++)
+          for (int dz = -1; dz <= 1; dz++)
+            if (dx || dy || dz) {
+              dirs[n][0] = dx; dirs[n][1] = dy; dirs[n][2] = dz;
+              for (int sc = 0; sc < 8; sc++)
+                for (int cc = 0; cc < 8; cc++) {
+                  const int sv[3] = {sc >> 2, (sc >> 1) & 1, sc & 1}, cv[3] = {cc >> 2, (cc >> 1) & 1, cc & 1};
+                  int o[3], linf = 0;
+                  for (int k = 0; k < 3; k++) {
+                    o[k] = 2 * dirs[n][k] + sv[k] - cv[k];
+                    linf = std::max(linf, std::abs(o[k]));
+                  }
+                  if (linf >= 2) optab[(n * 8 + sc) * 8 + cc] = (o[0] + 3) * 49 + (o[1] + 3) * 7 + (o[2] + 3);
+                }
+              n++;
+            }
+    }
+    P.optab = pool_i(P.I, optab);
+    std::vector<int> sslot_of(T.nnode, -1), tslot_of(T.nnode, -1);
     for (int L = 2; L < T.nlevel; L++) {
       std::vector<int> trg, src, vptr(1, 0), vsrc, vop;
+      long npair = 0;
       for (int g : by_depth[L]) {
         if (!(T.tag[g] & NODE_TRG)) continue;
         auto V = list(LIST_V, g);
         if (V.first == V.second) continue;
+        tslot_of[g] = (int)trg.size();
         trg.push_back(T.tndeidx[g]);
         for (auto it = V.first; it != V.second; ++it) {
           int v = *it;
-          if (slot[v] < 0) {
-            slot[v] = (int)src.size();
+          if (sslot_of[v] < 0) {
+            sslot_of[v] = (int)src.size();
             src.push_back(T.sndeidx[v]);
           }
           int d[3], linf = 0;
@@ -371,22 +405,79 @@ void Engine::build_plan()
             linf = std::max(linf, std::abs(d[k]));
           }
           if (linf < 2 || linf > 3) throw Error(KIFMM3D_EINVAL, "internal: V-list offset out of range");
-          vsrc.push_back(slot[v]);
-          vop.push_back((d[0] + 3) * 49 + (d[1] + 3) * 7 + (d[2] + 3));
+          npair++;
+          if (P.m2l_pairs) {
+            vsrc.push_back(sslot_of[v]);
+            vop.push_back((d[0] + 3) * 49 + (d[1] + 3) * 7 + (d[2] + 3));
+          }
         }
-        vptr.push_back((int)vsrc.size());
+        if (P.m2l_pairs) vptr.push_back((int)vsrc.size());
       }
       if (trg.empty()) continue;
       VLevel vl;
       vl.depth = L;
       vl.nsrc = (int)src.size();
       vl.ntrg = (int)trg.size();
-      vl.npair = (int)vsrc.size();
+      vl.npair = (int)npair;
       vl.src_node = pool_i(P.I, src);
       vl.trg_node = pool_i(P.I, trg);
-      vl.vptr = pool_i(P.I, vptr);
-      vl.vsrc = pool_i(P.I, vsrc);
-      vl.vop = pool_i(P.I, vop);
+      if (P.m2l_pairs) {
+        vl.vptr = pool_i(P.I, vptr);
+        vl.vsrc = pool_i(P.I, vsrc);
+        vl.vop = pool_i(P.I, vop);
+      } else {
+        // parents at depth L-1 with a child that has a V list
+        std::unordered_map<uint64_t, int> cell;
+        for (int g : by_depth[L - 1])
+          cell[((uint64_t)T.path[3 * g] << 40) | ((uint64_t)T.path[3 * g + 1] << 20) | (uint64_t)T.path[3 * g + 2]] = g;
+        std::vector<int> tsl, ssl;
+        long checked = 0;
+        std::vector<int> mine;
+        for (int pg : by_depth[L - 1]) {
+          if (T.terminal(pg)) continue;
+          bool any = false;
+          for (int c = 0; c < 8; c++) any = any || tslot_of[T.chd[pg] + c] >= 0;
+          if (!any) continue;
+          for (int c = 0; c < 8; c++) tsl.push_back(tslot_of[T.chd[pg] + c]);
+          const int lim = 1 << (L - 1);
+          for (int n = 0; n < 26; n++) {
+            int q[3];
+            bool in = true;
+            for (int k = 0; k < 3; k++) {
+              q[k] = T.path[3 * pg + k] + dirs[n][k];
+              in = in && q[k] >= 0 && q[k] < lim;
+            }
+            int qg = -1;
+            if (in) {
+              auto it = cell.find(((uint64_t)q[0] << 40) | ((uint64_t)q[1] << 20) | (uint64_t)q[2]);
+              if (it != cell.end() && !T.terminal(it->second)) qg = it->second;
+            }
+            for (int sc = 0; sc < 8; sc++) ssl.push_back(qg >= 0 ? sslot_of[T.chd[qg] + sc] : -1);
+          }
+          // the grouped pairs of each target child must be exactly its V list
+          const int* sp = &ssl[ssl.size() - 26 * 8];
+          for (int c = 0; c < 8; c++) {
+            int g = T.chd[pg] + c;
+            if (tslot_of[g] < 0) continue;
+            mine.clear();
+            for (int n = 0; n < 26; n++)
+              for (int sc = 0; sc < 8; sc++)
+                if (sp[n * 8 + sc] >= 0 && optab[(n * 8 + sc) * 8 + c] >= 0) mine.push_back(sp[n * 8 + sc]);
+            auto V = list(LIST_V, g);
+            std::vector<int> ref;
+            for (auto it = V.first; it != V.second; ++it) ref.push_back(sslot_of[*it]);
+            std::sort(mine.begin(), mine.end());
+            std::sort(ref.begin(), ref.end());
+            if (mine != ref) throw Error(KIFMM3D_EINVAL, "internal: grouped M2L plan differs from the V list");
+            checked += (long)ref.size();
+          }
+        }
+        if (checked != npair) throw Error(KIFMM3D_EINVAL, "internal: grouped M2L plan misses V pairs");
+        vl.npar = (int)(tsl.size() / 8);
+        vl.tslot = pool_i(P.I, tsl);
+        vl.sslot = pool_i(P.I, ssl);
+      }
+      for (int g : by_depth[L]) tslot_of[g] = -1;
       vl.dscale = (long)pool_d(P.D, scales(L + rootlvl_, +1, sm));  // plnden2effden (matmgnt3d.cpp:226-235)
       P.max_vsrc = std::max(P.max_vsrc, vl.nsrc);
       P.max_vtrg = std::max(P.max_vtrg, vl.ntrg);
@@ -605,8 +696,12 @@ void Engine::run_chunk(const double* srcden, double* trgval, int rc, hipEvent_t*
                    (double2*)effden_.p, (const double2*)tw_.p);
     });
     timed(KIFMM3D_K_M2L, [&] {
-      dev::m2l(s, sm, tm, O.C, vl.ntrg, rc, I + vl.vptr, I + vl.vsrc, I + vl.vop, (const double2*)m2l_.p,
-               (const double2*)effden_.p, (double2*)effval_.p);
+      if (P.m2l_pairs)
+        dev::m2l(s, sm, tm, O.C, vl.ntrg, rc, I + vl.vptr, I + vl.vsrc, I + vl.vop, (const double2*)m2l_.p,
+                 (const double2*)effden_.p, (double2*)effval_.p);
+      else
+        dev::m2l_grouped(s, sm, tm, O.C, vl.npar, rc, I + vl.tslot, I + vl.sslot, I + P.optab,
+                         (const double2*)m2l_.p, (const double2*)effden_.p, (double2*)effval_.p);
     });
     timed(KIFMM3D_K_FFT_INV, [&] {
       dev::fft_inv(s, np_, vl.ntrg, I + vl.trg_node, rc, (const double2*)effval_.p, tm, s2c_.p, O.nsp[SURF_DC],

@@ -55,7 +55,10 @@ struct ChunkList {  // chunks of <= 128 leaf targets
 
 struct VLevel {
   int depth = 0, nsrc = 0, ntrg = 0, npair = 0;
-  size_t src_node = 0, trg_node = 0, vptr = 0, vsrc = 0, vop = 0;
+  size_t src_node = 0, trg_node = 0;
+  size_t vptr = 0, vsrc = 0, vop = 0;  // per-pair form (V-list CSR with operator index)
+  int npar = 0;                        // grouped form: target parents of the level
+  size_t tslot = 0, sslot = 0;         // [npar][8] target-child slots, [npar][26][8] source-child slots
   long dscale = -1;
 };
 
@@ -65,6 +68,8 @@ struct Plan {
   std::vector<GemmOp> up, down;
   std::vector<VLevel> vlev;
   int max_vsrc = 0, max_vtrg = 0;
+  bool m2l_pairs = false;      // per-pair M2L kernel instead of the parent-grouped one
+  size_t optab = 0;            // [26 directions][8 source children][8 target children] -> operator or -1
   int nlow_s = 0, nlow_t = 0;  // source / target node indices of depth < 2 form these prefixes
   long near_pairs = 0;         // source-target pairs of the near field (one RHS)
   long far_pairs = 0;          // surface point - target pairs of L2T + M2T
