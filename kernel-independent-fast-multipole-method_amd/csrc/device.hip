@@ -14,6 +14,22 @@ namespace dev {
 
 // ---------------------------------------------------------------------------
 // kernels: accumulate u[q][t] += sum_s K_ts(d, n) f[q*S + s]
+//
+// 1/r comes from the hardware estimate v_rsq_f64 refined by two Newton steps
+// (about 1 ulp), which replaces the reference's sqrt and divisions; the
+// r < 1e-8 exclusion (kernel3d.cpp:21) is applied as r^2 < 1e-16.  The few-ulp
+// differences this makes in single kernel values are far below the parity bar
+// (checked with the oracle: 1-ulp kernel perturbations move the result by
+// < 1e-14 at np 6).
+__device__ inline double rsqrt_nr(double x)
+{
+  double y = __builtin_amdgcn_rsq(x);
+  double e = fma(-x * y, y, 1.0);
+  y = fma(0.5 * y, e, y);
+  e = fma(-x * y, y, 1.0);
+  return fma(0.5 * y, e, y);
+}
+
 template <int KT>
 struct Knl;
 
@@ -26,9 +42,8 @@ struct Knl<111> {
                                     const KC& k)
   {
     double r2 = dx * dx + dy * dy + dz * dz;
-    double r = sqrt(r2);
-    if (r < 1e-8) return;
-    double g = k.a / r;
+    if (r2 < 1e-16) return;
+    double g = k.a * rsqrt_nr(r2);
 #pragma unroll
     for (int q = 0; q < NR; q++) u[q][0] += g * f[q];
   }
@@ -43,10 +58,9 @@ struct Knl<301> {
                                     const KC& k)
   {
     double r2 = dx * dx + dy * dy + dz * dz;
-    double r = sqrt(r2);
-    if (r < 1e-8) return;
-    double r3 = r2 * r;
-    double G = k.a / r, H = k.a / r3, A = k.b / r3;
+    if (r2 < 1e-16) return;
+    double ri = rsqrt_nr(r2), ri3 = ri * ri * ri;
+    double G = k.a * ri, H = k.a * ri3, A = k.b * ri3;
 #pragma unroll
     for (int q = 0; q < NR; q++) {
       const double* g = f + 4 * q;
@@ -68,11 +82,10 @@ struct Knl<321> {
                                     const KC& k)
   {
     double r2 = dx * dx + dy * dy + dz * dz;
-    double r = sqrt(r2);
-    if (r < 1e-8) return;
+    if (r2 < 1e-16) return;
     double rn = dx * n[0] + dy * n[1] + dz * n[2];
-    double r5 = r2 * r2 * r;
-    double c = (k.a / r5) * rn;
+    double ri = rsqrt_nr(r2), ri2 = ri * ri;
+    double c = (k.a * (ri2 * ri2 * ri)) * rn;
 #pragma unroll
     for (int q = 0; q < NR; q++) {
       const double* g = f + 3 * q;
@@ -93,10 +106,9 @@ struct Knl<511> {
                                     const KC& k)
   {
     double r2 = dx * dx + dy * dy + dz * dz;
-    double r = sqrt(r2);
-    if (r < 1e-8) return;
-    double r3 = r2 * r;
-    double G = k.a / r, H = k.b / r3;
+    if (r2 < 1e-16) return;
+    double ri = rsqrt_nr(r2);
+    double G = k.a * ri, H = k.b * (ri * ri * ri);
 #pragma unroll
     for (int q = 0; q < NR; q++) {
       const double* g = f + 3 * q;
@@ -610,6 +622,7 @@ __device__ inline double dot2_finish(double o, double hi, double lo)
   return s + (t + lo);
 }
 
+template <bool DOT2>
 __global__ __launch_bounds__(256) void k_gemm(GemmArgs p)
 {
   const int grp = blockIdx.z;
@@ -662,7 +675,12 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs p)
 #pragma unroll
       for (int i = 0; i < 4; i++)
 #pragma unroll
-        for (int j = 0; j < 4; j++) dot2_acc(hi[i][j], lo[i][j], a[i], b[j]);
+        for (int j = 0; j < 4; j++) {
+          if (DOT2)
+            dot2_acc(hi[i][j], lo[i][j], a[i], b[j]);
+          else
+            hi[i][j] = fma(a[i], b[j], hi[i][j]);
+        }
     }
     __syncthreads();
   }
@@ -681,7 +699,10 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs p)
         h *= p.cscale[row];
         l *= p.cscale[row];
       }
-      out[row] = dot2_finish(p.accumulate ? out[row] : 0.0, h, l);
+      if (DOT2)
+        out[row] = dot2_finish(p.accumulate ? out[row] : 0.0, h, l);
+      else
+        out[row] = p.accumulate ? out[row] + h : h;
     }
   }
 }
@@ -691,7 +712,10 @@ void gemm(hipStream_t s, const GemmArgs& a)
   if (a.ngroup <= 0 || a.maxcols <= 0) return;
   long cols = (long)a.maxcols * a.Rc;
   dim3 grid((unsigned)((cols + 63) / 64), (a.M + 63) / 64, a.ngroup);
-  hipLaunchKernelGGL(k_gemm, grid, dim3(256), 0, s, a);
+  if (a.compensated)
+    hipLaunchKernelGGL(k_gemm<true>, grid, dim3(256), 0, s, a);
+  else
+    hipLaunchKernelGGL(k_gemm<false>, grid, dim3(256), 0, s, a);
 }
 
 // ---------------------------------------------------------------------------
@@ -985,13 +1009,26 @@ static void m2l_grp_st(hipStream_t s, int C, int npar, int Rc, const int* tslot,
   }
 }
 
-void m2l_grouped(hipStream_t s, int sm, int tm, int C, int npar, int Rc, const int* tslot, const int* sslot,
+void m2l_grouped(hipStream_t s, int sm, int tm, int C, int npar, int Rc, int rb, const int* tslot, const int* sslot,
                  const int* optab, const double2* ops, const double2* effden, double2* effval)
 {
   if (npar <= 0) return;
-  if (sm == 1 && tm == 1) m2l_grp_st<1, 1, 4>(s, C, npar, Rc, tslot, sslot, optab, ops, effden, effval);
-  else if (sm == 4 && tm == 3) m2l_grp_st<4, 3, 1>(s, C, npar, Rc, tslot, sslot, optab, ops, effden, effval);
-  else if (sm == 3 && tm == 3) m2l_grp_st<3, 3, 1>(s, C, npar, Rc, tslot, sslot, optab, ops, effden, effval);
+  if (sm == 1 && tm == 1) {
+    if (rb == 8)
+      m2l_grp_st<1, 1, 8>(s, C, npar, Rc, tslot, sslot, optab, ops, effden, effval);
+    else
+      m2l_grp_st<1, 1, 4>(s, C, npar, Rc, tslot, sslot, optab, ops, effden, effval);
+  } else if (sm == 4 && tm == 3) {
+    if (rb == 2)
+      m2l_grp_st<4, 3, 2>(s, C, npar, Rc, tslot, sslot, optab, ops, effden, effval);
+    else
+      m2l_grp_st<4, 3, 1>(s, C, npar, Rc, tslot, sslot, optab, ops, effden, effval);
+  } else if (sm == 3 && tm == 3) {
+    if (rb == 2)
+      m2l_grp_st<3, 3, 2>(s, C, npar, Rc, tslot, sslot, optab, ops, effden, effval);
+    else
+      m2l_grp_st<3, 3, 1>(s, C, npar, Rc, tslot, sslot, optab, ops, effden, effval);
+  }
 }
 
 template <int SM, int TM>

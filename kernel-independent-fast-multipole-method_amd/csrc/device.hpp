@@ -67,6 +67,7 @@ struct GemmArgs {
   const double* bscale;
   const double* cscale;
   int accumulate;
+  int compensated;  // 1: TwoProd/TwoSum accumulation (pseudoinverse applies), 0: plain FMA
 };
 void gemm(hipStream_t s, const GemmArgs& a);
 
@@ -80,7 +81,7 @@ void m2l(hipStream_t s, int sm, int tm, int C, int ntrg, int Rc, const int* vptr
 // M2L grouped by target parent: for parent p, effval[tslot[p][c]] =
 // sum over the 26 neighbour directions n and source children s of
 // T_{optab[n][s][c]} effden[sslot[p][n][s]] (entries < 0 are skipped).
-void m2l_grouped(hipStream_t s, int sm, int tm, int C, int npar, int Rc, const int* tslot, const int* sslot,
+void m2l_grouped(hipStream_t s, int sm, int tm, int C, int npar, int Rc, int rb, const int* tslot, const int* sslot,
                  const int* optab, const double2* ops, const double2* effden, double2* effval);
 // c2r and gather onto the DC surface: dncval[(node*Rc + r)*pln + p*tm + f] += ...
 void fft_inv(hipStream_t s, int np, int nslot, const int* slot_node, int Rc, const double2* effval, int tm,

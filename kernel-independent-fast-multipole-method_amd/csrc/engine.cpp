@@ -91,6 +91,7 @@ Engine::Engine(int kt, const double* coefs, int ncoefs, int np, int ptsmax, int 
   tree_.maxlevel = maxlevel;
   tree_.rootlvl = rootlvl;
   for (int d = 0; d < 3; d++) tree_.ctr[d] = ctr_[d];
+  if (const char* rb = std::getenv("KIFMM3D_M2L_RB")) m2l_rb_ = std::atoi(rb);
 }
 
 Engine::~Engine()
@@ -264,7 +265,7 @@ void Engine::build_plan()
       GemmOp op;
       op.M = pUE; op.Kseg = pUC; op.nseg = 1; op.ngroup = 1;
       op.ncol = op.maxcols = (int)all.size();
-      op.a_which = 0; op.b_buf = 0; op.c_buf = 1;
+      op.a_which = 0; op.b_buf = 0; op.c_buf = 1; op.compensated = 1;
       std::vector<int> idx(all.size());
       for (size_t n = 0; n < all.size(); n++) idx[n] = T.sndeidx[all[n]];
       op.bnode = op.cnode = pool_i(P.I, idx);
@@ -522,7 +523,7 @@ void Engine::build_plan()
     GemmOp op;
     op.M = pDE; op.Kseg = pDC; op.nseg = 1; op.ngroup = 1;
     op.ncol = op.maxcols = (int)all.size();
-    op.a_which = 2; op.b_buf = 2; op.c_buf = 3;
+    op.a_which = 2; op.b_buf = 2; op.c_buf = 3; op.compensated = 1;
     std::vector<int> idx(all.size());
     for (size_t n = 0; n < all.size(); n++) idx[n] = T.tndeidx[all[n]];
     op.bnode = op.cnode = pool_i(P.I, idx);
@@ -645,6 +646,7 @@ void Engine::run_chunk(const double* srcden, double* trgval, int rc, hipEvent_t*
     a.bscale = op.bscale >= 0 ? D + op.bscale : nullptr;
     a.cscale = op.cscale >= 0 ? D + op.cscale : nullptr;
     a.accumulate = op.accumulate;
+    a.compensated = op.compensated;
     dev::gemm(s, a);
   };
 
@@ -700,7 +702,7 @@ void Engine::run_chunk(const double* srcden, double* trgval, int rc, hipEvent_t*
         dev::m2l(s, sm, tm, O.C, vl.ntrg, rc, I + vl.vptr, I + vl.vsrc, I + vl.vop, (const double2*)m2l_.p,
                  (const double2*)effden_.p, (double2*)effval_.p);
       else
-        dev::m2l_grouped(s, sm, tm, O.C, vl.npar, rc, I + vl.tslot, I + vl.sslot, I + P.optab,
+        dev::m2l_grouped(s, sm, tm, O.C, vl.npar, rc, m2l_rb_, I + vl.tslot, I + vl.sslot, I + P.optab,
                          (const double2*)m2l_.p, (const double2*)effden_.p, (double2*)effval_.p);
     });
     timed(KIFMM3D_K_FFT_INV, [&] {

@@ -39,6 +39,7 @@ struct GemmOp {
   size_t bnode = 0, cnode = 0, gbeg = 0;
   long bscale = -1, cscale = -1;
   int accumulate = 0;
+  int compensated = 0;  // pseudoinverse applies (UC2UE, DC2DE) accumulate compensated
 };
 
 struct ItemList {  // point -> surface items
@@ -124,6 +125,7 @@ class Engine {
   bool setup_done_ = false;
   bool host_done_ = false;
   int last_rc_ = 0;
+  int m2l_rb_ = 0;  // right-hand sides per lane in the grouped M2L (0 = default; KIFMM3D_M2L_RB)
   hipStream_t stream_ = nullptr;
   std::vector<hipEvent_t> evpool_;
   std::vector<int> evcls_;
