@@ -15,19 +15,30 @@ namespace dev {
 // ---------------------------------------------------------------------------
 // kernels: accumulate u[q][t] += sum_s K_ts(d, n) f[q*S + s]
 //
-// 1/r comes from the hardware estimate v_rsq_f64 refined by two Newton steps
-// (about 1 ulp), which replaces the reference's sqrt and divisions; the
-// r < 1e-8 exclusion (kernel3d.cpp:21) is applied as r^2 < 1e-16.  The few-ulp
-// differences this makes in single kernel values are far below the parity bar
-// (checked with the oracle: 1-ulp kernel perturbations move the result by
-// < 1e-14 at np 6).
+// 1/r comes from the hardware estimate v_rsq_f64 refined by one third-order
+// step (about 1 ulp), which replaces the reference's sqrt and divisions; the
+// r < 1e-8 exclusion (kernel3d.cpp:21) is applied as r^2 < 1e-16 by zeroing
+// 1/r.  The few-ulp differences this makes in single kernel values are far
+// below the parity bar (checked with the oracle: 1-ulp kernel perturbations
+// move the result by < 1e-14 at np 6).
+//
+// The estimate y0 has relative error below 2^-24 on gfx950 (measured over 4M
+// inputs spanning 120 binades); with e = 1 - x y0^2 the step
+// y = y0 (1 + e/2 + 3e^2/8) leaves a truncation error of (5/16) e^3 < 2^-70.
 __device__ inline double rsqrt_nr(double x)
 {
   double y = __builtin_amdgcn_rsq(x);
   double e = fma(-x * y, y, 1.0);
-  y = fma(0.5 * y, e, y);
-  e = fma(-x * y, y, 1.0);
-  return fma(0.5 * y, e, y);
+  double p = fma(e, 0.375, 0.5);
+  return fma(y * e, p, y);
+}
+
+// 1/r from r^2, or 0 inside the exclusion radius: every kernel term carries a
+// factor of 1/r, so a zero here drops the pair without a branch.
+__device__ inline double rinv(double r2)
+{
+  double ri = rsqrt_nr(r2);
+  return r2 < 1e-16 ? 0.0 : ri;
 }
 
 template <int KT>
@@ -41,9 +52,7 @@ struct Knl<111> {
   __device__ static inline void add(double dx, double dy, double dz, const double*, const double* f, double (&u)[NR][T],
                                     const KC& k)
   {
-    double r2 = dx * dx + dy * dy + dz * dz;
-    if (r2 < 1e-16) return;
-    double g = k.a * rsqrt_nr(r2);
+    double g = k.a * rinv(dx * dx + dy * dy + dz * dz);
 #pragma unroll
     for (int q = 0; q < NR; q++) u[q][0] += g * f[q];
   }
@@ -57,9 +66,7 @@ struct Knl<301> {
   __device__ static inline void add(double dx, double dy, double dz, const double*, const double* f, double (&u)[NR][T],
                                     const KC& k)
   {
-    double r2 = dx * dx + dy * dy + dz * dz;
-    if (r2 < 1e-16) return;
-    double ri = rsqrt_nr(r2), ri3 = ri * ri * ri;
+    double ri = rinv(dx * dx + dy * dy + dz * dz), ri3 = ri * ri * ri;
     double G = k.a * ri, H = k.a * ri3, A = k.b * ri3;
 #pragma unroll
     for (int q = 0; q < NR; q++) {
@@ -81,10 +88,8 @@ struct Knl<321> {
   __device__ static inline void add(double dx, double dy, double dz, const double* n, const double* f, double (&u)[NR][T],
                                     const KC& k)
   {
-    double r2 = dx * dx + dy * dy + dz * dz;
-    if (r2 < 1e-16) return;
     double rn = dx * n[0] + dy * n[1] + dz * n[2];
-    double ri = rsqrt_nr(r2), ri2 = ri * ri;
+    double ri = rinv(dx * dx + dy * dy + dz * dz), ri2 = ri * ri;
     double c = (k.a * (ri2 * ri2 * ri)) * rn;
 #pragma unroll
     for (int q = 0; q < NR; q++) {
@@ -105,9 +110,7 @@ struct Knl<511> {
   __device__ static inline void add(double dx, double dy, double dz, const double*, const double* f, double (&u)[NR][T],
                                     const KC& k)
   {
-    double r2 = dx * dx + dy * dy + dz * dz;
-    if (r2 < 1e-16) return;
-    double ri = rsqrt_nr(r2);
+    double ri = rinv(dx * dx + dy * dy + dz * dz);
     double G = k.a * ri, H = k.b * (ri * ri * ri);
 #pragma unroll
     for (int q = 0; q < NR; q++) {
@@ -123,15 +126,14 @@ struct Knl<511> {
 // ---------------------------------------------------------------------------
 // permutations
 __global__ void k_gather(int n, int dof, int nr, const int* __restrict__ perm, const double* __restrict__ src,
-                         long src_rs, double* __restrict__ dst, long dst_rs)
+                         long src_rs, double* __restrict__ dst)
 {
   long tot = (long)n * dof * nr;
+  const int per = nr * dof;  // i = k*per + r*dof + d, in destination order
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < tot; i += (long)gridDim.x * blockDim.x) {
-    long per = (long)n * dof;
-    int r = (int)(i / per);
-    long rem = i - r * per;
-    int k = (int)(rem / dof), d = (int)(rem - (long)k * dof);
-    dst[r * dst_rs + rem] = src[r * src_rs + (long)perm[k] * dof + d];
+    int k = (int)(i / per), rem = (int)(i - (long)k * per);
+    int r = rem / dof, d = rem - r * dof;
+    dst[i] = src[r * src_rs + (long)perm[k] * dof + d];
   }
 }
 
@@ -155,12 +157,11 @@ static int grid_for(long tot, int bs)
   return (int)(g > 0 ? g : 1);
 }
 
-void gather(hipStream_t s, int n, int dof, int nr, const int* perm, const double* src, long src_rs, double* dst,
-            long dst_rs)
+void gather(hipStream_t s, int n, int dof, int nr, const int* perm, const double* src, long src_rs, double* dst)
 {
   long tot = (long)n * dof * nr;
   if (tot == 0) return;
-  hipLaunchKernelGGL(k_gather, dim3(grid_for(tot, 256)), dim3(256), 0, s, n, dof, nr, perm, src, src_rs, dst, dst_rs);
+  hipLaunchKernelGGL(k_gather, dim3(grid_for(tot, 256)), dim3(256), 0, s, n, dof, nr, perm, src, src_rs, dst);
 }
 
 void scatter(hipStream_t s, int n, int dof, int nr, const int* perm, const double* src, long src_rs, double* dst,
@@ -175,6 +176,19 @@ void scatter(hipStream_t s, int n, int dof, int nr, const int* perm, const doubl
 // near field (U list and direct W/X)
 constexpr int P2P_BS = 128;
 constexpr int TILE = 64;
+
+// Densities of sources j0 .. j0+cnt for NR right-hand sides into LDS,
+// s_den[jj][q*S + d]; den points at right-hand side r0 of the block.
+template <int S, int NR, int BS>
+__device__ inline void stage_den(double (*s_den)[NR * S], const double* den, const Points& src, int j0, int cnt, int t)
+{
+  constexpr int ND = NR * S;
+  for (int i = t; i < cnt * ND; i += BS) {
+    int jj = i / ND, rem = i - jj * ND;
+    int q = rem / S, d = rem - q * S;
+    s_den[jj][rem] = den[(long)(j0 + jj) * src.den_js + (long)q * src.den_rs + d];
+  }
+}
 
 template <int KT, int NR>
 __global__ __launch_bounds__(P2P_BS) void k_p2p(const int* __restrict__ tbeg, const int* __restrict__ tnum,
@@ -218,12 +232,7 @@ __global__ __launch_bounds__(P2P_BS) void k_p2p(const int* __restrict__ tbeg, co
           s_pos[i][NPOS - 1] = src.nz[j0 + i];
         }
       }
-      const int per = cnt * K::S;
-      for (int i = t; i < NR * per; i += P2P_BS) {
-        int q = i / per, rem = i - q * per;
-        int jj = rem / K::S, sd = rem - jj * K::S;
-        s_den[jj][q * K::S + sd] = den[(long)q * src.den_rs + (long)j0 * K::S + rem];
-      }
+      stage_den<K::S, NR, P2P_BS>(s_den, den, src, j0, cnt, t);
       __syncthreads();
       if (act)
         for (int jj = 0; jj < cnt; jj++)
@@ -237,6 +246,132 @@ __global__ __launch_bounds__(P2P_BS) void k_p2p(const int* __restrict__ tbeg, co
     for (int q = 0; q < NR; q++)
 #pragma unroll
       for (int d = 0; d < K::T; d++) val[(long)q * trg.val_rs + d] += u[q][d];
+  }
+}
+
+// Near field with the sources read through the scalar data cache: one wave
+// per half chunk (64 targets), the wave walks the chunk's source leaves with
+// wave-uniform addresses, so positions and the contiguous NR*S densities of a
+// source arrive in scalar registers and feed the FMAs directly (no LDS, no
+// barriers).  Requires src.den_rs == S.
+template <int KT, int NR>
+__global__ __launch_bounds__(64) void k_p2p_s(const int* __restrict__ tbeg, const int* __restrict__ tnum,
+                                              const int* __restrict__ eptr, const int2* __restrict__ ent, Points src,
+                                              Targets trg, int roff, KC kc)
+{
+  using K = Knl<KT>;
+  const int c = blockIdx.x >> 1, h = (blockIdx.x & 1) * 64, t = threadIdx.x;
+  const int r0 = roff + blockIdx.y * NR;
+  const int tb = tbeg[c] + h, tn = tnum[c] - h;
+  if (tn <= 0) return;
+  const bool act = t < tn;
+  const int it = act ? t : 0;
+  const double x = trg.x[tb + it], y = trg.y[tb + it], z = trg.z[tb + it];
+  double u[NR][K::T];
+#pragma unroll
+  for (int q = 0; q < NR; q++)
+#pragma unroll
+    for (int d = 0; d < K::T; d++) u[q][d] = 0.0;
+  const double* __restrict__ den = src.den + (long)r0 * K::S;
+  const long js = src.den_js;
+  for (int e = eptr[c]; e < eptr[c + 1]; e++) {
+    const int2 se = ent[e];
+#pragma unroll 2
+    for (int j = se.x; j < se.x + se.y; j++) {
+      double nrm[3];
+      if (K::NRM) {
+        nrm[0] = src.nx[j];
+        nrm[1] = src.ny[j];
+        nrm[2] = src.nz[j];
+      }
+      K::template add<NR>(x - src.x[j], y - src.y[j], z - src.z[j], nrm, den + j * js, u, kc);
+    }
+  }
+  if (act) {
+    double* val = trg.val + (long)r0 * trg.val_rs + (long)(tb + t) * K::T;
+#pragma unroll
+    for (int q = 0; q < NR; q++)
+#pragma unroll
+      for (int d = 0; d < K::T; d++) val[(long)q * trg.val_rs + d] += u[q][d];
+  }
+}
+
+// Laplace near field for blocks of 16 right-hand sides on the FP64 matrix
+// cores.  For a group of 4 sources the wave forms
+//   U[q][t] += sum_k F[k][q] G[t][k]      (v_mfma_f64_16x16x4f64)
+// with A = F^T (16 RHS x 4 sources, lane l holds F[k = l/16][q = l%16]) and
+// B = G^T (4 sources x 16 targets, lane l holds G(target l%16, source l/16)),
+// so every lane evaluates distinct pairs and the densities come from LDS as
+// one contiguous 512-byte row per group instead of a broadcast per pair.  A
+// wave covers 64 targets as four 16-target column blocks; the accumulator
+// lane l, item v holds right-hand side l/16 + 4*v of target l%16 (the D layout
+// of v_mfma_f64_16x16x4f64: row l/16 + 4v, column l%16).  Source
+// groups are padded to a multiple of 4 with zero densities.  Summation order
+// differs from the scalar kernels only by FP64 rounding.
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(P2P_BS) void k_p2p_mfma111(const int* __restrict__ tbeg, const int* __restrict__ tnum,
+                                                        const int* __restrict__ eptr, const int2* __restrict__ ent,
+                                                        Points src, Targets trg, int roff, KC kc)
+{
+  constexpr int NR = 16;
+  __shared__ double s_pos[3][TILE];
+  __shared__ double s_den[TILE][NR];
+  const int c = blockIdx.x, t = threadIdx.x, w = t >> 6, l = t & 63;
+  const int col = l & 15, kk = l >> 4;
+  const int r0 = roff + blockIdx.y * NR;
+  const int tb = tbeg[c], tn = tnum[c];
+  const bool wact = 64 * w < tn;  // wave-uniform
+  double tx[4], ty[4], tz[4];
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    int tt = 64 * w + 16 * i + col;
+    tt = tt < tn ? tt : 0;
+    tx[i] = trg.x[tb + tt];
+    ty[i] = trg.y[tb + tt];
+    tz[i] = trg.z[tb + tt];
+  }
+  v4d acc[4];
+#pragma unroll
+  for (int i = 0; i < 4; i++) acc[i] = v4d{0.0, 0.0, 0.0, 0.0};
+  const double* den = src.den + (long)r0 * src.den_rs;
+  for (int e = eptr[c]; e < eptr[c + 1]; e++) {
+    const int2 se = ent[e];
+    for (int b0 = 0; b0 < se.y; b0 += TILE) {
+      const int cnt = min(TILE, se.y - b0), cnt4 = (cnt + 3) & ~3;
+      const int j0 = se.x + b0;
+      __syncthreads();
+      for (int i = t; i < cnt4; i += P2P_BS) {
+        const bool in = i < cnt;
+        s_pos[0][i] = in ? src.x[j0 + i] : 0.0;
+        s_pos[1][i] = in ? src.y[j0 + i] : 0.0;
+        s_pos[2][i] = in ? src.z[j0 + i] : 0.0;
+      }
+      stage_den<1, NR, P2P_BS>(s_den, den, src, j0, cnt, t);
+      for (int i = cnt * NR + t; i < cnt4 * NR; i += P2P_BS) s_den[i / NR][i % NR] = 0.0;
+      __syncthreads();
+      if (wact)
+        for (int g = 0; g < cnt4; g += 4) {
+          const double sx = s_pos[0][g + kk], sy = s_pos[1][g + kk], sz = s_pos[2][g + kk];
+          const double f = s_den[g + kk][col];
+#pragma unroll
+          for (int i = 0; i < 4; i++) {
+            const double dx = tx[i] - sx, dy = ty[i] - sy, dz = tz[i] - sz;
+            const double gv = rinv(dx * dx + dy * dy + dz * dz);
+            acc[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(f, gv, acc[i], 0, 0, 0);
+          }
+        }
+    }
+  }
+  if (!wact) return;
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const int tt = 64 * w + 16 * i + col;
+    if (tt < tn) {
+      double* val = trg.val + (long)(r0 + kk) * trg.val_rs + tb + tt;
+#pragma unroll
+      for (int v = 0; v < 4; v++) val[(long)(4 * v) * trg.val_rs] += kc.a * acc[i][v];
+    }
   }
 }
 
@@ -293,12 +428,7 @@ __global__ __launch_bounds__(P2P_BS) void k_direct_part(int nchk, const double* 
         s_pos[i][NPOS - 1] = src.nz[b0 + i];
       }
     }
-    const int per = cnt * K::S;
-    for (int i = t; i < NR * per; i += P2P_BS) {
-      int q = i / per, rem = i - q * per;
-      int jj = rem / K::S, sd = rem - jj * K::S;
-      s_den[jj][q * K::S + sd] = den[(long)q * src.den_rs + (long)b0 * K::S + rem];
-    }
+    stage_den<K::S, NR, P2P_BS>(s_den, den, src, b0, cnt, t);
     __syncthreads();
     if (act)
       for (int jj = 0; jj < cnt; jj++)
@@ -406,12 +536,7 @@ __global__ __launch_bounds__(P2S_BS) void k_p2s(const int* __restrict__ out_node
           s_pos[i][NPOS - 1] = src.nz[j0 + i];
         }
       }
-      const int per = cnt * K::S;
-      for (int i = t; i < NR * per; i += P2S_BS) {
-        int q = i / per, rem = i - q * per;
-        int jj = rem / K::S, sd = rem - jj * K::S;
-        s_den[jj][q * K::S + sd] = den[(long)q * src.den_rs + (long)j0 * K::S + rem];
-      }
+      stage_den<K::S, NR, P2S_BS>(s_den, den, src, j0, cnt, t);
       __syncthreads();
       if (act)
         for (int jj = 0; jj < cnt; jj++)
@@ -494,29 +619,42 @@ __global__ __launch_bounds__(P2P_BS) void k_s2p(const int* __restrict__ tbeg, co
   }
 }
 
+template <int KT, int NR>
+static void p2p_launch(hipStream_t s, int variant, int nchunk, int nb, const int* tbeg, const int* tnum,
+                       const int* eptr, const int2* ent, Points src, Targets trg, int roff, KC kc)
+{
+  if (variant == 2 && KT == 111 && NR == 16 && src.den_rs == 1)
+    hipLaunchKernelGGL(k_p2p_mfma111, dim3(nchunk, nb), dim3(P2P_BS), 0, s, tbeg, tnum, eptr, ent, src, trg, roff, kc);
+  else if (variant == 1)
+    hipLaunchKernelGGL((k_p2p_s<KT, NR>), dim3(2 * nchunk, nb), dim3(64), 0, s, tbeg, tnum, eptr, ent, src, trg, roff,
+                       kc);
+  else
+    hipLaunchKernelGGL((k_p2p<KT, NR>), dim3(nchunk, nb), dim3(P2P_BS), 0, s, tbeg, tnum, eptr, ent, src, trg, roff,
+                       kc);
+}
+
 template <int KT>
 static void p2p_kt(hipStream_t s, int nchunk, const int* tbeg, const int* tnum, const int* eptr, const int2* ent,
-                   Points src, Targets trg, int nr, KC kc)
+                   Points src, Targets trg, int nr, KC kc, int variant)
 {
   rhs_blocks(nr, [&](int NR, int nb, int roff) {
-    dim3 grid(nchunk, nb);
     if (NR == 16)
-      hipLaunchKernelGGL((k_p2p<KT, 16>), grid, dim3(P2P_BS), 0, s, tbeg, tnum, eptr, ent, src, trg, roff, kc);
+      p2p_launch<KT, 16>(s, variant, nchunk, nb, tbeg, tnum, eptr, ent, src, trg, roff, kc);
     else if (NR == 4)
-      hipLaunchKernelGGL((k_p2p<KT, 4>), grid, dim3(P2P_BS), 0, s, tbeg, tnum, eptr, ent, src, trg, roff, kc);
+      p2p_launch<KT, 4>(s, variant, nchunk, nb, tbeg, tnum, eptr, ent, src, trg, roff, kc);
     else
-      hipLaunchKernelGGL((k_p2p<KT, 1>), grid, dim3(P2P_BS), 0, s, tbeg, tnum, eptr, ent, src, trg, roff, kc);
+      p2p_launch<KT, 1>(s, variant, nchunk, nb, tbeg, tnum, eptr, ent, src, trg, roff, kc);
   });
 }
 
 void p2p(hipStream_t s, int kt, int nchunk, const int* tbeg, const int* tnum, const int* eptr, const int2* ent,
-         Points src, Targets trg, int nr, KC kc)
+         Points src, Targets trg, int nr, KC kc, int variant)
 {
   if (nchunk <= 0) return;
   switch (kt) {
-  case 111: p2p_kt<111>(s, nchunk, tbeg, tnum, eptr, ent, src, trg, nr, kc); break;
-  case 321: p2p_kt<321>(s, nchunk, tbeg, tnum, eptr, ent, src, trg, nr, kc); break;
-  case 511: p2p_kt<511>(s, nchunk, tbeg, tnum, eptr, ent, src, trg, nr, kc); break;
+  case 111: p2p_kt<111>(s, nchunk, tbeg, tnum, eptr, ent, src, trg, nr, kc, variant); break;
+  case 321: p2p_kt<321>(s, nchunk, tbeg, tnum, eptr, ent, src, trg, nr, kc, variant); break;
+  case 511: p2p_kt<511>(s, nchunk, tbeg, tnum, eptr, ent, src, trg, nr, kc, variant); break;
   }
 }
 
@@ -924,17 +1062,20 @@ __global__ __launch_bounds__(256) void k_m2l(int C, int Rc, const int* __restric
 // is read once per target parent instead of once per V pair, and each
 // operator value serves RB right-hand sides.  Blocks are ordered parent-
 // fastest within a frequency block and parents are dealt to the 8 XCDs in
-// contiguous ranges, so neighbouring parents share an L2.
-template <int SM, int TM, int RB>
-__global__ __launch_bounds__(256) void k_m2l_grp(int C, int Rc, int npar, const int* __restrict__ tslot,
-                                                 const int* __restrict__ sslot, const int* __restrict__ optab,
-                                                 const double2* __restrict__ ops, const double2* __restrict__ effden,
-                                                 double2* __restrict__ effval, int roff)
+// contiguous ranges, so neighbouring parents share an L2.  A block covers BS
+// frequencies: the spectra of one (frequency block, right-hand-side block)
+// over a whole level are the working set of the parent sweep, so BS sets how
+// much of it stays in the L2 and Infinity Cache.
+template <int SM, int TM, int RB, int BS>
+__global__ __launch_bounds__(BS) void k_m2l_grp(int C, int Rc, int npar, const int* __restrict__ tslot,
+                                                const int* __restrict__ sslot, const int* __restrict__ optab,
+                                                const double2* __restrict__ ops, const double2* __restrict__ effden,
+                                                double2* __restrict__ effval, int roff)
 {
   const int per = (npar + 7) / 8;
   const int b = blockIdx.x;
   const int pidx = (b & 7) * per + (b >> 3);
-  const int q = blockIdx.y * 256 + threadIdx.x;
+  const int q = blockIdx.y * BS + threadIdx.x;
   const int r0 = roff + blockIdx.z * RB;
   if (pidx >= npar || q >= C) return;
   double2 acc[8][RB][TM];
@@ -988,47 +1129,64 @@ __global__ __launch_bounds__(256) void k_m2l_grp(int C, int Rc, int npar, const 
   }
 }
 
-template <int SM, int TM, int RBMAX>
+template <int SM, int TM, int RB, int BS>
+static void m2l_grp_launch(hipStream_t s, int C, int npar, int Rc, int nb, int roff, const int* tslot,
+                           const int* sslot, const int* optab, const double2* ops, const double2* effden,
+                           double2* effval)
+{
+  const int per = (npar + 7) / 8;
+  dim3 grid(8 * per, (C + BS - 1) / BS, nb);
+  hipLaunchKernelGGL((k_m2l_grp<SM, TM, RB, BS>), grid, dim3(BS), 0, s, C, Rc, npar, tslot, sslot, optab, ops,
+                     effden, effval, roff);
+}
+
+// Right-hand sides are covered by launches of RBMAX per lane and a remainder
+// of single right-hand sides.  (Lanes sharing one frequency across right-hand
+// side groups, so that operator loads coincide, measured slower on MI355X: the
+// texture path costs per lane, not per distinct address.)
+template <int SM, int TM, int RBMAX, int BS>
 static void m2l_grp_st(hipStream_t s, int C, int npar, int Rc, const int* tslot, const int* sslot, const int* optab,
                        const double2* ops, const double2* effden, double2* effval)
 {
-  const int per = (npar + 7) / 8;
   int r = 0;
   while (r < Rc) {
-    int left = Rc - r;
-    int RB = left >= RBMAX ? RBMAX : 1;
-    int nb = left / RB;
-    dim3 grid(8 * per, (C + 255) / 256, nb);
-    if (RB == RBMAX)
-      hipLaunchKernelGGL((k_m2l_grp<SM, TM, RBMAX>), grid, dim3(256), 0, s, C, Rc, npar, tslot, sslot, optab, ops,
-                         effden, effval, r);
-    else
-      hipLaunchKernelGGL((k_m2l_grp<SM, TM, 1>), grid, dim3(256), 0, s, C, Rc, npar, tslot, sslot, optab, ops,
-                         effden, effval, r);
-    r += nb * RB;
+    const int left = Rc - r;
+    if (left >= RBMAX) {
+      const int nb = left / RBMAX;
+      m2l_grp_launch<SM, TM, RBMAX, BS>(s, C, npar, Rc, nb, r, tslot, sslot, optab, ops, effden, effval);
+      r += nb * RBMAX;
+    } else {
+      m2l_grp_launch<SM, TM, 1, BS>(s, C, npar, Rc, left, r, tslot, sslot, optab, ops, effden, effval);
+      r += left;
+    }
   }
 }
 
-void m2l_grouped(hipStream_t s, int sm, int tm, int C, int npar, int Rc, int rb, const int* tslot, const int* sslot,
-                 const int* optab, const double2* ops, const double2* effden, double2* effval)
+void m2l_grouped(hipStream_t s, int sm, int tm, int C, int npar, int Rc, int rb, int bs, const int* tslot,
+                 const int* sslot, const int* optab, const double2* ops, const double2* effden, double2* effval)
 {
   if (npar <= 0) return;
+  const bool wide = bs == 256;
+#define KIFMM_M2L_ST(SMv, TMv, RBv)                                                          \
+  (wide ? m2l_grp_st<SMv, TMv, RBv, 256>(s, C, npar, Rc, tslot, sslot, optab, ops, effden, effval) \
+        : m2l_grp_st<SMv, TMv, RBv, 64>(s, C, npar, Rc, tslot, sslot, optab, ops, effden, effval))
   if (sm == 1 && tm == 1) {
     if (rb == 8)
-      m2l_grp_st<1, 1, 8>(s, C, npar, Rc, tslot, sslot, optab, ops, effden, effval);
+      KIFMM_M2L_ST(1, 1, 8);
     else
-      m2l_grp_st<1, 1, 4>(s, C, npar, Rc, tslot, sslot, optab, ops, effden, effval);
+      KIFMM_M2L_ST(1, 1, 4);
   } else if (sm == 4 && tm == 3) {
     if (rb == 2)
-      m2l_grp_st<4, 3, 2>(s, C, npar, Rc, tslot, sslot, optab, ops, effden, effval);
+      KIFMM_M2L_ST(4, 3, 2);
     else
-      m2l_grp_st<4, 3, 1>(s, C, npar, Rc, tslot, sslot, optab, ops, effden, effval);
+      KIFMM_M2L_ST(4, 3, 1);
   } else if (sm == 3 && tm == 3) {
     if (rb == 2)
-      m2l_grp_st<3, 3, 2>(s, C, npar, Rc, tslot, sslot, optab, ops, effden, effval);
+      KIFMM_M2L_ST(3, 3, 2);
     else
-      m2l_grp_st<3, 3, 1>(s, C, npar, Rc, tslot, sslot, optab, ops, effden, effval);
+      KIFMM_M2L_ST(3, 3, 1);
   }
+#undef KIFMM_M2L_ST
 }
 
 template <int SM, int TM>

@@ -11,17 +11,20 @@ struct KC {  // kernel constants (see kernel_consts in kernels_host.cpp)
   double a, b, c;
 };
 
-// dst[r*dst_rs + k*dof + d] = src[r*src_rs + perm[k]*dof + d]
-void gather(hipStream_t s, int n, int dof, int nr, const int* perm, const double* src, long src_rs, double* dst,
-            long dst_rs);
+// Source-major gather: dst[(k*nr + r)*dof + d] = src[r*src_rs + perm[k]*dof + d]
+void gather(hipStream_t s, int n, int dof, int nr, const int* perm, const double* src, long src_rs, double* dst);
 // dst[r*dst_rs + perm[k]*dof + d] = src[r*src_rs + k*dof + d]
 void scatter(hipStream_t s, int n, int dof, int nr, const int* perm, const double* src, long src_rs, double* dst,
              long dst_rs);
 
-struct Points {  // leaf-ordered source points (SoA) and densities [rhs][n*sdof]
+// Leaf-ordered source points (SoA) and densities: density d of source j for
+// right-hand side r is den[j*den_js + r*den_rs + d].  The evaluate pass keeps
+// them source-major (den_js = Rc*sdof, den_rs = sdof) so that the densities of
+// one source for a block of right-hand sides are contiguous.
+struct Points {
   const double *x, *y, *z, *nx, *ny, *nz;
   const double* den;
-  long den_rs;
+  long den_js, den_rs;
 };
 struct Targets {  // leaf-ordered targets (SoA) and values [rhs][n*tdof]
   const double *x, *y, *z;
@@ -31,8 +34,10 @@ struct Targets {  // leaf-ordered targets (SoA) and values [rhs][n*tdof]
 
 // Near field: for every chunk of <= 128 targets, sum the user kernel over the
 // source leaves ent[eptr[c] .. eptr[c+1]) (each ent = {sextbeg, sextnum}).
+// variant 0: sources staged in LDS, 128-thread blocks; variant 1: sources read
+// through the scalar cache (needs den_rs == sdof), one wave per 64 targets.
 void p2p(hipStream_t s, int kt, int nchunk, const int* tbeg, const int* tnum, const int* eptr, const int2* ent,
-         Points src, Targets trg, int nr, KC kc);
+         Points src, Targets trg, int nr, KC kc, int variant);
 
 // FP64 direct sum of the user kernel over all nsrc sources at nchk points
 // (cx, cy, cz): out[r][k*tdof + t]; part holds nr * nseg * nchk * tdof doubles.
@@ -81,8 +86,9 @@ void m2l(hipStream_t s, int sm, int tm, int C, int ntrg, int Rc, const int* vptr
 // M2L grouped by target parent: for parent p, effval[tslot[p][c]] =
 // sum over the 26 neighbour directions n and source children s of
 // T_{optab[n][s][c]} effden[sslot[p][n][s]] (entries < 0 are skipped).
-void m2l_grouped(hipStream_t s, int sm, int tm, int C, int npar, int Rc, int rb, const int* tslot, const int* sslot,
-                 const int* optab, const double2* ops, const double2* effden, double2* effval);
+// rb: right-hand sides per lane (0 = default), bs: frequencies per block (64 or 256).
+void m2l_grouped(hipStream_t s, int sm, int tm, int C, int npar, int Rc, int rb, int bs, const int* tslot,
+                 const int* sslot, const int* optab, const double2* ops, const double2* effden, double2* effval);
 // c2r and gather onto the DC surface: dncval[(node*Rc + r)*pln + p*tm + f] += ...
 void fft_inv(hipStream_t s, int np, int nslot, const int* slot_node, int Rc, const double2* effval, int tm,
              const int* surf2cube, int nsp, double* dncval, int pln, const double2* tw);

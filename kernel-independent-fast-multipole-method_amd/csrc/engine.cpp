@@ -92,6 +92,8 @@ Engine::Engine(int kt, const double* coefs, int ncoefs, int np, int ptsmax, int 
   tree_.rootlvl = rootlvl;
   for (int d = 0; d < 3; d++) tree_.ctr[d] = ctr_[d];
   if (const char* rb = std::getenv("KIFMM3D_M2L_RB")) m2l_rb_ = std::atoi(rb);
+  if (const char* bs = std::getenv("KIFMM3D_M2L_BS")) m2l_bs_ = std::atoi(bs);
+  if (const char* v = std::getenv("KIFMM3D_P2P")) p2p_variant_ = std::atoi(v);
 }
 
 Engine::~Engine()
@@ -633,8 +635,8 @@ void Engine::run_chunk(const double* srcden, double* trgval, int rc, hipEvent_t*
   const dev::KC kcu{kcu_.a, kcu_.b, kcu_.c}, kcm{kcm_.a, kcm_.b, kcm_.c};
   double* bufs[4] = {upcval_.p, upeden_.p, dncval_.p, dneden_.p};
   const double* opsA[4] = {uc2ue_.p, ue2uc_.p, dc2de_.p, de2dc_.p};
-  const long srs = (long)T.sextcnt * sdof, trs = (long)T.textcnt * tdof;
-  dev::Points src{sx_.p, sy_.p, sz_.p, snx_.p, sny_.p, snz_.p, sden_.p, srs};
+  const long trs = (long)T.textcnt * tdof;
+  dev::Points src{sx_.p, sy_.p, sz_.p, snx_.p, sny_.p, snz_.p, sden_.p, (long)rc * sdof, sdof};
   dev::Targets trg{tx_.p, ty_.p, tz_.p, tval_.p, trs};
 
   auto run_gemm = [&](const GemmOp& op) {
@@ -676,7 +678,7 @@ void Engine::run_chunk(const double* srcden, double* trgval, int rc, hipEvent_t*
   if (P.nlow_s) HIPCHK(hipMemsetAsync(upeden_.p, 0, sizeof(double) * (size_t)rc * P.nlow_s * pUE, s));
   if (P.nlow_t) HIPCHK(hipMemsetAsync(dneden_.p, 0, sizeof(double) * (size_t)rc * P.nlow_t * pDE, s));
   // load (fmm3d_eval.cpp:50-64)
-  timed(KIFMM3D_K_GATHER, [&] { dev::gather(s, T.sextcnt, sdof, rc, sperm_.p, srcden, (long)ns_ * sdof, sden_.p, srs); });
+  timed(KIFMM3D_K_GATHER, [&] { dev::gather(s, T.sextcnt, sdof, rc, sperm_.p, srcden, (long)ns_ * sdof, sden_.p); });
   HIPCHK(hipEventRecord(ev[1], s));
   // up: S2M, then M2M + UC2UE level by level
   timed(KIFMM3D_K_S2M, [&] {
@@ -688,7 +690,7 @@ void Engine::run_chunk(const double* srcden, double* trgval, int rc, hipEvent_t*
   // U list and direct W/X
   timed(KIFMM3D_K_P2P, [&] {
     dev::p2p(s, user_.kt, P.near.n, I + P.near.tbeg, I + P.near.tnum, I + P.near.eptr,
-             (const int2*)(I + P.near.ent), src, trg, rc, kcu);
+             (const int2*)(I + P.near.ent), src, trg, rc, kcu, p2p_variant_);
   });
   HIPCHK(hipEventRecord(ev[3], s));
   // V list
@@ -702,7 +704,7 @@ void Engine::run_chunk(const double* srcden, double* trgval, int rc, hipEvent_t*
         dev::m2l(s, sm, tm, O.C, vl.ntrg, rc, I + vl.vptr, I + vl.vsrc, I + vl.vop, (const double2*)m2l_.p,
                  (const double2*)effden_.p, (double2*)effval_.p);
       else
-        dev::m2l_grouped(s, sm, tm, O.C, vl.npar, rc, m2l_rb_, I + vl.tslot, I + vl.sslot, I + P.optab,
+        dev::m2l_grouped(s, sm, tm, O.C, vl.npar, rc, m2l_rb_, m2l_bs_, I + vl.tslot, I + vl.sslot, I + P.optab,
                          (const double2*)m2l_.p, (const double2*)effden_.p, (double2*)effval_.p);
     });
     timed(KIFMM3D_K_FFT_INV, [&] {
@@ -778,8 +780,8 @@ void Engine::direct(const double* srcden, bool on_device, int nrhs, const int* i
   sden.alloc((size_t)nrhs * srs);
   part_.ensure((size_t)nrhs * nseg * nchk * tdof);
   dout_.ensure((size_t)nrhs * nchk * tdof);
-  dev::gather(stream_, tree_.sextcnt, sdof, nrhs, sperm_.p, dsrc, (long)ns_ * sdof, sden.p, srs);
-  dev::Points src{sx_.p, sy_.p, sz_.p, snx_.p, sny_.p, snz_.p, sden.p, srs};
+  dev::gather(stream_, tree_.sextcnt, sdof, nrhs, sperm_.p, dsrc, (long)ns_ * sdof, sden.p);
+  dev::Points src{sx_.p, sy_.p, sz_.p, snx_.p, sny_.p, snz_.p, sden.p, (long)nrhs * sdof, sdof};
   const dev::KC kcu{kcu_.a, kcu_.b, kcu_.c};
   dev::direct(stream_, user_.kt, tdof, nchk, chk_.p, chk_.p + nchk, chk_.p + 2 * nchk, ns_, nseg, src, part_.p,
               dout_.p, nrhs, kcu);

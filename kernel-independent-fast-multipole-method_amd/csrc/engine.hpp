@@ -125,7 +125,9 @@ class Engine {
   bool setup_done_ = false;
   bool host_done_ = false;
   int last_rc_ = 0;
-  int m2l_rb_ = 0;  // right-hand sides per lane in the grouped M2L (0 = default; KIFMM3D_M2L_RB)
+  int m2l_rb_ = 0;   // right-hand sides per lane in the grouped M2L (0 = default; KIFMM3D_M2L_RB)
+  int m2l_bs_ = 64;  // frequencies per block in the grouped M2L (64 or 256; KIFMM3D_M2L_BS)
+  int p2p_variant_ = 2;  // near field: 0 LDS broadcast, 1 scalar-cache sources, 2 matrix cores (KIFMM3D_P2P)
   hipStream_t stream_ = nullptr;
   std::vector<hipEvent_t> evpool_;
   std::vector<int> evcls_;

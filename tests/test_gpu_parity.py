@@ -134,3 +134,25 @@ def test_repeat_eval_is_deterministic():
     a = f.eval(den).copy()
     b = f.eval(den).copy()
     assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("variant", [0, 1, 2])
+def test_near_field_variants_21rhs(variant, monkeypatch):
+    """21 right-hand sides run as blocks of 16 + 4 + 1 through every near-field
+    kernel form (LDS broadcast, scalar-cache sources, matrix cores)."""
+    monkeypatch.setenv("KIFMM3D_P2P", str(variant))
+    p = clusters(4000, 14)
+    run_case(f"laplace-21rhs-p2p{variant}", 111, 4, 40, p, None, p, (0.5, 0.5, 0.5), 1, nrhs=21)
+    # leaves of up to 150 targets: chunks of 128 plus a remainder, two waves per block
+    p = uniform_cube(6000, 16)
+    run_case(f"laplace-16rhs-big-leaves-p2p{variant}", 111, 4, 150, p, None, p, (0.5, 0.5, 0.5), 1, nrhs=16)
+
+
+@pytest.mark.parametrize("kt", [321, 511])
+def test_vector_kernels_16rhs(kt):
+    p = clusters(2500, 15)
+    nrm = None
+    if kt == 321:
+        p, nrm = sphere(2500, 15)
+    ctr, lvl = ((0.0, 0.0, 0.0), 0) if kt == 321 else ((0.5, 0.5, 0.5), 1)
+    run_case(f"kt{kt}-16rhs", kt, 4, 40, p, nrm, p, ctr, lvl, nrhs=16)
