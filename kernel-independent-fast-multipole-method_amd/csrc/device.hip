@@ -861,13 +861,26 @@ void gemm(hipStream_t s, const GemmArgs& a)
 // Only the np^3 cube at offset np/2 holds nonzero data (matmgnt3d.cpp:256,
 // :286), so the forward pass sums over np points per dimension and the
 // inverse evaluates only the surface points.
+//
+// Spectra are stored per slot either right-hand-side major ([slot][r][f][C],
+// rhs_inner = 0) or right-hand-side innermost ([slot][f][C][Rc], rhs_inner =
+// 1, the layout of the tiled matrix-core M2L); CT is double2 or float2.
 __device__ inline double2 cmul(double2 a, double2 b) { return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
 
-template <int NP>
+__device__ inline void cstore(double2* p, double2 v) { *p = v; }
+__device__ inline void cstore(float2* p, double2 v) { *p = make_float2((float)v.x, (float)v.y); }
+__device__ inline double2 cload(const double2* p) { return *p; }
+__device__ inline double2 cload(const float2* p)
+{
+  float2 v = *p;
+  return make_double2(v.x, v.y);
+}
+
+template <int NP, class CT>
 __global__ __launch_bounds__(256) void k_fft_fwd(const int* __restrict__ slot_node, int Rc,
                                                  const double* __restrict__ upeden, int pln, int sm,
                                                  const double* __restrict__ dscale, const short* __restrict__ c2s,
-                                                 double2* __restrict__ effden, const double2* __restrict__ tw)
+                                                 CT* __restrict__ effden, const double2* __restrict__ tw, int rhs_inner)
 {
   constexpr int N = 2 * NP, H = NP + 1, C = N * N * H, NC = NP * NP * NP;
   constexpr int NSP = NP * NP * NP - (NP - 2) * (NP - 2) * (NP - 2);
@@ -882,7 +895,8 @@ __global__ __launch_bounds__(256) void k_fft_fwd(const int* __restrict__ slot_no
   for (int i = t; i < NC; i += 256) s_c2s[i] = c2s[i];
   if (t < N) s_w[t] = tw[t];
   __syncthreads();
-  double2* out = effden + ((long)slot * Rc + r) * sm * C;
+  const long os = rhs_inner ? Rc : 1;
+  CT* out = effden + (long)slot * Rc * sm * C + (rhs_inner ? (long)r : (long)r * sm * C);
   for (int f = 0; f < sm; f++) {
     for (int o = t; o < NP * NP * H; o += 256) {  // along x: X1[k][j][kx]
       int kx = o % H, j = (o / H) % NP, k = o / (H * NP);
@@ -921,17 +935,17 @@ __global__ __launch_bounds__(256) void k_fft_fwd(const int* __restrict__ slot_no
         acc.x += v.x;
         acc.y += v.y;
       }
-      out[(long)f * C + o] = acc;
+      cstore(out + ((long)f * C + o) * os, acc);
     }
     __syncthreads();
   }
 }
 
-template <int NP>
+template <int NP, class CT>
 __global__ __launch_bounds__(256) void k_fft_inv(const int* __restrict__ slot_node, int Rc,
-                                                 const double2* __restrict__ effval, int tm,
+                                                 const CT* __restrict__ effval, int tm,
                                                  const int* __restrict__ s2c, int nsp, double* __restrict__ dncval,
-                                                 int pln, const double2* __restrict__ tw)
+                                                 int pln, const double2* __restrict__ tw, int rhs_inner)
 {
   constexpr int N = 2 * NP, H = NP + 1, C = N * N * H;
   __shared__ double2 s_x[C];           // spectrum, later Y2[k][j][kx]
@@ -940,10 +954,11 @@ __global__ __launch_bounds__(256) void k_fft_inv(const int* __restrict__ slot_no
   const int slot = blockIdx.x, r = blockIdx.y, t = threadIdx.x;
   if (t < N) s_w[t] = tw[t];
   double* out = dncval + ((long)slot_node[slot] * Rc + r) * pln;
-  const double2* in = effval + ((long)slot * Rc + r) * tm * C;
+  const long os = rhs_inner ? Rc : 1;
+  const CT* in = effval + (long)slot * Rc * tm * C + (rhs_inner ? (long)r : (long)r * tm * C);
   for (int f = 0; f < tm; f++) {
     __syncthreads();
-    for (int o = t; o < C; o += 256) s_x[o] = in[(long)f * C + o];
+    for (int o = t; o < C; o += 256) s_x[o] = cload(in + ((long)f * C + o) * os);
     __syncthreads();
     for (int o = t; o < NP * N * H; o += 256) {  // along z, only the cube's planes
       int kx = o % H, ky = (o / H) % N, k = o / (H * N);
@@ -987,28 +1002,197 @@ __global__ __launch_bounds__(256) void k_fft_inv(const int* __restrict__ slot_no
   }
 }
 
-void fft_fwd(hipStream_t s, int np, int nslot, const int* slot_node, int Rc, const double* upeden, int pln, int sm,
-             const double* dscale, const short* cube2surf, double2* effden, const double2* tw)
+// Right-hand-side-innermost forms for scalar kernels (sm = tm = 1): a block
+// transforms R right-hand sides of one slot and moves the spectra through LDS
+// so that global accesses cover R consecutive right-hand sides per frequency.
+template <int NP, class CT, int R>
+__global__ __launch_bounds__(256) void k_fft_fwd_ri(const int* __restrict__ slot_node, int Rc,
+                                                    const double* __restrict__ upeden, int pln,
+                                                    const double* __restrict__ dscale, const short* __restrict__ c2s,
+                                                    CT* __restrict__ effden, const double2* __restrict__ tw)
 {
-  if (nslot <= 0) return;
-  dim3 grid(nslot, Rc);
-  switch (np) {
-  case 4: hipLaunchKernelGGL(k_fft_fwd<4>, grid, dim3(256), 0, s, slot_node, Rc, upeden, pln, sm, dscale, cube2surf, effden, tw); break;
-  case 6: hipLaunchKernelGGL(k_fft_fwd<6>, grid, dim3(256), 0, s, slot_node, Rc, upeden, pln, sm, dscale, cube2surf, effden, tw); break;
-  case 8: hipLaunchKernelGGL(k_fft_fwd<8>, grid, dim3(256), 0, s, slot_node, Rc, upeden, pln, sm, dscale, cube2surf, effden, tw); break;
+  constexpr int N = 2 * NP, H = NP + 1, C = N * N * H, NC = NP * NP * NP;
+  constexpr int NSP = NP * NP * NP - (NP - 2) * (NP - 2) * (NP - 2);
+  __shared__ double s_den[NSP];
+  __shared__ short s_c2s[NC];
+  __shared__ double2 s_w[N];
+  __shared__ double2 X1[NP * NP * H];
+  __shared__ double2 X2[R][NP * N * H];
+  const int slot = blockIdx.x, r0 = blockIdx.y * R, t = threadIdx.x;
+  for (int i = t; i < NC; i += 256) s_c2s[i] = c2s[i];
+  if (t < N) s_w[t] = tw[t];
+  const double ds = dscale[0];
+  for (int rr = 0; rr < R; rr++) {
+    __syncthreads();
+    const double* den = upeden + ((long)slot_node[slot] * Rc + r0 + rr) * pln;
+    for (int i = t; i < NSP; i += 256) s_den[i] = den[i] * ds;
+    __syncthreads();
+    for (int o = t; o < NP * NP * H; o += 256) {  // along x: X1[k][j][kx]
+      int kx = o % H, j = (o / H) % NP, k = o / (H * NP);
+      double re = 0, im = 0;
+#pragma unroll
+      for (int i = 0; i < NP; i++) {
+        int si = s_c2s[(i * NP + j) * NP + k];
+        if (si >= 0) {
+          double v = s_den[si];
+          double2 w = s_w[((i + NP / 2) * kx) % N];
+          re += v * w.x;
+          im += v * w.y;
+        }
+      }
+      X1[o] = make_double2(re, im);
+    }
+    __syncthreads();
+    for (int o = t; o < NP * N * H; o += 256) {  // along y: X2[rr][k][ky][kx]
+      int kx = o % H, ky = (o / H) % N, k = o / (H * N);
+      double2 acc = make_double2(0, 0);
+#pragma unroll
+      for (int j = 0; j < NP; j++) {
+        double2 v = cmul(X1[(k * NP + j) * H + kx], s_w[((j + NP / 2) * ky) % N]);
+        acc.x += v.x;
+        acc.y += v.y;
+      }
+      X2[rr][o] = acc;
+    }
+  }
+  __syncthreads();
+  CT* out = effden + (long)slot * C * Rc + r0;
+  for (int e = t; e < C * R; e += 256) {  // along z, the R right-hand sides innermost
+    const int o = e / R, rr = e - o * R;
+    int kx = o % H, ky = (o / H) % N, kz = o / (H * N);
+    double2 acc = make_double2(0, 0);
+#pragma unroll
+    for (int k = 0; k < NP; k++) {
+      double2 v = cmul(X2[rr][(k * N + ky) * H + kx], s_w[((k + NP / 2) * kz) % N]);
+      acc.x += v.x;
+      acc.y += v.y;
+    }
+    cstore(out + (long)o * Rc + rr, acc);
   }
 }
 
-void fft_inv(hipStream_t s, int np, int nslot, const int* slot_node, int Rc, const double2* effval, int tm,
-             const int* surf2cube, int nsp, double* dncval, int pln, const double2* tw)
+template <int NP, class CT, int R>
+__global__ __launch_bounds__(256) void k_fft_inv_ri(const int* __restrict__ slot_node, int Rc,
+                                                    const CT* __restrict__ effval, const int* __restrict__ s2c,
+                                                    int nsp, double* __restrict__ dncval, int pln,
+                                                    const double2* __restrict__ tw)
+{
+  constexpr int N = 2 * NP, H = NP + 1, C = N * N * H;
+  __shared__ double2 s_x[R][C];       // spectra, later Y2[k][j][kx]
+  __shared__ double2 Y1[NP * N * H];  // [k][ky][kx]
+  __shared__ double2 s_w[N];
+  const int slot = blockIdx.x, r0 = blockIdx.y * R, t = threadIdx.x;
+  if (t < N) s_w[t] = tw[t];
+  const CT* in = effval + (long)slot * C * Rc + r0;
+  for (int e = t; e < C * R; e += 256) {
+    const int o = e / R, rr = e - o * R;
+    s_x[rr][o] = cload(in + (long)o * Rc + rr);
+  }
+  for (int rr = 0; rr < R; rr++) {
+    __syncthreads();
+    for (int o = t; o < NP * N * H; o += 256) {  // along z, only the cube's planes
+      int kx = o % H, ky = (o / H) % N, k = o / (H * N);
+      double2 acc = make_double2(0, 0);
+#pragma unroll
+      for (int kz = 0; kz < N; kz++) {
+        double2 w = s_w[((k + NP / 2) * kz) % N];
+        double2 v = cmul(s_x[rr][(kz * N + ky) * H + kx], make_double2(w.x, -w.y));
+        acc.x += v.x;
+        acc.y += v.y;
+      }
+      Y1[o] = acc;
+    }
+    __syncthreads();
+    for (int o = t; o < NP * NP * H; o += 256) {  // along y, only the cube's rows
+      int kx = o % H, j = (o / H) % NP, k = o / (H * NP);
+      double2 acc = make_double2(0, 0);
+#pragma unroll
+      for (int ky = 0; ky < N; ky++) {
+        double2 w = s_w[((j + NP / 2) * ky) % N];
+        double2 v = cmul(Y1[(k * N + ky) * H + kx], make_double2(w.x, -w.y));
+        acc.x += v.x;
+        acc.y += v.y;
+      }
+      s_x[rr][o] = acc;
+    }
+    __syncthreads();
+    double* out = dncval + ((long)slot_node[slot] * Rc + r0 + rr) * pln;
+    for (int p = t; p < nsp; p += 256) {  // along x (c2r) at the surface points
+      int cc = s2c[p];
+      int i = cc / (NP * NP), j = (cc / NP) % NP, k = cc % NP;
+      const double2* y = &s_x[rr][(k * NP + j) * H];
+      int xi = i + NP / 2;
+      double v = y[0].x + ((xi & 1) ? -y[NP].x : y[NP].x);
+#pragma unroll
+      for (int kx = 1; kx < NP; kx++) {
+        double2 w = s_w[(xi * kx) % N];
+        v += 2.0 * (y[kx].x * w.x + y[kx].y * w.y);
+      }
+      out[p] += v;
+    }
+  }
+}
+
+template <class CT>
+static void fft_fwd_t(hipStream_t s, int np, int nslot, const int* slot_node, int Rc, const double* upeden, int pln,
+                      int sm, const double* dscale, const short* cube2surf, CT* effden, const double2* tw, int ri)
 {
   if (nslot <= 0) return;
+  if (ri && sm == 1) {
+    constexpr int RW = sizeof(CT) == 8 ? 8 : 4;  // 64-byte runs of right-hand sides
+    switch (np) {
+    case 4: hipLaunchKernelGGL((k_fft_fwd_ri<4, CT, RW>), dim3(nslot, Rc / RW), dim3(256), 0, s, slot_node, Rc, upeden, pln, dscale, cube2surf, effden, tw); break;
+    case 6: hipLaunchKernelGGL((k_fft_fwd_ri<6, CT, RW>), dim3(nslot, Rc / RW), dim3(256), 0, s, slot_node, Rc, upeden, pln, dscale, cube2surf, effden, tw); break;
+    case 8: hipLaunchKernelGGL((k_fft_fwd_ri<8, CT, RW / 2>), dim3(nslot, Rc / (RW / 2)), dim3(256), 0, s, slot_node, Rc, upeden, pln, dscale, cube2surf, effden, tw); break;
+    }
+    return;
+  }
   dim3 grid(nslot, Rc);
   switch (np) {
-  case 4: hipLaunchKernelGGL(k_fft_inv<4>, grid, dim3(256), 0, s, slot_node, Rc, effval, tm, surf2cube, nsp, dncval, pln, tw); break;
-  case 6: hipLaunchKernelGGL(k_fft_inv<6>, grid, dim3(256), 0, s, slot_node, Rc, effval, tm, surf2cube, nsp, dncval, pln, tw); break;
-  case 8: hipLaunchKernelGGL(k_fft_inv<8>, grid, dim3(256), 0, s, slot_node, Rc, effval, tm, surf2cube, nsp, dncval, pln, tw); break;
+  case 4: hipLaunchKernelGGL((k_fft_fwd<4, CT>), grid, dim3(256), 0, s, slot_node, Rc, upeden, pln, sm, dscale, cube2surf, effden, tw, ri); break;
+  case 6: hipLaunchKernelGGL((k_fft_fwd<6, CT>), grid, dim3(256), 0, s, slot_node, Rc, upeden, pln, sm, dscale, cube2surf, effden, tw, ri); break;
+  case 8: hipLaunchKernelGGL((k_fft_fwd<8, CT>), grid, dim3(256), 0, s, slot_node, Rc, upeden, pln, sm, dscale, cube2surf, effden, tw, ri); break;
   }
+}
+
+template <class CT>
+static void fft_inv_t(hipStream_t s, int np, int nslot, const int* slot_node, int Rc, const CT* effval, int tm,
+                      const int* surf2cube, int nsp, double* dncval, int pln, const double2* tw, int ri)
+{
+  if (nslot <= 0) return;
+  if (ri && tm == 1) {
+    constexpr int RW = sizeof(CT) == 8 ? 8 : 4;
+    switch (np) {
+    case 4: hipLaunchKernelGGL((k_fft_inv_ri<4, CT, RW>), dim3(nslot, Rc / RW), dim3(256), 0, s, slot_node, Rc, effval, surf2cube, nsp, dncval, pln, tw); break;
+    case 6: hipLaunchKernelGGL((k_fft_inv_ri<6, CT, RW>), dim3(nslot, Rc / RW), dim3(256), 0, s, slot_node, Rc, effval, surf2cube, nsp, dncval, pln, tw); break;
+    case 8: hipLaunchKernelGGL((k_fft_inv_ri<8, CT, RW / 4>), dim3(nslot, Rc / (RW / 4)), dim3(256), 0, s, slot_node, Rc, effval, surf2cube, nsp, dncval, pln, tw); break;
+    }
+    return;
+  }
+  dim3 grid(nslot, Rc);
+  switch (np) {
+  case 4: hipLaunchKernelGGL((k_fft_inv<4, CT>), grid, dim3(256), 0, s, slot_node, Rc, effval, tm, surf2cube, nsp, dncval, pln, tw, ri); break;
+  case 6: hipLaunchKernelGGL((k_fft_inv<6, CT>), grid, dim3(256), 0, s, slot_node, Rc, effval, tm, surf2cube, nsp, dncval, pln, tw, ri); break;
+  case 8: hipLaunchKernelGGL((k_fft_inv<8, CT>), grid, dim3(256), 0, s, slot_node, Rc, effval, tm, surf2cube, nsp, dncval, pln, tw, ri); break;
+  }
+}
+
+void fft_fwd(hipStream_t s, int np, int nslot, const int* slot_node, int Rc, const double* upeden, int pln, int sm,
+             const double* dscale, const short* cube2surf, void* effden, int single, int rhs_inner, const double2* tw)
+{
+  if (single)
+    fft_fwd_t(s, np, nslot, slot_node, Rc, upeden, pln, sm, dscale, cube2surf, (float2*)effden, tw, rhs_inner);
+  else
+    fft_fwd_t(s, np, nslot, slot_node, Rc, upeden, pln, sm, dscale, cube2surf, (double2*)effden, tw, rhs_inner);
+}
+
+void fft_inv(hipStream_t s, int np, int nslot, const int* slot_node, int Rc, const void* effval, int single,
+             int rhs_inner, int tm, const int* surf2cube, int nsp, double* dncval, int pln, const double2* tw)
+{
+  if (single)
+    fft_inv_t(s, np, nslot, slot_node, Rc, (const float2*)effval, tm, surf2cube, nsp, dncval, pln, tw, rhs_inner);
+  else
+    fft_inv_t(s, np, nslot, slot_node, Rc, (const double2*)effval, tm, surf2cube, nsp, dncval, pln, tw, rhs_inner);
 }
 
 // ---------------------------------------------------------------------------
@@ -1214,6 +1398,186 @@ void m2l(hipStream_t s, int sm, int tm, int C, int ntrg, int Rc, const int* vptr
   if (sm == 1 && tm == 1) m2l_st<1, 1>(s, C, ntrg, Rc, vptr, vsrc, vop, ops, effden, effval);
   else if (sm == 4 && tm == 3) m2l_st<4, 3>(s, C, ntrg, Rc, vptr, vsrc, vop, ops, effden, effval);
   else if (sm == 3 && tm == 3) m2l_st<3, 3>(s, C, ntrg, Rc, vptr, vsrc, vop, ops, effden, effval);
+}
+
+// ---------------------------------------------------------------------------
+// Tiled M2L on the matrix cores (scalar kernels, 16 right-hand sides).
+//
+// A tile is the 2x2x2 block of target parents under one grandparent; their 26
+// neighbours lie in the 4x4x4 block of parents around it (the halo).  Block
+// (tile, frequency f, group of 16 right-hand sides) stages the spectra of the
+// halo's children at f in LDS once (each is then used by up to 8 target
+// parents instead of being re-read per parent) and wave p of the block forms,
+// for target parent p and each neighbour direction n,
+//   Y[c][r] += sum_s M_n(f)[c][s] X[s][r]          (c, s children; r RHS)
+// with M_n(f)[c][s] = T_{optab[n][s][c]}(f), as the real 16x16x16 product
+//   [Yr; Yi] += [[Mr, -Mi], [Mi, Mr]] [Xr; Xi]
+// in four 16x16x4 MFMAs.  Lane l holds A[i = l%16][k = l/16] and
+// B[k][j = l%16]; the K index of slice ks is source child k (ks even) or 4+k
+// (ks odd), real part for ks < 2 and imaginary part otherwise.  Rows of A are
+// ordered so that lane (k, r) ends up with Y of children 2k and 2k+1 as
+// (re, im) pairs: accumulator item v holds child 2k + v/2, part v%2.  The
+// position of item v in the 16x16 result is row k + 4v for the FP64 MFMA and
+// row 4k + v for the FP32 one (DROW).
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+template <class R>
+struct Mt;
+template <>
+struct Mt<double> {
+  typedef double2 C2;
+  typedef v4d V4;
+  static constexpr bool KV = true;  // D row of lane k, item v is k + 4v
+  __device__ static inline V4 mfma(double a, double b, V4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+};
+template <>
+struct Mt<float> {
+  typedef float2 C2;
+  typedef v4f V4;
+  static constexpr bool KV = false;  // D row of lane k, item v is 4k + v
+  __device__ static inline V4 mfma(float a, float b, V4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+};
+
+constexpr int M2LT_BS = 512;  // 8 waves: one per target parent of the tile
+constexpr int M2LT_FPB = 16;  // frequencies per block, visited in a software-pipelined loop
+constexpr int NOPS = 343;     // operator table entries ((o+3) in [0,7)^3)
+
+template <class R>
+__global__ __launch_bounds__(M2LT_BS) void k_m2l_tile(int C, int Rc, int ntile, const int* __restrict__ ttile,
+                                                      const int* __restrict__ htile, const int* __restrict__ optab,
+                                                      const typename Mt<R>::C2* __restrict__ opsT,
+                                                      const typename Mt<R>::C2* __restrict__ effden,
+                                                      typename Mt<R>::C2* __restrict__ effval, int roff)
+{
+  using C2 = typename Mt<R>::C2;
+  using V4 = typename Mt<R>::V4;
+  __shared__ C2 xs[64 * 8 * 16];  // [halo parent][source child][rhs] at the current frequency
+  __shared__ int s_opt[26 * 64];
+  __shared__ C2 s_op[NOPS];
+  __shared__ int s_hm[64];
+  // tiles are dealt to the 8 XCDs in contiguous ranges so that neighbouring
+  // tiles (whose halos overlap) share an L2
+  const int per = (ntile + 7) / 8, b = blockIdx.x;
+  const int tile = (b & 7) * per + (b >> 3);
+  if (tile >= ntile) return;
+  const int f0 = blockIdx.y * M2LT_FPB, f1 = min(C, f0 + M2LT_FPB), r0 = roff + blockIdx.z * 16;
+  const int t = threadIdx.x, w = t >> 6, l = t & 63;
+  const int* hs = htile + (long)tile * 512;
+  for (int i = t; i < 26 * 64; i += M2LT_BS) s_opt[i] = optab[i];
+  if (t < 64) {
+    int m = 0;
+#pragma unroll
+    for (int c = 0; c < 8; c++) m |= hs[t * 8 + c] >= 0;
+    s_hm[t] = m;
+  }
+  // staging: thread t moves entries e = it*512 + t (row e/16 = halo child, rhs e%16)
+  long src[16];
+#pragma unroll
+  for (int it = 0; it < 16; it++) {
+    const int e = it * M2LT_BS + t, sl = hs[e >> 4];
+    src[it] = sl >= 0 ? (long)sl * C * Rc + r0 + (e & 15) : -1;
+  }
+  C2 pre[16], pop;
+  auto fetch = [&](int f) {
+#pragma unroll
+    for (int it = 0; it < 16; it++) {
+      pre[it].x = 0;
+      pre[it].y = 0;
+      if (src[it] >= 0) pre[it] = effden[src[it] + (long)f * Rc];
+    }
+    pop.x = 0;
+    pop.y = 0;
+    if (t < NOPS) pop = opsT[(long)f * NOPS + t];
+  };
+  fetch(f0);
+  const int* ts = ttile + ((long)tile * 8 + w) * 8;
+  int live = 0;
+#pragma unroll
+  for (int c = 0; c < 8; c++) live |= ts[c] >= 0;
+  const int px = w >> 2, py = (w >> 1) & 1, pz = w & 1;
+  const int i = l & 15, k = l >> 4;
+  // output (child, part) of row i of A
+  int kr, vr;
+  if (Mt<R>::KV) {
+    kr = i & 3;
+    vr = i >> 2;
+  } else {
+    kr = i >> 2;
+    vr = i & 3;
+  }
+  const int ca = 2 * kr + (vr >> 1), pim = vr & 1;
+  for (int f = f0; f < f1; f++) {
+    __syncthreads();  // the previous frequency's products are done with xs and s_op
+#pragma unroll
+    for (int it = 0; it < 16; it++) xs[it * M2LT_BS + t] = pre[it];
+    if (t < NOPS) s_op[t] = pop;
+    __syncthreads();
+    if (f + 1 < f1) fetch(f + 1);  // in flight while the matrix cores work on f
+    if (!live) continue;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    V4 d = {0, 0, 0, 0};
+    int n = 0;
+    for (int dx = -1; dx <= 1; dx++)
+      for (int dy = -1; dy <= 1; dy++)
+        for (int dz = -1; dz <= 1; dz++) {
+          if (!(dx || dy || dz)) continue;
+          const int h = ((px + 1 + dx) * 4 + (py + 1 + dy)) * 4 + (pz + 1 + dz);
+          if (s_hm[h]) {
+            const int o0 = s_opt[(n * 8 + k) * 8 + ca], o1 = s_opt[(n * 8 + 4 + k) * 8 + ca];
+            C2 m0, m1;
+            m0.x = m0.y = m1.x = m1.y = 0;
+            if (o0 >= 0) m0 = s_op[o0];
+            if (o1 >= 0) m1 = s_op[o1];
+            const R a0 = pim ? m0.y : m0.x, a1 = pim ? m1.y : m1.x;
+            const R a2 = pim ? m0.x : -m0.y, a3 = pim ? m1.x : -m1.y;
+            const C2 x0 = xs[(h * 8 + k) * 16 + i], x1 = xs[(h * 8 + 4 + k) * 16 + i];
+            if (Mt<R>::KV) {
+              d = Mt<R>::mfma(a0, x0.x, d);
+              d = Mt<R>::mfma(a1, x1.x, d);
+              d = Mt<R>::mfma(a2, x0.y, d);
+              d = Mt<R>::mfma(a3, x1.y, d);
+            } else {
+              // single precision: one direction per FP32 product, summed in FP64
+              V4 e = {0, 0, 0, 0};
+              e = Mt<R>::mfma(a0, x0.x, e);
+              e = Mt<R>::mfma(a1, x1.x, e);
+              e = Mt<R>::mfma(a2, x0.y, e);
+              e = Mt<R>::mfma(a3, x1.y, e);
+#pragma unroll
+              for (int v = 0; v < 4; v++) acc[v] += (double)e[v];
+            }
+          }
+          n++;
+        }
+    if (Mt<R>::KV) {
+#pragma unroll
+      for (int v = 0; v < 4; v++) acc[v] = (double)d[v];
+    }
+#pragma unroll
+    for (int h2 = 0; h2 < 2; h2++) {
+      const int sl = ts[2 * k + h2];
+      if (sl >= 0) {
+        C2 v;
+        v.x = (R)acc[2 * h2];
+        v.y = (R)acc[2 * h2 + 1];
+        effval[((long)sl * C + f) * Rc + r0 + i] = v;
+      }
+    }
+  }
+}
+
+void m2l_tiled(hipStream_t s, int C, int ntile, int Rc, const int* ttile, const int* htile, const int* optab,
+               const void* opsT, const void* effden, void* effval, int single)
+{
+  if (ntile <= 0 || Rc < 16) return;
+  const int per = (ntile + 7) / 8;
+  dim3 grid(8 * per, (C + M2LT_FPB - 1) / M2LT_FPB, Rc / 16);
+  if (single)
+    hipLaunchKernelGGL(k_m2l_tile<float>, grid, dim3(M2LT_BS), 0, s, C, Rc, ntile, ttile, htile, optab,
+                       (const float2*)opsT, (const float2*)effden, (float2*)effval, 0);
+  else
+    hipLaunchKernelGGL(k_m2l_tile<double>, grid, dim3(M2LT_BS), 0, s, C, Rc, ntile, ttile, htile, optab,
+                       (const double2*)opsT, (const double2*)effden, (double2*)effval, 0);
 }
 
 }  // namespace dev

@@ -76,10 +76,13 @@ struct GemmArgs {
 };
 void gemm(hipStream_t s, const GemmArgs& a);
 
-// r2c of the scaled upward equivalent densities on the (2np)^3 grid
-// -> effden[slot][Rc][sm][C] (complex).
+// r2c of the scaled upward equivalent densities on the (2np)^3 grid into
+// complex spectra, effden[slot][Rc][sm][C] (rhs_inner 0) or
+// effden[slot][sm][C][Rc] (rhs_inner 1), stored as double2 or, with single,
+// as float2.
 void fft_fwd(hipStream_t s, int np, int nslot, const int* slot_node, int Rc, const double* upeden, int pln, int sm,
-             const double* dscale, const short* cube2surf, double2* effden, const double2* tw);
+             const double* dscale, const short* cube2surf, void* effden, int single, int rhs_inner,
+             const double2* tw);
 // M2L products summed over V pairs -> effval[trg][Rc][tm][C] (overwritten).
 void m2l(hipStream_t s, int sm, int tm, int C, int ntrg, int Rc, const int* vptr, const int* vsrc, const int* vop,
          const double2* ops, const double2* effden, double2* effval);
@@ -90,8 +93,18 @@ void m2l(hipStream_t s, int sm, int tm, int C, int ntrg, int Rc, const int* vptr
 void m2l_grouped(hipStream_t s, int sm, int tm, int C, int npar, int Rc, int rb, int bs, const int* tslot,
                  const int* sslot, const int* optab, const double2* ops, const double2* effden, double2* effval);
 // c2r and gather onto the DC surface: dncval[(node*Rc + r)*pln + p*tm + f] += ...
-void fft_inv(hipStream_t s, int np, int nslot, const int* slot_node, int Rc, const double2* effval, int tm,
-             const int* surf2cube, int nsp, double* dncval, int pln, const double2* tw);
+// (effval in the layout and precision described for fft_fwd)
+void fft_inv(hipStream_t s, int np, int nslot, const int* slot_node, int Rc, const void* effval, int single,
+             int rhs_inner, int tm, const int* surf2cube, int nsp, double* dncval, int pln, const double2* tw);
+// Tiled M2L on the matrix cores for scalar kernels, spectra with the
+// right-hand sides innermost and Rc a multiple of 16: for every tile (2x2x2
+// target parents, ttile [ntile][8][8] child slots) and its 4x4x4 halo of
+// neighbour parents (htile [ntile][64][8] child slots), effval of each target
+// child = sum over its V list of T_o effden.  opsT is [C][343] complex;
+// single selects FP32 spectra and operators (products summed in FP64 per
+// neighbour direction).
+void m2l_tiled(hipStream_t s, int C, int ntile, int Rc, const int* ttile, const int* htile, const int* optab,
+               const void* opsT, const void* effden, void* effval, int single);
 
 }  // namespace dev
 }  // namespace kifmm

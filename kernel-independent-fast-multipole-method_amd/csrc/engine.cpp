@@ -52,6 +52,7 @@ void DVec<T>::ensure(size_t count)
   if (count > n || p == nullptr) alloc(count);
 }
 template struct DVec<double>;
+template struct DVec<float>;
 template struct DVec<int>;
 template struct DVec<short>;
 
@@ -94,6 +95,7 @@ Engine::Engine(int kt, const double* coefs, int ncoefs, int np, int ptsmax, int 
   if (const char* rb = std::getenv("KIFMM3D_M2L_RB")) m2l_rb_ = std::atoi(rb);
   if (const char* bs = std::getenv("KIFMM3D_M2L_BS")) m2l_bs_ = std::atoi(bs);
   if (const char* v = std::getenv("KIFMM3D_P2P")) p2p_variant_ = std::atoi(v);
+  if (const char* m = std::getenv("KIFMM3D_M2L")) m2l_tile_ = std::string(m) != "group" && std::string(m) != "pair";
 }
 
 Engine::~Engine()
@@ -173,6 +175,19 @@ void Engine::upload_static()
   dc2de_.upload(O.dc2de);
   de2dc_.upload(O.de2dc);
   m2l_.upload(O.m2l);
+  if (mm_.sdof == 1 && mm_.tdof == 1) {  // frequency-major copies for the tiled M2L
+    std::vector<double> t((size_t)2 * 343 * O.C);
+    std::vector<float> t32(t.size());
+    for (int o = 0; o < 343; o++)
+      for (int f = 0; f < O.C; f++)
+        for (int c = 0; c < 2; c++) {
+          const double v = O.m2l[((size_t)o * O.C + f) * 2 + c];
+          t[((size_t)f * 343 + o) * 2 + c] = v;
+          t32[((size_t)f * 343 + o) * 2 + c] = (float)v;
+        }
+    opsT_.upload(t);
+    opsT32_.upload(t32);
+  }
   c2s_.upload(O.cube2surf);
   std::vector<int> s2c(O.nsp[SURF_DC], 0);
   for (int c = 0; c < np_ * np_ * np_; c++)
@@ -433,7 +448,7 @@ void Engine::build_plan()
         std::unordered_map<uint64_t, int> cell;
         for (int g : by_depth[L - 1])
           cell[((uint64_t)T.path[3 * g] << 40) | ((uint64_t)T.path[3 * g + 1] << 20) | (uint64_t)T.path[3 * g + 2]] = g;
-        std::vector<int> tsl, ssl;
+        std::vector<int> tsl, ssl, gpar;
         long checked = 0;
         std::vector<int> mine;
         for (int pg : by_depth[L - 1]) {
@@ -441,6 +456,7 @@ void Engine::build_plan()
           bool any = false;
           for (int c = 0; c < 8; c++) any = any || tslot_of[T.chd[pg] + c] >= 0;
           if (!any) continue;
+          gpar.push_back(pg);
           for (int c = 0; c < 8; c++) tsl.push_back(tslot_of[T.chd[pg] + c]);
           const int lim = 1 << (L - 1);
           for (int n = 0; n < 26; n++) {
@@ -479,6 +495,38 @@ void Engine::build_plan()
         vl.npar = (int)(tsl.size() / 8);
         vl.tslot = pool_i(P.I, tsl);
         vl.sslot = pool_i(P.I, ssl);
+        // tiled form: the grouped parents under one grandparent form a tile; the
+        // neighbour n of the parent at position (x, y, z) of the tile is halo
+        // entry ((x+1+dx)*4 + (y+1+dy))*4 + (z+1+dz)
+        if (sm == 1 && mm_.tdof == 1) {
+          std::unordered_map<int, int> tile_of;
+          std::vector<int> tt, ht;
+          for (int q = 0; q < vl.npar; q++) {
+            const int pg = gpar[q], gp = T.par[pg];
+            auto ins = tile_of.emplace(gp, (int)(tt.size() / 64));
+            if (ins.second) {
+              tt.resize(tt.size() + 64, -1);
+              ht.resize(ht.size() + 512, -1);
+            }
+            const int tile = ins.first->second;
+            int pv[3];
+            for (int k = 0; k < 3; k++) pv[k] = T.path[3 * pg + k] - 2 * T.path[3 * gp + k];
+            const int pos = pv[0] * 4 + pv[1] * 2 + pv[2];
+            for (int c = 0; c < 8; c++) tt[((size_t)tile * 8 + pos) * 8 + c] = tsl[(size_t)q * 8 + c];
+            for (int n = 0; n < 26; n++) {
+              const int h = ((pv[0] + 1 + dirs[n][0]) * 4 + (pv[1] + 1 + dirs[n][1])) * 4 + (pv[2] + 1 + dirs[n][2]);
+              for (int sc = 0; sc < 8; sc++) {
+                const int v = ssl[((size_t)q * 26 + n) * 8 + sc];
+                int& e = ht[((size_t)tile * 64 + h) * 8 + sc];
+                if (e >= 0 && v >= 0 && e != v) throw Error(KIFMM3D_EINVAL, "internal: inconsistent M2L tile halo");
+                if (v >= 0) e = v;
+              }
+            }
+          }
+          vl.ntile = (int)(tt.size() / 64);
+          vl.ttile = pool_i(P.I, tt);
+          vl.htile = pool_i(P.I, ht);
+        }
       }
       for (int g : by_depth[L]) tslot_of[g] = -1;
       vl.dscale = (long)pool_d(P.D, scales(L + rootlvl_, +1, sm));  // plnden2effden (matmgnt3d.cpp:226-235)
@@ -550,8 +598,9 @@ int Engine::choose_rhs_chunk(int nrhs)
   HIPCHK(hipMemGetInfo(&fr, &tot));
   double budget = 0.6 * (double)fr + 8.0 * (double)(sden_.n + tval_.n + upcval_.n + upeden_.n + dncval_.n + dneden_.n) +
                   8.0 * (double)(effden_.n + effval_.n);
-  int rc = (int)std::max(1.0, std::floor(budget / std::max(per, 1.0)));
-  return std::min(rc, nrhs);
+  int rc = std::min((int)std::max(1.0, std::floor(budget / std::max(per, 1.0))), nrhs);
+  if (rc >= 16) rc -= rc % 16;  // chunks of 16k right-hand sides take the matrix-core kernels
+  return rc;
 }
 
 void Engine::ensure_work(int rc)
@@ -693,14 +742,20 @@ void Engine::run_chunk(const double* srcden, double* trgval, int rc, hipEvent_t*
              (const int2*)(I + P.near.ent), src, trg, rc, kcu, p2p_variant_);
   });
   HIPCHK(hipEventRecord(ev[3], s));
-  // V list
+  // V list: tiled matrix-core M2L for scalar kernels on chunks of 16k
+  // right-hand sides, the grouped (or per-pair) form otherwise
+  const bool tiled = m2l_tile_ && !P.m2l_pairs && sm == 1 && tm == 1 && rc % 16 == 0;
+  const int ri = tiled ? 1 : 0, single = 0;
   for (const VLevel& vl : P.vlev) {
     timed(KIFMM3D_K_FFT_FWD, [&] {
-      dev::fft_fwd(s, np_, vl.nsrc, I + vl.src_node, rc, upeden_.p, pUE, sm, D + vl.dscale, c2s_.p,
-                   (double2*)effden_.p, (const double2*)tw_.p);
+      dev::fft_fwd(s, np_, vl.nsrc, I + vl.src_node, rc, upeden_.p, pUE, sm, D + vl.dscale, c2s_.p, effden_.p, single,
+                   ri, (const double2*)tw_.p);
     });
     timed(KIFMM3D_K_M2L, [&] {
-      if (P.m2l_pairs)
+      if (tiled)
+        dev::m2l_tiled(s, O.C, vl.ntile, rc, I + vl.ttile, I + vl.htile, I + P.optab, opsT_.p, effden_.p,
+                       effval_.p, single);
+      else if (P.m2l_pairs)
         dev::m2l(s, sm, tm, O.C, vl.ntrg, rc, I + vl.vptr, I + vl.vsrc, I + vl.vop, (const double2*)m2l_.p,
                  (const double2*)effden_.p, (double2*)effval_.p);
       else
@@ -708,7 +763,7 @@ void Engine::run_chunk(const double* srcden, double* trgval, int rc, hipEvent_t*
                          (const double2*)m2l_.p, (const double2*)effden_.p, (double2*)effval_.p);
     });
     timed(KIFMM3D_K_FFT_INV, [&] {
-      dev::fft_inv(s, np_, vl.ntrg, I + vl.trg_node, rc, (const double2*)effval_.p, tm, s2c_.p, O.nsp[SURF_DC],
+      dev::fft_inv(s, np_, vl.ntrg, I + vl.trg_node, rc, effval_.p, single, ri, tm, s2c_.p, O.nsp[SURF_DC],
                    dncval_.p, pDC, (const double2*)tw_.p);
     });
   }
