@@ -23,6 +23,7 @@ sys.path.insert(0, PKG)
 
 METRIC = "evaluated targets/sec at rel err ≤1e-6 vs FP64 direct (chosen vs plain FP64 GPU)"
 FP64_PEAK_TF = 78.6      # MI355X FP64 (vector and matrix) TFLOP/s
+FP32_PEAK_TF = 157.3     # MI355X FP32 dense matrix TFLOP/s
 HBM_PEAK_GBS = 8000.0    # MI355X HBM3E GB/s
 
 # seeds 0x4B1F0000 + config number
@@ -33,8 +34,9 @@ CONFIGS = {
                                   ctr=(0.0, 0.0, 0.0), rootlvl=0, nchk=200),
     "navier-clusters-1m": dict(cfg=3, kt=511, coefs=[1.0, 0.3], n=1048576, np=8, ptsmax=150, nrhs=1,
                                gen="clusters32", ctr=(0.5, 0.5, 0.5), rootlvl=1, nchk=200),
-    "laplace-uniform-4m-16rhs": dict(cfg=4, kt=111, coefs=[1.0], n=4194304, np=6, ptsmax=128, nrhs=16, gen="cube",
-                                     ctr=(0.5, 0.5, 0.5), rootlvl=1, nchk=1000),
+    "laplace-uniform-4m-16rhs": dict(cfg=4, kt=111, coefs=[1.0], n=4194304, np=6, ptsmax="auto", nrhs=16,
+                                     gen="cube", ctr=(0.5, 0.5, 0.5), rootlvl=1, nchk=1000,
+                                     ptsmax_cands=(64, 128, 256)),
 }
 DEFAULT = "laplace-uniform-4m-16rhs"
 
@@ -106,14 +108,90 @@ def cpu_baseline(c, seconds_target=15.0):
     return n / dt, f"{c['gen']} N={n}, np={c['np']}, ptsmax={c['ptsmax']}, 1 RHS, warm eval {dt:.2f} s"
 
 
+def timed_steps(f, K, d_src, d_trg, nrhs, mode, steps, warmup, label):
+    """Warm-up steps, then `steps` timed evaluate passes between device
+    synchronisations.  Returns (ms per step, mean stage times)."""
+    for i in range(warmup):
+        f.eval_device(d_src, d_trg, nrhs, mode)
+        log(f"{label} warmup {i + 1}/{warmup}: {f.stage_times()['total']:.1f} ms")
+    stage_sum = {}
+    K.synchronize()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        f.eval_device(d_src, d_trg, nrhs, mode)
+        for k, v in f.stage_times().items():
+            stage_sum[k] = stage_sum.get(k, 0.0) + v
+    K.synchronize()
+    dt = time.perf_counter() - t0
+    return 1000.0 * dt / steps, {k: v / steps for k, v in stage_sum.items()}
+
+
+def direct_error(f, d_src, d_trg, c, nrhs, n, tdof, seed, idx_cache={}):
+    val = np.empty((nrhs, n * tdof))
+    d_trg.download(val)
+    rng = np.random.default_rng(seed + 2)
+    idx = np.sort(rng.choice(n, c["nchk"], replace=False)).astype(np.int32)
+    key = (id(f), nrhs)
+    if key not in idx_cache:
+        idx_cache[key] = f.direct(d_src, idx, nrhs)
+    d = idx_cache[key]
+    v = val.reshape(nrhs, n, tdof)[:, idx, :].reshape(nrhs, -1)
+    return float(np.linalg.norm(v - d) / np.linalg.norm(d))
+
+
+def kernel_rates(f, c, stats, nrhs, mixed):
+    """Per kernel class: (ms, launch groups, rate, unit, bound, peak) of the last step."""
+    kt_mm = {111: 111, 321: 301, 511: 511}[c["kt"]]
+    sm, tm = {111: (1, 1), 301: (4, 3), 511: (3, 3)}[kt_mm]
+    nsp = {4: 56, 6: 152, 8: 296}[c["np"]]
+    mfma32 = mixed and sm == 1 and tm == 1 and nrhs % 16 == 0  # FP32 matrix-core classes of the mixed form
+    work = {  # (algorithmic amount per step, unit, bound, peak)
+        "p2p": (stats["near_pairs"] * PAIR_FLOPS[c["kt"]](nrhs), "flop", "mfma",
+                FP32_PEAK_TF if mfma32 else FP64_PEAK_TF),
+        "s2m": (stats["s2m_pairs"] * PAIR_FLOPS[c["kt"]](nrhs), "flop", "mfma", FP64_PEAK_TF),
+        "s2l": (stats["s2l_pairs"] * PAIR_FLOPS[c["kt"]](nrhs), "flop", "mfma", FP64_PEAK_TF),
+        "s2p": (stats["far_pairs"] * PAIR_FLOPS[kt_mm](nrhs), "flop", "mfma", FP64_PEAK_TF),
+        "m2l": (stats["v_pairs"] * nrhs * stats["C"] * 8.0 * sm * tm, "flop", "mfma",
+                FP32_PEAK_TF if mfma32 else FP64_PEAK_TF),
+        "gemm_up": (stats["up_flops"] * nrhs, "flop", "mfma", FP64_PEAK_TF),
+        "gemm_down": (stats["down_flops"] * nrhs, "flop", "mfma", FP64_PEAK_TF),
+        "fft_fwd": (stats["fft_fwd"] * nrhs * (nsp * 8.0 + stats["C"] * (8.0 if mfma32 else 16.0)), "byte", "hbm",
+                    HBM_PEAK_GBS),
+        "fft_inv": (stats["fft_inv"] * nrhs * (nsp * 8.0 + stats["C"] * (8.0 if mfma32 else 16.0)), "byte", "hbm",
+                    HBM_PEAK_GBS),
+    }
+    rates = {}
+    kt_times = f.kernel_times()
+    for k, (amt, unit, bound, peak) in work.items():
+        ms, calls = kt_times[k]
+        if ms > 0 and calls > 0:
+            rate = amt / (ms * 1e-3) / (1e12 if unit == "flop" else 1e9)
+            rates[k] = (ms, calls, rate, unit, bound, peak)
+            log(f"  kernel {k:9s}: {ms:9.3f} ms in {calls:3d} launch groups, "
+                f"{rate:9.3f} {'TFLOP/s' if unit == 'flop' else 'GB/s'} ({100 * rate / peak:.1f}% of peak)")
+    return rates
+
+
+def setup_fmm(K, c, pos, nor, ptsmax):
+    f = K.FMM3d("fmm3d_")
+    f.srcpos, f.srcnor, f.trgpos = pos, nor, pos
+    f.ctr, f.rootlvl = c["ctr"], c["rootlvl"]
+    f.knl = K.Kernel3d(c["kt"], c["coefs"])
+    t0 = time.perf_counter()
+    f.setup({"-fmm3d_np": str(c["np"]), "-fmm3d_let3d_ptsmax": str(ptsmax), "-fmm3d_let3d_maxlevel": "10"})
+    return f, time.perf_counter() - t0
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default=DEFAULT, choices=sorted(CONFIGS))
-    ap.add_argument("--ptsmax", type=int, default=None)
+    ap.add_argument("--ptsmax", default=None, help="leaf size, or 'auto' to pick the fastest of the candidates")
     ap.add_argument("--nrhs", type=int, default=None)
+    ap.add_argument("--form", default="both", choices=["both", "plain", "mixed"],
+                    help="both: time the plain FP64 form and the mixed form; value is the mixed form's")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-check", action="store_true", help="skip the direct-sum error check")
     args = ap.parse_args()
@@ -125,113 +203,101 @@ def main():
     import kifmm3d as K
 
     c = dict(CONFIGS[args.config])
-    if args.ptsmax:
-        c["ptsmax"] = args.ptsmax
     if args.nrhs:
         c["nrhs"] = args.nrhs
     if K.device_count() < 1:
         raise SystemExit("no HIP device visible")
     seed = 0x4B1F0000 + c["cfg"]
     pos, nor = make_points(c, seed)
-    f = K.FMM3d("fmm3d_")
-    f.srcpos, f.srcnor, f.trgpos = pos, nor, pos
-    f.ctr, f.rootlvl = c["ctr"], c["rootlvl"]
-    f.knl = K.Kernel3d(c["kt"], c["coefs"])
-    t0 = time.perf_counter()
-    f.setup({"-fmm3d_np": str(c["np"]), "-fmm3d_let3d_ptsmax": str(c["ptsmax"]), "-fmm3d_let3d_maxlevel": "10"})
-    t_setup = time.perf_counter() - t0
+    nrhs, n = c["nrhs"], c["n"]
+    forms = {"both": [("plain", K.MODE_PLAIN), ("mixed", K.MODE_MIXED)], "plain": [("plain", K.MODE_PLAIN)],
+             "mixed": [("mixed", K.MODE_MIXED)]}[args.form]
+    chosen_mode = forms[-1][1]
+
+    ptsmax = args.ptsmax or c["ptsmax"]
+    tune = None
+    if str(ptsmax) == "auto":
+        # pick the leaf size by timing one warm evaluate of the chosen form per candidate
+        tune = {}
+        best = None
+        for p in c["ptsmax_cands"]:
+            f, ts = setup_fmm(K, c, pos, nor, p)
+            sdof, tdof = f.knl.sdof(), f.knl.tdof()
+            den = make_density(c, sdof, seed)
+            d_src = K.DeviceBuffer(den.nbytes)
+            d_src.upload(den)
+            d_trg = K.DeviceBuffer(nrhs * n * tdof * 8)
+            ms, _ = timed_steps(f, K, d_src, d_trg, nrhs, chosen_mode, 1, 1, f"tune ptsmax={p}")
+            tune[p] = round(ms, 2)
+            log(f"tune ptsmax={p}: setup {ts:.2f} s, {ms:.2f} ms/step")
+            if best is None or ms < best[1]:
+                best = (p, ms)
+            del f, d_src, d_trg
+        ptsmax = best[0]
+    ptsmax = int(ptsmax)
+    f, t_setup = setup_fmm(K, c, pos, nor, ptsmax)
     info, stats = f.info(), f.plan_stats()
-    log(f"{args.config}: setup {t_setup:.2f} s, {info['nnode']} nodes, {info['nlevel']} levels, "
+    log(f"{args.config}: ptsmax {ptsmax}, setup {t_setup:.2f} s, {info['nnode']} nodes, {info['nlevel']} levels, "
         f"near pairs {stats['near_pairs']:.3e}, V pairs {stats['v_pairs']:.3e}")
-    sdof, tdof, nrhs, n = f.knl.sdof(), f.knl.tdof(), c["nrhs"], c["n"]
+    sdof, tdof = f.knl.sdof(), f.knl.tdof()
     den = make_density(c, sdof, seed)
     d_src = K.DeviceBuffer(den.nbytes)
     d_src.upload(den)
     d_trg = K.DeviceBuffer(nrhs * n * tdof * 8)
 
-    for i in range(args.warmup):
-        f.eval_device(d_src, d_trg, nrhs)
-        log(f"warmup {i + 1}/{args.warmup}: {f.stage_times()['total']:.1f} ms")
-    stage_sum = {}
-    K.synchronize()
-    # (the per-kernel event timings read after the loop are those of the last timed step)
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        f.eval_device(d_src, d_trg, nrhs)
-        for k, v in f.stage_times().items():
-            stage_sum[k] = stage_sum.get(k, 0.0) + v
-    K.synchronize()
-    dt = time.perf_counter() - t0
-    ms_step = 1000.0 * dt / args.steps
-    stages = {k: v / args.steps for k, v in stage_sum.items()}
-    value = n * nrhs / (dt / args.steps)
-    log(f"timed {args.steps} steps: {ms_step:.2f} ms/step, {value:.4e} targets/s, stages {stages}")
+    res = {}
+    for name, mode in forms:
+        ms, stages = timed_steps(f, K, d_src, d_trg, nrhs, mode, args.steps, args.warmup, name)
+        value = n * nrhs / (ms * 1e-3)
+        log(f"{name}: {args.steps} steps, {ms:.2f} ms/step, {value:.4e} targets/s")
+        rates = kernel_rates(f, c, stats, nrhs, mode == K.MODE_MIXED)
+        err = None
+        if not args.no_check:
+            err = direct_error(f, d_src, d_trg, c, nrhs, n, tdof, seed)
+            log(f"{name}: rel L2 error vs FP64 direct sum on {c['nchk']} targets x {nrhs} RHS: {err:.3e}")
+        res[name] = dict(ms=ms, value=value, stages=stages, rates=rates, err=err)
 
-    err = None
-    if not args.no_check:
-        val = np.empty((nrhs, n * tdof))
-        d_trg.download(val)
-        rng = np.random.default_rng(seed + 2)
-        idx = np.sort(rng.choice(n, c["nchk"], replace=False)).astype(np.int32)
-        d = f.direct(d_src, idx, nrhs)
-        v = val.reshape(nrhs, n, tdof)[:, idx, :].reshape(nrhs, -1)
-        err = float(np.linalg.norm(v - d) / np.linalg.norm(d))
-        log(f"rel L2 error vs FP64 direct sum on {c['nchk']} targets x {nrhs} RHS: {err:.3e}")
-
-    # roofline of the dominant kernel class, timed live with HIP events on the
-    # library's stream during the last timed step
-    kt_mm = {111: 111, 321: 301, 511: 511}[c["kt"]]
-    sm, tm = {111: (1, 1), 301: (4, 3), 511: (3, 3)}[kt_mm]
-    kt_times = f.kernel_times()
-    nsp = {4: 56, 6: 152, 8: 296}[c["np"]]
-    work = {  # (algorithmic amount per step, unit, bound)
-        "p2p": (stats["near_pairs"] * PAIR_FLOPS[c["kt"]](nrhs), "flop", "mfma"),
-        "s2m": (stats["s2m_pairs"] * PAIR_FLOPS[c["kt"]](nrhs), "flop", "mfma"),
-        "s2l": (stats["s2l_pairs"] * PAIR_FLOPS[c["kt"]](nrhs), "flop", "mfma"),
-        "s2p": (stats["far_pairs"] * PAIR_FLOPS[kt_mm](nrhs), "flop", "mfma"),
-        "m2l": (stats["v_pairs"] * nrhs * stats["C"] * 8.0 * sm * tm, "flop", "mfma"),
-        "gemm_up": (stats["up_flops"] * nrhs, "flop", "mfma"),
-        "gemm_down": (stats["down_flops"] * nrhs, "flop", "mfma"),
-        "fft_fwd": (stats["fft_fwd"] * nrhs * (nsp * 8.0 + stats["C"] * 16.0), "byte", "hbm"),
-        "fft_inv": (stats["fft_inv"] * nrhs * (nsp * 8.0 + stats["C"] * 16.0), "byte", "hbm"),
-    }
-    rates = {}
-    for k, (amt, unit, bound) in work.items():
-        ms, calls = kt_times[k]
-        if ms > 0 and calls > 0:
-            rate = amt / (ms * 1e-3) / (1e12 if unit == "flop" else 1e9)
-            rates[k] = (ms, calls, rate, unit, bound)
-            log(f"kernel {k:9s}: {ms:9.3f} ms in {calls:3d} launch groups, "
-                f"{rate:9.3f} {'TFLOP/s' if unit == 'flop' else 'GB/s'}")
-    dom = max(rates, key=lambda k: rates[k][0])
-    ms, calls, rate, unit, bound = rates[dom]
-    peak = FP64_PEAK_TF if unit == "flop" else HBM_PEAK_GBS
+    chosen = forms[-1][0]
+    r = res[chosen]
+    # roofline of the chosen form's dominant kernel class, timed live with HIP
+    # events on the library's stream during its last timed step
+    dom = max(r["rates"], key=lambda k: r["rates"][k][0])
+    ms, calls, rate, unit, bound, peak = r["rates"][dom]
     roofline = {"kernel": dom, "bound": bound, "achieved": round(rate, 4), "peak": peak,
                 "unit": "TFLOP/s" if unit == "flop" else "GB/s", "frac": round(rate / peak, 5),
                 "traffic": None, "ms_per_step": round(ms, 3), "launch_groups": calls}
 
     cpu = None
     if not args.no_cpu:
-        cv, sample = cpu_baseline(c)
+        cv, sample = cpu_baseline(dict(c, ptsmax=ptsmax))
         cpu = {"value": cv, "unit": "targets/s", "cores": 1, "kind": "port", "sample": sample}
         log(f"cpu baseline: {cv:.4e} targets/s ({sample})")
 
+    cfg = {"workload": args.config, "form": "mixed FP32/FP64" if chosen == "mixed" else "plain FP64",
+           "kernel": c["kt"], "n_points": n, "nrhs": nrhs, "np": c["np"], "ptsmax": ptsmax,
+           "rel_err_vs_direct": r["err"], "setup_s": round(t_setup, 3),
+           "stage_ms": {k: round(v, 3) for k, v in r["stages"].items()}}
+    if tune:
+        cfg["ptsmax_tuning_ms"] = tune
+    if "plain" in res and chosen != "plain":
+        p = res["plain"]
+        cfg["plain_fp64"] = {"value": p["value"], "ms_per_step": round(p["ms"], 3), "rel_err_vs_direct": p["err"],
+                             "stage_ms": {k: round(v, 3) for k, v in p["stages"].items()}}
+        cfg["speedup_vs_plain"] = round(r["value"] / p["value"], 3)
     out = {
         "metric": METRIC,
-        "value": value,
+        "value": r["value"],
         "unit": "targets/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_step,
+        "ms_per_step": r["ms"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32+f64" if chosen == "mixed" else "f64",
         "data": "synthetic",
-        "config": {"workload": args.config, "form": "plain FP64", "kernel": c["kt"], "n_points": n, "nrhs": nrhs,
-                   "np": c["np"], "ptsmax": c["ptsmax"], "rel_err_vs_direct": err, "setup_s": round(t_setup, 3),
-                   "stage_ms": {k: round(v, 3) for k, v in stages.items()}},
+        "config": cfg,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -309,6 +309,7 @@ __global__ __launch_bounds__(64) void k_p2p_s(const int* __restrict__ tbeg, cons
 // groups are padded to a multiple of 4 with zero densities.  Summation order
 // differs from the scalar kernels only by FP64 rounding.
 typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
 
 __global__ __launch_bounds__(P2P_BS) void k_p2p_mfma111(const int* __restrict__ tbeg, const int* __restrict__ tnum,
                                                         const int* __restrict__ eptr, const int2* __restrict__ ent,
@@ -371,6 +372,92 @@ __global__ __launch_bounds__(P2P_BS) void k_p2p_mfma111(const int* __restrict__ 
       double* val = trg.val + (long)(r0 + kk) * trg.val_rs + tb + tt;
 #pragma unroll
       for (int v = 0; v < 4; v++) val[(long)(4 * v) * trg.val_rs] += kc.a * acc[i][v];
+    }
+  }
+}
+
+// Mixed-precision form of k_p2p_mfma111: the same wave layout on the FP32
+// matrix cores (v_mfma_f32_16x16x4f32, accumulator item v of lane l holds
+// right-hand side 4*(l/16) + v of target l%16).  Coordinates are taken
+// relative to the chunk's first target in FP64 and then rounded, so FP32
+// keeps the differences of nearby points to ~1e-7 of the leaf size; 1/r
+// comes from v_rsq_f32.  The FP32 products of one staged source tile (at most
+// 64 sources) are summed into FP64 accumulators.
+__global__ __launch_bounds__(P2P_BS) void k_p2p_mfma111_f32(const int* __restrict__ tbeg, const int* __restrict__ tnum,
+                                                            const int* __restrict__ eptr, const int2* __restrict__ ent,
+                                                            Points src, Targets trg, int roff, KC kc)
+{
+  constexpr int NR = 16;
+  __shared__ float s_pos[3][TILE];
+  __shared__ float s_den[TILE][NR];
+  const int c = blockIdx.x, t = threadIdx.x, w = t >> 6, l = t & 63;
+  const int col = l & 15, kk = l >> 4;
+  const int r0 = roff + blockIdx.y * NR;
+  const int tb = tbeg[c], tn = tnum[c];
+  const bool wact = 64 * w < tn;  // wave-uniform
+  const double ox = trg.x[tb], oy = trg.y[tb], oz = trg.z[tb];
+  float tx[4], ty[4], tz[4];
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    int tt = 64 * w + 16 * i + col;
+    tt = tt < tn ? tt : 0;
+    tx[i] = (float)(trg.x[tb + tt] - ox);
+    ty[i] = (float)(trg.y[tb + tt] - oy);
+    tz[i] = (float)(trg.z[tb + tt] - oz);
+  }
+  double acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int v = 0; v < 4; v++) acc[i][v] = 0.0;
+  const double* den = src.den + (long)r0 * src.den_rs;
+  for (int e = eptr[c]; e < eptr[c + 1]; e++) {
+    const int2 se = ent[e];
+    for (int b0 = 0; b0 < se.y; b0 += TILE) {
+      const int cnt = min(TILE, se.y - b0), cnt4 = (cnt + 3) & ~3;
+      const int j0 = se.x + b0;
+      __syncthreads();
+      for (int i = t; i < cnt4; i += P2P_BS) {
+        const bool in = i < cnt;
+        s_pos[0][i] = in ? (float)(src.x[j0 + i] - ox) : 0.0f;
+        s_pos[1][i] = in ? (float)(src.y[j0 + i] - oy) : 0.0f;
+        s_pos[2][i] = in ? (float)(src.z[j0 + i] - oz) : 0.0f;
+      }
+      for (int i = t; i < cnt4 * NR; i += P2P_BS) {
+        const int jj = i / NR, q = i - jj * NR;
+        s_den[jj][q] = jj < cnt ? (float)den[(long)(j0 + jj) * src.den_js + (long)q * src.den_rs] : 0.0f;
+      }
+      __syncthreads();
+      if (wact) {
+        v4f part[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) part[i] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+        for (int g = 0; g < cnt4; g += 4) {
+          const float sx = s_pos[0][g + kk], sy = s_pos[1][g + kk], sz = s_pos[2][g + kk];
+          const float f = s_den[g + kk][col];
+#pragma unroll
+          for (int i = 0; i < 4; i++) {
+            const float dx = tx[i] - sx, dy = ty[i] - sy, dz = tz[i] - sz;
+            const float r2 = dx * dx + dy * dy + dz * dz;
+            const float gv = r2 < 1e-16f ? 0.0f : __builtin_amdgcn_rsqf(r2);
+            part[i] = __builtin_amdgcn_mfma_f32_16x16x4f32(f, gv, part[i], 0, 0, 0);
+          }
+        }
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+          for (int v = 0; v < 4; v++) acc[i][v] += (double)part[i][v];
+      }
+    }
+  }
+  if (!wact) return;
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const int tt = 64 * w + 16 * i + col;
+    if (tt < tn) {
+      double* val = trg.val + (long)(r0 + 4 * kk) * trg.val_rs + tb + tt;
+#pragma unroll
+      for (int v = 0; v < 4; v++) val[(long)v * trg.val_rs] += kc.a * acc[i][v];
     }
   }
 }
@@ -623,7 +710,10 @@ template <int KT, int NR>
 static void p2p_launch(hipStream_t s, int variant, int nchunk, int nb, const int* tbeg, const int* tnum,
                        const int* eptr, const int2* ent, Points src, Targets trg, int roff, KC kc)
 {
-  if (variant == 2 && KT == 111 && NR == 16 && src.den_rs == 1)
+  if (variant == 3 && KT == 111 && NR == 16)
+    hipLaunchKernelGGL(k_p2p_mfma111_f32, dim3(nchunk, nb), dim3(P2P_BS), 0, s, tbeg, tnum, eptr, ent, src, trg, roff,
+                       kc);
+  else if (variant >= 2 && KT == 111 && NR == 16 && src.den_rs == 1)
     hipLaunchKernelGGL(k_p2p_mfma111, dim3(nchunk, nb), dim3(P2P_BS), 0, s, tbeg, tnum, eptr, ent, src, trg, roff, kc);
   else if (variant == 1)
     hipLaunchKernelGGL((k_p2p_s<KT, NR>), dim3(2 * nchunk, nb), dim3(64), 0, s, tbeg, tnum, eptr, ent, src, trg, roff,
@@ -845,6 +935,84 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs p)
   }
 }
 
+// Plain (uncompensated) form on the FP64 matrix cores: the same 64 x 64 tile
+// and operand staging; wave w owns the 32 x 32 quarter (rows 32*(w/2), columns
+// 32*(w%2)) as 2 x 2 tiles of v_mfma_f64_16x16x4f64 (A lane l: row l%16, k
+// l/16; B lane l: k l/16, column l%16; D item v of lane l: row l/16 + 4v).
+__global__ __launch_bounds__(256) void k_gemm_mfma(GemmArgs p)
+{
+  const int grp = blockIdx.z;
+  const int n0 = p.gbeg[grp], n1 = p.gbeg[grp + 1];
+  const long cb = (long)n0 * p.Rc + (long)blockIdx.x * 64, ce = (long)n1 * p.Rc;
+  if (cb >= ce) return;
+  const int rb = blockIdx.y * 64;
+  const double* A = p.A + grp * p.a_gstride;
+  const int K = p.nseg * p.Kseg;
+  __shared__ double As[16][65];
+  __shared__ double Bs[16][65];
+  const int t = threadIdx.x, w = t >> 6, l = t & 63, li = l & 15, lk = l >> 4;
+  const int wr = (w >> 1) * 32, wc = (w & 1) * 32;
+  v4d acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
+  for (int k0 = 0; k0 < K; k0 += 16) {
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      int e = t + 256 * i, row = e & 63, kk = e >> 6;
+      int gr = rb + row, gk = k0 + kk;
+      As[kk][row] = (gr < p.M && gk < K) ? A[(long)gk * p.M + gr] : 0.0;
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      int e = t + 256 * i, kk = e & 15, cl = e >> 4;
+      int gk = k0 + kk;
+      long j = cb + cl;
+      double v = 0.0;
+      if (j < ce && gk < K) {
+        int n = (int)(j / p.Rc), r = (int)(j - (long)n * p.Rc);
+        int seg = gk / p.Kseg, kin = gk - seg * p.Kseg;
+        int bn = p.bnode[(long)seg * p.ncol + n];
+        if (bn >= 0) {
+          v = p.B[((long)bn * p.Rc + r) * p.Kseg + kin];
+          if (p.bscale) v *= p.bscale[kin];  // power of two: exact
+        }
+      }
+      Bs[kk][cl] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < 4; ks++) {
+      const int kk = ks * 4 + lk;
+      const double a0 = As[kk][wr + li], a1 = As[kk][wr + 16 + li];
+      const double b0 = Bs[kk][wc + li], b1 = Bs[kk][wc + 16 + li];
+      acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+      acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+      acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+      acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int j = 0; j < 2; j++) {
+    long jj = cb + wc + 16 * j + li;
+    if (jj >= ce) continue;
+    int n = (int)(jj / p.Rc), r = (int)(jj - (long)n * p.Rc);
+    double* out = p.C + ((long)p.cnode[n] * p.Rc + r) * p.M;
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        int row = rb + wr + 16 * i + lk + 4 * v;
+        if (row >= p.M) continue;
+        double h = acc[i][j][v];
+        if (p.cscale) h *= p.cscale[row];  // power of two: exact
+        out[row] = p.accumulate ? out[row] + h : h;
+      }
+  }
+}
+
 void gemm(hipStream_t s, const GemmArgs& a)
 {
   if (a.ngroup <= 0 || a.maxcols <= 0) return;
@@ -852,6 +1020,8 @@ void gemm(hipStream_t s, const GemmArgs& a)
   dim3 grid((unsigned)((cols + 63) / 64), (a.M + 63) / 64, a.ngroup);
   if (a.compensated)
     hipLaunchKernelGGL(k_gemm<true>, grid, dim3(256), 0, s, a);
+  else if (a.mfma)
+    hipLaunchKernelGGL(k_gemm_mfma, grid, dim3(256), 0, s, a);
   else
     hipLaunchKernelGGL(k_gemm<false>, grid, dim3(256), 0, s, a);
 }
@@ -1418,8 +1588,7 @@ void m2l(hipStream_t s, int sm, int tm, int C, int ntrg, int Rc, const int* vptr
 // ordered so that lane (k, r) ends up with Y of children 2k and 2k+1 as
 // (re, im) pairs: accumulator item v holds child 2k + v/2, part v%2.  The
 // position of item v in the 16x16 result is row k + 4v for the FP64 MFMA and
-// row 4k + v for the FP32 one (DROW).
-typedef float v4f __attribute__((ext_vector_type(4)));
+// row 4k + v for the FP32 one.
 
 template <class R>
 struct Mt;

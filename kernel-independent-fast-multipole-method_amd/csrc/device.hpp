@@ -35,7 +35,10 @@ struct Targets {  // leaf-ordered targets (SoA) and values [rhs][n*tdof]
 // Near field: for every chunk of <= 128 targets, sum the user kernel over the
 // source leaves ent[eptr[c] .. eptr[c+1]) (each ent = {sextbeg, sextnum}).
 // variant 0: sources staged in LDS, 128-thread blocks; variant 1: sources read
-// through the scalar cache (needs den_rs == sdof), one wave per 64 targets.
+// through the scalar cache (needs den_rs == sdof), one wave per 64 targets;
+// variant 2: Laplace blocks of 16 right-hand sides on the FP64 matrix cores;
+// variant 3: the same on the FP32 matrix cores (mixed precision).  Variants
+// 2 and 3 fall back to variant 0 for other kernels and smaller blocks.
 void p2p(hipStream_t s, int kt, int nchunk, const int* tbeg, const int* tnum, const int* eptr, const int2* ent,
          Points src, Targets trg, int nr, KC kc, int variant);
 
@@ -73,6 +76,7 @@ struct GemmArgs {
   const double* cscale;
   int accumulate;
   int compensated;  // 1: TwoProd/TwoSum accumulation (pseudoinverse applies), 0: plain FMA
+  int mfma;         // plain form on the FP64 matrix cores (ignored when compensated)
 };
 void gemm(hipStream_t s, const GemmArgs& a);
 

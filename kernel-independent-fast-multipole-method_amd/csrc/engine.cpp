@@ -95,6 +95,7 @@ Engine::Engine(int kt, const double* coefs, int ncoefs, int np, int ptsmax, int 
   if (const char* rb = std::getenv("KIFMM3D_M2L_RB")) m2l_rb_ = std::atoi(rb);
   if (const char* bs = std::getenv("KIFMM3D_M2L_BS")) m2l_bs_ = std::atoi(bs);
   if (const char* v = std::getenv("KIFMM3D_P2P")) p2p_variant_ = std::atoi(v);
+  if (const char* g = std::getenv("KIFMM3D_GEMM")) gemm_mfma_ = std::string(g) != "valu";
   if (const char* m = std::getenv("KIFMM3D_M2L")) m2l_tile_ = std::string(m) != "group" && std::string(m) != "pair";
 }
 
@@ -621,7 +622,7 @@ void Engine::evaluate_device(const double* d_srcden, double* d_trgval, int nrhs,
 {
   if (!setup_done_) throw Error(KIFMM3D_ESTATE, "evaluate called before setup");
   if (nrhs < 1) throw Error(KIFMM3D_EINVAL, "nrhs must be >= 1");
-  if (mode != KIFMM3D_MODE_PLAIN) throw Error(KIFMM3D_EINVAL, "unsupported evaluation mode");
+  if (mode != KIFMM3D_MODE_PLAIN && mode != KIFMM3D_MODE_MIXED) throw Error(KIFMM3D_EINVAL, "unsupported evaluation mode");
   if ((ns_ > 0 && !d_srcden) || (nt_ > 0 && !d_trgval)) throw Error(KIFMM3D_EINVAL, "null density or value pointer");
   for (double& m : stage_ms) m = 0.0;
   for (double& m : kernel_ms) m = 0.0;
@@ -634,7 +635,7 @@ void Engine::evaluate_device(const double* d_srcden, double* d_trgval, int nrhs,
   try {
     for (int r0 = 0; r0 < nrhs; r0 += rc) {
       int n = std::min(rc, nrhs - r0);
-      run_chunk(d_srcden + (size_t)r0 * ns_ * user_.sdof, d_trgval + (size_t)r0 * nt_ * user_.tdof, n, ev);
+      run_chunk(d_srcden + (size_t)r0 * ns_ * user_.sdof, d_trgval + (size_t)r0 * nt_ * user_.tdof, n, mode, ev);
       HIPCHK(hipEventSynchronize(ev[8]));
       for (int s = 0; s < 8; s++) {
         float ms = 0;
@@ -671,8 +672,9 @@ void Engine::evaluate_host(const double* srcden, double* trgval, int nrhs, int m
   if (nt) HIPCHK(hipMemcpy(trgval, htrg_.p, nt * sizeof(double), hipMemcpyDeviceToHost));
 }
 
-void Engine::run_chunk(const double* srcden, double* trgval, int rc, hipEvent_t* ev)
+void Engine::run_chunk(const double* srcden, double* trgval, int rc, int mode, hipEvent_t* ev)
 {
+  const bool mixed = mode == KIFMM3D_MODE_MIXED;
   const Operators& O = *ops_;
   const Tree& T = tree_;
   const Plan& P = plan_;
@@ -697,7 +699,8 @@ void Engine::run_chunk(const double* srcden, double* trgval, int rc, hipEvent_t*
     a.bscale = op.bscale >= 0 ? D + op.bscale : nullptr;
     a.cscale = op.cscale >= 0 ? D + op.cscale : nullptr;
     a.accumulate = op.accumulate;
-    a.compensated = op.compensated;
+    a.compensated = mixed ? 0 : op.compensated;
+    a.mfma = gemm_mfma_ ? 1 : 0;
     dev::gemm(s, a);
   };
 
@@ -739,13 +742,13 @@ void Engine::run_chunk(const double* srcden, double* trgval, int rc, hipEvent_t*
   // U list and direct W/X
   timed(KIFMM3D_K_P2P, [&] {
     dev::p2p(s, user_.kt, P.near.n, I + P.near.tbeg, I + P.near.tnum, I + P.near.eptr,
-             (const int2*)(I + P.near.ent), src, trg, rc, kcu, p2p_variant_);
+             (const int2*)(I + P.near.ent), src, trg, rc, kcu, mixed ? 3 : p2p_variant_);
   });
   HIPCHK(hipEventRecord(ev[3], s));
   // V list: tiled matrix-core M2L for scalar kernels on chunks of 16k
   // right-hand sides, the grouped (or per-pair) form otherwise
   const bool tiled = m2l_tile_ && !P.m2l_pairs && sm == 1 && tm == 1 && rc % 16 == 0;
-  const int ri = tiled ? 1 : 0, single = 0;
+  const int ri = tiled ? 1 : 0, single = tiled && mixed ? 1 : 0;
   for (const VLevel& vl : P.vlev) {
     timed(KIFMM3D_K_FFT_FWD, [&] {
       dev::fft_fwd(s, np_, vl.nsrc, I + vl.src_node, rc, upeden_.p, pUE, sm, D + vl.dscale, c2s_.p, effden_.p, single,
@@ -753,8 +756,8 @@ void Engine::run_chunk(const double* srcden, double* trgval, int rc, hipEvent_t*
     });
     timed(KIFMM3D_K_M2L, [&] {
       if (tiled)
-        dev::m2l_tiled(s, O.C, vl.ntile, rc, I + vl.ttile, I + vl.htile, I + P.optab, opsT_.p, effden_.p,
-                       effval_.p, single);
+        dev::m2l_tiled(s, O.C, vl.ntile, rc, I + vl.ttile, I + vl.htile, I + P.optab,
+                       single ? (const void*)opsT32_.p : (const void*)opsT_.p, effden_.p, effval_.p, single);
       else if (P.m2l_pairs)
         dev::m2l(s, sm, tm, O.C, vl.ntrg, rc, I + vl.vptr, I + vl.vsrc, I + vl.vop, (const double2*)m2l_.p,
                  (const double2*)effden_.p, (double2*)effval_.p);

@@ -114,7 +114,7 @@ class Engine {
   void upload_static();
   int choose_rhs_chunk(int nrhs);
   void ensure_work(int rc);
-  void run_chunk(const double* srcden, double* trgval, int rc, hipEvent_t* ev);
+  void run_chunk(const double* srcden, double* trgval, int rc, int mode, hipEvent_t* ev);
 
   KernelInfo user_, mm_;
   KernelConsts kcu_, kcm_;
@@ -130,6 +130,7 @@ class Engine {
   int m2l_rb_ = 0;   // right-hand sides per lane in the grouped M2L (0 = default; KIFMM3D_M2L_RB)
   int m2l_bs_ = 64;  // frequencies per block in the grouped M2L (64 or 256; KIFMM3D_M2L_BS)
   int p2p_variant_ = 2;  // near field: 0 LDS broadcast, 1 scalar-cache sources, 2 matrix cores (KIFMM3D_P2P)
+  bool gemm_mfma_ = true; // uncompensated GEMMs on the matrix cores (KIFMM3D_GEMM=valu disables)
   bool m2l_tile_ = true; // tiled matrix-core M2L for scalar kernels and 16-RHS chunks (KIFMM3D_M2L=group disables)
   hipStream_t stream_ = nullptr;
   std::vector<hipEvent_t> evpool_;

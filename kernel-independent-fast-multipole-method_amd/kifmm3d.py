@@ -62,6 +62,7 @@ SYMBOLS = [
 
 OK, EINVAL, EBOX, ESTATE, EDEVICE, ENOMEM = 0, -1, -2, -3, -4, -5
 MODE_PLAIN = 0
+MODE_MIXED = 1
 STAGES = ["load", "up", "near", "vlist", "xlist", "down", "eval", "save", "total"]
 KCLASSES = ["gather", "s2m", "gemm_up", "p2p", "fft_fwd", "m2l", "fft_inv", "s2l", "gemm_down", "s2p", "scatter"]
 

@@ -1,0 +1,63 @@
+"""GPU checks of the mixed-precision form (KIFMM3D_MODE_MIXED).
+
+The mixed form has no counterpart in the reference: it is bounded against the
+plain FP64 form on the same tree (the FP32 products must not move the result
+by more than MIXED_VS_PLAIN relative L2) and against the FP64 direct sum (the
+FMM error band of np).  Blocks of 16 right-hand sides take the FP32
+matrix-core kernels; the 5-RHS remainder of the 21-RHS case takes the FP64
+fallbacks, and other kernels take the FP64 kernels throughout.
+"""
+import numpy as np
+import pytest
+
+import kifmm3d as K
+import oracle as O
+from problems import COEFS, clusters, densities, relerr, sphere, uniform_cube
+
+pytestmark = pytest.mark.gpu
+
+MIXED_VS_PLAIN = 1e-6
+FMM_BAND = {4: 1e-2, 6: 2e-4, 8: 2e-5}
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _need_gpu():
+    if K.device_count() < 1:
+        pytest.fail("no HIP device visible: the GPU tests need an MI355X")
+
+
+def run(kt, np_, ptsmax, spos, snor, ctr, rootlvl, nrhs, seed):
+    print(f"\n[case] mixed kt={kt} np={np_} ptsmax={ptsmax} n={len(spos)} nrhs={nrhs}", flush=True)
+    f = K.FMM3d("fmm3d_")
+    f.srcpos, f.srcnor, f.trgpos = spos, snor, spos
+    f.ctr, f.rootlvl = ctr, rootlvl
+    f.knl = K.Kernel3d(kt, COEFS[kt])
+    f.setup({"-fmm3d_np": str(np_), "-fmm3d_let3d_ptsmax": str(ptsmax), "-fmm3d_let3d_maxlevel": "10"})
+    den = densities(len(spos), f.knl.sdof(), nrhs, seed)
+    plain = f.eval(den, nrhs=nrhs, mode=K.MODE_PLAIN).reshape(nrhs, -1).copy()
+    mixed = f.eval(den, nrhs=nrhs, mode=K.MODE_MIXED).reshape(nrhs, -1).copy()
+    assert np.all(np.isfinite(mixed))
+    worst = max(relerr(mixed[r], plain[r]) for r in range(nrhs))
+    rng = np.random.default_rng(seed + 5)
+    idx = rng.choice(len(spos), 100, replace=False)
+    tdof = f.knl.tdof()
+    d = O.direct(kt, COEFS[kt], spos, snor, den[0], spos[idx])
+    ferr = relerr(mixed[0].reshape(-1, tdof)[idx], d)
+    print(f"  mixed-vs-plain rel L2 {worst:.3e}   mixed fmm-vs-direct {ferr:.3e}", flush=True)
+    assert worst <= MIXED_VS_PLAIN
+    assert ferr <= FMM_BAND[np_]
+
+
+def test_mixed_laplace_16rhs_np6():
+    p = uniform_cube(60000, 21)
+    run(111, 6, 150, p, None, (0.5, 0.5, 0.5), 1, 16, 21)
+
+
+def test_mixed_laplace_21rhs_adaptive():
+    p = clusters(8000, 22)
+    run(111, 4, 60, p, None, (0.5, 0.5, 0.5), 1, 21, 22)
+
+
+def test_mixed_stokes_falls_back_to_fp64():
+    p, n = sphere(3000, 23)
+    run(321, 4, 40, p, n, (0.0, 0.0, 0.0), 0, 16, 23)
