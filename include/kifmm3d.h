@@ -149,6 +149,30 @@ int kifmm3d_plan_stats(const kifmm3d* h, double* out, int n);
 const char* kifmm3d_last_error(const kifmm3d* h);
 void kifmm3d_destroy(kifmm3d* h);
 
+/* Graph replay for repeated evaluations (no reference counterpart; for
+ * callers that evaluate the same buffers many times): with on != 0,
+ * kifmm3d_evaluate_device captures the whole pass into one hipGraph the first
+ * time it sees a (buffers, nrhs, mode) combination and afterwards only
+ * launches it.  Stage and kernel times then report the total only. */
+int kifmm3d_set_graph(kifmm3d* h, int on);
+
+/* Batched evaluation of independent problems (no reference counterpart; the
+ * reference evaluates one FMM3d at a time).  A batch groups set-up handles;
+ * kifmm3d_batch_evaluate_device evaluates handle i on d_srcden[i] ->
+ * d_trgval[i] with nrhs[i] right-hand sides, all in one hipGraph whose
+ * problems run on up to 8 concurrent branches.  The graph is captured on the
+ * first call and replayed while the pointers, nrhs and mode stay the same.
+ * The handles must outlive the batch and must not be evaluated on their own
+ * while a batch call runs. */
+typedef struct kifmm3d_batch kifmm3d_batch;
+int kifmm3d_batch_create(kifmm3d_batch** out, kifmm3d* const* handles, int n);
+int kifmm3d_batch_evaluate_device(kifmm3d_batch* b, const double* const* d_srcden, double* const* d_trgval,
+                                  const int* nrhs, int mode);
+/* Wall time (ms, HIP events) of the last batch launch. */
+int kifmm3d_batch_time(const kifmm3d_batch* b, double* ms);
+const char* kifmm3d_batch_last_error(const kifmm3d_batch* b);
+void kifmm3d_batch_destroy(kifmm3d_batch* b);
+
 /* Device memory helpers for callers without a HIP runtime of their own. */
 int kifmm3d_device_alloc(void** ptr, long bytes);
 int kifmm3d_device_free(void* ptr);

@@ -4,12 +4,19 @@
 #include <memory>
 #include <new>
 #include <string>
+#include <vector>
 
 #include "engine.hpp"
 #include "kifmm3d.h"
 
 struct kifmm3d {
   std::unique_ptr<kifmm::Engine> eng;
+  std::string err;
+};
+
+struct kifmm3d_batch {
+  std::vector<kifmm::Engine*> engines;
+  std::unique_ptr<kifmm::BatchRunner> run;
   std::string err;
 };
 
@@ -193,6 +200,71 @@ const char* kifmm3d_last_error(const kifmm3d* h)
 }
 
 void kifmm3d_destroy(kifmm3d* h) { delete h; }
+
+int kifmm3d_set_graph(kifmm3d* h, int on)
+{
+  if (!h) return KIFMM3D_EINVAL;
+  return guarded(h, [&] { h->eng->set_graph(on != 0); });
+}
+
+int kifmm3d_batch_create(kifmm3d_batch** out, kifmm3d* const* handles, int n)
+{
+  if (!out) return KIFMM3D_EINVAL;
+  *out = nullptr;
+  if (n < 1 || !handles) return KIFMM3D_EINVAL;
+  try {
+    auto b = std::make_unique<kifmm3d_batch>();
+    for (int i = 0; i < n; i++) {
+      if (!handles[i]) return KIFMM3D_EINVAL;
+      if (!handles[i]->eng->is_setup()) {
+        g_create_error = "batch handle " + std::to_string(i) + " is not set up";
+        return KIFMM3D_ESTATE;
+      }
+      b->engines.push_back(handles[i]->eng.get());
+    }
+    b->run = std::make_unique<kifmm::BatchRunner>(b->engines);
+    *out = b.release();
+    return KIFMM3D_OK;
+  } catch (const kifmm::Error& e) {
+    g_create_error = e.what();
+    return e.code;
+  } catch (const std::exception& e) {
+    g_create_error = e.what();
+    return KIFMM3D_EINVAL;
+  }
+}
+
+int kifmm3d_batch_evaluate_device(kifmm3d_batch* b, const double* const* d_srcden, double* const* d_trgval,
+                                  const int* nrhs, int mode)
+{
+  if (!b || !d_srcden || !d_trgval || !nrhs) return KIFMM3D_EINVAL;
+  try {
+    b->run->evaluate(d_srcden, d_trgval, nrhs, mode);
+    b->err.clear();
+    return KIFMM3D_OK;
+  } catch (const kifmm::Error& e) {
+    b->err = e.what();
+    return e.code;
+  } catch (const std::exception& e) {
+    b->err = e.what();
+    return KIFMM3D_EINVAL;
+  }
+}
+
+int kifmm3d_batch_time(const kifmm3d_batch* b, double* ms)
+{
+  if (!b || !ms) return KIFMM3D_EINVAL;
+  *ms = b->run->last_ms();
+  return KIFMM3D_OK;
+}
+
+const char* kifmm3d_batch_last_error(const kifmm3d_batch* b)
+{
+  if (!b) return g_create_error.c_str();
+  return b->err.c_str();
+}
+
+void kifmm3d_batch_destroy(kifmm3d_batch* b) { delete b; }
 
 int kifmm3d_device_alloc(void** ptr, long bytes)
 {

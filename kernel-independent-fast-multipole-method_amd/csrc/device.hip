@@ -157,6 +157,18 @@ static int grid_for(long tot, int bs)
   return (int)(g > 0 ? g : 1);
 }
 
+// Zero fill by kernel (also inside captured graphs, where memset nodes are avoided)
+__global__ void k_zero(long n, double* __restrict__ p)
+{
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) p[i] = 0.0;
+}
+
+void zero(hipStream_t s, double* p, long n)
+{
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_zero, dim3(grid_for(n, 256)), dim3(256), 0, s, n, p);
+}
+
 void gather(hipStream_t s, int n, int dof, int nr, const int* perm, const double* src, long src_rs, double* dst)
 {
   long tot = (long)n * dof * nr;
@@ -176,6 +188,16 @@ void scatter(hipStream_t s, int n, int dof, int nr, const int* perm, const doubl
 // near field (U list and direct W/X)
 constexpr int P2P_BS = 128;
 constexpr int TILE = 64;
+
+// Blocks are dealt to the 8 XCDs round-robin by block index; handing each XCD
+// a contiguous range of chunks instead keeps the source leaves that
+// neighbouring chunks share in one L2.  Grid x is 8 * ceil(n / 8).
+__device__ inline int xcd_item(int b, int n)
+{
+  const int per = (n + 7) / 8;
+  return (b & 7) * per + (b >> 3);
+}
+static inline unsigned xcd_grid(int n) { return 8u * (unsigned)((n + 7) / 8); }
 
 // Densities of sources j0 .. j0+cnt for NR right-hand sides into LDS,
 // s_den[jj][q*S + d]; den points at right-hand side r0 of the block.
@@ -311,14 +333,16 @@ __global__ __launch_bounds__(64) void k_p2p_s(const int* __restrict__ tbeg, cons
 typedef double v4d __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 
-__global__ __launch_bounds__(P2P_BS) void k_p2p_mfma111(const int* __restrict__ tbeg, const int* __restrict__ tnum,
-                                                        const int* __restrict__ eptr, const int2* __restrict__ ent,
-                                                        Points src, Targets trg, int roff, KC kc)
+__global__ __launch_bounds__(P2P_BS) void k_p2p_mfma111(int nchunk, const int* __restrict__ tbeg,
+                                                        const int* __restrict__ tnum, const int* __restrict__ eptr,
+                                                        const int2* __restrict__ ent, Points src, Targets trg, int roff,
+                                                        KC kc)
 {
   constexpr int NR = 16;
   __shared__ double s_pos[3][TILE];
   __shared__ double s_den[TILE][NR];
-  const int c = blockIdx.x, t = threadIdx.x, w = t >> 6, l = t & 63;
+  const int c = xcd_item(blockIdx.x, nchunk), t = threadIdx.x, w = t >> 6, l = t & 63;
+  if (c >= nchunk) return;
   const int col = l & 15, kk = l >> 4;
   const int r0 = roff + blockIdx.y * NR;
   const int tb = tbeg[c], tn = tnum[c];
@@ -383,14 +407,16 @@ __global__ __launch_bounds__(P2P_BS) void k_p2p_mfma111(const int* __restrict__ 
 // keeps the differences of nearby points to ~1e-7 of the leaf size; 1/r
 // comes from v_rsq_f32.  The FP32 products of one staged source tile (at most
 // 64 sources) are summed into FP64 accumulators.
-__global__ __launch_bounds__(P2P_BS) void k_p2p_mfma111_f32(const int* __restrict__ tbeg, const int* __restrict__ tnum,
-                                                            const int* __restrict__ eptr, const int2* __restrict__ ent,
-                                                            Points src, Targets trg, int roff, KC kc)
+__global__ __launch_bounds__(P2P_BS) void k_p2p_mfma111_f32(int nchunk, const int* __restrict__ tbeg,
+                                                            const int* __restrict__ tnum, const int* __restrict__ eptr,
+                                                            const int2* __restrict__ ent, Points src, Targets trg,
+                                                            int roff, KC kc)
 {
   constexpr int NR = 16;
   __shared__ float s_pos[3][TILE];
   __shared__ float s_den[TILE][NR];
-  const int c = blockIdx.x, t = threadIdx.x, w = t >> 6, l = t & 63;
+  const int c = xcd_item(blockIdx.x, nchunk), t = threadIdx.x, w = t >> 6, l = t & 63;
+  if (c >= nchunk) return;
   const int col = l & 15, kk = l >> 4;
   const int r0 = roff + blockIdx.y * NR;
   const int tb = tbeg[c], tn = tnum[c];
@@ -711,10 +737,11 @@ static void p2p_launch(hipStream_t s, int variant, int nchunk, int nb, const int
                        const int* eptr, const int2* ent, Points src, Targets trg, int roff, KC kc)
 {
   if (variant == 3 && KT == 111 && NR == 16)
-    hipLaunchKernelGGL(k_p2p_mfma111_f32, dim3(nchunk, nb), dim3(P2P_BS), 0, s, tbeg, tnum, eptr, ent, src, trg, roff,
-                       kc);
+    hipLaunchKernelGGL(k_p2p_mfma111_f32, dim3(xcd_grid(nchunk), nb), dim3(P2P_BS), 0, s, nchunk, tbeg, tnum, eptr,
+                       ent, src, trg, roff, kc);
   else if (variant >= 2 && KT == 111 && NR == 16 && src.den_rs == 1)
-    hipLaunchKernelGGL(k_p2p_mfma111, dim3(nchunk, nb), dim3(P2P_BS), 0, s, tbeg, tnum, eptr, ent, src, trg, roff, kc);
+    hipLaunchKernelGGL(k_p2p_mfma111, dim3(xcd_grid(nchunk), nb), dim3(P2P_BS), 0, s, nchunk, tbeg, tnum, eptr, ent,
+                       src, trg, roff, kc);
   else if (variant == 1)
     hipLaunchKernelGGL((k_p2p_s<KT, NR>), dim3(2 * nchunk, nb), dim3(64), 0, s, tbeg, tnum, eptr, ent, src, trg, roff,
                        kc);
@@ -1248,15 +1275,16 @@ __global__ __launch_bounds__(256) void k_fft_inv_ri(const int* __restrict__ slot
                                                     const double2* __restrict__ tw)
 {
   constexpr int N = 2 * NP, H = NP + 1, C = N * N * H;
-  __shared__ double2 s_x[R][C];       // spectra, later Y2[k][j][kx]
+  __shared__ CT s_x[R][C];            // spectra as stored
   __shared__ double2 Y1[NP * N * H];  // [k][ky][kx]
+  __shared__ double2 Y2[NP * NP * H]; // [k][j][kx]
   __shared__ double2 s_w[N];
   const int slot = blockIdx.x, r0 = blockIdx.y * R, t = threadIdx.x;
   if (t < N) s_w[t] = tw[t];
   const CT* in = effval + (long)slot * C * Rc + r0;
   for (int e = t; e < C * R; e += 256) {
     const int o = e / R, rr = e - o * R;
-    s_x[rr][o] = cload(in + (long)o * Rc + rr);
+    s_x[rr][o] = in[(long)o * Rc + rr];
   }
   for (int rr = 0; rr < R; rr++) {
     __syncthreads();
@@ -1266,7 +1294,7 @@ __global__ __launch_bounds__(256) void k_fft_inv_ri(const int* __restrict__ slot
 #pragma unroll
       for (int kz = 0; kz < N; kz++) {
         double2 w = s_w[((k + NP / 2) * kz) % N];
-        double2 v = cmul(s_x[rr][(kz * N + ky) * H + kx], make_double2(w.x, -w.y));
+        double2 v = cmul(cload(&s_x[rr][(kz * N + ky) * H + kx]), make_double2(w.x, -w.y));
         acc.x += v.x;
         acc.y += v.y;
       }
@@ -1283,14 +1311,14 @@ __global__ __launch_bounds__(256) void k_fft_inv_ri(const int* __restrict__ slot
         acc.x += v.x;
         acc.y += v.y;
       }
-      s_x[rr][o] = acc;
+      Y2[o] = acc;
     }
     __syncthreads();
     double* out = dncval + ((long)slot_node[slot] * Rc + r0 + rr) * pln;
     for (int p = t; p < nsp; p += 256) {  // along x (c2r) at the surface points
       int cc = s2c[p];
       int i = cc / (NP * NP), j = (cc / NP) % NP, k = cc % NP;
-      const double2* y = &s_x[rr][(k * NP + j) * H];
+      const double2* y = &Y2[(k * NP + j) * H];
       int xi = i + NP / 2;
       double v = y[0].x + ((xi & 1) ? -y[NP].x : y[NP].x);
 #pragma unroll
@@ -1335,7 +1363,7 @@ static void fft_inv_t(hipStream_t s, int np, int nslot, const int* slot_node, in
     switch (np) {
     case 4: hipLaunchKernelGGL((k_fft_inv_ri<4, CT, RW>), dim3(nslot, Rc / RW), dim3(256), 0, s, slot_node, Rc, effval, surf2cube, nsp, dncval, pln, tw); break;
     case 6: hipLaunchKernelGGL((k_fft_inv_ri<6, CT, RW>), dim3(nslot, Rc / RW), dim3(256), 0, s, slot_node, Rc, effval, surf2cube, nsp, dncval, pln, tw); break;
-    case 8: hipLaunchKernelGGL((k_fft_inv_ri<8, CT, RW / 4>), dim3(nslot, Rc / (RW / 4)), dim3(256), 0, s, slot_node, Rc, effval, surf2cube, nsp, dncval, pln, tw); break;
+    case 8: hipLaunchKernelGGL((k_fft_inv_ri<8, CT, RW / 2>), dim3(nslot, Rc / (RW / 2)), dim3(256), 0, s, slot_node, Rc, effval, surf2cube, nsp, dncval, pln, tw); break;
     }
     return;
   }
@@ -1611,8 +1639,10 @@ constexpr int M2LT_BS = 512;  // 8 waves: one per target parent of the tile
 constexpr int M2LT_FPB = 16;  // frequencies per block, visited in a software-pipelined loop
 constexpr int NOPS = 343;     // operator table entries ((o+3) in [0,7)^3)
 
+// (FP32: two blocks per CU fit the LDS, so registers are capped to match: the
+// second bound is waves per SIMD)
 template <class R>
-__global__ __launch_bounds__(M2LT_BS) void k_m2l_tile(int C, int Rc, int ntile, const int* __restrict__ ttile,
+__global__ __launch_bounds__(M2LT_BS, sizeof(R) == 4 ? 4 : 2) void k_m2l_tile(int C, int Rc, int ntile, const int* __restrict__ ttile,
                                                       const int* __restrict__ htile, const int* __restrict__ optab,
                                                       const typename Mt<R>::C2* __restrict__ opsT,
                                                       const typename Mt<R>::C2* __restrict__ effden,
@@ -1639,20 +1669,19 @@ __global__ __launch_bounds__(M2LT_BS) void k_m2l_tile(int C, int Rc, int ntile, 
     for (int c = 0; c < 8; c++) m |= hs[t * 8 + c] >= 0;
     s_hm[t] = m;
   }
-  // staging: thread t moves entries e = it*512 + t (row e/16 = halo child, rhs e%16)
-  long src[16];
+  // staging: thread t moves entries e = it*512 + t, i.e. halo child row
+  // it*32 + t/16 and right-hand side t%16
+  int hsl[16];
 #pragma unroll
-  for (int it = 0; it < 16; it++) {
-    const int e = it * M2LT_BS + t, sl = hs[e >> 4];
-    src[it] = sl >= 0 ? (long)sl * C * Rc + r0 + (e & 15) : -1;
-  }
+  for (int it = 0; it < 16; it++) hsl[it] = hs[it * 32 + (t >> 4)];
+  const C2* eb = effden + r0 + (t & 15);
   C2 pre[16], pop;
   auto fetch = [&](int f) {
 #pragma unroll
     for (int it = 0; it < 16; it++) {
       pre[it].x = 0;
       pre[it].y = 0;
-      if (src[it] >= 0) pre[it] = effden[src[it] + (long)f * Rc];
+      if (hsl[it] >= 0) pre[it] = eb[((long)hsl[it] * C + f) * Rc];
     }
     pop.x = 0;
     pop.y = 0;
@@ -1684,43 +1713,44 @@ __global__ __launch_bounds__(M2LT_BS) void k_m2l_tile(int C, int Rc, int ntile, 
     if (f + 1 < f1) fetch(f + 1);  // in flight while the matrix cores work on f
     if (!live) continue;
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    V4 d = {0, 0, 0, 0};
-    int n = 0;
-    for (int dx = -1; dx <= 1; dx++)
-      for (int dy = -1; dy <= 1; dy++)
-        for (int dz = -1; dz <= 1; dz++) {
-          if (!(dx || dy || dz)) continue;
-          const int h = ((px + 1 + dx) * 4 + (py + 1 + dy)) * 4 + (pz + 1 + dz);
-          if (s_hm[h]) {
-            const int o0 = s_opt[(n * 8 + k) * 8 + ca], o1 = s_opt[(n * 8 + 4 + k) * 8 + ca];
-            C2 m0, m1;
-            m0.x = m0.y = m1.x = m1.y = 0;
-            if (o0 >= 0) m0 = s_op[o0];
-            if (o1 >= 0) m1 = s_op[o1];
-            const R a0 = pim ? m0.y : m0.x, a1 = pim ? m1.y : m1.x;
-            const R a2 = pim ? m0.x : -m0.y, a3 = pim ? m1.x : -m1.y;
-            const C2 x0 = xs[(h * 8 + k) * 16 + i], x1 = xs[(h * 8 + 4 + k) * 16 + i];
-            if (Mt<R>::KV) {
-              d = Mt<R>::mfma(a0, x0.x, d);
-              d = Mt<R>::mfma(a1, x1.x, d);
-              d = Mt<R>::mfma(a2, x0.y, d);
-              d = Mt<R>::mfma(a3, x1.y, d);
-            } else {
-              // single precision: one direction per FP32 product, summed in FP64
-              V4 e = {0, 0, 0, 0};
-              e = Mt<R>::mfma(a0, x0.x, e);
-              e = Mt<R>::mfma(a1, x1.x, e);
-              e = Mt<R>::mfma(a2, x0.y, e);
-              e = Mt<R>::mfma(a3, x1.y, e);
+    V4 d0 = {0, 0, 0, 0}, d1 = {0, 0, 0, 0};  // two chains for matrix-core overlap
+    const int hc = ((px + 1) * 4 + (py + 1)) * 4 + (pz + 1);
+    // FP64 measured faster fully unrolled; FP32 needs the registers for 4 waves/SIMD
+    constexpr int UNR = Mt<R>::KV ? 26 : 2;
+#pragma unroll UNR
+    for (int n = 0; n < 26; n++) {
+      // direction n in (dx, dy, dz) order with dz fastest, the centre skipped
+      const int m = n + (n >= 13);
+      const int h = hc + (m / 9 - 1) * 16 + ((m / 3) % 3 - 1) * 4 + (m % 3 - 1);
+      if (s_hm[h]) {
+        const int o0 = s_opt[(n * 8 + k) * 8 + ca], o1 = s_opt[(n * 8 + 4 + k) * 8 + ca];
+        C2 m0, m1;
+        m0.x = m0.y = m1.x = m1.y = 0;
+        if (o0 >= 0) m0 = s_op[o0];
+        if (o1 >= 0) m1 = s_op[o1];
+        const R a0 = pim ? m0.y : m0.x, a1 = pim ? m1.y : m1.x;
+        const R a2 = pim ? m0.x : -m0.y, a3 = pim ? m1.x : -m1.y;
+        const C2 x0 = xs[(h * 8 + k) * 16 + i], x1 = xs[(h * 8 + 4 + k) * 16 + i];
+        if (Mt<R>::KV) {
+          d0 = Mt<R>::mfma(a0, x0.x, d0);
+          d1 = Mt<R>::mfma(a2, x0.y, d1);
+          d0 = Mt<R>::mfma(a1, x1.x, d0);
+          d1 = Mt<R>::mfma(a3, x1.y, d1);
+        } else {
+          // single precision: one direction per FP32 product, summed in FP64
+          V4 e0 = {0, 0, 0, 0}, e1 = {0, 0, 0, 0};
+          e0 = Mt<R>::mfma(a0, x0.x, e0);
+          e1 = Mt<R>::mfma(a2, x0.y, e1);
+          e0 = Mt<R>::mfma(a1, x1.x, e0);
+          e1 = Mt<R>::mfma(a3, x1.y, e1);
 #pragma unroll
-              for (int v = 0; v < 4; v++) acc[v] += (double)e[v];
-            }
-          }
-          n++;
+          for (int v = 0; v < 4; v++) acc[v] += (double)(e0[v] + e1[v]);
         }
+      }
+    }
     if (Mt<R>::KV) {
 #pragma unroll
-      for (int v = 0; v < 4; v++) acc[v] = (double)d[v];
+      for (int v = 0; v < 4; v++) acc[v] = (double)d0[v] + (double)d1[v];
     }
 #pragma unroll
     for (int h2 = 0; h2 < 2; h2++) {

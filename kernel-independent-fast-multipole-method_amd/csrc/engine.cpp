@@ -97,10 +97,12 @@ Engine::Engine(int kt, const double* coefs, int ncoefs, int np, int ptsmax, int 
   if (const char* v = std::getenv("KIFMM3D_P2P")) p2p_variant_ = std::atoi(v);
   if (const char* g = std::getenv("KIFMM3D_GEMM")) gemm_mfma_ = std::string(g) != "valu";
   if (const char* m = std::getenv("KIFMM3D_M2L")) m2l_tile_ = std::string(m) != "group" && std::string(m) != "pair";
+  if (const char* gph = std::getenv("KIFMM3D_GRAPH")) use_graph_ = std::atoi(gph) != 0;
 }
 
 Engine::~Engine()
 {
+  drop_graph();
   for (hipEvent_t e : evpool_) (void)hipEventDestroy(e);
   if (stream_) (void)hipStreamDestroy(stream_);
 }
@@ -618,24 +620,90 @@ void Engine::ensure_work(int rc)
   effval_.ensure((size_t)2 * rc * plan_.max_vtrg * mm_.tdof * O.C);
 }
 
-void Engine::evaluate_device(const double* d_srcden, double* d_trgval, int nrhs, int mode)
+void Engine::check_eval_args(const double* d_srcden, const double* d_trgval, int nrhs, int mode) const
 {
   if (!setup_done_) throw Error(KIFMM3D_ESTATE, "evaluate called before setup");
   if (nrhs < 1) throw Error(KIFMM3D_EINVAL, "nrhs must be >= 1");
   if (mode != KIFMM3D_MODE_PLAIN && mode != KIFMM3D_MODE_MIXED) throw Error(KIFMM3D_EINVAL, "unsupported evaluation mode");
   if ((ns_ > 0 && !d_srcden) || (nt_ > 0 && !d_trgval)) throw Error(KIFMM3D_EINVAL, "null density or value pointer");
+}
+
+int Engine::prepare(int nrhs)
+{
+  const int rc = choose_rhs_chunk(nrhs);
+  ensure_work(rc);
+  return rc;
+}
+
+void Engine::record(const double* d_srcden, double* d_trgval, int nrhs, int rc, int mode, hipStream_t s)
+{
+  if (nt_ == 0) return;
+  for (int r0 = 0; r0 < nrhs; r0 += rc) {
+    const int n = std::min(rc, nrhs - r0);
+    run_chunk(d_srcden + (size_t)r0 * ns_ * user_.sdof, d_trgval + (size_t)r0 * nt_ * user_.tdof, n, mode, s, nullptr);
+    last_rc_ = n;
+  }
+}
+
+void Engine::drop_graph()
+{
+  if (gexec_) (void)hipGraphExecDestroy(gexec_);
+  gexec_ = nullptr;
+}
+
+void Engine::evaluate_device(const double* d_srcden, double* d_trgval, int nrhs, int mode)
+{
+  check_eval_args(d_srcden, d_trgval, nrhs, mode);
   for (double& m : stage_ms) m = 0.0;
   for (double& m : kernel_ms) m = 0.0;
   for (int& c : kernel_calls) c = 0;
   if (nt_ == 0) return;
-  int rc = choose_rhs_chunk(nrhs);
-  ensure_work(rc);
+  int rc = prepare(nrhs);
+  if (use_graph_) {
+    // one captured graph per (buffers, nrhs, mode); replayed while they stay the same
+    if (!gexec_ || gsrc_ != d_srcden || gtrg_ != d_trgval || gnrhs_ != nrhs || gmode_ != mode || grc_ != rc) {
+      drop_graph();
+      HIPCHK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+      hipGraph_t g = nullptr;
+      try {
+        record(d_srcden, d_trgval, nrhs, rc, mode, stream_);
+      } catch (...) {
+        (void)hipStreamEndCapture(stream_, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+      }
+      HIPCHK(hipStreamEndCapture(stream_, &g));
+      hipError_t e = hipGraphInstantiate(&gexec_, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (e != hipSuccess) {
+        gexec_ = nullptr;
+        throw Error(KIFMM3D_EDEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+      }
+      gsrc_ = d_srcden; gtrg_ = d_trgval; gnrhs_ = nrhs; gmode_ = mode; grc_ = rc;
+    }
+    hipEvent_t a, b;
+    HIPCHK(hipEventCreate(&a));
+    HIPCHK(hipEventCreate(&b));
+    float ms = 0;
+    hipError_t e = hipEventRecord(a, stream_);
+    if (e == hipSuccess) e = hipGraphLaunch(gexec_, stream_);
+    if (e == hipSuccess) e = hipEventRecord(b, stream_);
+    if (e == hipSuccess) e = hipEventSynchronize(b);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (e != hipSuccess) throw Error(KIFMM3D_EDEVICE, std::string("graph launch: ") + hipGetErrorString(e));
+    stage_ms[KIFMM3D_STAGE_TOTAL] = ms;
+    last_rc_ = std::min(rc, nrhs - (nrhs - 1) / rc * rc);
+    return;
+  }
   hipEvent_t ev[9];
   for (auto& e : ev) HIPCHK(hipEventCreate(&e));
   try {
     for (int r0 = 0; r0 < nrhs; r0 += rc) {
       int n = std::min(rc, nrhs - r0);
-      run_chunk(d_srcden + (size_t)r0 * ns_ * user_.sdof, d_trgval + (size_t)r0 * nt_ * user_.tdof, n, mode, ev);
+      run_chunk(d_srcden + (size_t)r0 * ns_ * user_.sdof, d_trgval + (size_t)r0 * nt_ * user_.tdof, n, mode, stream_,
+                ev);
       HIPCHK(hipEventSynchronize(ev[8]));
       for (int s = 0; s < 8; s++) {
         float ms = 0;
@@ -672,13 +740,12 @@ void Engine::evaluate_host(const double* srcden, double* trgval, int nrhs, int m
   if (nt) HIPCHK(hipMemcpy(trgval, htrg_.p, nt * sizeof(double), hipMemcpyDeviceToHost));
 }
 
-void Engine::run_chunk(const double* srcden, double* trgval, int rc, int mode, hipEvent_t* ev)
+void Engine::run_chunk(const double* srcden, double* trgval, int rc, int mode, hipStream_t s, hipEvent_t* ev)
 {
   const bool mixed = mode == KIFMM3D_MODE_MIXED;
   const Operators& O = *ops_;
   const Tree& T = tree_;
   const Plan& P = plan_;
-  hipStream_t s = stream_;
   const int* I = I_.p;
   const double* D = D_.p;
   const int sdof = user_.sdof, tdof = user_.tdof, sm = mm_.sdof, tm = mm_.tdof;
@@ -704,9 +771,17 @@ void Engine::run_chunk(const double* srcden, double* trgval, int rc, int mode, h
     dev::gemm(s, a);
   };
 
-  // kernel-class timing: events around each launch group on this stream
+  // stage and kernel-class timing: events on this stream (none when ev is
+  // null, e.g. while the sequence is captured into a graph)
+  auto mark = [&](int k) {
+    if (ev) HIPCHK(hipEventRecord(ev[k], s));
+  };
   size_t nev = 0;
   auto timed = [&](int cls, auto&& fn) {
+    if (!ev) {
+      fn();
+      return;
+    }
     if (evpool_.size() < nev + 2) {
       hipEvent_t a, b;
       HIPCHK(hipEventCreate(&a));
@@ -722,29 +797,29 @@ void Engine::run_chunk(const double* srcden, double* trgval, int rc, int mode, h
     nev += 2;
   };
 
-  HIPCHK(hipEventRecord(ev[0], s));
+  mark(0);
   // zero the accumulated buffers (fmm3d_eval.cpp:33-42)
-  HIPCHK(hipMemsetAsync(tval_.p, 0, sizeof(double) * (size_t)rc * trs, s));
-  HIPCHK(hipMemsetAsync(upcval_.p, 0, sizeof(double) * (size_t)rc * T.sndecnt * pUC, s));
-  HIPCHK(hipMemsetAsync(dncval_.p, 0, sizeof(double) * (size_t)rc * T.tndecnt * pDC, s));
-  if (P.nlow_s) HIPCHK(hipMemsetAsync(upeden_.p, 0, sizeof(double) * (size_t)rc * P.nlow_s * pUE, s));
-  if (P.nlow_t) HIPCHK(hipMemsetAsync(dneden_.p, 0, sizeof(double) * (size_t)rc * P.nlow_t * pDE, s));
+  dev::zero(s, tval_.p, (long)rc * trs);
+  dev::zero(s, upcval_.p, (long)rc * T.sndecnt * pUC);
+  dev::zero(s, dncval_.p, (long)rc * T.tndecnt * pDC);
+  if (P.nlow_s) dev::zero(s, upeden_.p, (long)rc * P.nlow_s * pUE);
+  if (P.nlow_t) dev::zero(s, dneden_.p, (long)rc * P.nlow_t * pDE);
   // load (fmm3d_eval.cpp:50-64)
   timed(KIFMM3D_K_GATHER, [&] { dev::gather(s, T.sextcnt, sdof, rc, sperm_.p, srcden, (long)ns_ * sdof, sden_.p); });
-  HIPCHK(hipEventRecord(ev[1], s));
+  mark(1);
   // up: S2M, then M2M + UC2UE level by level
   timed(KIFMM3D_K_S2M, [&] {
     dev::p2s(s, user_.kt, P.s2m.n, I + P.s2m.out, (const double4*)(D + P.s2m.geom), I + P.s2m.eptr,
              (const int2*)(I + P.s2m.ent), unit_[SURF_UC].p, O.nsp[SURF_UC], src, upcval_.p, rc, pUC, rc, kcu);
   });
   for (const GemmOp& op : P.up) timed(KIFMM3D_K_GEMM_UP, [&] { run_gemm(op); });
-  HIPCHK(hipEventRecord(ev[2], s));
+  mark(2);
   // U list and direct W/X
   timed(KIFMM3D_K_P2P, [&] {
     dev::p2p(s, user_.kt, P.near.n, I + P.near.tbeg, I + P.near.tnum, I + P.near.eptr,
              (const int2*)(I + P.near.ent), src, trg, rc, kcu, mixed ? 3 : p2p_variant_);
   });
-  HIPCHK(hipEventRecord(ev[3], s));
+  mark(3);
   // V list: tiled matrix-core M2L for scalar kernels on chunks of 16k
   // right-hand sides, the grouped (or per-pair) form otherwise
   const bool tiled = m2l_tile_ && !P.m2l_pairs && sm == 1 && tm == 1 && rc % 16 == 0;
@@ -770,28 +845,28 @@ void Engine::run_chunk(const double* srcden, double* trgval, int rc, int mode, h
                    dncval_.p, pDC, (const double2*)tw_.p);
     });
   }
-  HIPCHK(hipEventRecord(ev[4], s));
+  mark(4);
   // X list S2L
   timed(KIFMM3D_K_S2L, [&] {
     dev::p2s(s, user_.kt, P.s2l.n, I + P.s2l.out, (const double4*)(D + P.s2l.geom), I + P.s2l.eptr,
              (const int2*)(I + P.s2l.ent), unit_[SURF_DC].p, O.nsp[SURF_DC], src, dncval_.p, rc, pDC, rc, kcu);
   });
-  HIPCHK(hipEventRecord(ev[5], s));
+  mark(5);
   // down: L2L + DC2DE level by level
   for (const GemmOp& op : P.down) timed(KIFMM3D_K_GEMM_DOWN, [&] { run_gemm(op); });
-  HIPCHK(hipEventRecord(ev[6], s));
+  mark(6);
   // L2T + M2T
   timed(KIFMM3D_K_S2P, [&] {
     dev::s2p(s, mm_.kt, P.far.n, I + P.far.tbeg, I + P.far.tnum, I + P.far.eptr, I + P.far.enode,
              I + P.far.ekind, (const double4*)(D + P.far.egeom), unit_[SURF_DE].p, unit_[SURF_UE].p,
              O.nsp[SURF_DE], dneden_.p, upeden_.p, rc, pDE, trg, rc, kcm);
   });
-  HIPCHK(hipEventRecord(ev[7], s));
+  mark(7);
   // save (fmm3d_eval.cpp:221-235)
   timed(KIFMM3D_K_SCATTER, [&] { dev::scatter(s, T.textcnt, tdof, rc, tperm_.p, tval_.p, trs, trgval, (long)nt_ * tdof); });
   HIPCHK(hipGetLastError());
-  HIPCHK(hipEventRecord(ev[8], s));
-  nev_used_ = nev;
+  mark(8);
+  if (ev) nev_used_ = nev;
 }
 
 long Engine::get_buffer(int what, double* buf, long n)
@@ -846,6 +921,89 @@ void Engine::direct(const double* srcden, bool on_device, int nrhs, const int* i
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(stream_));
   HIPCHK(hipMemcpy(out, dout_.p, sizeof(double) * (size_t)nrhs * nchk * tdof, hipMemcpyDeviceToHost));
+}
+
+BatchRunner::BatchRunner(const std::vector<Engine*>& engines) : eng_(engines)
+{
+  const int nb = (int)std::min<size_t>(8, eng_.size());
+  br_.resize(nb, nullptr);
+  for (auto& s : br_) HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  fork_join_.resize(2 * nb, nullptr);
+  for (auto& e : fork_join_) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+}
+
+BatchRunner::~BatchRunner()
+{
+  drop();
+  for (auto s : br_)
+    if (s) (void)hipStreamDestroy(s);
+  for (auto e : fork_join_)
+    if (e) (void)hipEventDestroy(e);
+}
+
+void BatchRunner::drop()
+{
+  if (exec_) (void)hipGraphExecDestroy(exec_);
+  exec_ = nullptr;
+}
+
+void BatchRunner::evaluate(const double* const* d_srcden, double* const* d_trgval, const int* nrhs, int mode)
+{
+  const size_t n = eng_.size();
+  std::vector<int> rc(n);
+  for (size_t i = 0; i < n; i++) {
+    eng_[i]->check_eval_args(d_srcden[i], d_trgval[i], nrhs[i], mode);
+    rc[i] = eng_[i]->prepare(nrhs[i]);  // allocations happen here, outside the capture
+  }
+  bool same = exec_ && kmode_ == mode;
+  for (size_t i = 0; same && i < n; i++)
+    same = ksrc_[i] == d_srcden[i] && ktrg_[i] == d_trgval[i] && knrhs_[i] == nrhs[i] && krc_[i] == rc[i];
+  hipStream_t s0 = br_[0];
+  if (!same) {
+    drop();
+    HIPCHK(hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal));
+    hipGraph_t g = nullptr;
+    try {
+      const int nb = (int)br_.size();
+      HIPCHK(hipEventRecord(fork_join_[0], s0));
+      for (int k = 1; k < nb; k++) HIPCHK(hipStreamWaitEvent(br_[k], fork_join_[0], 0));
+      for (size_t i = 0; i < n; i++)
+        eng_[i]->record(d_srcden[i], d_trgval[i], nrhs[i], rc[i], mode, br_[i % nb]);
+      for (int k = 1; k < nb; k++) {
+        HIPCHK(hipEventRecord(fork_join_[nb + k], br_[k]));
+        HIPCHK(hipStreamWaitEvent(s0, fork_join_[nb + k], 0));
+      }
+    } catch (...) {
+      (void)hipStreamEndCapture(s0, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    HIPCHK(hipStreamEndCapture(s0, &g));
+    hipError_t e = hipGraphInstantiate(&exec_, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+      exec_ = nullptr;
+      throw Error(KIFMM3D_EDEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    }
+    ksrc_.assign(d_srcden, d_srcden + n);
+    ktrg_.assign(d_trgval, d_trgval + n);
+    knrhs_.assign(nrhs, nrhs + n);
+    krc_ = rc;
+    kmode_ = mode;
+  }
+  hipEvent_t a, b;
+  HIPCHK(hipEventCreate(&a));
+  HIPCHK(hipEventCreate(&b));
+  float ms = 0;
+  hipError_t e = hipEventRecord(a, s0);
+  if (e == hipSuccess) e = hipGraphLaunch(exec_, s0);
+  if (e == hipSuccess) e = hipEventRecord(b, s0);
+  if (e == hipSuccess) e = hipEventSynchronize(b);
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  if (e != hipSuccess) throw Error(KIFMM3D_EDEVICE, std::string("batch graph launch: ") + hipGetErrorString(e));
+  last_ms_ = ms;
 }
 
 }  // namespace kifmm

@@ -94,6 +94,21 @@ class Engine {
   void setup(const double* spos, const double* snor, int ns, const double* tpos, int nt);
   void evaluate_device(const double* d_srcden, double* d_trgval, int nrhs, int mode);
   void evaluate_host(const double* srcden, double* trgval, int nrhs, int mode);
+  // Graph replay: with it on, evaluate_device captures the whole pass into a
+  // hipGraph on first use for a given (buffers, nrhs, mode) and then only
+  // launches it (stage_times then reports the total only).
+  void set_graph(bool on)
+  {
+    use_graph_ = on;
+    if (!on) drop_graph();
+  }
+  bool graph() const { return use_graph_; }
+  // Pieces of evaluate_device for callers that build their own graph (the
+  // batch interface): argument checks, work buffers for nrhs (returns the RHS
+  // chunk), and the whole pass enqueued on stream s without timing or sync.
+  void check_eval_args(const double* d_srcden, const double* d_trgval, int nrhs, int mode) const;
+  int prepare(int nrhs);
+  void record(const double* d_srcden, double* d_trgval, int nrhs, int rc, int mode, hipStream_t s);
 
   const Tree& tree() const { return tree_; }
   const Operators& ops() const { return *ops_; }
@@ -114,7 +129,8 @@ class Engine {
   void upload_static();
   int choose_rhs_chunk(int nrhs);
   void ensure_work(int rc);
-  void run_chunk(const double* srcden, double* trgval, int rc, int mode, hipEvent_t* ev);
+  void run_chunk(const double* srcden, double* trgval, int rc, int mode, hipStream_t s, hipEvent_t* ev);
+  void drop_graph();
 
   KernelInfo user_, mm_;
   KernelConsts kcu_, kcm_;
@@ -133,6 +149,11 @@ class Engine {
   bool gemm_mfma_ = true; // uncompensated GEMMs on the matrix cores (KIFMM3D_GEMM=valu disables)
   bool m2l_tile_ = true; // tiled matrix-core M2L for scalar kernels and 16-RHS chunks (KIFMM3D_M2L=group disables)
   hipStream_t stream_ = nullptr;
+  bool use_graph_ = false;  // KIFMM3D_GRAPH
+  hipGraphExec_t gexec_ = nullptr;
+  const double* gsrc_ = nullptr;
+  double* gtrg_ = nullptr;
+  int gnrhs_ = 0, gmode_ = -1, grc_ = 0;
   std::vector<hipEvent_t> evpool_;
   std::vector<int> evcls_;
   size_t nev_used_ = 0;
@@ -153,6 +174,30 @@ class Engine {
   DVec<double> hsrc_, htrg_;  // staging for the host-pointer entry point
   DVec<double> chk_, part_, dout_;  // direct-sum scratch
   std::vector<double> tpos_host_;  // caller-order target positions (for direct sums)
+};
+
+// Several set-up engines evaluated in one captured graph: problem i runs on
+// branch i % nbranch (forked from and joined back to one capture stream), so
+// independent problems overlap on the GPU.  The graph is kept while the
+// buffers, right-hand-side counts and mode stay the same.
+class BatchRunner {
+ public:
+  explicit BatchRunner(const std::vector<Engine*>& engines);
+  ~BatchRunner();
+  void evaluate(const double* const* d_srcden, double* const* d_trgval, const int* nrhs, int mode);
+  double last_ms() const { return last_ms_; }
+
+ private:
+  void drop();
+  std::vector<Engine*> eng_;
+  std::vector<hipStream_t> br_;
+  std::vector<hipEvent_t> fork_join_;
+  hipGraphExec_t exec_ = nullptr;
+  std::vector<const double*> ksrc_;
+  std::vector<double*> ktrg_;
+  std::vector<int> knrhs_, krc_;
+  int kmode_ = -1;
+  double last_ms_ = 0.0;
 };
 
 }  // namespace kifmm

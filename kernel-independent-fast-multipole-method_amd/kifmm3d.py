@@ -58,6 +58,12 @@ SYMBOLS = [
     ("kifmm3d_memcpy_d2h", ctypes.c_int, [_VP, _VP, ctypes.c_long]),
     ("kifmm3d_device_synchronize", ctypes.c_int, []),
     ("kifmm3d_device_count", ctypes.c_int, [_I]),
+    ("kifmm3d_set_graph", ctypes.c_int, [_VP, ctypes.c_int]),
+    ("kifmm3d_batch_create", ctypes.c_int, [ctypes.POINTER(_VP), ctypes.POINTER(_VP), ctypes.c_int]),
+    ("kifmm3d_batch_evaluate_device", ctypes.c_int, [_VP, ctypes.POINTER(_VP), ctypes.POINTER(_VP), _I, ctypes.c_int]),
+    ("kifmm3d_batch_time", ctypes.c_int, [_VP, _D]),
+    ("kifmm3d_batch_last_error", ctypes.c_char_p, [_VP]),
+    ("kifmm3d_batch_destroy", None, [_VP]),
 ]
 
 OK, EINVAL, EBOX, ESTATE, EDEVICE, ENOMEM = 0, -1, -2, -3, -4, -5
@@ -252,6 +258,10 @@ class FMM3d:
         """Evaluate with densities and values already resident in HBM."""
         self._check(lib().kifmm3d_evaluate_device(self.h, d_srcden.ptr, d_trgval.ptr, nrhs, mode))
 
+    def set_graph(self, on: bool = True):
+        """Capture the evaluate pass into a hipGraph and replay it on repeat calls."""
+        self._check(lib().kifmm3d_set_graph(self.h, 1 if on else 0))
+
     def destroy(self):
         if getattr(self, "h", None):
             lib().kifmm3d_destroy(self.h)
@@ -361,3 +371,41 @@ class FMM3d:
         if r < 0:
             self._check(int(r))
         return out[:n]
+
+
+class Batch:
+    """Independent set-up FMM3d problems evaluated together in one hipGraph
+    (kifmm3d_batch_*); the handles must outlive the batch."""
+
+    def __init__(self, fmms):
+        self.fmms = list(fmms)
+        arr = (_VP * len(self.fmms))(*[f.h for f in self.fmms])
+        self.h = _VP()
+        rc = lib().kifmm3d_batch_create(ctypes.byref(self.h), arr, len(self.fmms))
+        if rc != 0:
+            raise KifmmError(rc, lib().kifmm3d_batch_last_error(None).decode())
+
+    def eval_device(self, d_srcden, d_trgval, nrhs, mode: int = MODE_PLAIN):
+        n = len(self.fmms)
+        src = (_VP * n)(*[b.ptr for b in d_srcden])
+        trg = (_VP * n)(*[b.ptr for b in d_trgval])
+        nr = (ctypes.c_int * n)(*[int(r) for r in nrhs])
+        rc = lib().kifmm3d_batch_evaluate_device(self.h, src, trg, nr, mode)
+        if rc != 0:
+            raise KifmmError(rc, lib().kifmm3d_batch_last_error(self.h).decode())
+
+    def time(self) -> float:
+        ms = ctypes.c_double(0)
+        lib().kifmm3d_batch_time(self.h, ctypes.byref(ms))
+        return ms.value
+
+    def destroy(self):
+        if getattr(self, "h", None):
+            lib().kifmm3d_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
