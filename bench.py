@@ -40,6 +40,13 @@ CONFIGS = {
 }
 DEFAULT = "laplace-uniform-4m-16rhs"
 
+# Memory traffic (bytes per evaluate, L2 <-> fabric, calibrated against a streaming copy) of a
+# kernel class, measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE; the commands and the
+# calibration are in the named profile.  Keyed by (workload, form, kernel class).
+TRAFFIC = {
+    ("laplace-uniform-4m-16rhs", "mixed", "m2l"): (14.83e9, "profiles/laplace-uniform-4m-16rhs_m2l-tile-f32_traffic.md"),
+}
+
 # flops per source-target pair for nrhs right-hand sides (FMA = 2, sqrt/div = 1)
 PAIR_FLOPS = {111: lambda r: 9 + 2 * r, 321: lambda r: 18 + 12 * r, 511: lambda r: 12 + 19 * r,
               301: lambda r: 14 + 20 * r}
@@ -106,6 +113,101 @@ def cpu_baseline(c, seconds_target=15.0):
     dt = time.perf_counter() - t0
     lib.orc_set_accurate(1)
     return n / dt, f"{c['gen']} N={n}, np={c['np']}, ptsmax={c['ptsmax']}, 1 RHS, warm eval {dt:.2f} s"
+
+
+def batch_problems(seed=0x4B1F0005):
+    """64 independent problems: 32 Laplace (uniform cube), 19 Stokes double
+    layer (spheres of radius U[0.25, 1] with outward normals), 13 Navier
+    (uniform cube); sizes log-uniform in [1e4, 5e5]; np from {4, 6}; 8 RHS."""
+    rng = np.random.default_rng(seed)
+    kinds = [111] * 32 + [321] * 19 + [511] * 13
+    rng.shuffle(kinds)
+    out = []
+    for i, kt in enumerate(kinds):
+        n = int(round(np.exp(rng.uniform(np.log(1e4), np.log(5e5)))))
+        np_ = int(rng.choice([4, 6]))
+        if kt == 321:
+            rad = rng.uniform(0.25, 1.0)
+            ct = rng.uniform(-1, 1, n)
+            ph = rng.uniform(0, 2 * np.pi, n)
+            st = np.sqrt(1 - ct * ct)
+            nor = np.stack([st * np.cos(ph), st * np.sin(ph), ct], axis=1)
+            out.append(dict(kt=kt, n=n, np=np_, pos=rad * nor, nor=nor, ctr=(0.0, 0.0, 0.0), rootlvl=0,
+                            coefs=[1.0], seed=seed + 1 + i))
+        else:
+            p = rng.random((n, 3))
+            out.append(dict(kt=kt, n=n, np=np_, pos=p, nor=p, ctr=(0.5, 0.5, 0.5), rootlvl=1,
+                            coefs=[1.0] if kt == 111 else [1.0, 0.3], seed=seed + 1 + i))
+    return out
+
+
+def run_batch(args, K):
+    """The 64-problem batch in one graph: aggregate targets/s of the chosen
+    form, with the sum of the problems' individual eager evaluations beside it."""
+    nrhs = 8
+    mode = K.MODE_MIXED if args.form in ("both", "mixed") else K.MODE_PLAIN
+    probs = batch_problems()
+    fmms, srcs, trgs = [], [], []
+    t0 = time.perf_counter()
+    for pr in probs:
+        f = K.FMM3d("fmm3d_")
+        f.srcpos, f.srcnor, f.trgpos = pr["pos"], pr["nor"], pr["pos"]
+        f.ctr, f.rootlvl = pr["ctr"], pr["rootlvl"]
+        f.knl = K.Kernel3d(pr["kt"], pr["coefs"])
+        f.setup({"-fmm3d_np": str(pr["np"]), "-fmm3d_let3d_ptsmax": "100", "-fmm3d_let3d_maxlevel": "10"})
+        rng = np.random.default_rng(pr["seed"])
+        sdof = f.knl.sdof()
+        den = rng.standard_normal((nrhs, pr["n"] * sdof)) if pr["kt"] == 511 else \
+            rng.uniform(-1, 1, (nrhs, pr["n"] * sdof))
+        d_src = K.DeviceBuffer(den.nbytes)
+        d_src.upload(den)
+        d_trg = K.DeviceBuffer(nrhs * pr["n"] * f.knl.tdof() * 8)
+        fmms.append(f); srcs.append(d_src); trgs.append(d_trg)
+    t_setup = time.perf_counter() - t0
+    ntot = sum(pr["n"] for pr in probs) * nrhs
+    log(f"batch: {len(probs)} problems, {ntot // nrhs} points, setup {t_setup:.1f} s")
+    # individual eager evaluations (warm), for comparison
+    for f, s_, t_ in zip(fmms, srcs, trgs):
+        f.eval_device(s_, t_, nrhs, mode)
+    K.synchronize()
+    t0 = time.perf_counter()
+    for f, s_, t_ in zip(fmms, srcs, trgs):
+        f.eval_device(s_, t_, nrhs, mode)
+    K.synchronize()
+    eager_ms = 1000.0 * (time.perf_counter() - t0)
+    b = K.Batch(fmms)
+    for _ in range(args.warmup):
+        b.eval_device(srcs, trgs, [nrhs] * len(fmms), mode)
+    K.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        b.eval_device(srcs, trgs, [nrhs] * len(fmms), mode)
+    K.synchronize()
+    ms = 1000.0 * (time.perf_counter() - t0) / args.steps
+    value = ntot / (ms * 1e-3)
+    log(f"batch: {ms:.2f} ms per graph launch ({value:.4e} targets/s); one-by-one eager {eager_ms:.2f} ms")
+    errs = {111: 0.0, 321: 0.0, 511: 0.0}
+    if not args.no_check:
+        for pr, f, s_, t_ in zip(probs, fmms, srcs, trgs):
+            tdof = f.knl.tdof()
+            val = np.empty((nrhs, pr["n"] * tdof))
+            t_.download(val)
+            idx = np.sort(np.random.default_rng(pr["seed"] + 2).choice(pr["n"], 200, replace=False)).astype(np.int32)
+            d = f.direct(s_, idx, nrhs)
+            v = val.reshape(nrhs, pr["n"], tdof)[:, idx, :].reshape(nrhs, -1)
+            errs[pr["kt"]] = max(errs[pr["kt"]], float(np.linalg.norm(v - d) / np.linalg.norm(d)))
+        log(f"batch: worst rel L2 error vs FP64 direct (200 targets x {nrhs} RHS) per kernel {errs}")
+    out = {
+        "metric": METRIC, "value": value, "unit": "targets/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+f64" if mode == K.MODE_MIXED else "f64", "data": "synthetic",
+        "config": {"workload": "batch-64", "form": "mixed FP32/FP64" if mode == K.MODE_MIXED else "plain FP64",
+                   "problems": len(probs), "points": ntot // nrhs, "nrhs": nrhs, "one_graph_ms": round(ms, 3),
+                   "one_by_one_eager_ms": round(eager_ms, 3), "setup_s": round(t_setup, 2),
+                   "worst_rel_err_vs_direct": {str(k): v for k, v in errs.items()} if not args.no_check else None},
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
 
 
 def timed_steps(f, K, d_src, d_trg, nrhs, mode, steps, warmup, label):
@@ -187,13 +289,15 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", default=DEFAULT, choices=sorted(CONFIGS))
+    ap.add_argument("--config", default=DEFAULT, choices=sorted(CONFIGS) + ["batch-64"])
     ap.add_argument("--ptsmax", default=None, help="leaf size, or 'auto' to pick the fastest of the candidates")
     ap.add_argument("--nrhs", type=int, default=None)
     ap.add_argument("--form", default="both", choices=["both", "plain", "mixed"],
                     help="both: time the plain FP64 form and the mixed form; value is the mixed form's")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-check", action="store_true", help="skip the direct-sum error check")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="only run five 1 GiB streaming copies (reference for memory-traffic counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU design is "
@@ -202,6 +306,21 @@ def main():
 
     import kifmm3d as K
 
+    if args.calibrate:
+        # a plain streaming copy of 1 GiB, five times: the reference for the
+        # memory-traffic counters (FETCH_SIZE / WRITE_SIZE per dispatch)
+        nb = 1 << 30
+        a_, b_ = K.DeviceBuffer(nb), K.DeviceBuffer(nb)
+        for _ in range(5):
+            rc = K.lib().kifmm3d_device_copy(b_.ptr, a_.ptr, nb)
+            if rc != 0:
+                raise SystemExit(f"device copy failed ({rc})")
+        log("calibration: 5 copies of 1 GiB done")
+        return
+    if args.config == "batch-64":
+        if K.device_count() < 1:
+            raise SystemExit("no HIP device visible")
+        return run_batch(args, K)
     c = dict(CONFIGS[args.config])
     if args.nrhs:
         c["nrhs"] = args.nrhs
@@ -263,9 +382,12 @@ def main():
     # events on the library's stream during its last timed step
     dom = max(r["rates"], key=lambda k: r["rates"][k][0])
     ms, calls, rate, unit, bound, peak = r["rates"][dom]
+    tr = TRAFFIC.get((args.config, chosen, dom))
     roofline = {"kernel": dom, "bound": bound, "achieved": round(rate, 4), "peak": peak,
                 "unit": "TFLOP/s" if unit == "flop" else "GB/s", "frac": round(rate / peak, 5),
-                "traffic": None, "ms_per_step": round(ms, 3), "launch_groups": calls}
+                "traffic": tr[0] if tr else None, "ms_per_step": round(ms, 3), "launch_groups": calls}
+    if tr:
+        roofline["traffic_note"] = f"bytes per evaluate over the {calls} launches, from {tr[1]}"
 
     cpu = None
     if not args.no_cpu:

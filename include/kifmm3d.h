@@ -179,6 +179,10 @@ int kifmm3d_device_free(void* ptr);
 int kifmm3d_memcpy_h2d(void* dst, const void* src, long bytes);
 int kifmm3d_memcpy_d2h(void* dst, const void* src, long bytes);
 int kifmm3d_device_synchronize(void);
+/* Device-to-device copy by a plain streaming kernel (bytes a multiple of 8);
+ * it reads and writes exactly `bytes` each, which makes it the reference for
+ * scaling hardware memory-traffic counters. */
+int kifmm3d_device_copy(void* dst, const void* src, long bytes);
 int kifmm3d_device_count(int* n);
 
 #ifdef __cplusplus

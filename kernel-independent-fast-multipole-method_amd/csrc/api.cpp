@@ -6,6 +6,7 @@
 #include <string>
 #include <vector>
 
+#include "device.hpp"
 #include "engine.hpp"
 #include "kifmm3d.h"
 
@@ -282,6 +283,15 @@ int kifmm3d_memcpy_d2h(void* dst, const void* src, long bytes)
   return hipMemcpy(dst, src, (size_t)bytes, hipMemcpyDeviceToHost) == hipSuccess ? KIFMM3D_OK : KIFMM3D_EDEVICE;
 }
 int kifmm3d_device_synchronize(void) { return hipDeviceSynchronize() == hipSuccess ? KIFMM3D_OK : KIFMM3D_EDEVICE; }
+
+int kifmm3d_device_copy(void* dst, const void* src, long bytes)
+{
+  if ((!dst || !src) && bytes > 0) return KIFMM3D_EINVAL;
+  if (bytes < 0 || bytes % 8) return KIFMM3D_EINVAL;
+  kifmm::dev::copy(nullptr, (const double*)src, (double*)dst, bytes / 8);
+  if (hipGetLastError() != hipSuccess) return KIFMM3D_EDEVICE;
+  return hipDeviceSynchronize() == hipSuccess ? KIFMM3D_OK : KIFMM3D_EDEVICE;
+}
 int kifmm3d_device_count(int* n)
 {
   if (!n) return KIFMM3D_EINVAL;

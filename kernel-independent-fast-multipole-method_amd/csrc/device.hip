@@ -169,6 +169,19 @@ void zero(hipStream_t s, double* p, long n)
   hipLaunchKernelGGL(k_zero, dim3(grid_for(n, 256)), dim3(256), 0, s, n, p);
 }
 
+// Streaming copy of n doubles (reads n*8 bytes, writes n*8 bytes): the
+// reference kernel for scaling memory-traffic counters.
+__global__ void k_copy(long n, const double* __restrict__ src, double* __restrict__ dst)
+{
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) dst[i] = src[i];
+}
+
+void copy(hipStream_t s, const double* src, double* dst, long n)
+{
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_copy, dim3(grid_for(n, 256)), dim3(256), 0, s, n, src, dst);
+}
+
 void gather(hipStream_t s, int n, int dof, int nr, const int* perm, const double* src, long src_rs, double* dst)
 {
   long tot = (long)n * dof * nr;

@@ -13,6 +13,8 @@ struct KC {  // kernel constants (see kernel_consts in kernels_host.cpp)
 
 // p[0 .. n) = 0
 void zero(hipStream_t s, double* p, long n);
+// dst[0 .. n) = src[0 .. n)
+void copy(hipStream_t s, const double* src, double* dst, long n);
 // Source-major gather: dst[(k*nr + r)*dof + d] = src[r*src_rs + perm[k]*dof + d]
 void gather(hipStream_t s, int n, int dof, int nr, const int* perm, const double* src, long src_rs, double* dst);
 // dst[r*dst_rs + perm[k]*dof + d] = src[r*src_rs + k*dof + d]

@@ -58,6 +58,7 @@ SYMBOLS = [
     ("kifmm3d_memcpy_d2h", ctypes.c_int, [_VP, _VP, ctypes.c_long]),
     ("kifmm3d_device_synchronize", ctypes.c_int, []),
     ("kifmm3d_device_count", ctypes.c_int, [_I]),
+    ("kifmm3d_device_copy", ctypes.c_int, [_VP, _VP, ctypes.c_long]),
     ("kifmm3d_set_graph", ctypes.c_int, [_VP, ctypes.c_int]),
     ("kifmm3d_batch_create", ctypes.c_int, [ctypes.POINTER(_VP), ctypes.POINTER(_VP), ctypes.c_int]),
     ("kifmm3d_batch_evaluate_device", ctypes.c_int, [_VP, ctypes.POINTER(_VP), ctypes.POINTER(_VP), _I, ctypes.c_int]),
