@@ -1651,22 +1651,25 @@ struct Mt<float> {
 constexpr int M2LT_BS = 512;  // 8 waves: one per target parent of the tile
 constexpr int M2LT_FPB = 16;  // frequencies per block, visited in a software-pipelined loop
 constexpr int NOPS = 343;     // operator table entries ((o+3) in [0,7)^3)
+constexpr int NAENT = 26 * 32;  // A-operand table entries: direction x (source k, target child c)
 
-// (FP32: two blocks per CU fit the LDS, so registers are capped to match: the
-// second bound is waves per SIMD)
+// The A operands depend only on (direction, frequency, lane), not on the
+// target parent, so the block builds them once per frequency: entry
+// n*32 + k*8 + c holds (T_{optab[n][k][c]}, T_{optab[n][4+k][c]}) (zero where
+// the child pair is not in a V list) and every wave reads its lane's entry
+// with one LDS load.  (FP32: two blocks per CU fit the LDS, so registers are
+// capped to match: the second launch bound is waves per SIMD.)
 template <class R>
-__global__ __launch_bounds__(M2LT_BS, sizeof(R) == 4 ? 4 : 2) void k_m2l_tile(int C, int Rc, int ntile, const int* __restrict__ ttile,
-                                                      const int* __restrict__ htile, const int* __restrict__ optab,
-                                                      const typename Mt<R>::C2* __restrict__ opsT,
-                                                      const typename Mt<R>::C2* __restrict__ effden,
-                                                      typename Mt<R>::C2* __restrict__ effval, int roff)
+__global__ __launch_bounds__(M2LT_BS, sizeof(R) == 4 ? 4 : 2) void k_m2l_tile(
+    int C, int Rc, int ntile, const int* __restrict__ ttile, const int* __restrict__ htile,
+    const int* __restrict__ optab, const typename Mt<R>::C2* __restrict__ opsT,
+    const typename Mt<R>::C2* __restrict__ effden, typename Mt<R>::C2* __restrict__ effval, int roff)
 {
   using C2 = typename Mt<R>::C2;
   using V4 = typename Mt<R>::V4;
   __shared__ C2 xs[64 * 8 * 16];  // [halo parent][source child][rhs] at the current frequency
-  __shared__ int s_opt[26 * 64];
-  __shared__ C2 s_op[NOPS];
-  __shared__ int s_hm[64];
+  __shared__ C2 s_A[NAENT][2];    // A-operand table at the current frequency
+  __shared__ int s_hm[64];        // halo parent has a child in some V list
   // tiles are dealt to the 8 XCDs in contiguous ranges so that neighbouring
   // tiles (whose halos overlap) share an L2
   const int per = (ntile + 7) / 8, b = blockIdx.x;
@@ -1675,7 +1678,6 @@ __global__ __launch_bounds__(M2LT_BS, sizeof(R) == 4 ? 4 : 2) void k_m2l_tile(in
   const int f0 = blockIdx.y * M2LT_FPB, f1 = min(C, f0 + M2LT_FPB), r0 = roff + blockIdx.z * 16;
   const int t = threadIdx.x, w = t >> 6, l = t & 63;
   const int* hs = htile + (long)tile * 512;
-  for (int i = t; i < 26 * 64; i += M2LT_BS) s_opt[i] = optab[i];
   if (t < 64) {
     int m = 0;
 #pragma unroll
@@ -1683,12 +1685,19 @@ __global__ __launch_bounds__(M2LT_BS, sizeof(R) == 4 ? 4 : 2) void k_m2l_tile(in
     s_hm[t] = m;
   }
   // staging: thread t moves entries e = it*512 + t, i.e. halo child row
-  // it*32 + t/16 and right-hand side t%16
+  // it*32 + t/16 and right-hand side t%16, and A-table entries t and t + 512
   int hsl[16];
 #pragma unroll
   for (int it = 0; it < 16; it++) hsl[it] = hs[it * 32 + (t >> 4)];
   const C2* eb = effden + r0 + (t & 15);
-  C2 pre[16], pop;
+  int aop[2][2];
+#pragma unroll
+  for (int j = 0; j < 2; j++) {
+    const int e = t + j * M2LT_BS, n = e >> 5, k = (e >> 3) & 3, c = e & 7;
+    aop[j][0] = e < NAENT ? optab[(n * 8 + k) * 8 + c] : -1;
+    aop[j][1] = e < NAENT ? optab[(n * 8 + 4 + k) * 8 + c] : -1;
+  }
+  C2 pre[16], pa[2][2];
   auto fetch = [&](int f) {
 #pragma unroll
     for (int it = 0; it < 16; it++) {
@@ -1696,9 +1705,14 @@ __global__ __launch_bounds__(M2LT_BS, sizeof(R) == 4 ? 4 : 2) void k_m2l_tile(in
       pre[it].y = 0;
       if (hsl[it] >= 0) pre[it] = eb[((long)hsl[it] * C + f) * Rc];
     }
-    pop.x = 0;
-    pop.y = 0;
-    if (t < NOPS) pop = opsT[(long)f * NOPS + t];
+#pragma unroll
+    for (int j = 0; j < 2; j++)
+#pragma unroll
+      for (int q = 0; q < 2; q++) {
+        pa[j][q].x = 0;
+        pa[j][q].y = 0;
+        if (aop[j][q] >= 0) pa[j][q] = opsT[(long)f * NOPS + aop[j][q]];
+      }
   };
   fetch(f0);
   const int* ts = ttile + ((long)tile * 8 + w) * 8;
@@ -1706,6 +1720,7 @@ __global__ __launch_bounds__(M2LT_BS, sizeof(R) == 4 ? 4 : 2) void k_m2l_tile(in
 #pragma unroll
   for (int c = 0; c < 8; c++) live |= ts[c] >= 0;
   const int px = w >> 2, py = (w >> 1) & 1, pz = w & 1;
+  const int hc = ((px + 1) * 4 + (py + 1)) * 4 + (pz + 1);
   const int i = l & 15, k = l >> 4;
   // output (child, part) of row i of A
   int kr, vr;
@@ -1717,48 +1732,56 @@ __global__ __launch_bounds__(M2LT_BS, sizeof(R) == 4 ? 4 : 2) void k_m2l_tile(in
     vr = i & 3;
   }
   const int ca = 2 * kr + (vr >> 1), pim = vr & 1;
+  const int aent = k * 8 + ca;
+  __syncthreads();
+  // directions n (in (dx, dy, dz) order, dz fastest, centre skipped) whose
+  // halo parent has children: a wave-uniform bit mask
+  unsigned dmask = 0;
+  for (int n = 0; n < 26; n++) {
+    const int m = n + (n >= 13);
+    const int h = hc + (m / 9 - 1) * 16 + ((m / 3) % 3 - 1) * 4 + (m % 3 - 1);
+    if (s_hm[h]) dmask |= 1u << n;
+  }
   for (int f = f0; f < f1; f++) {
-    __syncthreads();  // the previous frequency's products are done with xs and s_op
+    __syncthreads();  // the previous frequency's products are done with xs and s_A
 #pragma unroll
     for (int it = 0; it < 16; it++) xs[it * M2LT_BS + t] = pre[it];
-    if (t < NOPS) s_op[t] = pop;
+#pragma unroll
+    for (int j = 0; j < 2; j++)
+      if (t + j * M2LT_BS < NAENT) {
+        s_A[t + j * M2LT_BS][0] = pa[j][0];
+        s_A[t + j * M2LT_BS][1] = pa[j][1];
+      }
     __syncthreads();
     if (f + 1 < f1) fetch(f + 1);  // in flight while the matrix cores work on f
     if (!live) continue;
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
     V4 d0 = {0, 0, 0, 0}, d1 = {0, 0, 0, 0};  // two chains for matrix-core overlap
-    const int hc = ((px + 1) * 4 + (py + 1)) * 4 + (pz + 1);
     // FP64 measured faster fully unrolled; FP32 needs the registers for 4 waves/SIMD
     constexpr int UNR = Mt<R>::KV ? 26 : 2;
 #pragma unroll UNR
     for (int n = 0; n < 26; n++) {
-      // direction n in (dx, dy, dz) order with dz fastest, the centre skipped
+      if (!(dmask >> n & 1u)) continue;
       const int m = n + (n >= 13);
       const int h = hc + (m / 9 - 1) * 16 + ((m / 3) % 3 - 1) * 4 + (m % 3 - 1);
-      if (s_hm[h]) {
-        const int o0 = s_opt[(n * 8 + k) * 8 + ca], o1 = s_opt[(n * 8 + 4 + k) * 8 + ca];
-        C2 m0, m1;
-        m0.x = m0.y = m1.x = m1.y = 0;
-        if (o0 >= 0) m0 = s_op[o0];
-        if (o1 >= 0) m1 = s_op[o1];
-        const R a0 = pim ? m0.y : m0.x, a1 = pim ? m1.y : m1.x;
-        const R a2 = pim ? m0.x : -m0.y, a3 = pim ? m1.x : -m1.y;
-        const C2 x0 = xs[(h * 8 + k) * 16 + i], x1 = xs[(h * 8 + 4 + k) * 16 + i];
-        if (Mt<R>::KV) {
-          d0 = Mt<R>::mfma(a0, x0.x, d0);
-          d1 = Mt<R>::mfma(a2, x0.y, d1);
-          d0 = Mt<R>::mfma(a1, x1.x, d0);
-          d1 = Mt<R>::mfma(a3, x1.y, d1);
-        } else {
-          // single precision: one direction per FP32 product, summed in FP64
-          V4 e0 = {0, 0, 0, 0}, e1 = {0, 0, 0, 0};
-          e0 = Mt<R>::mfma(a0, x0.x, e0);
-          e1 = Mt<R>::mfma(a2, x0.y, e1);
-          e0 = Mt<R>::mfma(a1, x1.x, e0);
-          e1 = Mt<R>::mfma(a3, x1.y, e1);
+      const C2 m0 = s_A[n * 32 + aent][0], m1 = s_A[n * 32 + aent][1];
+      const R a0 = pim ? m0.y : m0.x, a1 = pim ? m1.y : m1.x;
+      const R a2 = pim ? m0.x : -m0.y, a3 = pim ? m1.x : -m1.y;
+      const C2 x0 = xs[(h * 8 + k) * 16 + i], x1 = xs[(h * 8 + 4 + k) * 16 + i];
+      if (Mt<R>::KV) {
+        d0 = Mt<R>::mfma(a0, x0.x, d0);
+        d1 = Mt<R>::mfma(a2, x0.y, d1);
+        d0 = Mt<R>::mfma(a1, x1.x, d0);
+        d1 = Mt<R>::mfma(a3, x1.y, d1);
+      } else {
+        // single precision: one direction per FP32 product, summed in FP64
+        V4 e0 = {0, 0, 0, 0}, e1 = {0, 0, 0, 0};
+        e0 = Mt<R>::mfma(a0, x0.x, e0);
+        e1 = Mt<R>::mfma(a2, x0.y, e1);
+        e0 = Mt<R>::mfma(a1, x1.x, e0);
+        e1 = Mt<R>::mfma(a3, x1.y, e1);
 #pragma unroll
-          for (int v = 0; v < 4; v++) acc[v] += (double)(e0[v] + e1[v]);
-        }
+        for (int v = 0; v < 4; v++) acc[v] += (double)(e0[v] + e1[v]);
       }
     }
     if (Mt<R>::KV) {
