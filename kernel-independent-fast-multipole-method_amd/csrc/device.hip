@@ -1212,135 +1212,177 @@ __global__ __launch_bounds__(256) void k_fft_inv(const int* __restrict__ slot_no
   }
 }
 
-// Right-hand-side-innermost forms for scalar kernels (sm = tm = 1): a block
-// transforms R right-hand sides of one slot and moves the spectra through LDS
-// so that global accesses cover R consecutive right-hand sides per frequency.
-template <int NP, class CT, int R>
-__global__ __launch_bounds__(256) void k_fft_fwd_ri(const int* __restrict__ slot_node, int Rc,
-                                                    const double* __restrict__ upeden, int pln,
-                                                    const double* __restrict__ dscale, const short* __restrict__ c2s,
-                                                    CT* __restrict__ effden, const double2* __restrict__ tw)
+// Right-hand-side-innermost forms for scalar kernels (sm = tm = 1).  A block
+// of 64 R threads transforms R right-hand sides of one slot.  Every stage of
+// the separable DFT gives a thread one (rhs, line) pair: it loads the line's
+// inputs into registers once and produces all of the line's outputs, with the
+// twiddles w((i + np/2) m) read from a small table as LDS broadcasts, so LDS
+// traffic is one read and one write per intermediate value.  Global accesses
+// put the R right-hand sides of a frequency next to each other.
+template <int NP>
+__device__ inline void load_twiddles(double2 (*s_wt)[2 * NP], const double2* tw, int t, int nt)
 {
-  constexpr int N = 2 * NP, H = NP + 1, C = N * N * H, NC = NP * NP * NP;
-  constexpr int NSP = NP * NP * NP - (NP - 2) * (NP - 2) * (NP - 2);
-  __shared__ double s_den[NSP];
-  __shared__ short s_c2s[NC];
-  __shared__ double2 s_w[N];
-  __shared__ double2 X1[NP * NP * H];
-  __shared__ double2 X2[R][NP * N * H];
-  const int slot = blockIdx.x, r0 = blockIdx.y * R, t = threadIdx.x;
-  for (int i = t; i < NC; i += 256) s_c2s[i] = c2s[i];
-  if (t < N) s_w[t] = tw[t];
-  const double ds = dscale[0];
-  for (int rr = 0; rr < R; rr++) {
-    __syncthreads();
-    const double* den = upeden + ((long)slot_node[slot] * Rc + r0 + rr) * pln;
-    for (int i = t; i < NSP; i += 256) s_den[i] = den[i] * ds;
-    __syncthreads();
-    for (int o = t; o < NP * NP * H; o += 256) {  // along x: X1[k][j][kx]
-      int kx = o % H, j = (o / H) % NP, k = o / (H * NP);
-      double re = 0, im = 0;
-#pragma unroll
-      for (int i = 0; i < NP; i++) {
-        int si = s_c2s[(i * NP + j) * NP + k];
-        if (si >= 0) {
-          double v = s_den[si];
-          double2 w = s_w[((i + NP / 2) * kx) % N];
-          re += v * w.x;
-          im += v * w.y;
-        }
-      }
-      X1[o] = make_double2(re, im);
-    }
-    __syncthreads();
-    for (int o = t; o < NP * N * H; o += 256) {  // along y: X2[rr][k][ky][kx]
-      int kx = o % H, ky = (o / H) % N, k = o / (H * N);
-      double2 acc = make_double2(0, 0);
-#pragma unroll
-      for (int j = 0; j < NP; j++) {
-        double2 v = cmul(X1[(k * NP + j) * H + kx], s_w[((j + NP / 2) * ky) % N]);
-        acc.x += v.x;
-        acc.y += v.y;
-      }
-      X2[rr][o] = acc;
-    }
-  }
-  __syncthreads();
-  CT* out = effden + (long)slot * C * Rc + r0;
-  for (int e = t; e < C * R; e += 256) {  // along z, the R right-hand sides innermost
-    const int o = e / R, rr = e - o * R;
-    int kx = o % H, ky = (o / H) % N, kz = o / (H * N);
-    double2 acc = make_double2(0, 0);
-#pragma unroll
-    for (int k = 0; k < NP; k++) {
-      double2 v = cmul(X2[rr][(k * N + ky) * H + kx], s_w[((k + NP / 2) * kz) % N]);
-      acc.x += v.x;
-      acc.y += v.y;
-    }
-    cstore(out + (long)o * Rc + rr, acc);
+  constexpr int N = 2 * NP;
+  for (int e = t; e < NP * N; e += nt) {
+    const int i = e / N, m = e - i * N;
+    s_wt[i][m] = tw[((i + NP / 2) * m) % N];
   }
 }
 
 template <int NP, class CT, int R>
-__global__ __launch_bounds__(256) void k_fft_inv_ri(const int* __restrict__ slot_node, int Rc,
-                                                    const CT* __restrict__ effval, const int* __restrict__ s2c,
-                                                    int nsp, double* __restrict__ dncval, int pln,
-                                                    const double2* __restrict__ tw)
+__global__ __launch_bounds__(64 * R) void k_fft_fwd_ri(const int* __restrict__ slot_node, int Rc,
+                                                       const double* __restrict__ upeden, int pln,
+                                                       const double* __restrict__ dscale, const short* __restrict__ c2s,
+                                                       CT* __restrict__ effden, const double2* __restrict__ tw)
 {
-  constexpr int N = 2 * NP, H = NP + 1, C = N * N * H;
-  __shared__ CT s_x[R][C];            // spectra as stored
-  __shared__ double2 Y1[NP * N * H];  // [k][ky][kx]
-  __shared__ double2 Y2[NP * NP * H]; // [k][j][kx]
-  __shared__ double2 s_w[N];
+  constexpr int N = 2 * NP, H = NP + 1, C = N * N * H, NC = NP * NP * NP, BS = 64 * R;
+  constexpr int NSP = NP * NP * NP - (NP - 2) * (NP - 2) * (NP - 2);
+  __shared__ double s_den[R][NSP];
+  __shared__ short s_c2s[NC];
+  __shared__ double2 s_wt[NP][N];
+  __shared__ double2 X1[R][NP * NP * H];  // [k][j][kx]
+  __shared__ double2 X2[R][NP * N * H];   // [k][ky][kx]
   const int slot = blockIdx.x, r0 = blockIdx.y * R, t = threadIdx.x;
-  if (t < N) s_w[t] = tw[t];
-  const CT* in = effval + (long)slot * C * Rc + r0;
-  for (int e = t; e < C * R; e += 256) {
-    const int o = e / R, rr = e - o * R;
-    s_x[rr][o] = in[(long)o * Rc + rr];
+  for (int i = t; i < NC; i += BS) s_c2s[i] = c2s[i];
+  load_twiddles<NP>(s_wt, tw, t, BS);
+  const double ds = dscale[0];
+  const double* den = upeden + ((long)slot_node[slot] * Rc + r0) * pln;
+  for (int e = t; e < R * NSP; e += BS) {
+    const int rr = e / NSP, i = e - rr * NSP;
+    s_den[rr][i] = den[(long)rr * pln + i] * ds;
   }
-  for (int rr = 0; rr < R; rr++) {
-    __syncthreads();
-    for (int o = t; o < NP * N * H; o += 256) {  // along z, only the cube's planes
-      int kx = o % H, ky = (o / H) % N, k = o / (H * N);
+  __syncthreads();
+  for (int e = t; e < R * NP * NP; e += BS) {  // along x: line (k, j)
+    asm volatile("" ::: "memory");  // keep the twiddle loads inside the loop (registers)
+    const int rr = e % R, col = e / R, k = col / NP, j = col - k * NP;
+    double2 acc[H];
+#pragma unroll
+    for (int kx = 0; kx < H; kx++) acc[kx] = make_double2(0, 0);
+#pragma unroll
+    for (int i = 0; i < NP; i++) {
+      const int si = s_c2s[(i * NP + j) * NP + k];
+      if (si >= 0) {
+        const double v = s_den[rr][si];
+#pragma unroll
+        for (int kx = 0; kx < H; kx++) {
+          acc[kx].x += v * s_wt[i][kx].x;
+          acc[kx].y += v * s_wt[i][kx].y;
+        }
+      }
+    }
+#pragma unroll
+    for (int kx = 0; kx < H; kx++) X1[rr][(k * NP + j) * H + kx] = acc[kx];
+  }
+  __syncthreads();
+  for (int e = t; e < R * NP * H; e += BS) {  // along y: line (k, kx)
+    asm volatile("" ::: "memory");  // keep the twiddle loads inside the loop (registers)
+    const int rr = e % R, col = e / R, k = col / H, kx = col - k * H;
+    double2 x[NP];
+#pragma unroll
+    for (int j = 0; j < NP; j++) x[j] = X1[rr][(k * NP + j) * H + kx];
+#pragma unroll
+    for (int ky = 0; ky < N; ky++) {
+      double2 acc = make_double2(0, 0);
+#pragma unroll
+      for (int j = 0; j < NP; j++) {
+        const double2 v = cmul(x[j], s_wt[j][ky]);
+        acc.x += v.x;
+        acc.y += v.y;
+      }
+      X2[rr][(k * N + ky) * H + kx] = acc;
+    }
+  }
+  __syncthreads();
+  CT* out = effden + (long)slot * C * Rc + r0;
+  for (int e = t; e < R * N * H; e += BS) {  // along z: line (ky, kx), right-hand sides innermost
+    asm volatile("" ::: "memory");  // keep the twiddle loads inside the loop (registers)
+    const int rr = e % R, col = e / R;
+    double2 x[NP];
+#pragma unroll
+    for (int k = 0; k < NP; k++) x[k] = X2[rr][k * N * H + col];
+#pragma unroll
+    for (int kz = 0; kz < N; kz++) {
+      double2 acc = make_double2(0, 0);
+#pragma unroll
+      for (int k = 0; k < NP; k++) {
+        const double2 v = cmul(x[k], s_wt[k][kz]);
+        acc.x += v.x;
+        acc.y += v.y;
+      }
+      cstore(out + (long)(kz * N * H + col) * Rc + rr, acc);
+    }
+  }
+}
+
+template <int NP, class CT, int R>
+__global__ __launch_bounds__(64 * R) void k_fft_inv_ri(const int* __restrict__ slot_node, int Rc,
+                                                       const CT* __restrict__ effval, const int* __restrict__ s2c,
+                                                       int nsp, double* __restrict__ dncval, int pln,
+                                                       const double2* __restrict__ tw)
+{
+  constexpr int N = 2 * NP, H = NP + 1, C = N * N * H, BS = 64 * R;
+  __shared__ double2 s_wt[NP][N];
+  __shared__ double2 Y1[R][NP * N * H];   // [k][ky][kx], only the cube's planes
+  __shared__ double2 Y2[R][NP * NP * H];  // [k][j][kx], only the cube's rows
+  const int slot = blockIdx.x, r0 = blockIdx.y * R, t = threadIdx.x;
+  load_twiddles<NP>(s_wt, tw, t, BS);
+  __syncthreads();
+  const CT* in = effval + (long)slot * C * Rc + r0;
+  for (int e = t; e < R * N * H; e += BS) {  // along z: line (ky, kx), spectra read once from HBM
+    asm volatile("" ::: "memory");  // keep the twiddle loads inside the loop (registers)
+    const int rr = e % R, col = e / R;
+    double2 x[N];
+#pragma unroll
+    for (int kz = 0; kz < N; kz++) x[kz] = cload(in + (long)(kz * N * H + col) * Rc + rr);
+#pragma unroll
+    for (int k = 0; k < NP; k++) {
       double2 acc = make_double2(0, 0);
 #pragma unroll
       for (int kz = 0; kz < N; kz++) {
-        double2 w = s_w[((k + NP / 2) * kz) % N];
-        double2 v = cmul(cload(&s_x[rr][(kz * N + ky) * H + kx]), make_double2(w.x, -w.y));
+        const double2 w = s_wt[k][kz];
+        const double2 v = cmul(x[kz], make_double2(w.x, -w.y));
         acc.x += v.x;
         acc.y += v.y;
       }
-      Y1[o] = acc;
+      Y1[rr][k * N * H + col] = acc;
     }
-    __syncthreads();
-    for (int o = t; o < NP * NP * H; o += 256) {  // along y, only the cube's rows
-      int kx = o % H, j = (o / H) % NP, k = o / (H * NP);
+  }
+  __syncthreads();
+  for (int e = t; e < R * NP * H; e += BS) {  // along y: line (k, kx)
+    asm volatile("" ::: "memory");  // keep the twiddle loads inside the loop (registers)
+    const int rr = e % R, col = e / R, k = col / H, kx = col - k * H;
+    double2 y[N];
+#pragma unroll
+    for (int ky = 0; ky < N; ky++) y[ky] = Y1[rr][(k * N + ky) * H + kx];
+#pragma unroll
+    for (int j = 0; j < NP; j++) {
       double2 acc = make_double2(0, 0);
 #pragma unroll
       for (int ky = 0; ky < N; ky++) {
-        double2 w = s_w[((j + NP / 2) * ky) % N];
-        double2 v = cmul(Y1[(k * N + ky) * H + kx], make_double2(w.x, -w.y));
+        const double2 w = s_wt[j][ky];
+        const double2 v = cmul(y[ky], make_double2(w.x, -w.y));
         acc.x += v.x;
         acc.y += v.y;
       }
-      Y2[o] = acc;
+      Y2[rr][(k * NP + j) * H + kx] = acc;
     }
-    __syncthreads();
-    double* out = dncval + ((long)slot_node[slot] * Rc + r0 + rr) * pln;
-    for (int p = t; p < nsp; p += 256) {  // along x (c2r) at the surface points
-      int cc = s2c[p];
-      int i = cc / (NP * NP), j = (cc / NP) % NP, k = cc % NP;
-      const double2* y = &Y2[(k * NP + j) * H];
-      int xi = i + NP / 2;
-      double v = y[0].x + ((xi & 1) ? -y[NP].x : y[NP].x);
+  }
+  __syncthreads();
+  double* out = dncval + ((long)slot_node[slot] * Rc + r0) * pln;
+  for (int e = t; e < R * nsp; e += BS) {  // along x (c2r) at the surface points
+    asm volatile("" ::: "memory");  // keep the twiddle loads inside the loop (registers)
+    const int rr = e / nsp, p = e - rr * nsp;
+    const int cc = s2c[p];
+    const int i = cc / (NP * NP), j = (cc / NP) % NP, k = cc % NP;
+    const double2* y = &Y2[rr][(k * NP + j) * H];
+    const int xi = i + NP / 2;
+    double v = y[0].x + ((xi & 1) ? -y[NP].x : y[NP].x);
 #pragma unroll
-      for (int kx = 1; kx < NP; kx++) {
-        double2 w = s_w[(xi * kx) % N];
-        v += 2.0 * (y[kx].x * w.x + y[kx].y * w.y);
-      }
-      out[p] += v;
+    for (int kx = 1; kx < NP; kx++) {
+      const double2 w = s_wt[i][kx];
+      v += 2.0 * (y[kx].x * w.x + y[kx].y * w.y);
     }
+    out[(long)rr * pln + p] += v;
   }
 }
 
@@ -1352,9 +1394,9 @@ static void fft_fwd_t(hipStream_t s, int np, int nslot, const int* slot_node, in
   if (ri && sm == 1) {
     constexpr int RW = sizeof(CT) == 8 ? 8 : 4;  // 64-byte runs of right-hand sides
     switch (np) {
-    case 4: hipLaunchKernelGGL((k_fft_fwd_ri<4, CT, RW>), dim3(nslot, Rc / RW), dim3(256), 0, s, slot_node, Rc, upeden, pln, dscale, cube2surf, effden, tw); break;
-    case 6: hipLaunchKernelGGL((k_fft_fwd_ri<6, CT, RW>), dim3(nslot, Rc / RW), dim3(256), 0, s, slot_node, Rc, upeden, pln, dscale, cube2surf, effden, tw); break;
-    case 8: hipLaunchKernelGGL((k_fft_fwd_ri<8, CT, RW / 2>), dim3(nslot, Rc / (RW / 2)), dim3(256), 0, s, slot_node, Rc, upeden, pln, dscale, cube2surf, effden, tw); break;
+    case 4: hipLaunchKernelGGL((k_fft_fwd_ri<4, CT, RW>), dim3(nslot, Rc / RW), dim3(64 * RW), 0, s, slot_node, Rc, upeden, pln, dscale, cube2surf, effden, tw); break;
+    case 6: hipLaunchKernelGGL((k_fft_fwd_ri<6, CT, RW>), dim3(nslot, Rc / RW), dim3(64 * RW), 0, s, slot_node, Rc, upeden, pln, dscale, cube2surf, effden, tw); break;
+    case 8: hipLaunchKernelGGL((k_fft_fwd_ri<8, CT, RW / 2>), dim3(nslot, Rc / (RW / 2)), dim3(32 * RW), 0, s, slot_node, Rc, upeden, pln, dscale, cube2surf, effden, tw); break;
     }
     return;
   }
@@ -1372,11 +1414,11 @@ static void fft_inv_t(hipStream_t s, int np, int nslot, const int* slot_node, in
 {
   if (nslot <= 0) return;
   if (ri && tm == 1) {
-    constexpr int RW = sizeof(CT) == 8 ? 8 : 4;
+    constexpr int RW = 4;
     switch (np) {
-    case 4: hipLaunchKernelGGL((k_fft_inv_ri<4, CT, RW>), dim3(nslot, Rc / RW), dim3(256), 0, s, slot_node, Rc, effval, surf2cube, nsp, dncval, pln, tw); break;
-    case 6: hipLaunchKernelGGL((k_fft_inv_ri<6, CT, RW>), dim3(nslot, Rc / RW), dim3(256), 0, s, slot_node, Rc, effval, surf2cube, nsp, dncval, pln, tw); break;
-    case 8: hipLaunchKernelGGL((k_fft_inv_ri<8, CT, RW / 2>), dim3(nslot, Rc / (RW / 2)), dim3(256), 0, s, slot_node, Rc, effval, surf2cube, nsp, dncval, pln, tw); break;
+    case 4: hipLaunchKernelGGL((k_fft_inv_ri<4, CT, RW>), dim3(nslot, Rc / RW), dim3(64 * RW), 0, s, slot_node, Rc, effval, surf2cube, nsp, dncval, pln, tw); break;
+    case 6: hipLaunchKernelGGL((k_fft_inv_ri<6, CT, RW>), dim3(nslot, Rc / RW), dim3(64 * RW), 0, s, slot_node, Rc, effval, surf2cube, nsp, dncval, pln, tw); break;
+    case 8: hipLaunchKernelGGL((k_fft_inv_ri<8, CT, RW / 2>), dim3(nslot, Rc / (RW / 2)), dim3(32 * RW), 0, s, slot_node, Rc, effval, surf2cube, nsp, dncval, pln, tw); break;
     }
     return;
   }

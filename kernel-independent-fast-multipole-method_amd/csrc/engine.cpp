@@ -98,6 +98,7 @@ Engine::Engine(int kt, const double* coefs, int ncoefs, int np, int ptsmax, int 
   if (const char* g = std::getenv("KIFMM3D_GEMM")) gemm_mfma_ = std::string(g) != "valu";
   if (const char* m = std::getenv("KIFMM3D_M2L")) m2l_tile_ = std::string(m) != "group" && std::string(m) != "pair";
   if (const char* gph = std::getenv("KIFMM3D_GRAPH")) use_graph_ = std::atoi(gph) != 0;
+  if (const char* ov = std::getenv("KIFMM3D_OVERLAP")) overlap_ = std::atoi(ov) != 0;
 }
 
 Engine::~Engine()
@@ -105,6 +106,9 @@ Engine::~Engine()
   drop_graph();
   for (hipEvent_t e : evpool_) (void)hipEventDestroy(e);
   if (stream_) (void)hipStreamDestroy(stream_);
+  if (side_) (void)hipStreamDestroy(side_);
+  if (fork_ev_) (void)hipEventDestroy(fork_ev_);
+  if (join_ev_) (void)hipEventDestroy(join_ev_);
 }
 
 void Engine::setup(const double* spos, const double* snor, int ns, const double* tpos, int nt)
@@ -141,6 +145,11 @@ void Engine::setup(const double* spos, const double* snor, int ns, const double*
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
     throw Error(KIFMM3D_EDEVICE, "no HIP device is visible (tree, lists and operators are built)");
   if (!stream_) HIPCHK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+  if (!side_) {
+    HIPCHK(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&fork_ev_, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&join_ev_, hipEventDisableTiming));
+  }
   // leaf-ordered coordinates (fmm3d_setup.cpp:65-123) as SoA
   std::vector<double> ax(ns), ay(ns), az(ns);
   for (int k = 0; k < tree_.sextcnt; k++) {
@@ -777,7 +786,7 @@ void Engine::run_chunk(const double* srcden, double* trgval, int rc, int mode, h
     if (ev) HIPCHK(hipEventRecord(ev[k], s));
   };
   size_t nev = 0;
-  auto timed = [&](int cls, auto&& fn) {
+  auto timed_on = [&](hipStream_t st, int cls, auto&& fn) {
     if (!ev) {
       fn();
       return;
@@ -789,13 +798,14 @@ void Engine::run_chunk(const double* srcden, double* trgval, int rc, int mode, h
       evpool_.push_back(a);
       evpool_.push_back(b);
     }
-    HIPCHK(hipEventRecord(evpool_[nev], s));
+    HIPCHK(hipEventRecord(evpool_[nev], st));
     fn();
-    HIPCHK(hipEventRecord(evpool_[nev + 1], s));
+    HIPCHK(hipEventRecord(evpool_[nev + 1], st));
     evcls_.resize(nev / 2 + 1);
     evcls_[nev / 2] = cls;
     nev += 2;
   };
+  auto timed = [&](int cls, auto&& fn) { timed_on(s, cls, fn); };
 
   mark(0);
   // zero the accumulated buffers (fmm3d_eval.cpp:33-42)
@@ -814,11 +824,19 @@ void Engine::run_chunk(const double* srcden, double* trgval, int rc, int mode, h
   });
   for (const GemmOp& op : P.up) timed(KIFMM3D_K_GEMM_UP, [&] { run_gemm(op); });
   mark(2);
-  // U list and direct W/X
-  timed(KIFMM3D_K_P2P, [&] {
-    dev::p2p(s, user_.kt, P.near.n, I + P.near.tbeg, I + P.near.tnum, I + P.near.eptr,
+  // U list and direct W/X.  It depends only on the gathered densities, so
+  // with overlap on it runs on a side stream concurrently with the up pass,
+  // the V list and the down pass, and joins before L2T/M2T adds to the values.
+  const bool side = overlap_ && side_;
+  if (side) {
+    HIPCHK(hipEventRecord(fork_ev_, s));
+    HIPCHK(hipStreamWaitEvent(side_, fork_ev_, 0));
+  }
+  timed_on(side ? side_ : s, KIFMM3D_K_P2P, [&] {
+    dev::p2p(side ? side_ : s, user_.kt, P.near.n, I + P.near.tbeg, I + P.near.tnum, I + P.near.eptr,
              (const int2*)(I + P.near.ent), src, trg, rc, kcu, mixed ? 3 : p2p_variant_);
   });
+  if (side) HIPCHK(hipEventRecord(join_ev_, side_));
   mark(3);
   // V list: tiled matrix-core M2L for scalar kernels on chunks of 16k
   // right-hand sides, the grouped (or per-pair) form otherwise
@@ -855,6 +873,7 @@ void Engine::run_chunk(const double* srcden, double* trgval, int rc, int mode, h
   // down: L2L + DC2DE level by level
   for (const GemmOp& op : P.down) timed(KIFMM3D_K_GEMM_DOWN, [&] { run_gemm(op); });
   mark(6);
+  if (side) HIPCHK(hipStreamWaitEvent(s, join_ev_, 0));
   // L2T + M2T
   timed(KIFMM3D_K_S2P, [&] {
     dev::s2p(s, mm_.kt, P.far.n, I + P.far.tbeg, I + P.far.tnum, I + P.far.eptr, I + P.far.enode,

@@ -149,6 +149,9 @@ class Engine {
   bool gemm_mfma_ = true; // uncompensated GEMMs on the matrix cores (KIFMM3D_GEMM=valu disables)
   bool m2l_tile_ = true; // tiled matrix-core M2L for scalar kernels and 16-RHS chunks (KIFMM3D_M2L=group disables)
   hipStream_t stream_ = nullptr;
+  hipStream_t side_ = nullptr;  // near field, concurrent with the far-field chain
+  hipEvent_t fork_ev_ = nullptr, join_ev_ = nullptr;
+  bool overlap_ = false;        // KIFMM3D_OVERLAP=1 (measured: -0.5 ms of 51 at 4M/16 RHS; off so that kernel times stay uncontended)
   bool use_graph_ = false;  // KIFMM3D_GRAPH
   hipGraphExec_t gexec_ = nullptr;
   const double* gsrc_ = nullptr;
