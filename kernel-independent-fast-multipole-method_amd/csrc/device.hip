@@ -1694,6 +1694,11 @@ constexpr int M2LT_BS = 512;  // 8 waves: one per target parent of the tile
 constexpr int M2LT_FPB = 16;  // frequencies per block, visited in a software-pipelined loop
 constexpr int NOPS = 343;     // operator table entries ((o+3) in [0,7)^3)
 constexpr int NAENT = 26 * 32;  // A-operand table entries: direction x (source k, target child c)
+// FP32 form: directions summed on the matrix cores in FP32 before their sum
+// is added to the FP64 accumulators (0: all 26 in FP32)
+#ifndef M2L_F32_FLUSH
+#define M2L_F32_FLUSH 0
+#endif
 
 // The A operands depend only on (direction, frequency, lane), not on the
 // target parent, so the block builds them once per frequency: entry
@@ -1799,6 +1804,7 @@ __global__ __launch_bounds__(M2LT_BS, sizeof(R) == 4 ? 4 : 2) void k_m2l_tile(
     if (!live) continue;
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
     V4 d0 = {0, 0, 0, 0}, d1 = {0, 0, 0, 0};  // two chains for matrix-core overlap
+    int nacc = 0;
     // FP64 measured faster fully unrolled; FP32 needs the registers for 4 waves/SIMD
     constexpr int UNR = Mt<R>::KV ? 26 : 2;
 #pragma unroll UNR
@@ -1810,26 +1816,23 @@ __global__ __launch_bounds__(M2LT_BS, sizeof(R) == 4 ? 4 : 2) void k_m2l_tile(
       const R a0 = pim ? m0.y : m0.x, a1 = pim ? m1.y : m1.x;
       const R a2 = pim ? m0.x : -m0.y, a3 = pim ? m1.x : -m1.y;
       const C2 x0 = xs[(h * 8 + k) * 16 + i], x1 = xs[(h * 8 + 4 + k) * 16 + i];
-      if (Mt<R>::KV) {
-        d0 = Mt<R>::mfma(a0, x0.x, d0);
-        d1 = Mt<R>::mfma(a2, x0.y, d1);
-        d0 = Mt<R>::mfma(a1, x1.x, d0);
-        d1 = Mt<R>::mfma(a3, x1.y, d1);
-      } else {
-        // single precision: one direction per FP32 product, summed in FP64
-        V4 e0 = {0, 0, 0, 0}, e1 = {0, 0, 0, 0};
-        e0 = Mt<R>::mfma(a0, x0.x, e0);
-        e1 = Mt<R>::mfma(a2, x0.y, e1);
-        e0 = Mt<R>::mfma(a1, x1.x, e0);
-        e1 = Mt<R>::mfma(a3, x1.y, e1);
+      d0 = Mt<R>::mfma(a0, x0.x, d0);
+      d1 = Mt<R>::mfma(a2, x0.y, d1);
+      d0 = Mt<R>::mfma(a1, x1.x, d0);
+      d1 = Mt<R>::mfma(a3, x1.y, d1);
+      if (!Mt<R>::KV && M2L_F32_FLUSH && ++nacc == M2L_F32_FLUSH) {
+        // single precision: partial sums of M2L_F32_FLUSH directions go to FP64
 #pragma unroll
-        for (int v = 0; v < 4; v++) acc[v] += (double)(e0[v] + e1[v]);
+        for (int v = 0; v < 4; v++) {
+          acc[v] += (double)d0[v] + (double)d1[v];
+          d0[v] = 0;
+          d1[v] = 0;
+        }
+        nacc = 0;
       }
     }
-    if (Mt<R>::KV) {
 #pragma unroll
-      for (int v = 0; v < 4; v++) acc[v] = (double)d0[v] + (double)d1[v];
-    }
+    for (int v = 0; v < 4; v++) acc[v] += (double)d0[v] + (double)d1[v];
 #pragma unroll
     for (int h2 = 0; h2 < 2; h2++) {
       const int sl = ts[2 * k + h2];
