@@ -77,9 +77,12 @@ enum {
  * (fmm3d/fmm3d.cpp:23-27, fmm3d/tt.cpp:85-90) and reads the three options the
  * reference's setup requires (-fmm3d_np, -fmm3d_let3d_ptsmax,
  * -fmm3d_let3d_maxlevel; fmm3d_setup.cpp:30, let3d.cpp:45-48).
- * kt: 111 (Laplace single layer), 321 (Stokes double layer), 511 (Navier
- * single layer) -- fmm3d/kernel3d.hpp:29-53.  coefs: {mu} for 321, {mu, nu}
- * for 511 (kernel3d.cpp:224-226, :318-319); ignored for 111. */
+ * kt (fmm3d/kernel3d.hpp:29-53): 111 Laplace single layer, 121 Laplace double
+ * layer, 311 / 312 Stokes single layer velocity / pressure, 321 / 322 Stokes
+ * double layer velocity / pressure, 511 / 521 Navier single / double layer
+ * displacement.  coefs: {mu} for 311, 312, 321, 322 and {mu, nu} for 511, 521
+ * (kernel3d.cpp:187-189, :224-226, :318-319, :341-342); ignored for 111, 121.
+ * Kernels 121, 321, 322 and 521 need source normals. */
 int kifmm3d_create(kifmm3d** out, int kt, const double* coefs, int ncoefs, int np, int ptsmax,
                    int maxlevel, const double ctr[3], int rootlvl);
 

@@ -66,8 +66,13 @@ struct Point3 {
 
 enum {  // fmm3d/kernel3d.hpp:29-53 (the kernel types this evaluator supports)
   KNL_LAP_S_U = 111,
+  KNL_LAP_D_U = 121,
+  KNL_STK_S_U = 311,
+  KNL_STK_S_P = 312,
   KNL_STK_D_U = 321,
+  KNL_STK_D_P = 322,
   KNL_NAV_S_U = 511,
+  KNL_NAV_D_U = 521,
 };
 
 class Kernel3d {
@@ -78,8 +83,9 @@ class Kernel3d {
   const int& kt() const { return kt_; }
   std::vector<double>& coefs() { return coefs_; }
   const std::vector<double>& coefs() const { return coefs_; }
-  int sdof() const { return kt_ == 111 ? 1 : 3; }
-  int tdof() const { return kt_ == 111 ? 1 : 3; }
+  // kernel3d.cpp:24-85
+  int sdof() const { return (kt_ == 111 || kt_ == 121) ? 1 : 3; }
+  int tdof() const { return (kt_ == 111 || kt_ == 121 || kt_ == 312 || kt_ == 322) ? 1 : 3; }
 
  private:
   int kt_;

@@ -123,6 +123,108 @@ struct Knl<511> {
   }
 };
 
+// kernel3d.cpp:139-154: -(1/4 pi) (r.n) / r^3
+template <>
+struct Knl<121> {
+  static constexpr int S = 1, T = 1;
+  static constexpr bool NRM = true;
+  template <int NR>
+  __device__ static inline void add(double dx, double dy, double dz, const double* n, const double* f, double (&u)[NR][T],
+                                    const KC& k)
+  {
+    double ri = rinv(dx * dx + dy * dy + dz * dz);
+    double g = k.a * (ri * ri * ri) * (dx * n[0] + dy * n[1] + dz * n[2]);
+#pragma unroll
+    for (int q = 0; q < NR; q++) u[q][0] += g * f[q];
+  }
+};
+
+// kernel3d.cpp:185-203: (1/mu)(1/8 pi) (I / r + r r^T / r^3)
+template <>
+struct Knl<311> {
+  static constexpr int S = 3, T = 3;
+  static constexpr bool NRM = false;
+  template <int NR>
+  __device__ static inline void add(double dx, double dy, double dz, const double*, const double* f, double (&u)[NR][T],
+                                    const KC& k)
+  {
+    double ri = rinv(dx * dx + dy * dy + dz * dz);
+    double G = k.a * ri, H = k.a * (ri * ri * ri);
+#pragma unroll
+    for (int q = 0; q < NR; q++) {
+      const double* g = f + 3 * q;
+      double h = H * (dx * g[0] + dy * g[1] + dz * g[2]);
+      u[q][0] += G * g[0] + h * dx;
+      u[q][1] += G * g[1] + h * dy;
+      u[q][2] += G * g[2] + h * dz;
+    }
+  }
+};
+
+// kernel3d.cpp:204-221: (1/4 pi) r^T / r^3
+template <>
+struct Knl<312> {
+  static constexpr int S = 3, T = 1;
+  static constexpr bool NRM = false;
+  template <int NR>
+  __device__ static inline void add(double dx, double dy, double dz, const double*, const double* f, double (&u)[NR][T],
+                                    const KC& k)
+  {
+    double ri = rinv(dx * dx + dy * dy + dz * dz);
+    double H = k.a * (ri * ri * ri);
+#pragma unroll
+    for (int q = 0; q < NR; q++) {
+      const double* g = f + 3 * q;
+      u[q][0] += H * (dx * g[0] + dy * g[1] + dz * g[2]);
+    }
+  }
+};
+
+// kernel3d.cpp:243-265: (mu/2 pi) (n^T / r^3 - 3 (r.n) r^T / r^5)
+template <>
+struct Knl<322> {
+  static constexpr int S = 3, T = 1;
+  static constexpr bool NRM = true;
+  template <int NR>
+  __device__ static inline void add(double dx, double dy, double dz, const double* n, const double* f, double (&u)[NR][T],
+                                    const KC& k)
+  {
+    double ri = rinv(dx * dx + dy * dy + dz * dz), ri2 = ri * ri, ri3 = ri2 * ri;
+    double rn = dx * n[0] + dy * n[1] + dz * n[2];
+    double A = k.a * ri3, B = -3.0 * k.a * (ri3 * ri2) * rn;
+#pragma unroll
+    for (int q = 0; q < NR; q++) {
+      const double* g = f + 3 * q;
+      u[q][0] += A * (n[0] * g[0] + n[1] * g[1] + n[2] * g[2]) + B * (dx * g[0] + dy * g[1] + dz * g[2]);
+    }
+  }
+};
+
+// kernel3d.cpp:338-369: A ((r.n) I + n r^T) + B r n^T + C (r.n) r r^T with
+// A = dc1 / r^3, B = dc2 / r^3, C = dc3 / r^5
+template <>
+struct Knl<521> {
+  static constexpr int S = 3, T = 3;
+  static constexpr bool NRM = true;
+  template <int NR>
+  __device__ static inline void add(double dx, double dy, double dz, const double* n, const double* f, double (&u)[NR][T],
+                                    const KC& k)
+  {
+    double ri = rinv(dx * dx + dy * dy + dz * dz), ri2 = ri * ri, ri3 = ri2 * ri;
+    double rn = dx * n[0] + dy * n[1] + dz * n[2];
+    double A = k.a * ri3, B = k.b * ri3, C = k.c * (ri3 * ri2) * rn;
+#pragma unroll
+    for (int q = 0; q < NR; q++) {
+      const double* g = f + 3 * q;
+      double df = dx * g[0] + dy * g[1] + dz * g[2], nf = n[0] * g[0] + n[1] * g[1] + n[2] * g[2];
+      double ar = A * rn, an = A * df, bt = B * nf + C * df;
+      u[q][0] += ar * g[0] + an * n[0] + bt * dx;
+      u[q][1] += ar * g[1] + an * n[1] + bt * dy;
+      u[q][2] += ar * g[2] + an * n[2] + bt * dz;
+    }
+  }
+};
+
 // ---------------------------------------------------------------------------
 // permutations
 __global__ void k_gather(int n, int dof, int nr, const int* __restrict__ perm, const double* __restrict__ src,
@@ -605,9 +707,11 @@ void direct(hipStream_t s, int kt, int tdof, int nchk, const double* cx, const d
   if (nchk <= 0) return;
   const int seglen = (nsrc + nseg - 1) / nseg;
   switch (kt) {
-  case 111: direct_kt<111>(s, nchk, cx, cy, cz, nsrc, nseg, seglen, src, part, nr, kc); break;
-  case 321: direct_kt<321>(s, nchk, cx, cy, cz, nsrc, nseg, seglen, src, part, nr, kc); break;
-  case 511: direct_kt<511>(s, nchk, cx, cy, cz, nsrc, nseg, seglen, src, part, nr, kc); break;
+#define KIFMM_DIRECT(KT) \
+  case KT: direct_kt<KT>(s, nchk, cx, cy, cz, nsrc, nseg, seglen, src, part, nr, kc); break;
+  KIFMM_DIRECT(111) KIFMM_DIRECT(121) KIFMM_DIRECT(311) KIFMM_DIRECT(312) KIFMM_DIRECT(321) KIFMM_DIRECT(322)
+  KIFMM_DIRECT(511) KIFMM_DIRECT(521)
+#undef KIFMM_DIRECT
   }
   long tot = (long)nr * nchk * tdof;
   hipLaunchKernelGGL(k_direct_reduce, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, nchk * tdof, nseg, nr,
@@ -782,9 +886,11 @@ void p2p(hipStream_t s, int kt, int nchunk, const int* tbeg, const int* tnum, co
 {
   if (nchunk <= 0) return;
   switch (kt) {
-  case 111: p2p_kt<111>(s, nchunk, tbeg, tnum, eptr, ent, src, trg, nr, kc, variant); break;
-  case 321: p2p_kt<321>(s, nchunk, tbeg, tnum, eptr, ent, src, trg, nr, kc, variant); break;
-  case 511: p2p_kt<511>(s, nchunk, tbeg, tnum, eptr, ent, src, trg, nr, kc, variant); break;
+#define KIFMM_P2P(KT) \
+  case KT: p2p_kt<KT>(s, nchunk, tbeg, tnum, eptr, ent, src, trg, nr, kc, variant); break;
+  KIFMM_P2P(111) KIFMM_P2P(121) KIFMM_P2P(311) KIFMM_P2P(312) KIFMM_P2P(321) KIFMM_P2P(322) KIFMM_P2P(511)
+  KIFMM_P2P(521)
+#undef KIFMM_P2P
   }
 }
 
@@ -812,9 +918,11 @@ void p2s(hipStream_t s, int kt, int nitem, const int* out_node, const double4* g
 {
   if (nitem <= 0) return;
   switch (kt) {
-  case 111: p2s_kt<111>(s, nitem, out_node, geom, eptr, ent, unit, npt, src, out, Rc, pln, nr, kc); break;
-  case 321: p2s_kt<321>(s, nitem, out_node, geom, eptr, ent, unit, npt, src, out, Rc, pln, nr, kc); break;
-  case 511: p2s_kt<511>(s, nitem, out_node, geom, eptr, ent, unit, npt, src, out, Rc, pln, nr, kc); break;
+#define KIFMM_P2S(KT) \
+  case KT: p2s_kt<KT>(s, nitem, out_node, geom, eptr, ent, unit, npt, src, out, Rc, pln, nr, kc); break;
+  KIFMM_P2S(111) KIFMM_P2S(121) KIFMM_P2S(311) KIFMM_P2S(312) KIFMM_P2S(321) KIFMM_P2S(322) KIFMM_P2S(511)
+  KIFMM_P2S(521)
+#undef KIFMM_P2S
   }
 }
 

@@ -19,6 +19,23 @@ KernelInfo kernel_info(int kt, const double* coefs, int ncoefs)
   case 111:  // KNL_LAP_S_U
     k.sdof = 1; k.tdof = 1; k.deg[0] = 1;
     break;
+  case 121:  // KNL_LAP_D_U: Laplace double layer
+    k.sdof = 1; k.tdof = 1; k.needs_normal = true;
+    break;
+  case 311:  // KNL_STK_S_U: Stokes single layer velocity (Stokeslet)
+    k.sdof = 3; k.tdof = 3;
+    break;
+  case 312:  // KNL_STK_S_P: Stokes single layer pressure
+    k.sdof = 3; k.tdof = 1;
+    break;
+  case 322:  // KNL_STK_D_P: Stokes double layer pressure
+    k.sdof = 3; k.tdof = 1; k.needs_normal = true;
+    break;
+  case 521:  // KNL_NAV_D_U: Navier double layer displacement
+    k.sdof = 3; k.tdof = 3; k.needs_normal = true;
+    if (ncoefs < 2) throw Error(KIFMM3D_EINVAL, "kernel 521 needs coefs {mu, nu}");
+    if (k.coefs[1] == 1.0) throw Error(KIFMM3D_EINVAL, "kernel 521 needs nu != 1");
+    break;
   case 301:  // KNL_STK_F_U: Stokeslet plus potential source, mm kernel of 321
     k.sdof = 4; k.tdof = 3; k.deg[0] = k.deg[1] = k.deg[2] = 1; k.deg[3] = 2;
     break;
@@ -31,9 +48,11 @@ KernelInfo kernel_info(int kt, const double* coefs, int ncoefs)
     if (k.coefs[1] == 1.0) throw Error(KIFMM3D_EINVAL, "kernel 511 needs nu != 1");
     break;
   default:
-    throw Error(KIFMM3D_EINVAL, "unsupported kernel type " + std::to_string(kt) + " (supported: 111, 321, 511)");
+    throw Error(KIFMM3D_EINVAL, "unsupported kernel type " + std::to_string(kt) +
+                                    " (supported: 111, 121, 311, 312, 321, 322, 511, 521)");
   }
-  if ((kt == 301 || kt == 321 || kt == 511) && k.coefs[0] == 0.0) throw Error(KIFMM3D_EINVAL, "mu must be nonzero");
+  if ((kt == 301 || kt == 311 || kt == 321 || kt == 511 || kt == 521) && k.coefs[0] == 0.0)
+    throw Error(KIFMM3D_EINVAL, "mu must be nonzero");
   return k;
 }
 
@@ -41,8 +60,13 @@ int mm_kernel_type(int kt)
 {
   switch (kt) {
   case 111: return 111;
+  case 121: return 111;
+  case 311: return 301;
+  case 312: return 111;
   case 321: return 301;
+  case 322: return 111;
   case 511: return 511;
+  case 521: return 511;
   default: return -1;
   }
 }
@@ -59,6 +83,17 @@ KernelConsts kernel_consts(const KernelInfo& k)
     break;
   }
   case 321: c.a = -(6.0 / (8.0 * M_PI)); break;
+  case 121: c.a = -(1.0 / (4.0 * M_PI)); break;
+  case 311: c.a = (1.0 / k.coefs[0]) * (1.0 / (8.0 * M_PI)); break;
+  case 312: c.a = 1.0 / (4.0 * M_PI); break;
+  case 322: c.a = k.coefs[0] * (1.0 / (2.0 * M_PI)); break;
+  case 521: {
+    double ve = k.coefs[1];
+    c.a = -(1 - 2.0 * ve) / (8.0 * M_PI * (1.0 - ve));
+    c.b = (1 - 2.0 * ve) / (8.0 * M_PI * (1.0 - ve));
+    c.c = -3.0 / (8.0 * M_PI * (1.0 - ve));
+    break;
+  }
   case 511: {
     double ve = k.coefs[1];
     c.a = (3.0 - 4.0 * ve) / (16.0 * M_PI * (1.0 - ve));
@@ -105,6 +140,31 @@ void kernel_block(const KernelInfo& k, const KernelConsts& kc, double x, double 
     double G = kc.a / r, H = kc.b / r3;
     for (int t = 0; t < 3; t++)
       for (int s = 0; s < 3; s++) b[t * 3 + s] = kc.c * ((t == s ? G : 0.0) + H * d[t] * d[s]);
+    break;
+  }
+  case 121: b[0] = kc.a / (r2 * r) * (x * nrm[0] + y * nrm[1] + z * nrm[2]); break;
+  case 311: {
+    double G = kc.a / r, H = kc.a / (r2 * r);
+    for (int t = 0; t < 3; t++)
+      for (int s = 0; s < 3; s++) b[t * 3 + s] = (t == s ? G : 0.0) + H * d[t] * d[s];
+    break;
+  }
+  case 312:
+    for (int s = 0; s < 3; s++) b[s] = kc.a * d[s] / (r2 * r);
+    break;
+  case 322: {
+    double rn = x * nrm[0] + y * nrm[1] + z * nrm[2];
+    double r3 = r2 * r, r5 = r3 * r2;
+    for (int s = 0; s < 3; s++) b[s] = kc.a * (nrm[s] / r3 - 3 * rn * d[s] / r5);
+    break;
+  }
+  case 521: {
+    double rn = x * nrm[0] + y * nrm[1] + z * nrm[2];
+    double r3 = r2 * r, r5 = r3 * r2;
+    double A = kc.a / r3, B = kc.b / r3, C = kc.c / r5;
+    for (int t = 0; t < 3; t++)
+      for (int s = 0; s < 3; s++)
+        b[t * 3 + s] = A * ((t == s ? rn : 0.0) + nrm[t] * d[s]) + B * (d[t] * nrm[s]) + C * rn * d[t] * d[s];
     break;
   }
   }

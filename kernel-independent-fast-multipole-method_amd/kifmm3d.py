@@ -75,9 +75,15 @@ KCLASSES = ["gather", "s2m", "gemm_up", "p2p", "fft_fwd", "m2l", "fft_inv", "s2l
 
 # kernel ids of fmm3d/kernel3d.hpp:29-53 supported by this evaluator
 KNL_LAP_S_U = 111
+KNL_LAP_D_U = 121
+KNL_STK_S_U = 311
+KNL_STK_S_P = 312
 KNL_STK_D_U = 321
+KNL_STK_D_P = 322
 KNL_NAV_S_U = 511
-_DOFS = {111: (1, 1), 321: (3, 3), 511: (3, 3), 301: (4, 3)}
+KNL_NAV_D_U = 521
+_DOFS = {111: (1, 1), 121: (1, 1), 311: (3, 3), 312: (3, 1), 321: (3, 3), 322: (3, 1), 511: (3, 3), 521: (3, 3),
+         301: (4, 3)}
 
 _lib = None
 

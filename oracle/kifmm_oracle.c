@@ -97,9 +97,14 @@ int orc_kernel_dofs(int kt, int* sdof, int* tdof)
 {
   switch (kt) {
   case 111: *sdof = 1; *tdof = 1; return OK;
+  case 121: *sdof = 1; *tdof = 1; return OK;
   case 301: *sdof = 4; *tdof = 3; return OK;
+  case 311: *sdof = 3; *tdof = 3; return OK;
+  case 312: *sdof = 3; *tdof = 1; return OK;
   case 321: *sdof = 3; *tdof = 3; return OK;
+  case 322: *sdof = 3; *tdof = 1; return OK;
   case 511: *sdof = 3; *tdof = 3; return OK;
+  case 521: *sdof = 3; *tdof = 3; return OK;
   default: return ERR;
   }
 }
@@ -108,8 +113,13 @@ int orc_kernel_mm(int kt)
 {
   switch (kt) {
   case 111: return 111;
+  case 121: return 111;
+  case 311: return 301;
+  case 312: return 111;
   case 321: return 301;
+  case 322: return 111;
   case 511: return 511;
+  case 521: return 511;
   default: return ERR;
   }
 }
@@ -125,7 +135,9 @@ static int kernel_homdeg(int kt, int* deg)
 }
 
 /* One tdof x sdof block K(x = trg - src, n = src normal); b[t*sdof+s].
- * kernel3d.cpp:124-138 (111), :160-184 (301), :222-242 (321), :315-337 (511). */
+ * kernel3d.cpp:124-138 (111), :139-154 (121), :160-184 (301), :185-203 (311),
+ * :204-221 (312), :222-242 (321), :243-265 (322), :315-337 (511),
+ * :338-369 (521). */
 static void kblock(const oknl* k, double x, double y, double z, const double* nrm, double* b)
 {
   double r2 = x * x + y * y + z * z;
@@ -154,6 +166,51 @@ static void kblock(const oknl* k, double x, double y, double z, const double* nr
     double d[3] = {x, y, z};
     for (int t = 0; t < 3; t++)
       for (int s = 0; s < 3; s++) b[t * 3 + s] = C * rn * d[t] * d[s];
+    return;
+  }
+  if (k->kt == 121) {
+    if (r < MINDIF) { b[0] = 0.0; return; }
+    double rn = x * nrm[0] + y * nrm[1] + z * nrm[2];
+    b[0] = -(1.0 / (4.0 * M_PI)) / (r2 * r) * rn;
+    return;
+  }
+  if (k->kt == 311) {
+    if (r < MINDIF) { for (int i = 0; i < 9; i++) b[i] = 0.0; return; }
+    double oomu = 1.0 / k->c[0], OOEP = 1.0 / (8.0 * M_PI);
+    double G = OOEP / r, H = OOEP / (r2 * r);
+    double d[3] = {x, y, z};
+    for (int t = 0; t < 3; t++)
+      for (int s = 0; s < 3; s++) b[t * 3 + s] = oomu * ((t == s ? G : 0.0) + H * d[t] * d[s]);
+    return;
+  }
+  if (k->kt == 312) {
+    if (r < MINDIF) { for (int i = 0; i < 3; i++) b[i] = 0.0; return; }
+    double OOFP = 1.0 / (4.0 * M_PI), r3 = r2 * r;
+    b[0] = OOFP * x / r3; b[1] = OOFP * y / r3; b[2] = OOFP * z / r3;
+    return;
+  }
+  if (k->kt == 322) {
+    if (r < MINDIF) { for (int i = 0; i < 3; i++) b[i] = 0.0; return; }
+    double coef = k->c[0] * (1.0 / (2.0 * M_PI));
+    double rn = x * nrm[0] + y * nrm[1] + z * nrm[2];
+    double r3 = r2 * r, r5 = r3 * r2;
+    b[0] = coef * (nrm[0] / r3 - 3 * rn * x / r5);
+    b[1] = coef * (nrm[1] / r3 - 3 * rn * y / r5);
+    b[2] = coef * (nrm[2] / r3 - 3 * rn * z / r5);
+    return;
+  }
+  if (k->kt == 521) {
+    if (r < MINDIF) { for (int i = 0; i < 9; i++) b[i] = 0.0; return; }
+    double ve = k->c[1];
+    double dc1 = -(1 - 2.0 * ve) / (8.0 * M_PI * (1.0 - ve));
+    double dc2 = (1 - 2.0 * ve) / (8.0 * M_PI * (1.0 - ve));
+    double dc3 = -3.0 / (8.0 * M_PI * (1.0 - ve));
+    double rn = x * nrm[0] + y * nrm[1] + z * nrm[2];
+    double r3 = r2 * r, r5 = r3 * r2;
+    double A = dc1 / r3, B = dc2 / r3, C = dc3 / r5;
+    double d[3] = {x, y, z};
+    for (int t = 0; t < 3; t++)
+      for (int s = 0; s < 3; s++) b[t * 3 + s] = A * ((t == s ? rn : 0.0) + nrm[t] * d[s]) + B * (d[t] * nrm[s]) + C * rn * d[t] * d[s];
     return;
   }
   if (k->kt == 511) {

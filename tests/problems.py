@@ -1,7 +1,7 @@
 """Seeded point sets and densities shared by the tests (numpy generators only)."""
 import numpy as np
 
-COEFS = {111: [1.0], 321: [1.0], 511: [1.0, 0.3]}
+COEFS = {111: [1.0], 121: [1.0], 311: [1.3], 312: [1.3], 321: [1.0], 322: [1.3], 511: [1.0, 0.3], 521: [1.2, 0.3]}
 
 
 def uniform_cube(n, seed):

@@ -156,3 +156,16 @@ def test_vector_kernels_16rhs(kt):
         p, nrm = sphere(2500, 15)
     ctr, lvl = ((0.0, 0.0, 0.0), 0) if kt == 321 else ((0.5, 0.5, 0.5), 1)
     run_case(f"kt{kt}-16rhs", kt, 4, 40, p, nrm, p, ctr, lvl, nrhs=16)
+
+
+@pytest.mark.parametrize("kt", [121, 311, 312, 322, 521])
+def test_other_kernel_types(kt):
+    """The remaining kernel types of the reference behind the same API
+    (kernel3d.cpp:139-369, translation kernels fmm3d_setup.cpp:41-54)."""
+    if kt in (121, 322, 521):
+        p, n = sphere(4000, 40 + kt % 7, 0.9)
+        ctr, lvl = (0.0, 0.0, 0.0), 0
+    else:
+        p, n = clusters(4000, 40 + kt % 7), None
+        ctr, lvl = (0.5, 0.5, 0.5), 1
+    run_case(f"kt{kt}", kt, 4, 40, p, n, p, ctr, lvl, nrhs=2)

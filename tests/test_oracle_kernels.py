@@ -54,11 +54,16 @@ X0 = np.array([0.31, -0.22, 0.47])
 
 
 def test_dofs_and_mm_map():
-    assert O.kernel_dofs(111) == (1, 1)
-    assert O.kernel_dofs(301) == (4, 3)
-    assert O.kernel_dofs(321) == (3, 3)
-    assert O.kernel_dofs(511) == (3, 3)
-    assert O.kernel_mm(111) == 111 and O.kernel_mm(321) == 301 and O.kernel_mm(511) == 511
+    # kernel3d.cpp:24-85 (dofs), fmm3d_setup.cpp:41-54 (translation kernels)
+    dofs = {111: (1, 1), 121: (1, 1), 301: (4, 3), 311: (3, 3), 312: (3, 1), 321: (3, 3), 322: (3, 1),
+            511: (3, 3), 521: (3, 3)}
+    for kt, d in dofs.items():
+        assert O.kernel_dofs(kt) == d
+    mm = {111: 111, 121: 111, 311: 301, 312: 111, 321: 301, 322: 111, 511: 511, 521: 511}
+    for kt, m in mm.items():
+        assert O.kernel_mm(kt) == m
+        # the user kernel's target dof equals the translation kernel's (check surfaces are shared)
+        assert O.kernel_dofs(kt)[1] == O.kernel_dofs(m)[1]
     with pytest.raises(ValueError):
         O.kernel_dofs(999)
 
@@ -76,7 +81,7 @@ def test_laplace_harmonic_and_flux():
     assert abs(flux - 1.0) < 1e-4
 
 
-@pytest.mark.parametrize("kt", [111, 301, 321, 511])
+@pytest.mark.parametrize("kt", [111, 121, 301, 311, 312, 321, 322, 511, 521])
 def test_self_interaction_is_zero(kt):
     m = K(kt, [0.0, 0.0, 0.0], n=[0, 0, 1])
     assert np.all(m == 0.0)
@@ -149,3 +154,68 @@ def test_homogeneity_degrees(kt, deg):
 @pytest.mark.parametrize("kt", [111, 511])
 def test_reciprocity(kt):
     assert np.allclose(K(kt, X0), K(kt, -X0).T, rtol=1e-15)
+
+
+# ---- the other kernel types of the reference (kernel3d.cpp:139-154, 185-221, 243-265, 338-369)
+
+def test_laplace_double_layer_is_normal_derivative_and_jump():
+    """121 = n . grad_x of the single layer (source at the origin, normal n), harmonic,
+    and its integral over the unit sphere with outward normals is 1 inside, 0 outside."""
+    n = np.array([0.2, -0.6, 0.77])
+    n = n / np.linalg.norm(n)
+    d = K(121, X0, n)[0, 0]
+    g = fd_grad(lambda x: K(111, x)[0, 0], X0)
+    assert abs(d - np.dot(n, g)) < 1e-7 * abs(d)
+    assert abs(fd_laplacian(lambda x: K(121, x, n)[0, 0], X0)) < 1e-4 * abs(d) / np.dot(X0, X0)
+    p, w = sphere_quadrature(20000)
+    for x, expect in [(np.array([0.1, -0.2, 0.15]), 1.0), (np.array([1.6, 0.3, -0.4]), 0.0)]:
+        m = O.kernel_matrix(121, [], p, p, x[None, :])
+        assert abs(np.dot(m[0], w) - expect) < 2e-3
+
+
+def test_stokes_single_layer_velocity_pressure_pair():
+    """311 is the Stokeslet (the velocity block of 301) and (u, p) = (311, 312)
+    solve mu lap u = grad p, div u = 0."""
+    assert np.allclose(K(311, X0), K(301, X0)[:, :3], rtol=1e-15, atol=0)
+    u = lambda x: K(311, x)
+    p = lambda x: K(312, x)[0]  # p[j]: pressure of a unit force along j
+    lap = fd_laplacian(u, X0)
+    gp = fd_grad(p, X0)
+    assert np.allclose(MU * lap, gp.T, atol=1e-5 * np.abs(gp).max())
+    assert np.all(np.abs(np.einsum("ddj->j", fd_grad(u, X0))) < 1e-7)
+
+
+def test_stokes_double_layer_velocity_pressure_pair():
+    """(u, p) = (321, 322) for a source normal n solve mu lap u = grad p."""
+    n = np.array([0.3, -0.5, 0.81])
+    n = n / np.linalg.norm(n)
+    u = lambda x: O.kernel_matrix(321, [MU], np.zeros((1, 3)), n[None, :], np.atleast_2d(x))
+    p = lambda x: O.kernel_matrix(322, [MU], np.zeros((1, 3)), n[None, :], np.atleast_2d(x))[0]
+    lap = fd_laplacian(u, X0, h=2e-3)
+    gp = fd_grad(p, X0, h=1e-4)
+    assert np.allclose(MU * lap, gp.T, atol=2e-4 * np.abs(gp).max())
+
+
+def test_navier_double_layer_solves_lame_and_jump():
+    """521 solves the Lame equation in the target and its integral over the unit
+    sphere with outward normals is a constant multiple of the identity inside
+    (magnitude 1, the double-layer jump) and zero outside."""
+    lam = 2 * MU * NU / (1 - 2 * NU)
+    n = np.array([-0.4, 0.1, 0.9])
+    n = n / np.linalg.norm(n)
+    u = lambda x: K(521, x, n)
+    lap = fd_laplacian(u, X0, h=2e-3)
+    div = lambda x: np.einsum("ddj->j", fd_grad(u, x, h=1e-4))
+    gd = fd_grad(div, X0, h=1e-3)
+    res = MU * lap + (lam + MU) * gd
+    assert np.abs(res).max() < 5e-4 * np.abs(MU * lap).max()
+    p, w = sphere_quadrature(20000)
+    tot_in = None
+    for x, inside in [(np.array([0.1, -0.2, 0.15]), True), (np.array([1.6, 0.3, -0.4]), False)]:
+        m = O.kernel_matrix(521, [MU, NU], p, p, x[None, :])
+        tot = sum(m[:, 3 * j:3 * j + 3] * w[j] for j in range(len(w)))
+        if inside:
+            c = tot[0, 0]
+            assert abs(abs(c) - 1.0) < 2e-3 and np.allclose(tot, c * np.eye(3), atol=2e-3)
+        else:
+            assert np.allclose(tot, 0.0, atol=2e-3)
