@@ -152,6 +152,19 @@ int kifmm3d_plan_stats(const kifmm3d* h, double* out, int n);
 const char* kifmm3d_last_error(const kifmm3d* h);
 void kifmm3d_destroy(kifmm3d* h);
 
+/* Operator cache (no reference counterpart; the reference rebuilds its
+ * pseudoinverses and M2L spectra lazily in the first eval of every process,
+ * matmgnt3d.cpp:96-105, 163-171, 307-325).  With a directory set (or
+ * KIFMM3D_OPS_CACHE in the environment), setup reads the operators of its
+ * (translation kernel, coefficients, np) from a validated binary file there
+ * and writes the file after building them; dir NULL or "" turns it off.
+ * Files whose header, sizes or checksum do not match are rebuilt. */
+int kifmm3d_set_operator_cache(const char* dir);
+/* Operator sets built and loaded from disk since the process started. */
+int kifmm3d_operator_cache_stats(int* built, int* loaded);
+/* Free the in-memory operator sets that no handle uses; returns how many. */
+int kifmm3d_release_operators(void);
+
 /* Graph replay for repeated evaluations (no reference counterpart; for
  * callers that evaluate the same buffers many times): with on != 0,
  * kifmm3d_evaluate_device captures the whole pass into one hipGraph the first

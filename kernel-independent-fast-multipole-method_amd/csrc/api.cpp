@@ -8,6 +8,7 @@
 
 #include "device.hpp"
 #include "engine.hpp"
+#include "operators.hpp"
 #include "kifmm3d.h"
 
 struct kifmm3d {
@@ -201,6 +202,21 @@ const char* kifmm3d_last_error(const kifmm3d* h)
 }
 
 void kifmm3d_destroy(kifmm3d* h) { delete h; }
+
+int kifmm3d_set_operator_cache(const char* dir)
+{
+  kifmm::set_operator_cache_dir(dir ? std::string(dir) : std::string());
+  return KIFMM3D_OK;
+}
+
+int kifmm3d_operator_cache_stats(int* built, int* loaded)
+{
+  if (!built || !loaded) return KIFMM3D_EINVAL;
+  kifmm::operator_cache_stats(built, loaded);
+  return KIFMM3D_OK;
+}
+
+int kifmm3d_release_operators(void) { return kifmm::release_operators(); }
 
 int kifmm3d_set_graph(kifmm3d* h, int on)
 {

@@ -4,6 +4,7 @@
 #pragma once
 
 #include <memory>
+#include <string>
 #include <vector>
 
 #include "common.hpp"
@@ -30,7 +31,13 @@ struct Operators {
   std::vector<short> cube2surf;  // np^3 -> surface point index (x-major order) or -1
 };
 
+// Operators for (mm kernel, np): from the in-memory cache, else from the
+// on-disk cache directory (set_operator_cache_dir or KIFMM3D_OPS_CACHE) when
+// a valid file is there, else built (and written to the directory).
 std::shared_ptr<const Operators> get_operators(const KernelInfo& mm, int np);
+void set_operator_cache_dir(const std::string& dir);  // "" disables the disk cache
+void operator_cache_stats(int* built, int* loaded);     // counts since process start
+int release_operators();                                // drop in-memory sets no handle uses
 
 // Unit surface grid of matmgnt3d.cpp:375-397 (boundary points of an n^3 grid on [-R, R]^3).
 std::vector<double> surface_grid(int n, double R);

@@ -59,6 +59,9 @@ SYMBOLS = [
     ("kifmm3d_device_synchronize", ctypes.c_int, []),
     ("kifmm3d_device_count", ctypes.c_int, [_I]),
     ("kifmm3d_device_copy", ctypes.c_int, [_VP, _VP, ctypes.c_long]),
+    ("kifmm3d_set_operator_cache", ctypes.c_int, [ctypes.c_char_p]),
+    ("kifmm3d_operator_cache_stats", ctypes.c_int, [_I, _I]),
+    ("kifmm3d_release_operators", ctypes.c_int, []),
     ("kifmm3d_set_graph", ctypes.c_int, [_VP, ctypes.c_int]),
     ("kifmm3d_batch_create", ctypes.c_int, [ctypes.POINTER(_VP), ctypes.POINTER(_VP), ctypes.c_int]),
     ("kifmm3d_batch_evaluate_device", ctypes.c_int, [_VP, ctypes.POINTER(_VP), ctypes.POINTER(_VP), _I, ctypes.c_int]),
@@ -107,6 +110,23 @@ def lib():
             f.argtypes = args
         _lib = L
     return _lib
+
+
+def set_operator_cache(directory):
+    """Read/write the translation operators from/to files in `directory` (None disables)."""
+    lib().kifmm3d_set_operator_cache(directory.encode() if directory else None)
+
+
+def operator_cache_stats():
+    """(operator sets built, operator sets loaded from disk) since the process started."""
+    b, l_ = ctypes.c_int(0), ctypes.c_int(0)
+    lib().kifmm3d_operator_cache_stats(ctypes.byref(b), ctypes.byref(l_))
+    return b.value, l_.value
+
+
+def release_operators() -> int:
+    """Free the in-memory operator sets no handle uses."""
+    return lib().kifmm3d_release_operators()
 
 
 def device_count() -> int:

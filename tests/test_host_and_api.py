@@ -143,3 +143,36 @@ def test_options_file_and_missing_keys(tmp_path):
     with pytest.raises(K.KifmmError) as e:
         f.setup(opts)
     assert "maxlevel" in str(e.value)
+
+
+def test_operator_disk_cache_roundtrip_and_corruption(tmp_path):
+    """Operators written to the cache directory are read back bit for bit; a
+    corrupted file is detected by its checksum and rebuilt."""
+    K.set_operator_cache(str(tmp_path))
+    try:
+        K.release_operators()
+        b0, l0 = K.operator_cache_stats()
+        f1, _ = setup_host(511, 4, n=1500, seed=3)
+        ref = [f1.operator(w).copy() for w in (0, 1)]
+        files = sorted(tmp_path.iterdir())
+        assert len(files) == 1 and files[0].name.startswith("kifmm3d_ops_v1_kt511_np4_")
+        b1, l1 = K.operator_cache_stats()
+        assert (b1, l1) == (b0 + 1, l0)
+        del f1
+        assert K.release_operators() >= 1
+        f2, _ = setup_host(511, 4, n=1500, seed=4)
+        assert K.operator_cache_stats() == (b1, l1 + 1)
+        for w, r in zip((0, 1), ref):
+            assert np.array_equal(f2.operator(w), r)
+        del f2
+        K.release_operators()
+        data = bytearray(files[0].read_bytes())
+        data[len(data) // 2] ^= 0x40
+        files[0].write_bytes(bytes(data))
+        f3, _ = setup_host(511, 4, n=1500, seed=5)
+        assert K.operator_cache_stats() == (b1 + 1, l1 + 1)
+        for w, r in zip((0, 1), ref):
+            assert np.array_equal(f3.operator(w), r)
+        assert files[0].read_bytes() != bytes(data)  # rewritten after the rebuild
+    finally:
+        K.set_operator_cache(None)
