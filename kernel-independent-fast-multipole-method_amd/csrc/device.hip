@@ -1968,5 +1968,167 @@ void m2l_tiled(hipStream_t s, int C, int ntile, int Rc, const int* ttile, const 
                        (const double2*)opsT, (const double2*)effden, (double2*)effval, 0);
 }
 
+// ---------------------------------------------------------------------------
+// M2L on the matrix cores for any kernel and any number of right-hand sides
+// (spectra in the [slot][rhs][dof][C] layout of the grouped plan).
+//
+// For one frequency f and neighbour direction n, the M2L of a target parent
+// T is the complex product Y_T[(c,t)] += sum_{(s,j)} M_n[(c,t)][(s,j)]
+// X_{T+n}[(s,j)] over the 8 TM outputs (child c, dof t) and the 8 SM inputs
+// (source child s, dof j), M_n[(c,t)][(s,j)] = T_{optab[n][s][c]}[j][t](f).
+// M_n does not depend on T, so the product is batched over columns =
+// (target parent, right-hand side) pairs, 16 per matrix-core tile, in real form
+//   [Yr; Yi] += [[Mr, -Mi], [Mi, Mr]] [Xr; Xi]
+// with v_mfma_f64_16x16x4f64.  Row tile rt < TM holds 16 real rows: row r16
+// is output rt*8 + r16%8 (flattened (c, t)), real part for r16 < 8.  The 16 SM
+// real inputs are cut into 4 SM slices of 4; input kappa is part kappa/(8 SM)
+// of flattened (s, j) = kappa % (8 SM).  With the output layout of the
+// instruction (item v of lane l is row l/16 + 4v) lane (k, col) ends with
+// (re, im) of outputs rt*8 + k and rt*8 + k + 4.
+//
+// A block covers one frequency and 4 G tiles of 16 columns (G per wave).  For
+// each direction the block builds [[Mr,-Mi],[Mi,Mr]] in LDS from the
+// frequency-major operator copy opsT [C][343][SM][TM] (one contiguous run per
+// block) and every wave applies it to its G tiles.  pmask[q] has bit n set if
+// parent q has a source child in direction n; a tile whose parents have none
+// skips the direction.  The blocks of one column block and 8 neighbouring
+// frequencies are dealt to one XCD side by side, so each 128-byte line of
+// spectra (8 frequencies) is fetched into its L2 once for all 8.
+constexpr int M2LV_BS = 256;
+
+template <int SM, int TM, int G>
+__global__ __launch_bounds__(M2LV_BS) void k_m2l_vec(int C, int Rc, long ncol, int ncb, const int* __restrict__ tslot,
+                                                     const int* __restrict__ sslot, const int* __restrict__ pmask,
+                                                     const int* __restrict__ optab, const double2* __restrict__ opsT,
+                                                     const double2* __restrict__ effden,
+                                                     double2* __restrict__ effval)
+{
+  constexpr int NKS = 4 * SM, NH = NKS / 2, ST = SM * TM;
+  __shared__ double s_A[TM][NKS][64];
+  __shared__ int s_ot[26 * 64];
+  // items (frequency group of 8, column block, frequency in group): the 8
+  // frequencies of one spectra line run side by side on one XCD
+  const int nitem = ((C + 7) / 8) * ncb * 8;
+  const int item = xcd_item(blockIdx.x, nitem);
+  const int fg = item / (8 * ncb), rem = item - fg * 8 * ncb, cb = rem >> 3, f = fg * 8 + (rem & 7);
+  if (item >= nitem || f >= C) return;  // block-uniform
+  const int t = threadIdx.x, w = t >> 6, l = t & 63, li = l & 15, lk = l >> 4;
+  const double2* Of = opsT + (long)f * 343 * ST;
+  for (int i = t; i < 26 * 64; i += M2LV_BS) s_ot[i] = optab[i];
+  int q[G], r[G], msk[G];
+#pragma unroll
+  for (int g = 0; g < G; g++) {
+    const long col = (((long)cb * 4 + w) * G + g) * 16 + li;
+    q[g] = col < ncol ? (int)(col / Rc) : -1;
+    r[g] = col < ncol ? (int)(col - (long)(col / Rc) * Rc) : 0;
+    msk[g] = q[g] >= 0 ? pmask[q[g]] : 0;
+  }
+  v4d acc[G][TM];
+#pragma unroll
+  for (int g = 0; g < G; g++)
+#pragma unroll
+    for (int rt = 0; rt < TM; rt++) acc[g][rt] = v4d{0.0, 0.0, 0.0, 0.0};
+  for (int n = 0; n < 26; n++) {
+    __syncthreads();  // s_ot is staged; the previous direction is done with s_A
+    // the block's 64 TM SM entries of the real-embedded operator, TM SM per
+    // thread: all loads issued (clamped to operator 0 where the pair has none)
+    // before the first use
+    {
+      int oo[ST];
+      double2 m[ST];
+#pragma unroll
+      for (int i = 0; i < ST; i++) {
+        const int e = t + M2LV_BS * i, ll = e & 63, kr = e >> 6, ks = kr % NKS, rt = kr / NKS;
+        const int qo = rt * 8 + (ll & 7), c = qo / TM, ti = qo - c * TM;
+        const int kap = ks * 4 + (ll >> 4), cs = kap % (8 * SM), sc = cs / SM, j = cs - sc * SM;
+        oo[i] = s_ot[n * 64 + sc * 8 + c];
+        m[i] = Of[(long)(oo[i] < 0 ? 0 : oo[i]) * ST + j * TM + ti];
+      }
+#pragma unroll
+      for (int i = 0; i < ST; i++) {
+        const int e = t + M2LV_BS * i, ll = e & 63, kr = e >> 6, ks = kr % NKS, rt = kr / NKS;
+        const int pin = (ks * 4 + (ll >> 4)) / (8 * SM);
+        const double v = ((ll & 15) < 8) ? (pin == 0 ? m[i].x : -m[i].y) : (pin == 0 ? m[i].y : m[i].x);
+        s_A[rt][ks][ll] = oo[i] < 0 ? 0.0 : v;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+      if (!__any((msk[g] >> n) & 1)) continue;  // wave-uniform
+      // this lane's NH source spectra for direction n, loads batched
+      // (clamped to slot 0 where the neighbour has no such child)
+      const int* ss = sslot + ((long)(q[g] < 0 ? 0 : q[g]) * 26 + n) * 8;
+      int sl[NH];
+      double2 x[NH];
+#pragma unroll
+      for (int kh = 0; kh < NH; kh++) sl[kh] = q[g] >= 0 ? ss[(kh * 4 + lk) / SM] : -1;
+#pragma unroll
+      for (int kh = 0; kh < NH; kh++) {
+        const int j = (kh * 4 + lk) % SM;
+        x[kh] = effden[(((long)(sl[kh] < 0 ? 0 : sl[kh]) * Rc + r[g]) * SM + j) * C + f];
+      }
+#pragma unroll
+      for (int kh = 0; kh < NH; kh++) {
+        const double xr = sl[kh] < 0 ? 0.0 : x[kh].x, xi = sl[kh] < 0 ? 0.0 : x[kh].y;
+#pragma unroll
+        for (int rt = 0; rt < TM; rt++) {
+          acc[g][rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(s_A[rt][kh][l], xr, acc[g][rt], 0, 0, 0);
+          acc[g][rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(s_A[rt][kh + NH][l], xi, acc[g][rt], 0, 0, 0);
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int g = 0; g < G; g++) {
+    if (q[g] < 0) continue;
+#pragma unroll
+    for (int rt = 0; rt < TM; rt++)
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        const int qo = rt * 8 + lk + 4 * h, c = qo / TM, ti = qo - c * TM;
+        const int tl = tslot[(long)q[g] * 8 + c];
+        if (tl >= 0)
+          effval[(((long)tl * Rc + r[g]) * TM + ti) * C + f] = make_double2(acc[g][rt][h], acc[g][rt][h + 2]);
+      }
+  }
+}
+
+template <int SM, int TM, int G>
+static void m2l_vec_launch(hipStream_t s, int C, int npar, int Rc, const int* tslot, const int* sslot,
+                           const int* pmask, const int* optab, const double2* opsT, const double2* effden,
+                           double2* effval)
+{
+  const long ncol = (long)npar * Rc;
+  const int ncb = (int)((ncol + 64 * G - 1) / (64 * G));
+  hipLaunchKernelGGL((k_m2l_vec<SM, TM, G>), dim3(xcd_grid((C + 7) / 8 * ncb * 8)), dim3(M2LV_BS), 0, s, C, Rc, ncol,
+                     ncb, tslot, sslot, pmask, optab, opsT, effden, effval);
+}
+
+void m2l_matrix(hipStream_t s, int sm, int tm, int C, int npar, int Rc, const int* tslot, const int* sslot,
+                const int* pmask, const int* optab, const double2* opsT, const double2* effden, double2* effval)
+{
+  if (npar <= 0) return;
+  const long ncol = (long)npar * Rc;
+  // fewer tiles per wave when the level is too small to fill the chip otherwise
+  const bool wide = ncol * C >= 64L * 4 * 256 * 8;
+  if (sm == 1 && tm == 1) {
+    if (wide)
+      m2l_vec_launch<1, 1, 4>(s, C, npar, Rc, tslot, sslot, pmask, optab, opsT, effden, effval);
+    else
+      m2l_vec_launch<1, 1, 1>(s, C, npar, Rc, tslot, sslot, pmask, optab, opsT, effden, effval);
+  } else if (sm == 4 && tm == 3) {
+    if (wide)
+      m2l_vec_launch<4, 3, 2>(s, C, npar, Rc, tslot, sslot, pmask, optab, opsT, effden, effval);
+    else
+      m2l_vec_launch<4, 3, 1>(s, C, npar, Rc, tslot, sslot, pmask, optab, opsT, effden, effval);
+  } else if (sm == 3 && tm == 3) {
+    if (wide)
+      m2l_vec_launch<3, 3, 2>(s, C, npar, Rc, tslot, sslot, pmask, optab, opsT, effden, effval);
+    else
+      m2l_vec_launch<3, 3, 1>(s, C, npar, Rc, tslot, sslot, pmask, optab, opsT, effden, effval);
+  }
+}
+
 }  // namespace dev
 }  // namespace kifmm

@@ -96,7 +96,11 @@ Engine::Engine(int kt, const double* coefs, int ncoefs, int np, int ptsmax, int 
   if (const char* bs = std::getenv("KIFMM3D_M2L_BS")) m2l_bs_ = std::atoi(bs);
   if (const char* v = std::getenv("KIFMM3D_P2P")) p2p_variant_ = std::atoi(v);
   if (const char* g = std::getenv("KIFMM3D_GEMM")) gemm_mfma_ = std::string(g) != "valu";
-  if (const char* m = std::getenv("KIFMM3D_M2L")) m2l_tile_ = std::string(m) != "group" && std::string(m) != "pair";
+  if (const char* m = std::getenv("KIFMM3D_M2L")) {
+    const std::string v(m);
+    m2l_tile_ = v != "group" && v != "pair" && v != "matrix";
+    m2l_matrix_ = v != "group" && v != "pair";
+  }
   if (const char* gph = std::getenv("KIFMM3D_GRAPH")) use_graph_ = std::atoi(gph) != 0;
   if (const char* ov = std::getenv("KIFMM3D_OVERLAP")) overlap_ = std::atoi(ov) != 0;
 }
@@ -187,18 +191,19 @@ void Engine::upload_static()
   dc2de_.upload(O.dc2de);
   de2dc_.upload(O.de2dc);
   m2l_.upload(O.m2l);
-  if (mm_.sdof == 1 && mm_.tdof == 1) {  // frequency-major copies for the tiled M2L
-    std::vector<double> t((size_t)2 * 343 * O.C);
-    std::vector<float> t32(t.size());
+  {  // frequency-major copies for the matrix-core M2L: [C][343][sdof][tdof] complex
+    const size_t st = (size_t)mm_.sdof * mm_.tdof;
+    std::vector<double> t((size_t)2 * 343 * st * O.C);
     for (int o = 0; o < 343; o++)
-      for (int f = 0; f < O.C; f++)
-        for (int c = 0; c < 2; c++) {
-          const double v = O.m2l[((size_t)o * O.C + f) * 2 + c];
-          t[((size_t)f * 343 + o) * 2 + c] = v;
-          t32[((size_t)f * 343 + o) * 2 + c] = (float)v;
-        }
+      for (size_t e = 0; e < st; e++)
+        for (int f = 0; f < O.C; f++)
+          for (int c = 0; c < 2; c++)
+            t[(((size_t)f * 343 + o) * st + e) * 2 + c] = O.m2l[(((size_t)o * st + e) * O.C + f) * 2 + c];
     opsT_.upload(t);
-    opsT32_.upload(t32);
+    if (st == 1) {
+      std::vector<float> t32(t.begin(), t.end());
+      opsT32_.upload(t32);
+    }
   }
   c2s_.upload(O.cube2surf);
   std::vector<int> s2c(O.nsp[SURF_DC], 0);
@@ -507,6 +512,32 @@ void Engine::build_plan()
         vl.npar = (int)(tsl.size() / 8);
         vl.tslot = pool_i(P.I, tsl);
         vl.sslot = pool_i(P.I, ssl);
+        std::vector<int> pm(vl.npar, 0);  // bit n: the parent has a source child in direction n
+        for (int q = 0; q < vl.npar; q++)
+          for (int n = 0; n < 26; n++)
+            for (int sc = 0; sc < 8; sc++)
+              if (ssl[((size_t)q * 26 + n) * 8 + sc] >= 0) pm[q] |= 1 << n;
+        vl.pmask = pool_i(P.I, pm);
+        if (std::getenv("KIFMM3D_TRACE")) {
+          // occupancy of the matrix-core M2L tiles (16 parents x 26 directions x
+          // 8 x 8 child pairs) by V pairs, for one right-hand side
+          long used = 0, run = 0;
+          for (int q0 = 0; q0 < vl.npar; q0 += 16)
+            for (int n = 0; n < 26; n++) {
+              bool any = false;
+              for (int q = q0; q < std::min(q0 + 16, vl.npar); q++) {
+                any = any || ((pm[q] >> n) & 1);
+                for (int sc = 0; sc < 8; sc++)
+                  for (int c = 0; c < 8; c++)
+                    used += ssl[((size_t)q * 26 + n) * 8 + sc] >= 0 && tsl[(size_t)q * 8 + c] >= 0 &&
+                            optab[(n * 8 + sc) * 8 + c] >= 0;
+              }
+              run += any ? 16 * 64 : 0;
+            }
+          std::fprintf(stderr, "[kifmm3d] V level %2d: %7d parents %8d sources %8d targets %10ld pairs, "
+                       "matrix tiles %5.1f%% occupied\n", L, vl.npar, (int)src.size(), (int)trg.size(), npair,
+                       run ? 100.0 * used / run : 0.0);
+        }
         // tiled form: the grouped parents under one grandparent form a tile; the
         // neighbour n of the parent at position (x, y, z) of the tile is halo
         // entry ((x+1+dx)*4 + (y+1+dy))*4 + (z+1+dz)
@@ -839,7 +870,8 @@ void Engine::run_chunk(const double* srcden, double* trgval, int rc, int mode, h
   if (side) HIPCHK(hipEventRecord(join_ev_, side_));
   mark(3);
   // V list: tiled matrix-core M2L for scalar kernels on chunks of 16k
-  // right-hand sides, the grouped (or per-pair) form otherwise
+  // right-hand sides, the parent-grouped matrix-core form otherwise (VALU
+  // grouped and per-pair forms for comparison)
   const bool tiled = m2l_tile_ && !P.m2l_pairs && sm == 1 && tm == 1 && rc % 16 == 0;
   const int ri = tiled ? 1 : 0, single = tiled && mixed ? 1 : 0;
   for (const VLevel& vl : P.vlev) {
@@ -854,6 +886,9 @@ void Engine::run_chunk(const double* srcden, double* trgval, int rc, int mode, h
       else if (P.m2l_pairs)
         dev::m2l(s, sm, tm, O.C, vl.ntrg, rc, I + vl.vptr, I + vl.vsrc, I + vl.vop, (const double2*)m2l_.p,
                  (const double2*)effden_.p, (double2*)effval_.p);
+      else if (m2l_matrix_)
+        dev::m2l_matrix(s, sm, tm, O.C, vl.npar, rc, I + vl.tslot, I + vl.sslot, I + vl.pmask, I + P.optab,
+                        (const double2*)opsT_.p, (const double2*)effden_.p, (double2*)effval_.p);
       else
         dev::m2l_grouped(s, sm, tm, O.C, vl.npar, rc, m2l_rb_, m2l_bs_, I + vl.tslot, I + vl.sslot, I + P.optab,
                          (const double2*)m2l_.p, (const double2*)effden_.p, (double2*)effval_.p);

@@ -60,6 +60,7 @@ struct VLevel {
   size_t vptr = 0, vsrc = 0, vop = 0;  // per-pair form (V-list CSR with operator index)
   int npar = 0;                        // grouped form: target parents of the level
   size_t tslot = 0, sslot = 0;         // [npar][8] target-child slots, [npar][26][8] source-child slots
+  size_t pmask = 0;                    // [npar] bit n: a source child in neighbour direction n
   int ntile = 0;                       // tiled form: 2x2x2 blocks of target parents
   size_t ttile = 0, htile = 0;         // [ntile][8][8] target-child slots, [ntile][64][8] halo source-child slots
   long dscale = -1;
@@ -147,7 +148,8 @@ class Engine {
   int m2l_bs_ = 64;  // frequencies per block in the grouped M2L (64 or 256; KIFMM3D_M2L_BS)
   int p2p_variant_ = 2;  // near field: 0 LDS broadcast, 1 scalar-cache sources, 2 matrix cores (KIFMM3D_P2P)
   bool gemm_mfma_ = true; // uncompensated GEMMs on the matrix cores (KIFMM3D_GEMM=valu disables)
-  bool m2l_tile_ = true; // tiled matrix-core M2L for scalar kernels and 16-RHS chunks (KIFMM3D_M2L=group disables)
+  bool m2l_tile_ = true;   // tiled matrix-core M2L for scalar kernels and 16-RHS chunks (KIFMM3D_M2L=matrix|group|pair disables)
+  bool m2l_matrix_ = true; // parent-grouped matrix-core M2L otherwise (KIFMM3D_M2L=group|pair disables)
   hipStream_t stream_ = nullptr;
   hipStream_t side_ = nullptr;  // near field, concurrent with the far-field chain
   hipEvent_t fork_ev_ = nullptr, join_ev_ = nullptr;
@@ -167,8 +169,8 @@ class Engine {
   DVec<short> c2s_;
   DVec<double> unit_[4];
   DVec<double> uc2ue_, ue2uc_, dc2de_, de2dc_, m2l_;
-  DVec<double> opsT_;  // scalar kernels: M2L operators as [C][343] complex (FP64)
-  DVec<float> opsT32_; // the same in FP32
+  DVec<double> opsT_;  // M2L operators as [C][343][sdof][tdof] complex (FP64)
+  DVec<float> opsT32_; // scalar kernels: the same in FP32
   DVec<double> tw_;
   DVec<int> I_;
   DVec<double> D_;

@@ -169,3 +169,22 @@ def test_other_kernel_types(kt):
         p, n = clusters(4000, 40 + kt % 7), None
         ctr, lvl = (0.5, 0.5, 0.5), 1
     run_case(f"kt{kt}", kt, 4, 40, p, n, p, ctr, lvl, nrhs=2)
+
+
+@pytest.mark.parametrize("form", ["matrix", "group", "pair"])
+@pytest.mark.parametrize("kt", [111, 321, 511])
+def test_m2l_forms(kt, form, monkeypatch):
+    """Every M2L form (parent-grouped on the matrix cores, parent-grouped on the
+    vector ALUs, per V pair) against the oracle, for the three translation
+    shapes 1x1, 4x3 (Stokes double layer, sdof 4) and 3x3, with 5 right-hand
+    sides so that the matrix-core tiles of 16 columns mix parents and
+    right-hand sides, and an adaptive tree whose boundary parents miss
+    neighbour directions."""
+    monkeypatch.setenv("KIFMM3D_M2L", form)
+    if kt == 321:
+        p, nrm = sphere(5000, 60)
+        ctr, lvl = (0.0, 0.0, 0.0), 0
+    else:
+        p, nrm = clusters(5000, 61), None
+        ctr, lvl = (0.5, 0.5, 0.5), 1
+    run_case(f"kt{kt}-m2l-{form}", kt, 4, 30, p, nrm, p, ctr, lvl, nrhs=5)
