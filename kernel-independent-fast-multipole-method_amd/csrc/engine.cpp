@@ -135,7 +135,15 @@ void Engine::setup(const double* spos, const double* snor, int ns, const double*
   t.maxlevel = tree_.maxlevel;
   t.rootlvl = tree_.rootlvl;
   for (int d = 0; d < 3; d++) t.ctr[d] = tree_.ctr[d];
-  t.build(spos, ns, tpos, nt);
+  // the per-point tree work runs on the GPU when one is visible (identical
+  // tree; KIFMM3D_TREE=host keeps it on the host)
+  int ndev = 0;
+  const bool have_dev = hipGetDeviceCount(&ndev) == hipSuccess && ndev > 0;
+  const char* tb = std::getenv("KIFMM3D_TREE");
+  if (have_dev && !(tb && std::string(tb) == "host"))
+    t.build_device(spos, ns, tpos, nt);
+  else
+    t.build(spos, ns, tpos, nt);
   tree_ = std::move(t);
   ns_ = ns;
   nt_ = nt;
@@ -145,9 +153,7 @@ void Engine::setup(const double* spos, const double* snor, int ns, const double*
   lap("plan");
   host_done_ = true;
   // device phase
-  int ndev = 0;
-  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
-    throw Error(KIFMM3D_EDEVICE, "no HIP device is visible (tree, lists and operators are built)");
+  if (!have_dev) throw Error(KIFMM3D_EDEVICE, "no HIP device is visible (tree, lists and operators are built)");
   if (!stream_) HIPCHK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
   if (!side_) {
     HIPCHK(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking));

@@ -7,6 +7,8 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
+#include <functional>
+#include <string>
 #include <numeric>
 #include <unordered_map>
 
@@ -78,17 +80,17 @@ void Tree::build(const double* spos, int ns, const double* tpos, int nt)
   lap("lists");
 }
 
-// Let3d::srcdata, let3d.cpp:56-137
-void Tree::build_sources(const double* pos, int ns)
+// Nodes of the source tree, level by level (Let3d::srcdata, let3d.cpp:56-137):
+// a node with more than ptsmax points above depth maxlevel-1 gets 8 children
+// numbered after the nodes already made, in child order 4a + 2b + c.
+// split(k, cc, c8) counts node k's points per octant of its centre cc and
+// leaves them octant-ordered in [beg[k], beg[k] + cnt[k]).
+void Tree::grow(int ns, std::vector<int>& beg, std::vector<int>& cnt,
+                const std::function<void(int, const double*, int*)>& split)
 {
-  for (int k = 0; k < ns; k++)
-    if (!in_box(*this, pos + 3 * (size_t)k))
-      throw Error(KIFMM3D_EBOX, "source point " + std::to_string(k) + " lies outside the root box");
   par.assign(1, -1); chd.assign(1, -1); depth.assign(1, 0); tag.assign(1, 0); path.assign(3, 0);
-  std::vector<int> perm(ns), beg(1, 0), cnt(1, ns);
-  std::iota(perm.begin(), perm.end(), 0);
-  std::vector<int> tmp;
-  std::vector<unsigned char> oct;
+  beg.assign(1, 0);
+  cnt.assign(1, ns);
   nnode = 1;
   int level = 0, arrbeg = 0, arrend = 1;
   while (arrbeg < arrend) {
@@ -109,7 +111,7 @@ void Tree::build_sources(const double* pos, int ns)
         double cc[3];
         center(k, cc);
         int c8[8];
-        split_range(pos, perm.data(), beg[k], cnt[k], cc, c8, tmp, oct);
+        split(k, cc, c8);
         int off = beg[k];
         for (int o = 0; o < 8; o++) {
           beg[chd[k] + o] = off;
@@ -123,8 +125,13 @@ void Tree::build_sources(const double* pos, int ns)
     arrend = arrcnt;
   }
   nlevel = level;
+}
+
+// Source tags, node indices and leaf ranges from per-node point counts
+// (let3d.cpp:118-135).
+void Tree::index_sources(const std::vector<int>& cnt)
+{
   sndeidx.assign(nnode, 0); sextbeg.assign(nnode, 0); sextnum.assign(nnode, 0);
-  sperm.resize(ns);
   int c = 0, sum = 0;
   for (int g = 0; g < nnode; g++) {
     if (cnt[g] > 0) {
@@ -133,7 +140,6 @@ void Tree::build_sources(const double* pos, int ns)
       if (chd[g] < 0) {
         sextbeg[g] = sum;
         sextnum[g] = cnt[g];
-        std::copy(perm.begin() + beg[g], perm.begin() + beg[g] + cnt[g], sperm.begin() + sum);
         sum += cnt[g];
       }
     }
@@ -142,12 +148,52 @@ void Tree::build_sources(const double* pos, int ns)
   sextcnt = sum;
 }
 
+// The same for targets (let3d.cpp:183-200).
+void Tree::index_targets(const std::vector<int>& cnt)
+{
+  tndeidx.assign(nnode, 0); textbeg.assign(nnode, 0); textnum.assign(nnode, 0);
+  int c = 0, sum = 0;
+  for (int g = 0; g < nnode; g++) {
+    if (cnt[g] > 0) {
+      tag[g] |= NODE_TRG;
+      tndeidx[g] = c++;
+      if (chd[g] < 0) {
+        textbeg[g] = sum;
+        textnum[g] = cnt[g];
+        sum += cnt[g];
+      }
+    }
+  }
+  tndecnt = c;
+  textcnt = sum;
+}
+
+void Tree::check_box(const double* pos, int n, const char* what) const
+{
+  for (int k = 0; k < n; k++)
+    if (!in_box(*this, pos + 3 * (size_t)k))
+      throw Error(KIFMM3D_EBOX, std::string(what) + " point " + std::to_string(k) + " lies outside the root box");
+}
+
+// Let3d::srcdata, let3d.cpp:56-137
+void Tree::build_sources(const double* pos, int ns)
+{
+  check_box(pos, ns, "source");
+  std::vector<int> perm(ns), beg, cnt;
+  std::iota(perm.begin(), perm.end(), 0);
+  std::vector<int> tmp;
+  std::vector<unsigned char> oct;
+  grow(ns, beg, cnt, [&](int k, const double* cc, int* c8) { split_range(pos, perm.data(), beg[k], cnt[k], cc, c8, tmp, oct); });
+  index_sources(cnt);
+  sperm.resize(ns);
+  for (int g = 0; g < nnode; g++)
+    if (cnt[g] > 0 && chd[g] < 0) std::copy(perm.begin() + beg[g], perm.begin() + beg[g] + cnt[g], sperm.begin() + sextbeg[g]);
+}
+
 // Let3d::trgdata, let3d.cpp:140-200
 void Tree::insert_targets(const double* pos, int nt)
 {
-  for (int k = 0; k < nt; k++)
-    if (!in_box(*this, pos + 3 * (size_t)k))
-      throw Error(KIFMM3D_EBOX, "target point " + std::to_string(k) + " lies outside the root box");
+  check_box(pos, nt, "target");
   std::vector<int> perm(nt), beg(nnode, 0), cnt(nnode, 0);
   std::iota(perm.begin(), perm.end(), 0);
   cnt[0] = nt;
@@ -166,23 +212,10 @@ void Tree::insert_targets(const double* pos, int nt)
       off += c8[o];
     }
   }
-  tndeidx.assign(nnode, 0); textbeg.assign(nnode, 0); textnum.assign(nnode, 0);
+  index_targets(cnt);
   tperm.resize(nt);
-  int c = 0, sum = 0;
-  for (int g = 0; g < nnode; g++) {
-    if (cnt[g] > 0) {
-      tag[g] |= NODE_TRG;
-      tndeidx[g] = c++;
-      if (chd[g] < 0) {
-        textbeg[g] = sum;
-        textnum[g] = cnt[g];
-        std::copy(perm.begin() + beg[g], perm.begin() + beg[g] + cnt[g], tperm.begin() + sum);
-        sum += cnt[g];
-      }
-    }
-  }
-  tndecnt = c;
-  textcnt = sum;
+  for (int g = 0; g < nnode; g++)
+    if (cnt[g] > 0 && chd[g] < 0) std::copy(perm.begin() + beg[g], perm.begin() + beg[g] + cnt[g], tperm.begin() + textbeg[g]);
 }
 
 // Let3d::calgnext for every target node, let3d.cpp:203-282

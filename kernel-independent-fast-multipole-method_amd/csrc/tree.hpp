@@ -5,6 +5,7 @@
 // findgnt, threads over target nodes).
 #pragma once
 
+#include <functional>
 #include <vector>
 
 namespace kifmm {
@@ -35,6 +36,10 @@ struct Tree {
 
   // Throws Error(KIFMM3D_EBOX) when a point is outside the root box.
   void build(const double* spos, int ns, const double* tpos, int nt);
+  // The same tree with the per-point work on the current HIP device: octant
+  // keys by the same centre comparisons, radix sorts, the node structure from
+  // the sorted keys; lists on the host.  Identical arrays to build().
+  void build_device(const double* spos, int ns, const double* tpos, int nt);
 
   void center(int g, double* c) const;  // let3d.cpp:309-319
   double radius(int g) const;           // let3d.cpp:320-323
@@ -42,6 +47,11 @@ struct Tree {
   int child(int g, int a, int b, int c) const { return chd[g] + a * 4 + b * 2 + c; }
 
  private:
+  void grow(int ns, std::vector<int>& beg, std::vector<int>& cnt,
+            const std::function<void(int, const double*, int*)>& split);
+  void index_sources(const std::vector<int>& cnt);
+  void index_targets(const std::vector<int>& cnt);
+  void check_box(const double* pos, int n, const char* what) const;
   void build_sources(const double* spos, int ns);
   void insert_targets(const double* tpos, int nt);
   void build_lists();
