@@ -1337,20 +1337,41 @@ __device__ inline void load_twiddles(double2 (*s_wt)[2 * NP], const double2* tw,
   }
 }
 
+// Intermediate spectra in LDS: FP64 for FP64 spectra, FP32 for FP32 spectra
+// (arithmetic stays FP64); per-RHS rows padded by one element so the R rows
+// of a wave's accesses fall in different banks.
+__device__ inline double2 lds_get(const double2& v) { return v; }
+__device__ inline double2 lds_get(const float2& v) { return make_double2(v.x, v.y); }
+__device__ inline void lds_put(double2& d, double2 v) { d = v; }
+__device__ inline void lds_put(float2& d, double2 v) { d = make_float2((float)v.x, (float)v.y); }
+
+// Blocks (slot, group of R right-hand sides): the groups of one slot share
+// the 128-byte lines of its spectra, so they are dealt to one XCD side by side.
+__device__ inline bool fft_block(int nslot, int nrg, int R, int& slot, int& r0)
+{
+  const int nitem = nslot * nrg, item = xcd_item(blockIdx.x, nitem);
+  slot = item / nrg;
+  r0 = (item - slot * nrg) * R;
+  return item < nitem;
+}
+
 template <int NP, class CT, int R>
-__global__ __launch_bounds__(64 * R) void k_fft_fwd_ri(const int* __restrict__ slot_node, int Rc,
+__global__ __launch_bounds__(64 * R) void k_fft_fwd_ri(int nslot, const int* __restrict__ slot_node, int Rc,
                                                        const double* __restrict__ upeden, int pln,
                                                        const double* __restrict__ dscale, const short* __restrict__ c2s,
                                                        CT* __restrict__ effden, const double2* __restrict__ tw)
 {
   constexpr int N = 2 * NP, H = NP + 1, C = N * N * H, NC = NP * NP * NP, BS = 64 * R;
   constexpr int NSP = NP * NP * NP - (NP - 2) * (NP - 2) * (NP - 2);
+  typedef CT LT;
   __shared__ double s_den[R][NSP];
   __shared__ short s_c2s[NC];
   __shared__ double2 s_wt[NP][N];
-  __shared__ double2 X1[R][NP * NP * H];  // [k][j][kx]
-  __shared__ double2 X2[R][NP * N * H];   // [k][ky][kx]
-  const int slot = blockIdx.x, r0 = blockIdx.y * R, t = threadIdx.x;
+  __shared__ LT X1[R][NP * NP * H + 1];  // [k][j][kx]
+  __shared__ LT X2[R][NP * N * H + 1];   // [k][ky][kx]
+  int slot, r0;
+  if (!fft_block(nslot, Rc / R, R, slot, r0)) return;  // block-uniform
+  const int t = threadIdx.x;
   for (int i = t; i < NC; i += BS) s_c2s[i] = c2s[i];
   load_twiddles<NP>(s_wt, tw, t, BS);
   const double ds = dscale[0];
@@ -1379,7 +1400,7 @@ __global__ __launch_bounds__(64 * R) void k_fft_fwd_ri(const int* __restrict__ s
       }
     }
 #pragma unroll
-    for (int kx = 0; kx < H; kx++) X1[rr][(k * NP + j) * H + kx] = acc[kx];
+    for (int kx = 0; kx < H; kx++) lds_put(X1[rr][(k * NP + j) * H + kx], acc[kx]);
   }
   __syncthreads();
   for (int e = t; e < R * NP * H; e += BS) {  // along y: line (k, kx)
@@ -1387,7 +1408,7 @@ __global__ __launch_bounds__(64 * R) void k_fft_fwd_ri(const int* __restrict__ s
     const int rr = e % R, col = e / R, k = col / H, kx = col - k * H;
     double2 x[NP];
 #pragma unroll
-    for (int j = 0; j < NP; j++) x[j] = X1[rr][(k * NP + j) * H + kx];
+    for (int j = 0; j < NP; j++) x[j] = lds_get(X1[rr][(k * NP + j) * H + kx]);
 #pragma unroll
     for (int ky = 0; ky < N; ky++) {
       double2 acc = make_double2(0, 0);
@@ -1397,7 +1418,7 @@ __global__ __launch_bounds__(64 * R) void k_fft_fwd_ri(const int* __restrict__ s
         acc.x += v.x;
         acc.y += v.y;
       }
-      X2[rr][(k * N + ky) * H + kx] = acc;
+      lds_put(X2[rr][(k * N + ky) * H + kx], acc);
     }
   }
   __syncthreads();
@@ -1407,7 +1428,7 @@ __global__ __launch_bounds__(64 * R) void k_fft_fwd_ri(const int* __restrict__ s
     const int rr = e % R, col = e / R;
     double2 x[NP];
 #pragma unroll
-    for (int k = 0; k < NP; k++) x[k] = X2[rr][k * N * H + col];
+    for (int k = 0; k < NP; k++) x[k] = lds_get(X2[rr][k * N * H + col]);
 #pragma unroll
     for (int kz = 0; kz < N; kz++) {
       double2 acc = make_double2(0, 0);
@@ -1423,16 +1444,19 @@ __global__ __launch_bounds__(64 * R) void k_fft_fwd_ri(const int* __restrict__ s
 }
 
 template <int NP, class CT, int R>
-__global__ __launch_bounds__(64 * R) void k_fft_inv_ri(const int* __restrict__ slot_node, int Rc,
+__global__ __launch_bounds__(64 * R) void k_fft_inv_ri(int nslot, const int* __restrict__ slot_node, int Rc,
                                                        const CT* __restrict__ effval, const int* __restrict__ s2c,
                                                        int nsp, double* __restrict__ dncval, int pln,
                                                        const double2* __restrict__ tw)
 {
   constexpr int N = 2 * NP, H = NP + 1, C = N * N * H, BS = 64 * R;
+  typedef CT LT;
   __shared__ double2 s_wt[NP][N];
-  __shared__ double2 Y1[R][NP * N * H];   // [k][ky][kx], only the cube's planes
-  __shared__ double2 Y2[R][NP * NP * H];  // [k][j][kx], only the cube's rows
-  const int slot = blockIdx.x, r0 = blockIdx.y * R, t = threadIdx.x;
+  __shared__ LT Y1[R][NP * N * H + 1];   // [k][ky][kx], only the cube's planes
+  __shared__ LT Y2[R][NP * NP * H + 1];  // [k][j][kx], only the cube's rows
+  int slot, r0;
+  if (!fft_block(nslot, Rc / R, R, slot, r0)) return;  // block-uniform
+  const int t = threadIdx.x;
   load_twiddles<NP>(s_wt, tw, t, BS);
   __syncthreads();
   const CT* in = effval + (long)slot * C * Rc + r0;
@@ -1452,7 +1476,7 @@ __global__ __launch_bounds__(64 * R) void k_fft_inv_ri(const int* __restrict__ s
         acc.x += v.x;
         acc.y += v.y;
       }
-      Y1[rr][k * N * H + col] = acc;
+      lds_put(Y1[rr][k * N * H + col], acc);
     }
   }
   __syncthreads();
@@ -1461,7 +1485,7 @@ __global__ __launch_bounds__(64 * R) void k_fft_inv_ri(const int* __restrict__ s
     const int rr = e % R, col = e / R, k = col / H, kx = col - k * H;
     double2 y[N];
 #pragma unroll
-    for (int ky = 0; ky < N; ky++) y[ky] = Y1[rr][(k * N + ky) * H + kx];
+    for (int ky = 0; ky < N; ky++) y[ky] = lds_get(Y1[rr][(k * N + ky) * H + kx]);
 #pragma unroll
     for (int j = 0; j < NP; j++) {
       double2 acc = make_double2(0, 0);
@@ -1472,7 +1496,7 @@ __global__ __launch_bounds__(64 * R) void k_fft_inv_ri(const int* __restrict__ s
         acc.x += v.x;
         acc.y += v.y;
       }
-      Y2[rr][(k * NP + j) * H + kx] = acc;
+      lds_put(Y2[rr][(k * NP + j) * H + kx], acc);
     }
   }
   __syncthreads();
@@ -1482,13 +1506,13 @@ __global__ __launch_bounds__(64 * R) void k_fft_inv_ri(const int* __restrict__ s
     const int rr = e / nsp, p = e - rr * nsp;
     const int cc = s2c[p];
     const int i = cc / (NP * NP), j = (cc / NP) % NP, k = cc % NP;
-    const double2* y = &Y2[rr][(k * NP + j) * H];
+    const LT* y = &Y2[rr][(k * NP + j) * H];
     const int xi = i + NP / 2;
-    double v = y[0].x + ((xi & 1) ? -y[NP].x : y[NP].x);
+    double v = lds_get(y[0]).x + ((xi & 1) ? -lds_get(y[NP]).x : lds_get(y[NP]).x);
 #pragma unroll
     for (int kx = 1; kx < NP; kx++) {
-      const double2 w = s_wt[i][kx];
-      v += 2.0 * (y[kx].x * w.x + y[kx].y * w.y);
+      const double2 w = s_wt[i][kx], yk = lds_get(y[kx]);
+      v += 2.0 * (yk.x * w.x + yk.y * w.y);
     }
     out[(long)rr * pln + p] += v;
   }
@@ -1502,9 +1526,9 @@ static void fft_fwd_t(hipStream_t s, int np, int nslot, const int* slot_node, in
   if (ri && sm == 1) {
     constexpr int RW = sizeof(CT) == 8 ? 8 : 4;  // 64-byte runs of right-hand sides
     switch (np) {
-    case 4: hipLaunchKernelGGL((k_fft_fwd_ri<4, CT, RW>), dim3(nslot, Rc / RW), dim3(64 * RW), 0, s, slot_node, Rc, upeden, pln, dscale, cube2surf, effden, tw); break;
-    case 6: hipLaunchKernelGGL((k_fft_fwd_ri<6, CT, RW>), dim3(nslot, Rc / RW), dim3(64 * RW), 0, s, slot_node, Rc, upeden, pln, dscale, cube2surf, effden, tw); break;
-    case 8: hipLaunchKernelGGL((k_fft_fwd_ri<8, CT, RW / 2>), dim3(nslot, Rc / (RW / 2)), dim3(32 * RW), 0, s, slot_node, Rc, upeden, pln, dscale, cube2surf, effden, tw); break;
+    case 4: hipLaunchKernelGGL((k_fft_fwd_ri<4, CT, RW>), dim3(xcd_grid(nslot * (Rc / RW))), dim3(64 * RW), 0, s, nslot, slot_node, Rc, upeden, pln, dscale, cube2surf, effden, tw); break;
+    case 6: hipLaunchKernelGGL((k_fft_fwd_ri<6, CT, RW>), dim3(xcd_grid(nslot * (Rc / RW))), dim3(64 * RW), 0, s, nslot, slot_node, Rc, upeden, pln, dscale, cube2surf, effden, tw); break;
+    case 8: hipLaunchKernelGGL((k_fft_fwd_ri<8, CT, RW / 2>), dim3(xcd_grid(nslot * (Rc / (RW / 2)))), dim3(32 * RW), 0, s, nslot, slot_node, Rc, upeden, pln, dscale, cube2surf, effden, tw); break;
     }
     return;
   }
@@ -1522,11 +1546,11 @@ static void fft_inv_t(hipStream_t s, int np, int nslot, const int* slot_node, in
 {
   if (nslot <= 0) return;
   if (ri && tm == 1) {
-    constexpr int RW = 4;
+    constexpr int RW = sizeof(CT) == 8 ? 8 : 4;
     switch (np) {
-    case 4: hipLaunchKernelGGL((k_fft_inv_ri<4, CT, RW>), dim3(nslot, Rc / RW), dim3(64 * RW), 0, s, slot_node, Rc, effval, surf2cube, nsp, dncval, pln, tw); break;
-    case 6: hipLaunchKernelGGL((k_fft_inv_ri<6, CT, RW>), dim3(nslot, Rc / RW), dim3(64 * RW), 0, s, slot_node, Rc, effval, surf2cube, nsp, dncval, pln, tw); break;
-    case 8: hipLaunchKernelGGL((k_fft_inv_ri<8, CT, RW / 2>), dim3(nslot, Rc / (RW / 2)), dim3(32 * RW), 0, s, slot_node, Rc, effval, surf2cube, nsp, dncval, pln, tw); break;
+    case 4: hipLaunchKernelGGL((k_fft_inv_ri<4, CT, RW>), dim3(xcd_grid(nslot * (Rc / RW))), dim3(64 * RW), 0, s, nslot, slot_node, Rc, effval, surf2cube, nsp, dncval, pln, tw); break;
+    case 6: hipLaunchKernelGGL((k_fft_inv_ri<6, CT, RW>), dim3(xcd_grid(nslot * (Rc / RW))), dim3(64 * RW), 0, s, nslot, slot_node, Rc, effval, surf2cube, nsp, dncval, pln, tw); break;
+    case 8: hipLaunchKernelGGL((k_fft_inv_ri<8, CT, RW / 2>), dim3(xcd_grid(nslot * (Rc / (RW / 2)))), dim3(32 * RW), 0, s, nslot, slot_node, Rc, effval, surf2cube, nsp, dncval, pln, tw); break;
     }
     return;
   }
