@@ -1161,10 +1161,150 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(GemmArgs p)
   }
 }
 
+// Plain product on the FP64 matrix cores, wide form.  A block covers RT x 16
+// rows (every row of an operator of up to 160 rows) and 64 G columns; wave w
+// owns columns [16 G w, 16 G (w + 1)) of the block as RT x G tiles of
+// v_mfma_f64_16x16x4f64.  K runs in steps of 8 -- every surface length
+// nS(n) x dof is a multiple of 8, so a step never crosses a segment.  The A
+// tile of a step (RT*16 x 8, with the power-of-two B scale of its K rows
+// folded in) is double-buffered in LDS; each lane loads the two B values it
+// feeds straight from its column (contiguous in K) one step ahead.  Lane
+// (k, col) feeds B[k0 + 2k + s][col] to MFMA slice s, so slice s reads A
+// rows at K index k0 + 2k + s.  One barrier per step.
+template <int RT, int G>
+__global__ __launch_bounds__(256) void k_gemm_mx(GemmArgs p)
+{
+  constexpr int MR = RT * 16, NAT = (8 * MR + 255) / 256;
+  const int grp = blockIdx.z;
+  const int n0 = p.gbeg[grp], n1 = p.gbeg[grp + 1];
+  const long cb = (long)n0 * p.Rc + (long)blockIdx.x * 64 * G, ce = (long)n1 * p.Rc;
+  if (cb >= ce) return;  // block-uniform
+  const int rb = blockIdx.y * MR;
+  const double* A = p.A + grp * p.a_gstride;
+  const int nstep = p.nseg * p.Kseg / 8, sps = p.Kseg / 8;  // steps, steps per segment
+  __shared__ double sA[2][8][MR];
+  const int t = threadIdx.x, w = t >> 6, l = t & 63, li = l & 15, lk = l >> 4;
+  int cn[G], cr[G];
+  bool cok[G];
+#pragma unroll
+  for (int g = 0; g < G; g++) {
+    const long j = cb + (long)(w * G + g) * 16 + li;
+    cok[g] = j < ce;
+    cn[g] = cok[g] ? (int)(j / p.Rc) : 0;
+    cr[g] = cok[g] ? (int)(j - (long)cn[g] * p.Rc) : 0;
+  }
+  // B column pointers of segment seg (clamped to column 0 of B, masked by bm)
+  const double* bp[G];
+  double bm[G];
+  auto seg_ptrs = [&](int seg) {
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+      const int bn = cok[g] ? p.bnode[(long)seg * p.ncol + cn[g]] : -1;
+      bp[g] = p.B + (bn >= 0 ? ((long)bn * p.Rc + cr[g]) * p.Kseg : 0) + 2 * lk;
+      bm[g] = bn >= 0 ? 1.0 : 0.0;
+    }
+  };
+  double ar[NAT];
+  auto load_a = [&](int st) {
+    const int k0 = st * 8;
+#pragma unroll
+    for (int i = 0; i < NAT; i++) {
+      const int e = t + 256 * i, kk = e / MR, row = e - kk * MR, gk = k0 + kk;
+      double v = 0.0;
+      if (e < 8 * MR && rb + row < p.M) {
+        v = A[(long)gk * p.M + rb + row];
+        if (p.bscale) v *= p.bscale[gk - (gk / p.Kseg) * p.Kseg];  // power of two: exact
+      }
+      ar[i] = v;
+    }
+  };
+  auto store_a = [&](int buf) {
+#pragma unroll
+    for (int i = 0; i < NAT; i++) {
+      const int e = t + 256 * i, kk = e / MR, row = e - kk * MR;
+      if (e < 8 * MR) sA[buf][kk][row] = ar[i];
+    }
+  };
+  double2 bc[G], bn2[G];
+  auto load_b = [&](int st, double2* dst) {
+    const int kin = (st - (st / sps) * sps) * 8;
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+      const double2 v = *(const double2*)(bp[g] + kin);
+      dst[g] = make_double2(v.x * bm[g], v.y * bm[g]);
+    }
+  };
+  v4d acc[RT][G];
+#pragma unroll
+  for (int rt = 0; rt < RT; rt++)
+#pragma unroll
+    for (int g = 0; g < G; g++) acc[rt][g] = v4d{0.0, 0.0, 0.0, 0.0};
+  seg_ptrs(0);
+  load_a(0);
+  load_b(0, bc);
+  store_a(0);
+  __syncthreads();
+  for (int st = 0; st < nstep; st++) {
+    const int buf = st & 1;
+    const bool more = st + 1 < nstep;
+    if (more) {
+      if ((st + 1) % sps == 0) seg_ptrs((st + 1) / sps);
+      load_a(st + 1);
+      load_b(st + 1, bn2);
+    }
+#pragma unroll
+    for (int sl = 0; sl < 2; sl++)
+#pragma unroll
+      for (int rt = 0; rt < RT; rt++) {
+        const double a = sA[buf][2 * lk + sl][rt * 16 + li];
+#pragma unroll
+        for (int g = 0; g < G; g++)
+          acc[rt][g] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, sl ? bc[g].y : bc[g].x, acc[rt][g], 0, 0, 0);
+      }
+    if (more) {
+      store_a(buf ^ 1);
+#pragma unroll
+      for (int g = 0; g < G; g++) bc[g] = bn2[g];
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int g = 0; g < G; g++) {
+    if (!cok[g]) continue;
+    double* out = p.C + ((long)p.cnode[cn[g]] * p.Rc + cr[g]) * p.M;
+#pragma unroll
+    for (int rt = 0; rt < RT; rt++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int row = rb + rt * 16 + lk + 4 * v;
+        if (row >= p.M) continue;
+        double h = acc[rt][g][v];
+        if (p.cscale) h *= p.cscale[row];  // power of two: exact
+        out[row] = p.accumulate ? out[row] + h : h;
+      }
+  }
+}
+
 void gemm(hipStream_t s, const GemmArgs& a)
 {
   if (a.ngroup <= 0 || a.maxcols <= 0) return;
   long cols = (long)a.maxcols * a.Rc;
+  // wide tiles when they still give at least 2 blocks per CU, 64 x 64 tiles otherwise
+  if (!a.compensated && a.mfma == 2 && a.Kseg % 8 == 0) {
+    if (a.M <= 64) {
+      dim3 grid((unsigned)((cols + 255) / 256), 1, a.ngroup);
+      if ((long)grid.x * grid.z >= 512) {
+        hipLaunchKernelGGL((k_gemm_mx<4, 4>), grid, dim3(256), 0, s, a);
+        return;
+      }
+    } else {
+      dim3 grid((unsigned)((cols + 127) / 128), (a.M + 159) / 160, a.ngroup);
+      if ((long)grid.x * grid.y * grid.z >= 512) {
+        hipLaunchKernelGGL((k_gemm_mx<10, 2>), grid, dim3(256), 0, s, a);
+        return;
+      }
+    }
+  }
   dim3 grid((unsigned)((cols + 63) / 64), (a.M + 63) / 64, a.ngroup);
   if (a.compensated)
     hipLaunchKernelGGL(k_gemm<true>, grid, dim3(256), 0, s, a);

@@ -80,7 +80,8 @@ struct GemmArgs {
   const double* cscale;
   int accumulate;
   int compensated;  // 1: TwoProd/TwoSum accumulation (pseudoinverse applies), 0: plain FMA
-  int mfma;         // plain form on the FP64 matrix cores (ignored when compensated)
+  int mfma;         // plain form on the FP64 matrix cores (ignored when compensated):
+                    // 2 wide tiles (up to 160 rows x 128 columns), 1 64 x 64 tiles, 0 VALU
 };
 void gemm(hipStream_t s, const GemmArgs& a);
 

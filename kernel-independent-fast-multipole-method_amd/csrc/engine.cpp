@@ -95,7 +95,7 @@ Engine::Engine(int kt, const double* coefs, int ncoefs, int np, int ptsmax, int 
   if (const char* rb = std::getenv("KIFMM3D_M2L_RB")) m2l_rb_ = std::atoi(rb);
   if (const char* bs = std::getenv("KIFMM3D_M2L_BS")) m2l_bs_ = std::atoi(bs);
   if (const char* v = std::getenv("KIFMM3D_P2P")) p2p_variant_ = std::atoi(v);
-  if (const char* g = std::getenv("KIFMM3D_GEMM")) gemm_mfma_ = std::string(g) != "valu";
+  if (const char* g = std::getenv("KIFMM3D_GEMM")) gemm_mfma_ = std::string(g) == "valu" ? 0 : std::string(g) == "tile64" ? 1 : 2;
   if (const char* m = std::getenv("KIFMM3D_M2L")) {
     const std::string v(m);
     m2l_tile_ = v != "group" && v != "pair" && v != "matrix";
@@ -813,7 +813,7 @@ void Engine::run_chunk(const double* srcden, double* trgval, int rc, int mode, h
     a.cscale = op.cscale >= 0 ? D + op.cscale : nullptr;
     a.accumulate = op.accumulate;
     a.compensated = mixed ? 0 : op.compensated;
-    a.mfma = gemm_mfma_ ? 1 : 0;
+    a.mfma = gemm_mfma_;
     dev::gemm(s, a);
   };
 
