@@ -7,6 +7,8 @@
 // 511: :315-337) including the r < 1e-8 exclusion (:21).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "device.hpp"
 
 namespace kifmm {
@@ -1171,17 +1173,23 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(GemmArgs p)
 // feeds straight from its column (contiguous in K) one step ahead.  Lane
 // (k, col) feeds B[k0 + 2k + s][col] to MFMA slice s, so slice s reads A
 // rows at K index k0 + 2k + s.  One barrier per step.
+//
+// With ks > 1 the K steps are split over ks blocks (blockIdx.z = group x ks +
+// split), each writing its raw partial product to work[split][column][row];
+// k_gemm_reduce then sums the partials in split order, so the result does not
+// depend on scheduling.
 template <int RT, int G>
-__global__ __launch_bounds__(256) void k_gemm_mx(GemmArgs p)
+__global__ __launch_bounds__(256) void k_gemm_mx(GemmArgs p, int ks)
 {
   constexpr int MR = RT * 16, NAT = (8 * MR + 255) / 256;
-  const int grp = blockIdx.z;
+  const int grp = blockIdx.z / ks, sp = blockIdx.z - grp * ks;
   const int n0 = p.gbeg[grp], n1 = p.gbeg[grp + 1];
   const long cb = (long)n0 * p.Rc + (long)blockIdx.x * 64 * G, ce = (long)n1 * p.Rc;
   if (cb >= ce) return;  // block-uniform
   const int rb = blockIdx.y * MR;
   const double* A = p.A + grp * p.a_gstride;
   const int nstep = p.nseg * p.Kseg / 8, sps = p.Kseg / 8;  // steps, steps per segment
+  const int st0 = (int)((long)nstep * sp / ks), st1 = (int)((long)nstep * (sp + 1) / ks);
   __shared__ double sA[2][8][MR];
   const int t = threadIdx.x, w = t >> 6, l = t & 63, li = l & 15, lk = l >> 4;
   int cn[G], cr[G];
@@ -1239,14 +1247,14 @@ __global__ __launch_bounds__(256) void k_gemm_mx(GemmArgs p)
   for (int rt = 0; rt < RT; rt++)
 #pragma unroll
     for (int g = 0; g < G; g++) acc[rt][g] = v4d{0.0, 0.0, 0.0, 0.0};
-  seg_ptrs(0);
-  load_a(0);
-  load_b(0, bc);
+  seg_ptrs(st0 / sps);
+  load_a(st0);
+  load_b(st0, bc);
   store_a(0);
   __syncthreads();
-  for (int st = 0; st < nstep; st++) {
-    const int buf = st & 1;
-    const bool more = st + 1 < nstep;
+  for (int st = st0; st < st1; st++) {
+    const int buf = (st - st0) & 1;
+    const bool more = st + 1 < st1;
     if (more) {
       if ((st + 1) % sps == 0) seg_ptrs((st + 1) / sps);
       load_a(st + 1);
@@ -1271,7 +1279,9 @@ __global__ __launch_bounds__(256) void k_gemm_mx(GemmArgs p)
 #pragma unroll
   for (int g = 0; g < G; g++) {
     if (!cok[g]) continue;
-    double* out = p.C + ((long)p.cnode[cn[g]] * p.Rc + cr[g]) * p.M;
+    const long jj = cb + (long)(w * G + g) * 16 + li;
+    double* out = ks > 1 ? p.work + ((long)sp * p.ctot + jj) * p.M
+                         : p.C + ((long)p.cnode[cn[g]] * p.Rc + cr[g]) * p.M;
 #pragma unroll
     for (int rt = 0; rt < RT; rt++)
 #pragma unroll
@@ -1279,30 +1289,72 @@ __global__ __launch_bounds__(256) void k_gemm_mx(GemmArgs p)
         const int row = rb + rt * 16 + lk + 4 * v;
         if (row >= p.M) continue;
         double h = acc[rt][g][v];
+        if (ks > 1) {
+          out[row] = h;
+          continue;
+        }
         if (p.cscale) h *= p.cscale[row];  // power of two: exact
         out[row] = p.accumulate ? out[row] + h : h;
       }
   }
 }
 
+__global__ __launch_bounds__(256) void k_gemm_reduce(GemmArgs p, int ks)
+{
+  const long n = p.ctot * p.M;
+  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
+    const long jj = i / p.M;
+    const int row = (int)(i - jj * p.M);
+    double h = 0.0;
+    for (int sp = 0; sp < ks; sp++) h += p.work[(long)sp * n + i];
+    if (p.cscale) h *= p.cscale[row];  // power of two: exact
+    const int cn = (int)(jj / p.Rc), cr = (int)(jj - (long)cn * p.Rc);
+    double* out = p.C + ((long)p.cnode[cn] * p.Rc + cr) * p.M + row;
+    *out = p.accumulate ? *out + h : h;
+  }
+}
+
+// Splits of K that bring a wide-tile product to about 4 blocks per CU
+// (1 = no split; each split keeps at least 4 steps of 8).
+static int gemm_split_of(const GemmArgs& a, long nb)
+{
+  if (nb >= 512) return 1;
+  const int nstep = a.nseg * a.Kseg / 8;
+  long ks = (1024 + nb - 1) / nb;
+  ks = std::min<long>(ks, std::min(16, nstep / 4));
+  return (int)std::max<long>(ks, 1);
+}
+
+int gemm_split(const GemmArgs& a)
+{
+  if (a.compensated || a.mfma != 2 || a.Kseg % 8 != 0 || a.ngroup <= 0 || a.maxcols <= 0) return 1;
+  const long cols = (long)a.maxcols * a.Rc;
+  const long nb = a.M <= 64 ? ((cols + 255) / 256) * a.ngroup : ((cols + 127) / 128) * ((a.M + 159) / 160) * a.ngroup;
+  return gemm_split_of(a, nb);
+}
+
 void gemm(hipStream_t s, const GemmArgs& a)
 {
   if (a.ngroup <= 0 || a.maxcols <= 0) return;
   long cols = (long)a.maxcols * a.Rc;
-  // wide tiles when they still give at least 2 blocks per CU, 64 x 64 tiles otherwise
+  // wide tiles, with K split over blocks (partials in a.work) when the
+  // columns alone give fewer than 2 blocks per CU; 64 x 64 tiles without a
+  // workspace
   if (!a.compensated && a.mfma == 2 && a.Kseg % 8 == 0) {
-    if (a.M <= 64) {
-      dim3 grid((unsigned)((cols + 255) / 256), 1, a.ngroup);
-      if ((long)grid.x * grid.z >= 512) {
-        hipLaunchKernelGGL((k_gemm_mx<4, 4>), grid, dim3(256), 0, s, a);
-        return;
+    const int ks = gemm_split(a);
+    if (ks == 1 || (a.work && (long)ks * a.ctot * a.M <= a.work_n)) {
+      if (a.M <= 64) {
+        dim3 grid((unsigned)((cols + 255) / 256), 1, a.ngroup * ks);
+        hipLaunchKernelGGL((k_gemm_mx<4, 4>), grid, dim3(256), 0, s, a, ks);
+      } else {
+        dim3 grid((unsigned)((cols + 127) / 128), (a.M + 159) / 160, a.ngroup * ks);
+        hipLaunchKernelGGL((k_gemm_mx<10, 2>), grid, dim3(256), 0, s, a, ks);
       }
-    } else {
-      dim3 grid((unsigned)((cols + 127) / 128), (a.M + 159) / 160, a.ngroup);
-      if ((long)grid.x * grid.y * grid.z >= 512) {
-        hipLaunchKernelGGL((k_gemm_mx<10, 2>), grid, dim3(256), 0, s, a);
-        return;
+      if (ks > 1) {
+        const long n = a.ctot * a.M;
+        hipLaunchKernelGGL(k_gemm_reduce, dim3((unsigned)std::min<long>((n + 255) / 256, 65536)), dim3(256), 0, s, a, ks);
       }
+      return;
     }
   }
   dim3 grid((unsigned)((cols + 63) / 64), (a.M + 63) / 64, a.ngroup);

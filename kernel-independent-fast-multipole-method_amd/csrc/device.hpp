@@ -82,8 +82,13 @@ struct GemmArgs {
   int compensated;  // 1: TwoProd/TwoSum accumulation (pseudoinverse applies), 0: plain FMA
   int mfma;         // plain form on the FP64 matrix cores (ignored when compensated):
                     // 2 wide tiles (up to 160 rows x 128 columns), 1 64 x 64 tiles, 0 VALU
+  long ctot;        // all columns of the product (ncol x Rc)
+  double* work;     // split-K partials, gemm_split(a) x ctot x M doubles (null or too small: no split)
+  long work_n;      // capacity of work in doubles
 };
 void gemm(hipStream_t s, const GemmArgs& a);
+// K splits the wide-tile product takes for these sizes (1: none).
+int gemm_split(const GemmArgs& a);
 
 // r2c of the scaled upward equivalent densities on the (2np)^3 grid into
 // complex spectra, effden[slot][Rc][sm][C] (rhs_inner 0) or

@@ -664,6 +664,20 @@ void Engine::ensure_work(int rc)
   dneden_.ensure((size_t)rc * T.tndecnt * O.pln[SURF_DE]);
   effden_.ensure((size_t)2 * rc * plan_.max_vsrc * mm_.sdof * O.C);
   effval_.ensure((size_t)2 * rc * plan_.max_vtrg * mm_.tdof * O.C);
+  // split-K partials of the matrix-core GEMMs with few columns, for every
+  // chunk size up to rc (the last chunk of an evaluate may be smaller)
+  size_t gw = 0;
+  for (const std::vector<GemmOp>* ops : {&plan_.up, &plan_.down})
+    for (const GemmOp& op : *ops)
+      for (int r = 1; r <= rc; r++) {
+        dev::GemmArgs a{};
+        a.M = op.M; a.Kseg = op.Kseg; a.nseg = op.nseg; a.Rc = r; a.ngroup = op.ngroup; a.maxcols = op.maxcols;
+        a.compensated = 0;
+        a.mfma = gemm_mfma_;
+        const int ks = dev::gemm_split(a);
+        if (ks > 1) gw = std::max(gw, (size_t)ks * op.ncol * r * op.M);
+      }
+  gwork_.ensure(gw);
 }
 
 void Engine::check_eval_args(const double* d_srcden, const double* d_trgval, int nrhs, int mode) const
@@ -814,6 +828,9 @@ void Engine::run_chunk(const double* srcden, double* trgval, int rc, int mode, h
     a.accumulate = op.accumulate;
     a.compensated = mixed ? 0 : op.compensated;
     a.mfma = gemm_mfma_;
+    a.ctot = (long)op.ncol * rc;
+    a.work = gwork_.p;
+    a.work_n = (long)gwork_.n;
     dev::gemm(s, a);
   };
 

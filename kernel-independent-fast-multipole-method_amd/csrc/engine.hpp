@@ -178,6 +178,7 @@ class Engine {
   DVec<double> sden_, tval_, upcval_, upeden_, dncval_, dneden_, effden_, effval_;
   DVec<double> hsrc_, htrg_;  // staging for the host-pointer entry point
   DVec<double> chk_, part_, dout_;  // direct-sum scratch
+  DVec<double> gwork_;              // split-K partials of the GEMMs
   std::vector<double> tpos_host_;  // caller-order target positions (for direct sums)
 };
 
