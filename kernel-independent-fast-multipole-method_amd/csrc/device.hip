@@ -450,6 +450,51 @@ __global__ __launch_bounds__(64) void k_p2p_s(const int* __restrict__ tbeg, cons
 typedef double v4d __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 
+// One staged source tile against the wave's first NG 16-target column blocks
+// (NG = the blocks that hold targets, so ragged leaves do not pay for empty
+// blocks).
+template <int NG>
+__device__ __forceinline__ void p2p_tile64(int cnt4, int kk, int col, const double (*s_pos)[TILE],
+                                           const double (*s_den)[16], const double* tx, const double* ty,
+                                           const double* tz, v4d* acc)
+{
+  for (int g = 0; g < cnt4; g += 4) {
+    const double sx = s_pos[0][g + kk], sy = s_pos[1][g + kk], sz = s_pos[2][g + kk];
+    const double f = s_den[g + kk][col];
+#pragma unroll
+    for (int i = 0; i < NG; i++) {
+      const double dx = tx[i] - sx, dy = ty[i] - sy, dz = tz[i] - sz;
+      const double gv = rinv(dx * dx + dy * dy + dz * dz);
+      acc[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(f, gv, acc[i], 0, 0, 0);
+    }
+  }
+}
+
+template <int NG>
+__device__ __forceinline__ void p2p_tile32(int cnt4, int kk, int col, const float (*s_pos)[TILE],
+                                           const float (*s_den)[16], const float* tx, const float* ty,
+                                           const float* tz, double (*acc)[4])
+{
+  v4f part[NG];
+#pragma unroll
+  for (int i = 0; i < NG; i++) part[i] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+  for (int g = 0; g < cnt4; g += 4) {
+    const float sx = s_pos[0][g + kk], sy = s_pos[1][g + kk], sz = s_pos[2][g + kk];
+    const float f = s_den[g + kk][col];
+#pragma unroll
+    for (int i = 0; i < NG; i++) {
+      const float dx = tx[i] - sx, dy = ty[i] - sy, dz = tz[i] - sz;
+      const float r2 = dx * dx + dy * dy + dz * dz;
+      const float gv = r2 < 1e-16f ? 0.0f : __builtin_amdgcn_rsqf(r2);
+      part[i] = __builtin_amdgcn_mfma_f32_16x16x4f32(f, gv, part[i], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < NG; i++)
+#pragma unroll
+    for (int v = 0; v < 4; v++) acc[i][v] += (double)part[i][v];
+}
+
 __global__ __launch_bounds__(P2P_BS) void k_p2p_mfma111(int nchunk, const int* __restrict__ tbeg,
                                                         const int* __restrict__ tnum, const int* __restrict__ eptr,
                                                         const int2* __restrict__ ent, Points src, Targets trg, int roff,
@@ -464,6 +509,7 @@ __global__ __launch_bounds__(P2P_BS) void k_p2p_mfma111(int nchunk, const int* _
   const int r0 = roff + blockIdx.y * NR;
   const int tb = tbeg[c], tn = tnum[c];
   const bool wact = 64 * w < tn;  // wave-uniform
+  const int ng = wact ? min(4, (tn - 64 * w + 15) >> 4) : 0;  // 16-target blocks with targets
   double tx[4], ty[4], tz[4];
 #pragma unroll
   for (int i = 0; i < 4; i++) {
@@ -492,17 +538,14 @@ __global__ __launch_bounds__(P2P_BS) void k_p2p_mfma111(int nchunk, const int* _
       stage_den<1, NR, P2P_BS>(s_den, den, src, j0, cnt, t);
       for (int i = cnt * NR + t; i < cnt4 * NR; i += P2P_BS) s_den[i / NR][i % NR] = 0.0;
       __syncthreads();
-      if (wact)
-        for (int g = 0; g < cnt4; g += 4) {
-          const double sx = s_pos[0][g + kk], sy = s_pos[1][g + kk], sz = s_pos[2][g + kk];
-          const double f = s_den[g + kk][col];
-#pragma unroll
-          for (int i = 0; i < 4; i++) {
-            const double dx = tx[i] - sx, dy = ty[i] - sy, dz = tz[i] - sz;
-            const double gv = rinv(dx * dx + dy * dy + dz * dz);
-            acc[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(f, gv, acc[i], 0, 0, 0);
-          }
-        }
+      if (ng == 4)
+        p2p_tile64<4>(cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc);
+      else if (ng == 3)
+        p2p_tile64<3>(cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc);
+      else if (ng == 2)
+        p2p_tile64<2>(cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc);
+      else if (ng == 1)
+        p2p_tile64<1>(cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc);
     }
   }
   if (!wact) return;
@@ -538,6 +581,7 @@ __global__ __launch_bounds__(P2P_BS) void k_p2p_mfma111_f32(int nchunk, const in
   const int r0 = roff + blockIdx.y * NR;
   const int tb = tbeg[c], tn = tnum[c];
   const bool wact = 64 * w < tn;  // wave-uniform
+  const int ng = wact ? min(4, (tn - 64 * w + 15) >> 4) : 0;  // 16-target blocks with targets
   const double ox = trg.x[tb], oy = trg.y[tb], oz = trg.z[tb];
   float tx[4], ty[4], tz[4];
 #pragma unroll
@@ -571,26 +615,14 @@ __global__ __launch_bounds__(P2P_BS) void k_p2p_mfma111_f32(int nchunk, const in
         s_den[jj][q] = jj < cnt ? (float)den[(long)(j0 + jj) * src.den_js + (long)q * src.den_rs] : 0.0f;
       }
       __syncthreads();
-      if (wact) {
-        v4f part[4];
-#pragma unroll
-        for (int i = 0; i < 4; i++) part[i] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
-        for (int g = 0; g < cnt4; g += 4) {
-          const float sx = s_pos[0][g + kk], sy = s_pos[1][g + kk], sz = s_pos[2][g + kk];
-          const float f = s_den[g + kk][col];
-#pragma unroll
-          for (int i = 0; i < 4; i++) {
-            const float dx = tx[i] - sx, dy = ty[i] - sy, dz = tz[i] - sz;
-            const float r2 = dx * dx + dy * dy + dz * dz;
-            const float gv = r2 < 1e-16f ? 0.0f : __builtin_amdgcn_rsqf(r2);
-            part[i] = __builtin_amdgcn_mfma_f32_16x16x4f32(f, gv, part[i], 0, 0, 0);
-          }
-        }
-#pragma unroll
-        for (int i = 0; i < 4; i++)
-#pragma unroll
-          for (int v = 0; v < 4; v++) acc[i][v] += (double)part[i][v];
-      }
+      if (ng == 4)
+        p2p_tile32<4>(cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc);
+      else if (ng == 3)
+        p2p_tile32<3>(cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc);
+      else if (ng == 2)
+        p2p_tile32<2>(cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc);
+      else if (ng == 1)
+        p2p_tile32<1>(cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc);
     }
   }
   if (!wact) return;
