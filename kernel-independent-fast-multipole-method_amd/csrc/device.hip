@@ -229,15 +229,28 @@ struct Knl<521> {
 
 // ---------------------------------------------------------------------------
 // permutations
-__global__ void k_gather(int n, int dof, int nr, const int* __restrict__ perm, const double* __restrict__ src,
+// One caller-order point per thread: its densities are read across the
+// right-hand sides (coalesced over the threads for each one) and written as
+// the point's contiguous destination row (a full line for 16 RHS), instead of
+// gathering 8 bytes from a different line per (point, RHS).
+__global__ void k_gather(int n, int dof, int nr, const int* __restrict__ iperm, const double* __restrict__ src,
                          long src_rs, double* __restrict__ dst)
 {
-  long tot = (long)n * dof * nr;
-  const int per = nr * dof;  // i = k*per + r*dof + d, in destination order
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < tot; i += (long)gridDim.x * blockDim.x) {
-    int k = (int)(i / per), rem = (int)(i - (long)k * per);
-    int r = rem / dof, d = rem - r * dof;
-    dst[i] = src[r * src_rs + (long)perm[k] * dof + d];
+  const int per = nr * dof;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    double* out = dst + (long)iperm[i] * per;
+    const double* in = src + i * dof;
+    for (int e0 = 0; e0 < per; e0 += 8) {  // 8 loads in flight, then 8 stores
+      double v[8];
+#pragma unroll
+      for (int j = 0; j < 8; j++) {
+        const int e = min(e0 + j, per - 1), r = e / dof;
+        v[j] = in[r * src_rs + (e - r * dof)];
+      }
+#pragma unroll
+      for (int j = 0; j < 8; j++)
+        if (e0 + j < per) out[e0 + j] = v[j];
+    }
   }
 }
 
@@ -286,11 +299,10 @@ void copy(hipStream_t s, const double* src, double* dst, long n)
   hipLaunchKernelGGL(k_copy, dim3(grid_for(n, 256)), dim3(256), 0, s, n, src, dst);
 }
 
-void gather(hipStream_t s, int n, int dof, int nr, const int* perm, const double* src, long src_rs, double* dst)
+void gather(hipStream_t s, int n, int dof, int nr, const int* iperm, const double* src, long src_rs, double* dst)
 {
-  long tot = (long)n * dof * nr;
-  if (tot == 0) return;
-  hipLaunchKernelGGL(k_gather, dim3(grid_for(tot, 256)), dim3(256), 0, s, n, dof, nr, perm, src, src_rs, dst);
+  if ((long)n * dof * nr == 0) return;
+  hipLaunchKernelGGL(k_gather, dim3(grid_for(n, 256)), dim3(256), 0, s, n, dof, nr, iperm, src, src_rs, dst);
 }
 
 void scatter(hipStream_t s, int n, int dof, int nr, const int* perm, const double* src, long src_rs, double* dst,

@@ -15,8 +15,9 @@ struct KC {  // kernel constants (see kernel_consts in kernels_host.cpp)
 void zero(hipStream_t s, double* p, long n);
 // dst[0 .. n) = src[0 .. n)
 void copy(hipStream_t s, const double* src, double* dst, long n);
-// Source-major gather: dst[(k*nr + r)*dof + d] = src[r*src_rs + perm[k]*dof + d]
-void gather(hipStream_t s, int n, int dof, int nr, const int* perm, const double* src, long src_rs, double* dst);
+// Source-major gather through the inverse permutation (iperm[perm[k]] = k):
+// dst[(iperm[i]*nr + r)*dof + d] = src[r*src_rs + i*dof + d] for i < n
+void gather(hipStream_t s, int n, int dof, int nr, const int* iperm, const double* src, long src_rs, double* dst);
 // dst[r*dst_rs + perm[k]*dof + d] = src[r*src_rs + k*dof + d]
 void scatter(hipStream_t s, int n, int dof, int nr, const int* perm, const double* src, long src_rs, double* dst,
              long dst_rs);

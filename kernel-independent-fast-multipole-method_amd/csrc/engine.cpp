@@ -180,7 +180,11 @@ void Engine::setup(const double* spos, const double* snor, int ns, const double*
     bx[k] = p[0]; by[k] = p[1]; bz[k] = p[2];
   }
   tx_.upload(bx); ty_.upload(by); tz_.upload(bz);
-  sperm_.upload(tree_.sperm);
+  {
+    std::vector<int> inv(tree_.sperm.size());
+    for (size_t k = 0; k < inv.size(); k++) inv[tree_.sperm[k]] = (int)k;
+    siperm_.upload(inv);
+  }
   tperm_.upload(tree_.tperm);
   upload_static();
   HIPCHK(hipDeviceSynchronize());
@@ -869,7 +873,7 @@ void Engine::run_chunk(const double* srcden, double* trgval, int rc, int mode, h
   if (P.nlow_s) dev::zero(s, upeden_.p, (long)rc * P.nlow_s * pUE);
   if (P.nlow_t) dev::zero(s, dneden_.p, (long)rc * P.nlow_t * pDE);
   // load (fmm3d_eval.cpp:50-64)
-  timed(KIFMM3D_K_GATHER, [&] { dev::gather(s, T.sextcnt, sdof, rc, sperm_.p, srcden, (long)ns_ * sdof, sden_.p); });
+  timed(KIFMM3D_K_GATHER, [&] { dev::gather(s, T.sextcnt, sdof, rc, siperm_.p, srcden, (long)ns_ * sdof, sden_.p); });
   mark(1);
   // up: S2M, then M2M + UC2UE level by level
   timed(KIFMM3D_K_S2M, [&] {
@@ -990,7 +994,7 @@ void Engine::direct(const double* srcden, bool on_device, int nrhs, const int* i
   sden.alloc((size_t)nrhs * srs);
   part_.ensure((size_t)nrhs * nseg * nchk * tdof);
   dout_.ensure((size_t)nrhs * nchk * tdof);
-  dev::gather(stream_, tree_.sextcnt, sdof, nrhs, sperm_.p, dsrc, (long)ns_ * sdof, sden.p);
+  dev::gather(stream_, tree_.sextcnt, sdof, nrhs, siperm_.p, dsrc, (long)ns_ * sdof, sden.p);
   dev::Points src{sx_.p, sy_.p, sz_.p, snx_.p, sny_.p, snz_.p, sden.p, (long)nrhs * sdof, sdof};
   const dev::KC kcu{kcu_.a, kcu_.b, kcu_.c};
   dev::direct(stream_, user_.kt, tdof, nchk, chk_.p, chk_.p + nchk, chk_.p + 2 * nchk, ns_, nseg, src, part_.p,
