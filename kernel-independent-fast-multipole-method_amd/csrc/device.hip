@@ -2256,15 +2256,48 @@ void m2l_tiled(hipStream_t s, int C, int ntile, int Rc, const int* ttile, const 
 // spectra (8 frequencies) is fetched into its L2 once for all 8.
 constexpr int M2LV_BS = 256;
 
-template <int SM, int TM, int G>
+// Precision traits of the parent-grouped M2L.  FP64: row r16 of a row tile is
+// output r16 % 8, part r16 / 8, so accumulator items (h, h + 2) of lane k are
+// (re, im) of output k + 4h.  FP32 (output item v of lane l at row
+// 4*(l/16) + v): rows interleave (re, im), row r16 is output r16 / 2, part
+// r16 % 2, so items (2h, 2h + 1) of lane k are (re, im) of output 2k + h.
+template <class R>
+struct Mv;
+template <>
+struct Mv<double> {
+  typedef double2 C2;
+  typedef v4d V4;
+  __device__ static inline V4 mfma(double a, double b, V4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+  __device__ static inline int row_out(int r16) { return r16 & 7; }
+  __device__ static inline int row_im(int r16) { return r16 >> 3; }
+  __device__ static inline int item_out(int k, int h) { return k + 4 * h; }
+  __device__ static inline int item_re(int h) { return h; }
+  __device__ static inline int item_im(int h) { return h + 2; }
+};
+template <>
+struct Mv<float> {
+  typedef float2 C2;
+  typedef v4f V4;
+  __device__ static inline V4 mfma(float a, float b, V4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+  __device__ static inline int row_out(int r16) { return r16 >> 1; }
+  __device__ static inline int row_im(int r16) { return r16 & 1; }
+  __device__ static inline int item_out(int k, int h) { return 2 * k + h; }
+  __device__ static inline int item_re(int h) { return 2 * h; }
+  __device__ static inline int item_im(int h) { return 2 * h + 1; }
+};
+
+template <class R, int SM, int TM, int G>
 __global__ __launch_bounds__(M2LV_BS) void k_m2l_vec(int C, int Rc, long ncol, int ncb, const int* __restrict__ tslot,
                                                      const int* __restrict__ sslot, const int* __restrict__ pmask,
-                                                     const int* __restrict__ optab, const double2* __restrict__ opsT,
-                                                     const double2* __restrict__ effden,
-                                                     double2* __restrict__ effval)
+                                                     const int* __restrict__ optab,
+                                                     const typename Mv<R>::C2* __restrict__ opsT,
+                                                     const typename Mv<R>::C2* __restrict__ effden,
+                                                     typename Mv<R>::C2* __restrict__ effval)
 {
+  typedef typename Mv<R>::C2 C2;
+  typedef typename Mv<R>::V4 V4;
   constexpr int NKS = 4 * SM, NH = NKS / 2, ST = SM * TM;
-  __shared__ double s_A[TM][NKS][64];
+  __shared__ R s_A[TM][NKS][64];
   __shared__ int s_ot[26 * 64];
   // items (frequency group of 8, column block, frequency in group): the 8
   // frequencies of one spectra line run side by side on one XCD
@@ -2273,7 +2306,7 @@ __global__ __launch_bounds__(M2LV_BS) void k_m2l_vec(int C, int Rc, long ncol, i
   const int fg = item / (8 * ncb), rem = item - fg * 8 * ncb, cb = rem >> 3, f = fg * 8 + (rem & 7);
   if (item >= nitem || f >= C) return;  // block-uniform
   const int t = threadIdx.x, w = t >> 6, l = t & 63, li = l & 15, lk = l >> 4;
-  const double2* Of = opsT + (long)f * 343 * ST;
+  const C2* Of = opsT + (long)f * 343 * ST;
   for (int i = t; i < 26 * 64; i += M2LV_BS) s_ot[i] = optab[i];
   int q[G], r[G], msk[G];
 #pragma unroll
@@ -2283,11 +2316,11 @@ __global__ __launch_bounds__(M2LV_BS) void k_m2l_vec(int C, int Rc, long ncol, i
     r[g] = col < ncol ? (int)(col - (long)(col / Rc) * Rc) : 0;
     msk[g] = q[g] >= 0 ? pmask[q[g]] : 0;
   }
-  v4d acc[G][TM];
+  V4 acc[G][TM];
 #pragma unroll
   for (int g = 0; g < G; g++)
 #pragma unroll
-    for (int rt = 0; rt < TM; rt++) acc[g][rt] = v4d{0.0, 0.0, 0.0, 0.0};
+    for (int rt = 0; rt < TM; rt++) acc[g][rt] = V4{0, 0, 0, 0};
   for (int n = 0; n < 26; n++) {
     __syncthreads();  // s_ot is staged; the previous direction is done with s_A
     // the block's 64 TM SM entries of the real-embedded operator, TM SM per
@@ -2295,11 +2328,11 @@ __global__ __launch_bounds__(M2LV_BS) void k_m2l_vec(int C, int Rc, long ncol, i
     // before the first use
     {
       int oo[ST];
-      double2 m[ST];
+      C2 m[ST];
 #pragma unroll
       for (int i = 0; i < ST; i++) {
         const int e = t + M2LV_BS * i, ll = e & 63, kr = e >> 6, ks = kr % NKS, rt = kr / NKS;
-        const int qo = rt * 8 + (ll & 7), c = qo / TM, ti = qo - c * TM;
+        const int qo = rt * 8 + Mv<R>::row_out(ll & 15), c = qo / TM, ti = qo - c * TM;
         const int kap = ks * 4 + (ll >> 4), cs = kap % (8 * SM), sc = cs / SM, j = cs - sc * SM;
         oo[i] = s_ot[n * 64 + sc * 8 + c];
         m[i] = Of[(long)(oo[i] < 0 ? 0 : oo[i]) * ST + j * TM + ti];
@@ -2308,8 +2341,8 @@ __global__ __launch_bounds__(M2LV_BS) void k_m2l_vec(int C, int Rc, long ncol, i
       for (int i = 0; i < ST; i++) {
         const int e = t + M2LV_BS * i, ll = e & 63, kr = e >> 6, ks = kr % NKS, rt = kr / NKS;
         const int pin = (ks * 4 + (ll >> 4)) / (8 * SM);
-        const double v = ((ll & 15) < 8) ? (pin == 0 ? m[i].x : -m[i].y) : (pin == 0 ? m[i].y : m[i].x);
-        s_A[rt][ks][ll] = oo[i] < 0 ? 0.0 : v;
+        const R v = Mv<R>::row_im(ll & 15) == 0 ? (pin == 0 ? m[i].x : -m[i].y) : (pin == 0 ? m[i].y : m[i].x);
+        s_A[rt][ks][ll] = oo[i] < 0 ? (R)0 : v;
       }
     }
     __syncthreads();
@@ -2320,7 +2353,7 @@ __global__ __launch_bounds__(M2LV_BS) void k_m2l_vec(int C, int Rc, long ncol, i
       // (clamped to slot 0 where the neighbour has no such child)
       const int* ss = sslot + ((long)(q[g] < 0 ? 0 : q[g]) * 26 + n) * 8;
       int sl[NH];
-      double2 x[NH];
+      C2 x[NH];
 #pragma unroll
       for (int kh = 0; kh < NH; kh++) sl[kh] = q[g] >= 0 ? ss[(kh * 4 + lk) / SM] : -1;
 #pragma unroll
@@ -2330,11 +2363,11 @@ __global__ __launch_bounds__(M2LV_BS) void k_m2l_vec(int C, int Rc, long ncol, i
       }
 #pragma unroll
       for (int kh = 0; kh < NH; kh++) {
-        const double xr = sl[kh] < 0 ? 0.0 : x[kh].x, xi = sl[kh] < 0 ? 0.0 : x[kh].y;
+        const R xr = sl[kh] < 0 ? (R)0 : x[kh].x, xi = sl[kh] < 0 ? (R)0 : x[kh].y;
 #pragma unroll
         for (int rt = 0; rt < TM; rt++) {
-          acc[g][rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(s_A[rt][kh][l], xr, acc[g][rt], 0, 0, 0);
-          acc[g][rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(s_A[rt][kh + NH][l], xi, acc[g][rt], 0, 0, 0);
+          acc[g][rt] = Mv<R>::mfma(s_A[rt][kh][l], xr, acc[g][rt]);
+          acc[g][rt] = Mv<R>::mfma(s_A[rt][kh + NH][l], xi, acc[g][rt]);
         }
       }
     }
@@ -2346,48 +2379,62 @@ __global__ __launch_bounds__(M2LV_BS) void k_m2l_vec(int C, int Rc, long ncol, i
     for (int rt = 0; rt < TM; rt++)
 #pragma unroll
       for (int h = 0; h < 2; h++) {
-        const int qo = rt * 8 + lk + 4 * h, c = qo / TM, ti = qo - c * TM;
+        const int qo = rt * 8 + Mv<R>::item_out(lk, h), c = qo / TM, ti = qo - c * TM;
         const int tl = tslot[(long)q[g] * 8 + c];
-        if (tl >= 0)
-          effval[(((long)tl * Rc + r[g]) * TM + ti) * C + f] = make_double2(acc[g][rt][h], acc[g][rt][h + 2]);
+        if (tl >= 0) {
+          C2 v;
+          v.x = acc[g][rt][Mv<R>::item_re(h)];
+          v.y = acc[g][rt][Mv<R>::item_im(h)];
+          effval[(((long)tl * Rc + r[g]) * TM + ti) * C + f] = v;
+        }
       }
   }
 }
 
-template <int SM, int TM, int G>
+template <class R, int SM, int TM, int G>
 static void m2l_vec_launch(hipStream_t s, int C, int npar, int Rc, const int* tslot, const int* sslot,
-                           const int* pmask, const int* optab, const double2* opsT, const double2* effden,
-                           double2* effval)
+                           const int* pmask, const int* optab, const void* opsT, const void* effden, void* effval)
 {
+  typedef typename Mv<R>::C2 C2;
   const long ncol = (long)npar * Rc;
   const int ncb = (int)((ncol + 64 * G - 1) / (64 * G));
-  hipLaunchKernelGGL((k_m2l_vec<SM, TM, G>), dim3(xcd_grid((C + 7) / 8 * ncb * 8)), dim3(M2LV_BS), 0, s, C, Rc, ncol,
-                     ncb, tslot, sslot, pmask, optab, opsT, effden, effval);
+  hipLaunchKernelGGL((k_m2l_vec<R, SM, TM, G>), dim3(xcd_grid((C + 7) / 8 * ncb * 8)), dim3(M2LV_BS), 0, s, C, Rc,
+                     ncol, ncb, tslot, sslot, pmask, optab, (const C2*)opsT, (const C2*)effden, (C2*)effval);
 }
 
-void m2l_matrix(hipStream_t s, int sm, int tm, int C, int npar, int Rc, const int* tslot, const int* sslot,
-                const int* pmask, const int* optab, const double2* opsT, const double2* effden, double2* effval)
+template <class R>
+static void m2l_matrix_t(hipStream_t s, int sm, int tm, int C, int npar, int Rc, const int* tslot, const int* sslot,
+                         const int* pmask, const int* optab, const void* opsT, const void* effden, void* effval)
 {
-  if (npar <= 0) return;
   const long ncol = (long)npar * Rc;
   // fewer tiles per wave when the level is too small to fill the chip otherwise
   const bool wide = ncol * C >= 64L * 4 * 256 * 8;
   if (sm == 1 && tm == 1) {
     if (wide)
-      m2l_vec_launch<1, 1, 4>(s, C, npar, Rc, tslot, sslot, pmask, optab, opsT, effden, effval);
+      m2l_vec_launch<R, 1, 1, 4>(s, C, npar, Rc, tslot, sslot, pmask, optab, opsT, effden, effval);
     else
-      m2l_vec_launch<1, 1, 1>(s, C, npar, Rc, tslot, sslot, pmask, optab, opsT, effden, effval);
+      m2l_vec_launch<R, 1, 1, 1>(s, C, npar, Rc, tslot, sslot, pmask, optab, opsT, effden, effval);
   } else if (sm == 4 && tm == 3) {
     if (wide)
-      m2l_vec_launch<4, 3, 2>(s, C, npar, Rc, tslot, sslot, pmask, optab, opsT, effden, effval);
+      m2l_vec_launch<R, 4, 3, 2>(s, C, npar, Rc, tslot, sslot, pmask, optab, opsT, effden, effval);
     else
-      m2l_vec_launch<4, 3, 1>(s, C, npar, Rc, tslot, sslot, pmask, optab, opsT, effden, effval);
+      m2l_vec_launch<R, 4, 3, 1>(s, C, npar, Rc, tslot, sslot, pmask, optab, opsT, effden, effval);
   } else if (sm == 3 && tm == 3) {
     if (wide)
-      m2l_vec_launch<3, 3, 2>(s, C, npar, Rc, tslot, sslot, pmask, optab, opsT, effden, effval);
+      m2l_vec_launch<R, 3, 3, 2>(s, C, npar, Rc, tslot, sslot, pmask, optab, opsT, effden, effval);
     else
-      m2l_vec_launch<3, 3, 1>(s, C, npar, Rc, tslot, sslot, pmask, optab, opsT, effden, effval);
+      m2l_vec_launch<R, 3, 3, 1>(s, C, npar, Rc, tslot, sslot, pmask, optab, opsT, effden, effval);
   }
+}
+
+void m2l_matrix(hipStream_t s, int sm, int tm, int C, int npar, int Rc, const int* tslot, const int* sslot,
+                const int* pmask, const int* optab, const void* opsT, const void* effden, void* effval, int single)
+{
+  if (npar <= 0) return;
+  if (single)
+    m2l_matrix_t<float>(s, sm, tm, C, npar, Rc, tslot, sslot, pmask, optab, opsT, effden, effval);
+  else
+    m2l_matrix_t<double>(s, sm, tm, C, npar, Rc, tslot, sslot, pmask, optab, opsT, effden, effval);
 }
 
 }  // namespace dev

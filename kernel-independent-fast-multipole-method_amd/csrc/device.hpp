@@ -107,12 +107,13 @@ void m2l(hipStream_t s, int sm, int tm, int C, int ntrg, int Rc, const int* vptr
 // rb: right-hand sides per lane (0 = default), bs: frequencies per block (64 or 256).
 void m2l_grouped(hipStream_t s, int sm, int tm, int C, int npar, int Rc, int rb, int bs, const int* tslot,
                  const int* sslot, const int* optab, const double2* ops, const double2* effden, double2* effval);
-// The same sums on the matrix cores (FP64): columns (parent, right-hand side)
-// in tiles of 16, operators from the frequency-major copy opsT
-// [C][343][sm][tm]; pmask[p] bit n set if parent p has a source child in
-// direction n.
+// The same sums on the matrix cores: columns (parent, right-hand side) in
+// tiles of 16, operators from the frequency-major copy opsT [C][343][sm][tm];
+// pmask[p] bit n set if parent p has a source child in direction n.  single
+// selects FP32 spectra and operators (float2, all directions accumulated in
+// FP32), otherwise FP64 (double2).
 void m2l_matrix(hipStream_t s, int sm, int tm, int C, int npar, int Rc, const int* tslot, const int* sslot,
-                const int* pmask, const int* optab, const double2* opsT, const double2* effden, double2* effval);
+                const int* pmask, const int* optab, const void* opsT, const void* effden, void* effval, int single);
 // c2r and gather onto the DC surface: dncval[(node*Rc + r)*pln + p*tm + f] += ...
 // (effval in the layout and precision described for fft_fwd)
 void fft_inv(hipStream_t s, int np, int nslot, const int* slot_node, int Rc, const void* effval, int single,

@@ -210,10 +210,8 @@ void Engine::upload_static()
           for (int c = 0; c < 2; c++)
             t[(((size_t)f * 343 + o) * st + e) * 2 + c] = O.m2l[(((size_t)o * st + e) * O.C + f) * 2 + c];
     opsT_.upload(t);
-    if (st == 1) {
-      std::vector<float> t32(t.begin(), t.end());
-      opsT32_.upload(t32);
-    }
+    std::vector<float> t32(t.begin(), t.end());
+    opsT32_.upload(t32);
   }
   c2s_.upload(O.cube2surf);
   std::vector<int> s2c(O.nsp[SURF_DC], 0);
@@ -900,7 +898,8 @@ void Engine::run_chunk(const double* srcden, double* trgval, int rc, int mode, h
   // right-hand sides, the parent-grouped matrix-core form otherwise (VALU
   // grouped and per-pair forms for comparison)
   const bool tiled = m2l_tile_ && !P.m2l_pairs && sm == 1 && tm == 1 && rc % 16 == 0;
-  const int ri = tiled ? 1 : 0, single = tiled && mixed ? 1 : 0;
+  const bool matrix = !tiled && m2l_matrix_ && !P.m2l_pairs;
+  const int ri = tiled ? 1 : 0, single = mixed && (tiled || matrix) ? 1 : 0;
   for (const VLevel& vl : P.vlev) {
     timed(KIFMM3D_K_FFT_FWD, [&] {
       dev::fft_fwd(s, np_, vl.nsrc, I + vl.src_node, rc, upeden_.p, pUE, sm, D + vl.dscale, c2s_.p, effden_.p, single,
@@ -915,7 +914,7 @@ void Engine::run_chunk(const double* srcden, double* trgval, int rc, int mode, h
                  (const double2*)effden_.p, (double2*)effval_.p);
       else if (m2l_matrix_)
         dev::m2l_matrix(s, sm, tm, O.C, vl.npar, rc, I + vl.tslot, I + vl.sslot, I + vl.pmask, I + P.optab,
-                        (const double2*)opsT_.p, (const double2*)effden_.p, (double2*)effval_.p);
+                        single ? (const void*)opsT32_.p : (const void*)opsT_.p, effden_.p, effval_.p, single);
       else
         dev::m2l_grouped(s, sm, tm, O.C, vl.npar, rc, m2l_rb_, m2l_bs_, I + vl.tslot, I + vl.sslot, I + P.optab,
                          (const double2*)m2l_.p, (const double2*)effden_.p, (double2*)effval_.p);

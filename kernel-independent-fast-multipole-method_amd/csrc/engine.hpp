@@ -171,7 +171,7 @@ class Engine {
   DVec<double> unit_[4];
   DVec<double> uc2ue_, ue2uc_, dc2de_, de2dc_, m2l_;
   DVec<double> opsT_;  // M2L operators as [C][343][sdof][tdof] complex (FP64)
-  DVec<float> opsT32_; // scalar kernels: the same in FP32
+  DVec<float> opsT32_; // the same in FP32
   DVec<double> tw_;
   DVec<int> I_;
   DVec<double> D_;

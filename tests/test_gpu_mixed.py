@@ -4,8 +4,9 @@ The mixed form has no counterpart in the reference: it is bounded against the
 plain FP64 form on the same tree (the FP32 products must not move the result
 by more than MIXED_VS_PLAIN relative L2) and against the FP64 direct sum (the
 FMM error band of np).  Blocks of 16 right-hand sides take the FP32
-matrix-core kernels; the 5-RHS remainder of the 21-RHS case takes the FP64
-fallbacks, and other kernels take the FP64 kernels throughout.
+matrix-core near field and tiled M2L; other right-hand-side counts and the
+vector kernels take the FP32 parent-grouped matrix-core M2L with FP64 near
+fields.
 """
 import numpy as np
 import pytest
@@ -58,6 +59,15 @@ def test_mixed_laplace_21rhs_adaptive():
     run(111, 4, 60, p, None, (0.5, 0.5, 0.5), 1, 21, 22)
 
 
-def test_mixed_stokes_falls_back_to_fp64():
+def test_mixed_stokes_dl_fp32_m2l():
     p, n = sphere(3000, 23)
     run(321, 4, 40, p, n, (0.0, 0.0, 0.0), 0, 16, 23)
+
+
+def test_mixed_vector_kernels_one_rhs():
+    """One right-hand side of the 4x3 and 3x3 translation shapes (np 6 and 8)
+    through the FP32 parent-grouped M2L."""
+    p, n = sphere(6000, 24)
+    run(321, 6, 60, p, n, (0.0, 0.0, 0.0), 0, 1, 24)
+    c = clusters(8000, 25)
+    run(511, 8, 80, c, None, (0.5, 0.5, 0.5), 1, 1, 25)
