@@ -482,8 +482,10 @@ __device__ __forceinline__ void p2p_tile64(int cnt4, int kk, int col, const doub
   }
 }
 
+constexpr int TILE32 = 128;  // sources per staged tile of the FP32 kernel (measured 2 % faster than 64)
+
 template <int NG>
-__device__ __forceinline__ void p2p_tile32(int cnt4, int kk, int col, const float (*s_pos)[TILE],
+__device__ __forceinline__ void p2p_tile32(int cnt4, int kk, int col, const float (*s_pos)[TILE32],
                                            const float (*s_den)[16], const float* tx, const float* ty,
                                            const float* tz, double (*acc)[4])
 {
@@ -578,15 +580,15 @@ __global__ __launch_bounds__(P2P_BS) void k_p2p_mfma111(int nchunk, const int* _
 // relative to the chunk's first target in FP64 and then rounded, so FP32
 // keeps the differences of nearby points to ~1e-7 of the leaf size; 1/r
 // comes from v_rsq_f32.  The FP32 products of one staged source tile (at most
-// 64 sources) are summed into FP64 accumulators.
+// 128 sources) are summed into FP64 accumulators.
 __global__ __launch_bounds__(P2P_BS) void k_p2p_mfma111_f32(int nchunk, const int* __restrict__ tbeg,
                                                             const int* __restrict__ tnum, const int* __restrict__ eptr,
                                                             const int2* __restrict__ ent, Points src, Targets trg,
                                                             int roff, KC kc)
 {
   constexpr int NR = 16;
-  __shared__ float s_pos[3][TILE];
-  __shared__ float s_den[TILE][NR];
+  __shared__ float s_pos[3][TILE32];
+  __shared__ float s_den[TILE32][NR];
   const int c = xcd_item(blockIdx.x, nchunk), t = threadIdx.x, w = t >> 6, l = t & 63;
   if (c >= nchunk) return;
   const int col = l & 15, kk = l >> 4;
@@ -612,8 +614,8 @@ __global__ __launch_bounds__(P2P_BS) void k_p2p_mfma111_f32(int nchunk, const in
   const double* den = src.den + (long)r0 * src.den_rs;
   for (int e = eptr[c]; e < eptr[c + 1]; e++) {
     const int2 se = ent[e];
-    for (int b0 = 0; b0 < se.y; b0 += TILE) {
-      const int cnt = min(TILE, se.y - b0), cnt4 = (cnt + 3) & ~3;
+    for (int b0 = 0; b0 < se.y; b0 += TILE32) {
+      const int cnt = min(TILE32, se.y - b0), cnt4 = (cnt + 3) & ~3;
       const int j0 = se.x + b0;
       __syncthreads();
       for (int i = t; i < cnt4; i += P2P_BS) {
@@ -1360,6 +1362,10 @@ __global__ __launch_bounds__(256) void k_gemm_reduce(GemmArgs p, int ks)
 
 // Splits of K that bring a wide-tile product to about 4 blocks per CU
 // (1 = no split; each split keeps at least 4 steps of 8).
+// wide tile: GMX_RT 16-row tiles x GMX_G 16-column tiles per wave (measured
+// best of 10x2, 10x1 and 5x4 on the 4M-point Laplace and 1M-point Navier runs)
+constexpr int GMX_RT = 10, GMX_G = 2;
+
 static int gemm_split_of(const GemmArgs& a, long nb)
 {
   if (nb >= 512) return 1;
@@ -1373,7 +1379,8 @@ int gemm_split(const GemmArgs& a)
 {
   if (a.compensated || a.mfma != 2 || a.Kseg % 8 != 0 || a.ngroup <= 0 || a.maxcols <= 0) return 1;
   const long cols = (long)a.maxcols * a.Rc;
-  const long nb = a.M <= 64 ? ((cols + 255) / 256) * a.ngroup : ((cols + 127) / 128) * ((a.M + 159) / 160) * a.ngroup;
+  const long nb = a.M <= 64 ? ((cols + 255) / 256) * a.ngroup
+                            : ((cols + 64 * GMX_G - 1) / (64 * GMX_G)) * ((a.M + 16 * GMX_RT - 1) / (16 * GMX_RT)) * a.ngroup;
   return gemm_split_of(a, nb);
 }
 
@@ -1391,8 +1398,8 @@ void gemm(hipStream_t s, const GemmArgs& a)
         dim3 grid((unsigned)((cols + 255) / 256), 1, a.ngroup * ks);
         hipLaunchKernelGGL((k_gemm_mx<4, 4>), grid, dim3(256), 0, s, a, ks);
       } else {
-        dim3 grid((unsigned)((cols + 127) / 128), (a.M + 159) / 160, a.ngroup * ks);
-        hipLaunchKernelGGL((k_gemm_mx<10, 2>), grid, dim3(256), 0, s, a, ks);
+        dim3 grid((unsigned)((cols + 64 * GMX_G - 1) / (64 * GMX_G)), (a.M + 16 * GMX_RT - 1) / (16 * GMX_RT), a.ngroup * ks);
+        hipLaunchKernelGGL((k_gemm_mx<GMX_RT, GMX_G>), grid, dim3(256), 0, s, a, ks);
       }
       if (ks > 1) {
         const long n = a.ctot * a.M;
