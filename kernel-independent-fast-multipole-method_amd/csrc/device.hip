@@ -2252,10 +2252,12 @@ void m2l_tiled(hipStream_t s, int C, int ntile, int Rc, const int* ttile, const 
 // instruction (item v of lane l is row l/16 + 4v) lane (k, col) ends with
 // (re, im) of outputs rt*8 + k and rt*8 + k + 4.
 //
-// A block covers one frequency and 4 G tiles of 16 columns (G per wave).  For
-// each direction the block builds [[Mr,-Mi],[Mi,Mr]] in LDS from the
-// frequency-major operator copy opsT [C][343][SM][TM] (one contiguous run per
-// block) and every wave applies it to its G tiles.  pmask[q] has bit n set if
+// A block covers FB consecutive frequencies and 4 tiles of 16 columns (one
+// per wave).  For each direction the block builds [[Mr,-Mi],[Mi,Mr]] of each
+// frequency in LDS from the frequency-major operator copy opsT [C][343][SM][TM]
+// through a frequency-independent index table, and every wave applies them to
+// its tile; a lane loads the FB frequencies of each source spectrum as one
+// contiguous run.  pmask[q] has bit n set if
 // parent q has a source child in direction n; a tile whose parents have none
 // skips the direction.  The blocks of one column block and 8 neighbouring
 // frequencies are dealt to one XCD side by side, so each 128-byte line of
@@ -2292,155 +2294,140 @@ struct Mv<float> {
   __device__ static inline int item_im(int h) { return 2 * h + 1; }
 };
 
-template <class R, int SM, int TM, int G>
+template <class R, int SM, int TM, int FB>
 __global__ __launch_bounds__(M2LV_BS) void k_m2l_vec(int C, int Rc, long ncol, int ncb, const int* __restrict__ tslot,
                                                      const int* __restrict__ sslot, const int* __restrict__ pmask,
-                                                     const int* __restrict__ optab,
+                                                     const int* __restrict__ aidx,
                                                      const typename Mv<R>::C2* __restrict__ opsT,
                                                      const typename Mv<R>::C2* __restrict__ effden,
                                                      typename Mv<R>::C2* __restrict__ effval)
 {
   typedef typename Mv<R>::C2 C2;
   typedef typename Mv<R>::V4 V4;
-  constexpr int NKS = 4 * SM, NH = NKS / 2, ST = SM * TM;
-  __shared__ R s_A[TM][NKS][64];
-  __shared__ int s_ot[26 * 64];
-  // items (frequency group of 8, column block, frequency in group): the 8
-  // frequencies of one spectra line run side by side on one XCD
-  const int nitem = ((C + 7) / 8) * ncb * 8;
+  constexpr int NKS = 4 * SM, NH = NKS / 2, ST = SM * TM, NA = TM * NKS * 64;
+  __shared__ R s_A[FB][NA];
+  // items (group of 8 frequencies, column block, FB-frequency run in the
+  // group): the runs of one 128-byte line of spectra run side by side on one XCD
+  constexpr int RUNS = 8 / FB;
+  const int ngrp = (C + 7) / 8, nitem = ngrp * ncb * RUNS;
   const int item = xcd_item(blockIdx.x, nitem);
-  const int fg = item / (8 * ncb), rem = item - fg * 8 * ncb, cb = rem >> 3, f = fg * 8 + (rem & 7);
-  if (item >= nitem || f >= C) return;  // block-uniform
+  const int fg = item / (RUNS * ncb), rem = item - fg * RUNS * ncb, cb = rem / RUNS;
+  const int f0 = fg * 8 + (rem - cb * RUNS) * FB;
+  if (item >= nitem || f0 >= C) return;  // block-uniform
+  const int nf = min(FB, C - f0);
   const int t = threadIdx.x, w = t >> 6, l = t & 63, li = l & 15, lk = l >> 4;
-  const C2* Of = opsT + (long)f * 343 * ST;
-  for (int i = t; i < 26 * 64; i += M2LV_BS) s_ot[i] = optab[i];
-  int q[G], r[G], msk[G];
+  const long col = ((long)cb * 4 + w) * 16 + li;
+  const int q = col < ncol ? (int)(col / Rc) : -1;
+  const int r = col < ncol ? (int)(col - (long)(col / Rc) * Rc) : 0;
+  const int msk = q >= 0 ? pmask[q] : 0;
+  V4 acc[FB][TM];
 #pragma unroll
-  for (int g = 0; g < G; g++) {
-    const long col = (((long)cb * 4 + w) * G + g) * 16 + li;
-    q[g] = col < ncol ? (int)(col / Rc) : -1;
-    r[g] = col < ncol ? (int)(col - (long)(col / Rc) * Rc) : 0;
-    msk[g] = q[g] >= 0 ? pmask[q[g]] : 0;
-  }
-  V4 acc[G][TM];
+  for (int fb = 0; fb < FB; fb++)
 #pragma unroll
-  for (int g = 0; g < G; g++)
-#pragma unroll
-    for (int rt = 0; rt < TM; rt++) acc[g][rt] = V4{0, 0, 0, 0};
+    for (int rt = 0; rt < TM; rt++) acc[fb][rt] = V4{0, 0, 0, 0};
   for (int n = 0; n < 26; n++) {
-    __syncthreads();  // s_ot is staged; the previous direction is done with s_A
-    // the block's 64 TM SM entries of the real-embedded operator, TM SM per
-    // thread: all loads issued (clamped to operator 0 where the pair has none)
-    // before the first use
+    __syncthreads();  // the previous direction is done with s_A
+    // the direction-n blocks [[Mr,-Mi],[Mi,Mr]] of the FB frequencies: entry
+    // e is aidx[n][e] = 2 x (index of the real value among one frequency's
+    // 343 sm tm complex operators) + 1 if negated, or -1 (adjacent pair)
     {
-      int oo[ST];
-      C2 m[ST];
+      int ai[ST];
+      R av[FB][ST];
 #pragma unroll
-      for (int i = 0; i < ST; i++) {
-        const int e = t + M2LV_BS * i, ll = e & 63, kr = e >> 6, ks = kr % NKS, rt = kr / NKS;
-        const int qo = rt * 8 + Mv<R>::row_out(ll & 15), c = qo / TM, ti = qo - c * TM;
-        const int kap = ks * 4 + (ll >> 4), cs = kap % (8 * SM), sc = cs / SM, j = cs - sc * SM;
-        oo[i] = s_ot[n * 64 + sc * 8 + c];
-        m[i] = Of[(long)(oo[i] < 0 ? 0 : oo[i]) * ST + j * TM + ti];
+      for (int i = 0; i < ST; i++) ai[i] = aidx[(long)n * NA + t + M2LV_BS * i];
+#pragma unroll
+      for (int fb = 0; fb < FB; fb++) {
+        const R* Of = (const R*)(opsT + (long)(f0 + (fb < nf ? fb : 0)) * 343 * ST);
+#pragma unroll
+        for (int i = 0; i < ST; i++) av[fb][i] = Of[ai[i] < 0 ? 0 : ai[i] >> 1];
       }
 #pragma unroll
-      for (int i = 0; i < ST; i++) {
-        const int e = t + M2LV_BS * i, ll = e & 63, kr = e >> 6, ks = kr % NKS, rt = kr / NKS;
-        const int pin = (ks * 4 + (ll >> 4)) / (8 * SM);
-        const R v = Mv<R>::row_im(ll & 15) == 0 ? (pin == 0 ? m[i].x : -m[i].y) : (pin == 0 ? m[i].y : m[i].x);
-        s_A[rt][ks][ll] = oo[i] < 0 ? (R)0 : v;
-      }
+      for (int fb = 0; fb < FB; fb++)
+#pragma unroll
+        for (int i = 0; i < ST; i++)
+          s_A[fb][t + M2LV_BS * i] = ai[i] < 0 ? (R)0 : ((ai[i] & 1) ? -av[fb][i] : av[fb][i]);
     }
     __syncthreads();
+    if (!__any((msk >> n) & 1)) continue;  // wave-uniform: no parent of the tile has direction n
+    // this lane's NH source spectra for the FB frequencies (contiguous in
+    // the spectra layout), loads batched and clamped to slot 0 / the run
+    const int* ss = sslot + ((long)(q < 0 ? 0 : q) * 26 + n) * 8;
+    int sl[NH];
 #pragma unroll
-    for (int g = 0; g < G; g++) {
-      if (!__any((msk[g] >> n) & 1)) continue;  // wave-uniform
-      // this lane's NH source spectra for direction n, loads batched
-      // (clamped to slot 0 where the neighbour has no such child)
-      const int* ss = sslot + ((long)(q[g] < 0 ? 0 : q[g]) * 26 + n) * 8;
-      int sl[NH];
-      C2 x[NH];
+    for (int kh = 0; kh < NH; kh++) sl[kh] = q >= 0 ? ss[(kh * 4 + lk) / SM] : -1;
+    C2 x[NH][FB];
 #pragma unroll
-      for (int kh = 0; kh < NH; kh++) sl[kh] = q[g] >= 0 ? ss[(kh * 4 + lk) / SM] : -1;
+    for (int kh = 0; kh < NH; kh++) {
+      const int j = (kh * 4 + lk) % SM;
+      const C2* xp = effden + (((long)(sl[kh] < 0 ? 0 : sl[kh]) * Rc + r) * SM + j) * C + f0;
 #pragma unroll
-      for (int kh = 0; kh < NH; kh++) {
-        const int j = (kh * 4 + lk) % SM;
-        x[kh] = effden[(((long)(sl[kh] < 0 ? 0 : sl[kh]) * Rc + r[g]) * SM + j) * C + f];
-      }
+      for (int fb = 0; fb < FB; fb++) x[kh][fb] = xp[fb < nf ? fb : 0];
+    }
 #pragma unroll
-      for (int kh = 0; kh < NH; kh++) {
-        const R xr = sl[kh] < 0 ? (R)0 : x[kh].x, xi = sl[kh] < 0 ? (R)0 : x[kh].y;
+    for (int kh = 0; kh < NH; kh++)
+#pragma unroll
+      for (int fb = 0; fb < FB; fb++) {
+        const R xr = sl[kh] < 0 ? (R)0 : x[kh][fb].x, xi = sl[kh] < 0 ? (R)0 : x[kh][fb].y;
 #pragma unroll
         for (int rt = 0; rt < TM; rt++) {
-          acc[g][rt] = Mv<R>::mfma(s_A[rt][kh][l], xr, acc[g][rt]);
-          acc[g][rt] = Mv<R>::mfma(s_A[rt][kh + NH][l], xi, acc[g][rt]);
+          acc[fb][rt] = Mv<R>::mfma(s_A[fb][(rt * NKS + kh) * 64 + l], xr, acc[fb][rt]);
+          acc[fb][rt] = Mv<R>::mfma(s_A[fb][(rt * NKS + kh + NH) * 64 + l], xi, acc[fb][rt]);
         }
       }
-    }
   }
+  if (q < 0) return;
 #pragma unroll
-  for (int g = 0; g < G; g++) {
-    if (q[g] < 0) continue;
+  for (int rt = 0; rt < TM; rt++)
 #pragma unroll
-    for (int rt = 0; rt < TM; rt++)
+    for (int h = 0; h < 2; h++) {
+      const int qo = rt * 8 + Mv<R>::item_out(lk, h), c = qo / TM, ti = qo - c * TM;
+      const int tl = tslot[(long)q * 8 + c];
+      if (tl < 0) continue;
+      C2* out = effval + (((long)tl * Rc + r) * TM + ti) * C + f0;
 #pragma unroll
-      for (int h = 0; h < 2; h++) {
-        const int qo = rt * 8 + Mv<R>::item_out(lk, h), c = qo / TM, ti = qo - c * TM;
-        const int tl = tslot[(long)q[g] * 8 + c];
-        if (tl >= 0) {
+      for (int fb = 0; fb < FB; fb++)
+        if (fb < nf) {
           C2 v;
-          v.x = acc[g][rt][Mv<R>::item_re(h)];
-          v.y = acc[g][rt][Mv<R>::item_im(h)];
-          effval[(((long)tl * Rc + r[g]) * TM + ti) * C + f] = v;
+          v.x = acc[fb][rt][Mv<R>::item_re(h)];
+          v.y = acc[fb][rt][Mv<R>::item_im(h)];
+          out[fb] = v;
         }
-      }
-  }
+    }
 }
 
-template <class R, int SM, int TM, int G>
+template <class R, int SM, int TM, int FB>
 static void m2l_vec_launch(hipStream_t s, int C, int npar, int Rc, const int* tslot, const int* sslot,
-                           const int* pmask, const int* optab, const void* opsT, const void* effden, void* effval)
+                           const int* pmask, const int* aidx, const void* opsT, const void* effden, void* effval)
 {
   typedef typename Mv<R>::C2 C2;
   const long ncol = (long)npar * Rc;
-  const int ncb = (int)((ncol + 64 * G - 1) / (64 * G));
-  hipLaunchKernelGGL((k_m2l_vec<R, SM, TM, G>), dim3(xcd_grid((C + 7) / 8 * ncb * 8)), dim3(M2LV_BS), 0, s, C, Rc,
-                     ncol, ncb, tslot, sslot, pmask, optab, (const C2*)opsT, (const C2*)effden, (C2*)effval);
+  const int ncb = (int)((ncol + 63) / 64);
+  hipLaunchKernelGGL((k_m2l_vec<R, SM, TM, FB>), dim3(xcd_grid((C + 7) / 8 * ncb * (8 / FB))), dim3(M2LV_BS), 0, s, C,
+                     Rc, ncol, ncb, tslot, sslot, pmask, aidx, (const C2*)opsT, (const C2*)effden, (C2*)effval);
 }
 
 template <class R>
 static void m2l_matrix_t(hipStream_t s, int sm, int tm, int C, int npar, int Rc, const int* tslot, const int* sslot,
-                         const int* pmask, const int* optab, const void* opsT, const void* effden, void* effval)
+                         const int* pmask, const int* aidx, const void* opsT, const void* effden, void* effval)
 {
-  const long ncol = (long)npar * Rc;
-  // fewer tiles per wave when the level is too small to fill the chip otherwise
-  const bool wide = ncol * C >= 64L * 4 * 256 * 8;
-  if (sm == 1 && tm == 1) {
-    if (wide)
-      m2l_vec_launch<R, 1, 1, 4>(s, C, npar, Rc, tslot, sslot, pmask, optab, opsT, effden, effval);
-    else
-      m2l_vec_launch<R, 1, 1, 1>(s, C, npar, Rc, tslot, sslot, pmask, optab, opsT, effden, effval);
-  } else if (sm == 4 && tm == 3) {
-    if (wide)
-      m2l_vec_launch<R, 4, 3, 2>(s, C, npar, Rc, tslot, sslot, pmask, optab, opsT, effden, effval);
-    else
-      m2l_vec_launch<R, 4, 3, 1>(s, C, npar, Rc, tslot, sslot, pmask, optab, opsT, effden, effval);
-  } else if (sm == 3 && tm == 3) {
-    if (wide)
-      m2l_vec_launch<R, 3, 3, 2>(s, C, npar, Rc, tslot, sslot, pmask, optab, opsT, effden, effval);
-    else
-      m2l_vec_launch<R, 3, 3, 1>(s, C, npar, Rc, tslot, sslot, pmask, optab, opsT, effden, effval);
-  }
+  // frequencies per block: 2 measured best in both precisions (of 1, 2, 4, 8)
+  constexpr int FB = 2;
+  if (sm == 1 && tm == 1)
+    m2l_vec_launch<R, 1, 1, FB>(s, C, npar, Rc, tslot, sslot, pmask, aidx, opsT, effden, effval);
+  else if (sm == 4 && tm == 3)
+    m2l_vec_launch<R, 4, 3, FB>(s, C, npar, Rc, tslot, sslot, pmask, aidx, opsT, effden, effval);
+  else if (sm == 3 && tm == 3)
+    m2l_vec_launch<R, 3, 3, FB>(s, C, npar, Rc, tslot, sslot, pmask, aidx, opsT, effden, effval);
 }
 
 void m2l_matrix(hipStream_t s, int sm, int tm, int C, int npar, int Rc, const int* tslot, const int* sslot,
-                const int* pmask, const int* optab, const void* opsT, const void* effden, void* effval, int single)
+                const int* pmask, const int* aidx, const void* opsT, const void* effden, void* effval, int single)
 {
   if (npar <= 0) return;
   if (single)
-    m2l_matrix_t<float>(s, sm, tm, C, npar, Rc, tslot, sslot, pmask, optab, opsT, effden, effval);
+    m2l_matrix_t<float>(s, sm, tm, C, npar, Rc, tslot, sslot, pmask, aidx, opsT, effden, effval);
   else
-    m2l_matrix_t<double>(s, sm, tm, C, npar, Rc, tslot, sslot, pmask, optab, opsT, effden, effval);
+    m2l_matrix_t<double>(s, sm, tm, C, npar, Rc, tslot, sslot, pmask, aidx, opsT, effden, effval);
 }
 
 }  // namespace dev

@@ -109,11 +109,13 @@ void m2l_grouped(hipStream_t s, int sm, int tm, int C, int npar, int Rc, int rb,
                  const int* sslot, const int* optab, const double2* ops, const double2* effden, double2* effval);
 // The same sums on the matrix cores: columns (parent, right-hand side) in
 // tiles of 16, operators from the frequency-major copy opsT [C][343][sm][tm];
-// pmask[p] bit n set if parent p has a source child in direction n.  single
-// selects FP32 spectra and operators (float2, all directions accumulated in
-// FP32), otherwise FP64 (double2).
+// pmask[p] bit n set if parent p has a source child in direction n; aidx
+// [26][tm][4 sm][64] the A-operand table of the precision's row layout
+// (2 x index of the real value among one frequency's operators + 1 if
+// negated, -1 for zero).  single selects FP32 spectra and operators (float2,
+// all directions accumulated in FP32), otherwise FP64 (double2).
 void m2l_matrix(hipStream_t s, int sm, int tm, int C, int npar, int Rc, const int* tslot, const int* sslot,
-                const int* pmask, const int* optab, const void* opsT, const void* effden, void* effval, int single);
+                const int* pmask, const int* aidx, const void* opsT, const void* effden, void* effval, int single);
 // c2r and gather onto the DC surface: dncval[(node*Rc + r)*pln + p*tm + f] += ...
 // (effval in the layout and precision described for fft_fwd)
 void fft_inv(hipStream_t s, int np, int nslot, const int* slot_node, int Rc, const void* effval, int single,

@@ -426,6 +426,31 @@ void Engine::build_plan()
             }
     }
     P.optab = pool_i(P.I, optab);
+    // A-operand tables of the parent-grouped matrix-core M2L: entry (n, rt,
+    // ks, lane) of the real embedding [[Mr, -Mi], [Mi, Mr]] of the direction-n
+    // operator block, for the FP64 row layout (row r16 = output r16 % 8, part
+    // r16 / 8) and the FP32 one (output r16 / 2, part r16 % 2)
+    for (int prec = 0; prec < 2; prec++) {
+      const int tmv = mm_.tdof, nks = 4 * sm;
+      std::vector<int> aidx((size_t)26 * tmv * nks * 64, -1);
+      for (int n = 0; n < 26; n++)
+        for (int rt = 0; rt < tmv; rt++)
+          for (int ks = 0; ks < nks; ks++)
+            for (int ll = 0; ll < 64; ll++) {
+              const int r16 = ll & 15, kk = ll >> 4;
+              const int ro = prec == 0 ? (r16 & 7) : (r16 >> 1);
+              const bool imrow = prec == 0 ? r16 >= 8 : (r16 & 1) == 1;
+              const int qo = rt * 8 + ro, c = qo / tmv, ti = qo % tmv;
+              const int kap = ks * 4 + kk, pin = kap / (8 * sm), cs = kap % (8 * sm), sc = cs / sm, j = cs % sm;
+              const int o = optab[(n * 8 + sc) * 8 + c];
+              if (o < 0) continue;
+              const int part = imrow ? (pin == 0 ? 1 : 0) : pin;  // Yr = Mr Xr - Mi Xi, Yi = Mi Xr + Mr Xi
+              const int neg = (!imrow && pin == 1) ? 1 : 0;
+              const int vidx = ((o * sm + j) * tmv + ti) * 2 + part;
+              aidx[(((size_t)n * tmv + rt) * nks + ks) * 64 + ll] = vidx * 2 + neg;
+            }
+      P.aidx[prec] = pool_i(P.I, aidx);
+    }
     std::vector<int> sslot_of(T.nnode, -1), tslot_of(T.nnode, -1);
     for (int L = 2; L < T.nlevel; L++) {
       std::vector<int> trg, src, vptr(1, 0), vsrc, vop;
@@ -913,7 +938,7 @@ void Engine::run_chunk(const double* srcden, double* trgval, int rc, int mode, h
         dev::m2l(s, sm, tm, O.C, vl.ntrg, rc, I + vl.vptr, I + vl.vsrc, I + vl.vop, (const double2*)m2l_.p,
                  (const double2*)effden_.p, (double2*)effval_.p);
       else if (m2l_matrix_)
-        dev::m2l_matrix(s, sm, tm, O.C, vl.npar, rc, I + vl.tslot, I + vl.sslot, I + vl.pmask, I + P.optab,
+        dev::m2l_matrix(s, sm, tm, O.C, vl.npar, rc, I + vl.tslot, I + vl.sslot, I + vl.pmask, I + P.aidx[single],
                         single ? (const void*)opsT32_.p : (const void*)opsT_.p, effden_.p, effval_.p, single);
       else
         dev::m2l_grouped(s, sm, tm, O.C, vl.npar, rc, m2l_rb_, m2l_bs_, I + vl.tslot, I + vl.sslot, I + P.optab,

@@ -74,6 +74,7 @@ struct Plan {
   int max_vsrc = 0, max_vtrg = 0;
   bool m2l_pairs = false;      // per-pair M2L kernel instead of the parent-grouped one
   size_t optab = 0;            // [26 directions][8 source children][8 target children] -> operator or -1
+  size_t aidx[2] = {0, 0};     // A-operand tables of the matrix-core M2L, FP64 and FP32 row layouts
   int nlow_s = 0, nlow_t = 0;  // source / target node indices of depth < 2 form these prefixes
   long near_pairs = 0;         // source-target pairs of the near field (one RHS)
   long far_pairs = 0;          // surface point - target pairs of L2T + M2T
