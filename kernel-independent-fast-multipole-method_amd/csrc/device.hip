@@ -1131,6 +1131,120 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs p)
   }
 }
 
+// Compensated (Dot2) product, staged like k_gemm_mx: 64 x 64 tile of C per
+// block, each thread a 4 x 4 block of (hi, lo) accumulators; K in steps of 8
+// (segments are multiples of 8), the A and B tiles double-buffered in LDS with
+// the next step's values loaded into registers while the current step is
+// accumulated; each thread's staging addresses are fixed per segment (no
+// per-element index arithmetic).  One barrier per step.
+__global__ __launch_bounds__(256) void k_gemm_dot2(GemmArgs p)
+{
+  const int grp = blockIdx.z;
+  const int n0 = p.gbeg[grp], n1 = p.gbeg[grp + 1];
+  const long cb = (long)n0 * p.Rc + (long)blockIdx.x * 64, ce = (long)n1 * p.Rc;
+  if (cb >= ce) return;  // block-uniform
+  const int rb = blockIdx.y * 64;
+  const double* A = p.A + grp * p.a_gstride;
+  const int nstep = p.nseg * p.Kseg / 8, sps = p.Kseg / 8;
+  __shared__ double As[2][8][64];
+  __shared__ double Bs[2][8][65];
+  const int t = threadIdx.x, tx = t & 15, ty = t >> 4;
+  // staging: A elements (kk = e >> 6, row = e & 63), B elements (kk = e & 7,
+  // column e >> 3) for e = t and t + 256
+  int akk[2], arow[2], bkk[2], bcl[2], bn_n[2], bn_r[2];
+  bool aok[2], bok[2];
+#pragma unroll
+  for (int i = 0; i < 2; i++) {
+    const int e = t + 256 * i;
+    akk[i] = e >> 6;
+    arow[i] = e & 63;
+    aok[i] = rb + arow[i] < p.M;
+    bkk[i] = e & 7;
+    bcl[i] = e >> 3;
+    const long j = cb + bcl[i];
+    bok[i] = j < ce;
+    bn_n[i] = bok[i] ? (int)(j / p.Rc) : 0;
+    bn_r[i] = bok[i] ? (int)(j - (long)bn_n[i] * p.Rc) : 0;
+  }
+  const double* bcol[2];
+  double bmul[2];
+  auto seg_ptrs = [&](int seg) {
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+      const int bn = bok[i] ? p.bnode[(long)seg * p.ncol + bn_n[i]] : -1;
+      bcol[i] = p.B + (bn >= 0 ? ((long)bn * p.Rc + bn_r[i]) * p.Kseg : 0) + bkk[i];
+      bmul[i] = bn >= 0 ? 1.0 : 0.0;
+    }
+  };
+  double ra[2], rbv[2];
+  auto load = [&](int st) {
+    const int k0 = st * 8, kin = (st - (st / sps) * sps) * 8;
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+      ra[i] = aok[i] ? A[(long)(k0 + akk[i]) * p.M + rb + arow[i]] : 0.0;
+      double v = bcol[i][kin] * bmul[i];
+      if (p.bscale) v *= p.bscale[kin + bkk[i]];  // power of two: exact
+      rbv[i] = v;
+    }
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+      As[buf][akk[i]][arow[i]] = ra[i];
+      Bs[buf][bkk[i]][bcl[i]] = rbv[i];
+    }
+  };
+  double hi[4][4], lo[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) hi[i][j] = lo[i][j] = 0.0;
+  seg_ptrs(0);
+  load(0);
+  store(0);
+  __syncthreads();
+  for (int st = 0; st < nstep; st++) {
+    const int buf = st & 1;
+    const bool more = st + 1 < nstep;
+    if (more) {
+      if ((st + 1) % sps == 0) seg_ptrs((st + 1) / sps);
+      load(st + 1);
+    }
+#pragma unroll 2
+    for (int kk = 0; kk < 8; kk++) {
+      double a[4], b[4];
+#pragma unroll
+      for (int i = 0; i < 4; i++) a[i] = As[buf][kk][tx + 16 * i];
+#pragma unroll
+      for (int j = 0; j < 4; j++) b[j] = Bs[buf][kk][ty + 16 * j];
+#pragma unroll
+      for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) dot2_acc(hi[i][j], lo[i][j], a[i], b[j]);
+    }
+    if (more) store(buf ^ 1);
+    __syncthreads();
+  }
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    const long jj = cb + ty + 16 * j;
+    if (jj >= ce) continue;
+    const int n = (int)(jj / p.Rc), r = (int)(jj - (long)n * p.Rc);
+    double* out = p.C + ((long)p.cnode[n] * p.Rc + r) * p.M;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int row = rb + tx + 16 * i;
+      if (row >= p.M) continue;
+      double h = hi[i][j], l = lo[i][j];
+      if (p.cscale) {  // power of two: exact
+        h *= p.cscale[row];
+        l *= p.cscale[row];
+      }
+      out[row] = dot2_finish(p.accumulate ? out[row] : 0.0, h, l);
+    }
+  }
+}
+
 // Plain (uncompensated) form on the FP64 matrix cores: the same 64 x 64 tile
 // and operand staging; wave w owns the 32 x 32 quarter (rows 32*(w/2), columns
 // 32*(w%2)) as 2 x 2 tiles of v_mfma_f64_16x16x4f64 (A lane l: row l%16, k
@@ -1409,7 +1523,9 @@ void gemm(hipStream_t s, const GemmArgs& a)
     }
   }
   dim3 grid((unsigned)((cols + 63) / 64), (a.M + 63) / 64, a.ngroup);
-  if (a.compensated)
+  if (a.compensated && a.Kseg % 8 == 0)
+    hipLaunchKernelGGL(k_gemm_dot2, grid, dim3(256), 0, s, a);
+  else if (a.compensated)
     hipLaunchKernelGGL(k_gemm<true>, grid, dim3(256), 0, s, a);
   else if (a.mfma)
     hipLaunchKernelGGL(k_gemm_mfma, grid, dim3(256), 0, s, a);
