@@ -1474,17 +1474,18 @@ __global__ __launch_bounds__(256) void k_gemm_reduce(GemmArgs p, int ks)
   }
 }
 
-// Splits of K that bring a wide-tile product to about 4 blocks per CU
-// (1 = no split; each split keeps at least 4 steps of 8).
 // wide tile: GMX_RT 16-row tiles x GMX_G 16-column tiles per wave (measured
 // best of 10x2, 10x1 and 5x4 on the 4M-point Laplace and 1M-point Navier runs)
 constexpr int GMX_RT = 10, GMX_G = 2;
 
+// Splits of K that bring a wide-tile product to about a.split_blocks blocks
+// (default 1024, 4 per CU; 1 = no split; each split keeps at least 4 steps of 8).
 static int gemm_split_of(const GemmArgs& a, long nb)
 {
-  if (nb >= 512) return 1;
+  const long target = a.split_blocks > 0 ? a.split_blocks : 1024;
+  if (nb >= target / 2) return 1;
   const int nstep = a.nseg * a.Kseg / 8;
-  long ks = (1024 + nb - 1) / nb;
+  long ks = (target + nb - 1) / nb;
   ks = std::min<long>(ks, std::min(16, nstep / 4));
   return (int)std::max<long>(ks, 1);
 }

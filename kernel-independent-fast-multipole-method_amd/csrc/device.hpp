@@ -86,6 +86,7 @@ struct GemmArgs {
   long ctot;        // all columns of the product (ncol x Rc)
   double* work;     // split-K partials, gemm_split(a) x ctot x M doubles (null or too small: no split)
   long work_n;      // capacity of work in doubles
+  int split_blocks; // blocks the split-K aims at (0: 1024); lower when other streams share the GPU
 };
 void gemm(hipStream_t s, const GemmArgs& a);
 // K splits the wide-tile product takes for these sizes (1: none).

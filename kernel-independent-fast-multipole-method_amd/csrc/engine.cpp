@@ -96,6 +96,7 @@ Engine::Engine(int kt, const double* coefs, int ncoefs, int np, int ptsmax, int 
   if (const char* bs = std::getenv("KIFMM3D_M2L_BS")) m2l_bs_ = std::atoi(bs);
   if (const char* v = std::getenv("KIFMM3D_P2P")) p2p_variant_ = std::atoi(v);
   if (const char* g = std::getenv("KIFMM3D_GEMM")) gemm_mfma_ = std::string(g) == "valu" ? 0 : std::string(g) == "tile64" ? 1 : 2;
+  if (const char* g = std::getenv("KIFMM3D_GEMM_SPLIT")) split_blocks_ = std::max(1, std::atoi(g));
   if (const char* m = std::getenv("KIFMM3D_M2L")) {
     const std::string v(m);
     m2l_tile_ = v != "group" && v != "pair" && v != "matrix";
@@ -701,6 +702,7 @@ void Engine::ensure_work(int rc)
         a.M = op.M; a.Kseg = op.Kseg; a.nseg = op.nseg; a.Rc = r; a.ngroup = op.ngroup; a.maxcols = op.maxcols;
         a.compensated = 0;
         a.mfma = gemm_mfma_;
+        a.split_blocks = split_blocks_;
         const int ks = dev::gemm_split(a);
         if (ks > 1) gw = std::max(gw, (size_t)ks * op.ncol * r * op.M);
       }
@@ -845,7 +847,7 @@ void Engine::run_chunk(const double* srcden, double* trgval, int rc, int mode, h
   dev::Targets trg{tx_.p, ty_.p, tz_.p, tval_.p, trs};
 
   auto run_gemm = [&](const GemmOp& op) {
-    dev::GemmArgs a;
+    dev::GemmArgs a{};
     a.M = op.M; a.Kseg = op.Kseg; a.nseg = op.nseg; a.Rc = rc; a.ngroup = op.ngroup; a.maxcols = op.maxcols;
     a.A = opsA[op.a_which]; a.a_gstride = op.a_gstride;
     a.B = bufs[op.b_buf]; a.bnode = I + op.bnode; a.ncol = op.ncol;
@@ -858,6 +860,7 @@ void Engine::run_chunk(const double* srcden, double* trgval, int rc, int mode, h
     a.ctot = (long)op.ncol * rc;
     a.work = gwork_.p;
     a.work_n = (long)gwork_.n;
+    a.split_blocks = split_blocks_;
     dev::gemm(s, a);
   };
 

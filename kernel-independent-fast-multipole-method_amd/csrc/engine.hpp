@@ -148,6 +148,7 @@ class Engine {
   int m2l_rb_ = 0;   // right-hand sides per lane in the grouped M2L (0 = default; KIFMM3D_M2L_RB)
   int m2l_bs_ = 64;  // frequencies per block in the grouped M2L (64 or 256; KIFMM3D_M2L_BS)
   int p2p_variant_ = 2;  // near field: 0 LDS broadcast, 1 scalar-cache sources, 2 matrix cores (KIFMM3D_P2P)
+  int split_blocks_ = 1024;  // blocks the split-K GEMMs aim at (KIFMM3D_GEMM_SPLIT)
   int gemm_mfma_ = 2;  // uncompensated GEMMs: 2 wide matrix-core tiles, 1 64x64 tiles (KIFMM3D_GEMM=tile64), 0 VALU (=valu)
   bool m2l_tile_ = true;   // tiled matrix-core M2L for scalar kernels and 16-RHS chunks (KIFMM3D_M2L=matrix|group|pair disables)
   bool m2l_matrix_ = true; // parent-grouped matrix-core M2L otherwise (KIFMM3D_M2L=group|pair disables)
