@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "device.hpp"
 
@@ -974,14 +975,133 @@ void p2s(hipStream_t s, int kt, int nitem, const int* out_node, const double4* g
   }
 }
 
+// L2T + M2T for the Laplace mm kernel and 16-RHS blocks on the matrix cores:
+// the wave layout of k_p2p_mfma111 (FP64) and k_p2p_mfma111_f32 (FP32, mixed
+// form) with the surface points of each entry as the sources -- positions
+// centre + radius x unit surface, densities from the node's [rhs][pln] row
+// of the downward (kind 0) or upward (kind 1) equivalent densities.
+template <bool F32>
+__global__ __launch_bounds__(P2P_BS) void k_s2p_mfma111(int nchunk, const int* __restrict__ tbeg,
+                                                        const int* __restrict__ tnum, const int* __restrict__ eptr,
+                                                        const int* __restrict__ enode, const int* __restrict__ ekind,
+                                                        const double4* __restrict__ egeom,
+                                                        const double* __restrict__ unit_de,
+                                                        const double* __restrict__ unit_ue, int nsp,
+                                                        const double* __restrict__ dneden,
+                                                        const double* __restrict__ upeden, int Rc, int pln,
+                                                        Targets trg, int roff, KC kc)
+{
+  typedef typename std::conditional<F32, float, double>::type R;
+  constexpr int NR = 16, TL = F32 ? TILE32 : TILE;
+  __shared__ R s_pos[3][TL];
+  __shared__ R s_den[TL][NR];
+  const int c = xcd_item(blockIdx.x, nchunk), t = threadIdx.x, w = t >> 6, l = t & 63;
+  if (c >= nchunk) return;  // block-uniform
+  const int col = l & 15, kk = l >> 4;
+  const int r0 = roff + blockIdx.y * NR;
+  const int tb = tbeg[c], tn = tnum[c];
+  const bool wact = 64 * w < tn;  // wave-uniform
+  const int ng = wact ? min(4, (tn - 64 * w + 15) >> 4) : 0;
+  // FP32: coordinates relative to the chunk's first target; FP64: absolute
+  const double ox = F32 ? trg.x[tb] : 0.0, oy = F32 ? trg.y[tb] : 0.0, oz = F32 ? trg.z[tb] : 0.0;
+  R tx[4], ty[4], tz[4];
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    int tt = 64 * w + 16 * i + col;
+    tt = tt < tn ? tt : 0;
+    tx[i] = (R)(trg.x[tb + tt] - ox);
+    ty[i] = (R)(trg.y[tb + tt] - oy);
+    tz[i] = (R)(trg.z[tb + tt] - oz);
+  }
+  double acc64[4][4];
+  v4d accm[4];
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    accm[i] = v4d{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int v = 0; v < 4; v++) acc64[i][v] = 0.0;
+  }
+  for (int e = eptr[c]; e < eptr[c + 1]; e++) {
+    const int kind = ekind[e];
+    const double* unit = kind == 0 ? unit_de : unit_ue;
+    const double* buf = (kind == 0 ? dneden : upeden) + ((long)enode[e] * Rc + r0) * pln;
+    const double4 g = egeom[e];
+    for (int b0 = 0; b0 < nsp; b0 += TL) {
+      const int cnt = min(TL, nsp - b0), cnt4 = (cnt + 3) & ~3;
+      __syncthreads();
+      for (int i = t; i < cnt4; i += P2P_BS) {
+        const bool in = i < cnt;
+        const int pi = in ? b0 + i : 0;
+        s_pos[0][i] = in ? (R)(g.x + g.w * unit[3 * pi] - ox) : (R)0;
+        s_pos[1][i] = in ? (R)(g.y + g.w * unit[3 * pi + 1] - oy) : (R)0;
+        s_pos[2][i] = in ? (R)(g.z + g.w * unit[3 * pi + 2] - oz) : (R)0;
+      }
+      for (int i = t; i < cnt4 * NR; i += P2P_BS) {
+        const int q = i / cnt4, jj = i - q * cnt4;
+        s_den[jj][q] = jj < cnt ? (R)buf[(long)q * pln + b0 + jj] : (R)0;
+      }
+      __syncthreads();
+      if constexpr (F32) {
+        if (ng == 4)
+          p2p_tile32<4>(cnt4, kk, col, (const float(*)[TILE32])s_pos, (const float(*)[16])s_den, (const float*)tx,
+                        (const float*)ty, (const float*)tz, acc64);
+        else if (ng == 3)
+          p2p_tile32<3>(cnt4, kk, col, (const float(*)[TILE32])s_pos, (const float(*)[16])s_den, (const float*)tx,
+                        (const float*)ty, (const float*)tz, acc64);
+        else if (ng == 2)
+          p2p_tile32<2>(cnt4, kk, col, (const float(*)[TILE32])s_pos, (const float(*)[16])s_den, (const float*)tx,
+                        (const float*)ty, (const float*)tz, acc64);
+        else if (ng == 1)
+          p2p_tile32<1>(cnt4, kk, col, (const float(*)[TILE32])s_pos, (const float(*)[16])s_den, (const float*)tx,
+                        (const float*)ty, (const float*)tz, acc64);
+      } else {
+        if (ng == 4)
+          p2p_tile64<4>(cnt4, kk, col, (const double(*)[TILE])s_pos, (const double(*)[16])s_den, (const double*)tx,
+                        (const double*)ty, (const double*)tz, accm);
+        else if (ng == 3)
+          p2p_tile64<3>(cnt4, kk, col, (const double(*)[TILE])s_pos, (const double(*)[16])s_den, (const double*)tx,
+                        (const double*)ty, (const double*)tz, accm);
+        else if (ng == 2)
+          p2p_tile64<2>(cnt4, kk, col, (const double(*)[TILE])s_pos, (const double(*)[16])s_den, (const double*)tx,
+                        (const double*)ty, (const double*)tz, accm);
+        else if (ng == 1)
+          p2p_tile64<1>(cnt4, kk, col, (const double(*)[TILE])s_pos, (const double(*)[16])s_den, (const double*)tx,
+                        (const double*)ty, (const double*)tz, accm);
+      }
+    }
+  }
+  if (!wact) return;
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const int tt = 64 * w + 16 * i + col;
+    if (tt >= tn) continue;
+    if constexpr (F32) {  // FP32 item v of lane l: right-hand side 4*(l/16) + v
+      double* val = trg.val + (long)(r0 + 4 * kk) * trg.val_rs + tb + tt;
+#pragma unroll
+      for (int v = 0; v < 4; v++) val[(long)v * trg.val_rs] += kc.a * acc64[i][v];
+    } else {  // FP64 item v of lane l: right-hand side l/16 + 4v
+      double* val = trg.val + (long)(r0 + kk) * trg.val_rs + tb + tt;
+#pragma unroll
+      for (int v = 0; v < 4; v++) val[(long)(4 * v) * trg.val_rs] += kc.a * accm[i][v];
+    }
+  }
+}
+
 template <int KT>
 static void s2p_kt(hipStream_t s, int nchunk, const int* tbeg, const int* tnum, const int* eptr, const int* enode,
                    const int* ekind, const double4* egeom, const double* unit_de, const double* unit_ue, int nsp,
-                   const double* dneden, const double* upeden, int Rc, int pln, Targets trg, int nr, KC kc)
+                   const double* dneden, const double* upeden, int Rc, int pln, Targets trg, int nr, KC kc,
+                   int variant)
 {
   rhs_blocks(nr, [&](int NR, int nb, int roff) {
     dim3 grid(nchunk, nb);
-    if (NR == 16)
+    if (KT == 111 && NR == 16 && variant == 3)
+      hipLaunchKernelGGL((k_s2p_mfma111<true>), dim3(xcd_grid(nchunk), nb), dim3(P2P_BS), 0, s, nchunk, tbeg, tnum,
+                         eptr, enode, ekind, egeom, unit_de, unit_ue, nsp, dneden, upeden, Rc, pln, trg, roff, kc);
+    else if (KT == 111 && NR == 16 && variant == 2)
+      hipLaunchKernelGGL((k_s2p_mfma111<false>), dim3(xcd_grid(nchunk), nb), dim3(P2P_BS), 0, s, nchunk, tbeg, tnum,
+                         eptr, enode, ekind, egeom, unit_de, unit_ue, nsp, dneden, upeden, Rc, pln, trg, roff, kc);
+    else if (NR == 16)
       hipLaunchKernelGGL((k_s2p<KT, 16>), grid, dim3(P2P_BS), 0, s, tbeg, tnum, eptr, enode, ekind, egeom, unit_de,
                          unit_ue, nsp, dneden, upeden, Rc, pln, trg, roff, kc);
     else if (NR == 4)
@@ -995,21 +1115,21 @@ static void s2p_kt(hipStream_t s, int nchunk, const int* tbeg, const int* tnum, 
 
 void s2p(hipStream_t s, int kt, int nchunk, const int* tbeg, const int* tnum, const int* eptr, const int* enode,
          const int* ekind, const double4* egeom, const double* unit_de, const double* unit_ue, int nsp,
-         const double* dneden, const double* upeden, int Rc, int pln, Targets trg, int nr, KC kc)
+         const double* dneden, const double* upeden, int Rc, int pln, Targets trg, int nr, KC kc, int variant)
 {
   if (nchunk <= 0) return;
   switch (kt) {
   case 111:
     s2p_kt<111>(s, nchunk, tbeg, tnum, eptr, enode, ekind, egeom, unit_de, unit_ue, nsp, dneden, upeden, Rc, pln, trg,
-                nr, kc);
+                nr, kc, variant);
     break;
   case 301:
     s2p_kt<301>(s, nchunk, tbeg, tnum, eptr, enode, ekind, egeom, unit_de, unit_ue, nsp, dneden, upeden, Rc, pln, trg,
-                nr, kc);
+                nr, kc, variant);
     break;
   case 511:
     s2p_kt<511>(s, nchunk, tbeg, tnum, eptr, enode, ekind, egeom, unit_de, unit_ue, nsp, dneden, upeden, Rc, pln, trg,
-                nr, kc);
+                nr, kc, variant);
     break;
   }
 }

@@ -60,9 +60,11 @@ void p2s(hipStream_t s, int kt, int nitem, const int* out_node, const double4* g
 // Equivalent surfaces to targets with the mm kernel (L2T from DE, M2T from UE).
 // Entry e: node enode[e] of buffer ekind[e] (0 = dneden, 1 = upeden), surface
 // ekind 0 -> unit_de, 1 -> unit_ue, placed at geom[e] = (cx, cy, cz, rad).
+// variant (Laplace, 16-RHS blocks): 2 FP64 matrix cores, 3 FP32 matrix cores
+// (mixed form), otherwise the FP64 vector kernels.
 void s2p(hipStream_t s, int kt, int nchunk, const int* tbeg, const int* tnum, const int* eptr, const int* enode,
          const int* ekind, const double4* egeom, const double* unit_de, const double* unit_ue, int nsp,
-         const double* dneden, const double* upeden, int Rc, int pln, Targets trg, int nr, KC kc);
+         const double* dneden, const double* upeden, int Rc, int pln, Targets trg, int nr, KC kc, int variant);
 
 // Batched, indexed FP64 GEMM over (node, rhs) columns:
 // C[:, cnode[n]*Rc + r] (+)= cscale .* sum_seg A_{grp,seg} (bscale .* B[:, bnode[seg*ncol + n]*Rc + r])

@@ -967,7 +967,7 @@ void Engine::run_chunk(const double* srcden, double* trgval, int rc, int mode, h
   timed(KIFMM3D_K_S2P, [&] {
     dev::s2p(s, mm_.kt, P.far.n, I + P.far.tbeg, I + P.far.tnum, I + P.far.eptr, I + P.far.enode,
              I + P.far.ekind, (const double4*)(D + P.far.egeom), unit_[SURF_DE].p, unit_[SURF_UE].p,
-             O.nsp[SURF_DE], dneden_.p, upeden_.p, rc, pDE, trg, rc, kcm);
+             O.nsp[SURF_DE], dneden_.p, upeden_.p, rc, pDE, trg, rc, kcm, mixed ? 3 : p2p_variant_);
   });
   mark(7);
   // save (fmm3d_eval.cpp:221-235)
