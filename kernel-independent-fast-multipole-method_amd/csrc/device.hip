@@ -943,14 +943,91 @@ void p2p(hipStream_t s, int kt, int nchunk, const int* tbeg, const int* tnum, co
   }
 }
 
+// S2M / S2L for the Laplace kernel and 16-RHS blocks on the FP64 matrix
+// cores: the wave layout of k_p2p_mfma111 with the check-surface points of an
+// item as the targets (128 per block, positions centre + radius x unit
+// surface) and the item's source points as the sources.  FP64 in both forms:
+// the check values feed the pseudoinverses.
+__global__ __launch_bounds__(P2P_BS) void k_p2s_mfma111(int nitem, int npc, const int* __restrict__ out_node,
+                                                        const double4* __restrict__ geom,
+                                                        const int* __restrict__ eptr, const int2* __restrict__ ent,
+                                                        const double* __restrict__ unit, int npt, Points src,
+                                                        double* __restrict__ out, int Rc, int pln, int roff, KC kc)
+{
+  constexpr int NR = 16;
+  __shared__ double s_pos[3][TILE];
+  __shared__ double s_den[TILE][NR];
+  const int nb = nitem * npc, b = xcd_item(blockIdx.x, nb);
+  if (b >= nb) return;  // block-uniform
+  const int it = b / npc, pc = b - it * npc;
+  const int t = threadIdx.x, w = t >> 6, l = t & 63, col = l & 15, kk = l >> 4;
+  const int r0 = roff + blockIdx.y * NR;
+  const int pb = pc * P2P_BS, tn = min(P2P_BS, npt - pb);  // surface points of this block
+  const bool wact = 64 * w < tn;                           // wave-uniform
+  const int ng = wact ? min(4, (tn - 64 * w + 15) >> 4) : 0;
+  const double4 g = geom[it];
+  double tx[4], ty[4], tz[4];
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    int tt = 64 * w + 16 * i + col;
+    tt = pb + (tt < tn ? tt : 0);
+    tx[i] = g.x + g.w * unit[3 * tt];
+    ty[i] = g.y + g.w * unit[3 * tt + 1];
+    tz[i] = g.z + g.w * unit[3 * tt + 2];
+  }
+  v4d acc[4];
+#pragma unroll
+  for (int i = 0; i < 4; i++) acc[i] = v4d{0.0, 0.0, 0.0, 0.0};
+  const double* den = src.den + (long)r0 * src.den_rs;
+  for (int e = eptr[it]; e < eptr[it + 1]; e++) {
+    const int2 se = ent[e];
+    for (int b0 = 0; b0 < se.y; b0 += TILE) {
+      const int cnt = min(TILE, se.y - b0), cnt4 = (cnt + 3) & ~3;
+      const int j0 = se.x + b0;
+      __syncthreads();
+      for (int i = t; i < cnt4; i += P2P_BS) {
+        const bool in = i < cnt;
+        s_pos[0][i] = in ? src.x[j0 + i] : 0.0;
+        s_pos[1][i] = in ? src.y[j0 + i] : 0.0;
+        s_pos[2][i] = in ? src.z[j0 + i] : 0.0;
+      }
+      stage_den<1, NR, P2P_BS>(s_den, den, src, j0, cnt, t);
+      for (int i = cnt * NR + t; i < cnt4 * NR; i += P2P_BS) s_den[i / NR][i % NR] = 0.0;
+      __syncthreads();
+      if (ng == 4)
+        p2p_tile64<4>(cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc);
+      else if (ng == 3)
+        p2p_tile64<3>(cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc);
+      else if (ng == 2)
+        p2p_tile64<2>(cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc);
+      else if (ng == 1)
+        p2p_tile64<1>(cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc);
+    }
+  }
+  if (!wact) return;
+  const long node = out_node[it];
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const int tt = 64 * w + 16 * i + col;
+    if (tt >= tn) continue;
+    double* o = out + (node * Rc + r0 + kk) * pln + pb + tt;  // item v: right-hand side kk + 4v
+#pragma unroll
+    for (int v = 0; v < 4; v++) o[(long)(4 * v) * pln] += kc.a * acc[i][v];
+  }
+}
+
 template <int KT>
 static void p2s_kt(hipStream_t s, int nitem, const int* out_node, const double4* geom, const int* eptr,
                    const int2* ent, const double* unit, int npt, Points src, double* out, int Rc, int pln, int nr,
-                   KC kc)
+                   KC kc, int variant)
 {
   rhs_blocks(nr, [&](int NR, int nb, int roff) {
     dim3 grid(nitem, (npt + P2S_BS - 1) / P2S_BS, nb);
-    if (NR == 16)
+    if (KT == 111 && NR == 16 && variant >= 2 && src.den_rs == 1) {
+      const int npc = (npt + P2P_BS - 1) / P2P_BS;
+      hipLaunchKernelGGL(k_p2s_mfma111, dim3(xcd_grid(nitem * npc), nb), dim3(P2P_BS), 0, s, nitem, npc, out_node,
+                         geom, eptr, ent, unit, npt, src, out, Rc, pln, roff, kc);
+    } else if (NR == 16)
       hipLaunchKernelGGL((k_p2s<KT, 16>), grid, dim3(P2S_BS), 0, s, out_node, geom, eptr, ent, unit, npt, src, out,
                          Rc, pln, roff, kc);
     else if (NR == 4)
@@ -963,12 +1040,12 @@ static void p2s_kt(hipStream_t s, int nitem, const int* out_node, const double4*
 }
 
 void p2s(hipStream_t s, int kt, int nitem, const int* out_node, const double4* geom, const int* eptr, const int2* ent,
-         const double* unit, int npt, Points src, double* out, int Rc, int pln, int nr, KC kc)
+         const double* unit, int npt, Points src, double* out, int Rc, int pln, int nr, KC kc, int variant)
 {
   if (nitem <= 0) return;
   switch (kt) {
 #define KIFMM_P2S(KT) \
-  case KT: p2s_kt<KT>(s, nitem, out_node, geom, eptr, ent, unit, npt, src, out, Rc, pln, nr, kc); break;
+  case KT: p2s_kt<KT>(s, nitem, out_node, geom, eptr, ent, unit, npt, src, out, Rc, pln, nr, kc, variant); break;
   KIFMM_P2S(111) KIFMM_P2S(121) KIFMM_P2S(311) KIFMM_P2S(312) KIFMM_P2S(321) KIFMM_P2S(322) KIFMM_P2S(511)
   KIFMM_P2S(521)
 #undef KIFMM_P2S

@@ -55,7 +55,8 @@ void direct(hipStream_t s, int kt, int tdof, int nchk, const double* cx, const d
 // Points to check surface (S2M onto UC, S2L onto DC): out[(node*Rc + r)*pln + p*tdof + t] +=
 // sum over the source leaves of item i of K(ctr_i + rad_i * unit_p <- y) den(y).
 void p2s(hipStream_t s, int kt, int nitem, const int* out_node, const double4* geom, const int* eptr,
-         const int2* ent, const double* unit, int npt, Points src, double* out, int Rc, int pln, int nr, KC kc);
+         const int2* ent, const double* unit, int npt, Points src, double* out, int Rc, int pln, int nr, KC kc,
+         int variant);
 
 // Equivalent surfaces to targets with the mm kernel (L2T from DE, M2T from UE).
 // Entry e: node enode[e] of buffer ekind[e] (0 = dneden, 1 = upeden), surface

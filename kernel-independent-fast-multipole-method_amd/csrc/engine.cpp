@@ -904,7 +904,8 @@ void Engine::run_chunk(const double* srcden, double* trgval, int rc, int mode, h
   // up: S2M, then M2M + UC2UE level by level
   timed(KIFMM3D_K_S2M, [&] {
     dev::p2s(s, user_.kt, P.s2m.n, I + P.s2m.out, (const double4*)(D + P.s2m.geom), I + P.s2m.eptr,
-             (const int2*)(I + P.s2m.ent), unit_[SURF_UC].p, O.nsp[SURF_UC], src, upcval_.p, rc, pUC, rc, kcu);
+             (const int2*)(I + P.s2m.ent), unit_[SURF_UC].p, O.nsp[SURF_UC], src, upcval_.p, rc, pUC, rc, kcu,
+             std::min(p2p_variant_, 2));  // FP64 in both forms: the check values feed the pseudoinverse
   });
   for (const GemmOp& op : P.up) timed(KIFMM3D_K_GEMM_UP, [&] { run_gemm(op); });
   mark(2);
@@ -956,7 +957,8 @@ void Engine::run_chunk(const double* srcden, double* trgval, int rc, int mode, h
   // X list S2L
   timed(KIFMM3D_K_S2L, [&] {
     dev::p2s(s, user_.kt, P.s2l.n, I + P.s2l.out, (const double4*)(D + P.s2l.geom), I + P.s2l.eptr,
-             (const int2*)(I + P.s2l.ent), unit_[SURF_DC].p, O.nsp[SURF_DC], src, dncval_.p, rc, pDC, rc, kcu);
+             (const int2*)(I + P.s2l.ent), unit_[SURF_DC].p, O.nsp[SURF_DC], src, dncval_.p, rc, pDC, rc, kcu,
+             std::min(p2p_variant_, 2));
   });
   mark(5);
   // down: L2L + DC2DE level by level
