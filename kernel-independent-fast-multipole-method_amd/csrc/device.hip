@@ -44,6 +44,19 @@ __device__ inline double rinv(double r2)
   return r2 < 1e-16 ? 0.0 : ri;
 }
 
+// r.n in the reference's order, (x nx + y ny) + z nz (kernel3d.cpp:150, :235,
+// :257, :353), with no fused multiply-add.  For a target on the source's own
+// smooth surface the displacement is nearly tangent, r.n ~ |r|^2 / (2 R), so
+// the three products cancel to about 1 part in 2 R / |r|; a fused evaluation
+// rounds differently by up to that factor (1e-11 of the pair's value for a
+// 1e-5 neighbour gap on the unit sphere), which is the largest single
+// difference from the reference's own FP64 sums at full problem sizes.
+__device__ inline double rdotn(double dx, double dy, double dz, const double* n)
+{
+#pragma clang fp contract(off)
+  return (dx * n[0] + dy * n[1]) + dz * n[2];
+}
+
 template <int KT>
 struct Knl;
 
@@ -91,7 +104,7 @@ struct Knl<321> {
   __device__ static inline void add(double dx, double dy, double dz, const double* n, const double* f, double (&u)[NR][T],
                                     const KC& k)
   {
-    double rn = dx * n[0] + dy * n[1] + dz * n[2];
+    double rn = rdotn(dx, dy, dz, n);
     double ri = rinv(dx * dx + dy * dy + dz * dz), ri2 = ri * ri;
     double c = (k.a * (ri2 * ri2 * ri)) * rn;
 #pragma unroll
@@ -136,7 +149,7 @@ struct Knl<121> {
                                     const KC& k)
   {
     double ri = rinv(dx * dx + dy * dy + dz * dz);
-    double g = k.a * (ri * ri * ri) * (dx * n[0] + dy * n[1] + dz * n[2]);
+    double g = k.a * (ri * ri * ri) * rdotn(dx, dy, dz, n);
 #pragma unroll
     for (int q = 0; q < NR; q++) u[q][0] += g * f[q];
   }
@@ -193,7 +206,7 @@ struct Knl<322> {
                                     const KC& k)
   {
     double ri = rinv(dx * dx + dy * dy + dz * dz), ri2 = ri * ri, ri3 = ri2 * ri;
-    double rn = dx * n[0] + dy * n[1] + dz * n[2];
+    double rn = rdotn(dx, dy, dz, n);
     double A = k.a * ri3, B = -3.0 * k.a * (ri3 * ri2) * rn;
 #pragma unroll
     for (int q = 0; q < NR; q++) {
@@ -214,7 +227,7 @@ struct Knl<521> {
                                     const KC& k)
   {
     double ri = rinv(dx * dx + dy * dy + dz * dz), ri2 = ri * ri, ri3 = ri2 * ri;
-    double rn = dx * n[0] + dy * n[1] + dz * n[2];
+    double rn = rdotn(dx, dy, dz, n);
     double A = k.a * ri3, B = k.b * ri3, C = k.c * (ri3 * ri2) * rn;
 #pragma unroll
     for (int q = 0; q < NR; q++) {

@@ -6,6 +6,7 @@ The oracle (oracle/kifmm_oracle.c) restates the reference's FMM3d::eval
 evaluate pass; the bar is 1e-12 relative L2 in FP64.  A second check against
 the FP64 direct sum (fmm3d/fmm3d_check.cpp) bounds the FMM error itself.
 """
+
 import numpy as np
 import pytest
 
@@ -188,3 +189,48 @@ def test_m2l_forms(kt, form, monkeypatch):
         p, nrm = clusters(5000, 61), None
         ctr, lvl = (0.5, 0.5, 0.5), 1
     run_case(f"kt{kt}-m2l-{form}", kt, 4, 30, p, nrm, p, ctr, lvl, nrhs=5)
+
+
+@pytest.mark.parametrize("kt", [121, 321, 322, 521])
+def test_double_layer_near_coincident_pairs(kt):
+    """Targets 1e-7 to 1e-5 from a source along the sphere's tangent plane:
+    r.n cancels by up to 1e7 there, so any reordering or fusing of the three
+    products of r.n (kernel3d.cpp:150, :235, :257, :353) shows up at well
+    above 1e-12 of the result, carried by the huge near-field values of these
+    few targets."""
+    p, nrm = sphere(12000, 90)
+    rng = np.random.default_rng(91)
+    k = 64
+    src = rng.choice(len(p), k, replace=False)
+    t = rng.normal(size=(k, 3))
+    t -= (t * nrm[src]).sum(1)[:, None] * nrm[src]
+    t /= np.linalg.norm(t, axis=1)[:, None]
+    gap = 10.0 ** rng.uniform(-7, -5, k)
+    q = p[src] + gap[:, None] * t
+    q /= np.linalg.norm(q, axis=1)[:, None]  # back onto the sphere, still 1e-7..1e-5 away
+    tp = np.concatenate([p, q])
+    run_case(f"kt{kt}-near-pairs", kt, 4, 40, p, nrm, tp, (0.0, 0.0, 0.0), 0, nrhs=2)
+
+
+def test_stokes_sphere_262k_all_targets():
+    """The Stokes double-layer sphere workload at full size (N = 262,144 on the
+    unit sphere, normals = positions, np 6, ptsmax 100, ctr 0, rootlvl 0): the
+    plain FP64 form against the oracle on every target, relative L2 and
+    max-abs relative to the largest value, both <= 1e-12.  The single-core
+    oracle takes about 35 s here."""
+    rng = np.random.default_rng(0x4B1F0002)
+    n = 262144
+    ct = rng.uniform(-1, 1, n)
+    ph = rng.uniform(0, 2 * np.pi, n)
+    st = np.sqrt(1 - ct * ct)
+    p = np.stack([st * np.cos(ph), st * np.sin(ph), ct], axis=1)
+    nrm = p.copy()
+    f = make_fmm(321, 6, 100, 10, (0.0, 0.0, 0.0), 0, p, nrm, p)
+    den = rng.uniform(-1, 1, n * 3)
+    val = f.eval(den)
+    o = make_oracle(f, 321, 6, 100, 10, (0.0, 0.0, 0.0), 0, p, nrm, p)
+    ref = o.eval(den)
+    e2 = relerr(val, ref)
+    emax = float(np.max(np.abs(val - ref)) / np.max(np.abs(ref)))
+    print(f"\n  stokes sphere 262k: gpu-vs-oracle rel L2 {e2:.3e}  max-abs/max {emax:.3e}", flush=True)
+    assert e2 <= PARITY and emax <= PARITY
