@@ -255,7 +255,7 @@ def kernel_rates(f, c, stats, nrhs, mixed):
         "s2p": (stats["far_pairs"] * PAIR_FLOPS[kt_mm](nrhs), "flop", "mfma", FP64_PEAK_TF),
         "m2l": (stats["v_pairs"] * nrhs * stats["C"] * 8.0 * sm * tm, "flop", "mfma",
                 FP32_PEAK_TF if mfma32 else FP64_PEAK_TF),
-        "gemm_up": (stats["up_flops"] * nrhs, "flop", "mfma", FP64_PEAK_TF),
+        "gemm_up": (stats["up_flops_mixed" if mixed else "up_flops"] * nrhs, "flop", "mfma", FP64_PEAK_TF),
         "gemm_down": (stats["down_flops"] * nrhs, "flop", "mfma", FP64_PEAK_TF),
         "fft_fwd": (stats["fft_fwd"] * nrhs * (nsp * 8.0 + stats["C"] * (8.0 if mfma32 else 16.0)), "byte", "hbm",
                     HBM_PEAK_GBS),

@@ -146,7 +146,9 @@ int kifmm3d_direct_device(kifmm3d* h, const double* d_srcden, int nrhs, const in
 /* Work counts of the plan after setup, per right-hand side: out[0] near-field
  * source-target pairs, [1] L2T + M2T surface-target pairs, [2] V-list pairs,
  * [3] S2M pairs, [4] S2L pairs, [5] up-pass GEMM flops, [6] down-pass GEMM
- * flops, [7] forward FFTs, [8] inverse FFTs, [9] complex spectrum length C. */
+ * flops, [7] forward FFTs, [8] inverse FFTs, [9] complex spectrum length C,
+ * [10] up-pass GEMM flops of the mixed form (inner nodes by the fused
+ * UC2UE x UE2UC product). */
 int kifmm3d_plan_stats(const kifmm3d* h, double* out, int n);
 
 const char* kifmm3d_last_error(const kifmm3d* h);

@@ -227,6 +227,21 @@ void Engine::upload_static()
   tw_.upload(tw);
   I_.upload(plan_.I);
   D_.upload(plan_.D);
+  {  // fused operator of the mixed up pass: m2me_ = UC2UE x [UE2UC_0 .. UE2UC_7],
+     // compensated (Dot2) like the pseudoinverse applies
+    const int pUE = O.pln[SURF_UE], pUC = O.pln[SURF_UC];
+    m2me_.alloc((size_t)8 * pUE * pUE);
+    dev::GemmArgs a{};
+    a.M = pUE; a.Kseg = pUC; a.nseg = 1; a.Rc = 1; a.ngroup = 1; a.maxcols = a.ncol = 8 * pUE;
+    a.A = uc2ue_.p; a.a_gstride = 0;
+    a.B = ue2uc_.p; a.bnode = I_.p + plan_.fuse_idx;
+    a.C = m2me_.p; a.cnode = I_.p + plan_.fuse_idx; a.gbeg = I_.p + plan_.fuse_idx_g;
+    a.compensated = 1;
+    a.mfma = gemm_mfma_;
+    a.ctot = a.ncol;
+    dev::gemm(stream_, a);
+    HIPCHK(hipGetLastError());
+  }
 }
 
 void Engine::build_plan()
@@ -303,6 +318,12 @@ void Engine::build_plan()
       op.bscale = (long)pool_d(P.D, scales(L + 1 + rootlvl_, +1, pUE));  // child level (fmm3d_eval.cpp:84)
       P.up_flops += 2.0 * op.M * op.Kseg * op.nseg * op.ncol;
       P.up.push_back(op);
+      GemmOp f = op;  // mixed form: child densities -> parent densities in one product
+      f.M = pUE;
+      f.a_which = 4; f.c_buf = 1;
+      f.cscale = (long)pool_d(P.D, scales(L + rootlvl_, -1, pUE));
+      P.up_flops_mixed += 2.0 * f.M * f.Kseg * f.nseg * f.ncol;
+      P.up_mixed.push_back(f);
     }
     if (!all.empty()) {
       GemmOp op;
@@ -316,7 +337,24 @@ void Engine::build_plan()
       op.cscale = (long)pool_d(P.D, scales(L + rootlvl_, -1, pUE));
       P.up_flops += 2.0 * op.M * op.Kseg * op.ncol;
       P.up.push_back(op);
+      std::vector<int> leaves;
+      for (int g : all)
+        if (T.terminal(g)) leaves.push_back(T.sndeidx[g]);
+      if (!leaves.empty()) {
+        GemmOp u = op;
+        u.ncol = u.maxcols = (int)leaves.size();
+        u.bnode = u.cnode = pool_i(P.I, leaves);
+        u.gbeg = pool_i(P.I, {0, (int)leaves.size()});
+        P.up_flops_mixed += 2.0 * u.M * u.Kseg * u.ncol;
+        P.up_mixed.push_back(u);
+      }
     }
+  }
+  {
+    std::vector<int> id(8 * (size_t)pUE);
+    for (size_t i = 0; i < id.size(); i++) id[i] = (int)i;
+    P.fuse_idx = pool_i(P.I, id);
+    P.fuse_idx_g = pool_i(P.I, {0, (int)id.size()});
   }
   // near field, S2L items and L2T/M2T chunks, per target leaf
   {
@@ -695,7 +733,7 @@ void Engine::ensure_work(int rc)
   // split-K partials of the matrix-core GEMMs with few columns, for every
   // chunk size up to rc (the last chunk of an evaluate may be smaller)
   size_t gw = 0;
-  for (const std::vector<GemmOp>* ops : {&plan_.up, &plan_.down})
+  for (const std::vector<GemmOp>* ops : {&plan_.up, &plan_.up_mixed, &plan_.down})
     for (const GemmOp& op : *ops)
       for (int r = 1; r <= rc; r++) {
         dev::GemmArgs a{};
@@ -841,7 +879,7 @@ void Engine::run_chunk(const double* srcden, double* trgval, int rc, int mode, h
   const int pUE = O.pln[SURF_UE], pUC = O.pln[SURF_UC], pDE = O.pln[SURF_DE], pDC = O.pln[SURF_DC];
   const dev::KC kcu{kcu_.a, kcu_.b, kcu_.c}, kcm{kcm_.a, kcm_.b, kcm_.c};
   double* bufs[4] = {upcval_.p, upeden_.p, dncval_.p, dneden_.p};
-  const double* opsA[4] = {uc2ue_.p, ue2uc_.p, dc2de_.p, de2dc_.p};
+  const double* opsA[5] = {uc2ue_.p, ue2uc_.p, dc2de_.p, de2dc_.p, m2me_.p};
   const long trs = (long)T.textcnt * tdof;
   dev::Points src{sx_.p, sy_.p, sz_.p, snx_.p, sny_.p, snz_.p, sden_.p, (long)rc * sdof, sdof};
   dev::Targets trg{tx_.p, ty_.p, tz_.p, tval_.p, trs};
@@ -907,7 +945,7 @@ void Engine::run_chunk(const double* srcden, double* trgval, int rc, int mode, h
              (const int2*)(I + P.s2m.ent), unit_[SURF_UC].p, O.nsp[SURF_UC], src, upcval_.p, rc, pUC, rc, kcu,
              std::min(p2p_variant_, 2));  // FP64 in both forms: the check values feed the pseudoinverse
   });
-  for (const GemmOp& op : P.up) timed(KIFMM3D_K_GEMM_UP, [&] { run_gemm(op); });
+  for (const GemmOp& op : mixed ? P.up_mixed : P.up) timed(KIFMM3D_K_GEMM_UP, [&] { run_gemm(op); });
   mark(2);
   // U list and direct W/X.  It depends only on the gathered densities, so
   // with overlap on it runs on a side stream concurrently with the up pass,

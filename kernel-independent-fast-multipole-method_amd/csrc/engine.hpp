@@ -33,7 +33,7 @@ struct DVec {
 // A GEMM step of the up or down pass (see dev::GemmArgs).
 struct GemmOp {
   int M = 0, Kseg = 0, nseg = 1, ngroup = 1, maxcols = 0, ncol = 0;
-  int a_which = 0;  // 0 uc2ue, 1 ue2uc, 2 dc2de, 3 de2dc
+  int a_which = 0;  // 0 uc2ue, 1 ue2uc, 2 dc2de, 3 de2dc, 4 uc2ue x ue2uc (fused M2M + UC2UE)
   long a_gstride = 0;
   int b_buf = 0, c_buf = 0;  // 0 upcval, 1 upeden, 2 dncval, 3 dneden
   size_t bnode = 0, cnode = 0, gbeg = 0;
@@ -70,6 +70,12 @@ struct Plan {
   ItemList s2m, s2l;
   ChunkList near, far;
   std::vector<GemmOp> up, down;
+  // Mixed form's up pass: an inner node's equivalent density comes straight
+  // from its children's through UC2UE x UE2UC_c (one product instead of M2M
+  // then UC2UE; its check values are not formed); leaves apply UC2UE.
+  std::vector<GemmOp> up_mixed;
+  size_t fuse_idx = 0;         // identity index 0 .. 8 plnUE - 1 (builds the fused operator)
+  size_t fuse_idx_g = 0;       // its group bounds {0, 8 plnUE}
   std::vector<VLevel> vlev;
   int max_vsrc = 0, max_vtrg = 0;
   bool m2l_pairs = false;      // per-pair M2L kernel instead of the parent-grouped one
@@ -82,6 +88,7 @@ struct Plan {
   long s2m_pairs = 0;          // source point - UC point pairs of S2M
   long s2l_pairs = 0;          // source point - DC point pairs of S2L
   double up_flops = 0;         // M2M + UC2UE multiply-adds x 2, per right-hand side
+  double up_flops_mixed = 0;   // the same for up_mixed
   double down_flops = 0;       // L2L + DC2DE multiply-adds x 2, per right-hand side
   long fft_fwd = 0, fft_inv = 0;  // transforms per right-hand side (times the dof count)
   std::vector<int> I;          // index pool (uploaded once)
@@ -172,6 +179,7 @@ class Engine {
   DVec<short> c2s_;
   DVec<double> unit_[4];
   DVec<double> uc2ue_, ue2uc_, dc2de_, de2dc_, m2l_;
+  DVec<double> m2me_;  // 8 blocks of plnUE x plnUE: UC2UE x UE2UC_c (Dot2 on the device at setup)
   DVec<double> opsT_;  // M2L operators as [C][343][sdof][tdof] complex (FP64)
   DVec<float> opsT32_; // the same in FP32
   DVec<double> tw_;

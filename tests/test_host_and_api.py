@@ -86,6 +86,16 @@ def test_plan_stats_and_info():
     assert len(wi) > 0 and len(xi) > 0
     assert st["near_pairs"] == pairs
     assert st["C"] == 5 * 8 * 8
+    # up pass (fmm3d_eval.cpp:69-92) at depth >= 2: plain = M2M over the 8
+    # children of inner nodes + UC2UE of every node; mixed = one fused
+    # (UC2UE x UE2UC_c) product per inner node + UC2UE of the leaves
+    ue, uc = i["plnUE"], i["plnUC"]
+    src = (nodes[:, 6] & 1 != 0) & (nodes[:, 5] >= 2)
+    inner = int(np.sum(src & (nodes[:, 1] >= 0)))
+    leaves = int(np.sum(src & (nodes[:, 1] < 0)))
+    assert inner > 0 and leaves > 0
+    assert st["up_flops"] == 2.0 * (inner * 8 * ue * uc + (inner + leaves) * ue * uc)
+    assert st["up_flops_mixed"] == 2.0 * (inner * 8 * ue * ue + leaves * ue * uc)
 
 
 def test_error_paths():
