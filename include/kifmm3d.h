@@ -131,7 +131,9 @@ long kifmm3d_get_operator(const kifmm3d* h, int what, double* buf, long n);
  * (sndecnt x plnUC), 2 downward equivalent densities (tndecnt x plnDE),
  * 3 downward check values (tndecnt x plnDC) -- the reference's _supeden,
  * _supcval, _tdneden, _tdncval (fmm3d/fmm3d.hpp:61-64).  Returns the element
- * count; buf may be NULL. */
+ * count; buf may be NULL.  After a mixed-mode evaluate the upward check values
+ * of inner nodes are not formed (their equivalent densities come from the
+ * children's directly) and hold no meaningful data. */
 long kifmm3d_get_buffer(kifmm3d* h, int what, double* buf, long n);
 
 /* FP64 direct sum at selected targets: out[r][k*tdof + t] = sum over all

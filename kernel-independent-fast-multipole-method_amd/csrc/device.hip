@@ -261,9 +261,62 @@ __global__ void k_gather(int n, int dof, int nr, const int* __restrict__ iperm, 
         const int e = min(e0 + j, per - 1), r = e / dof;
         v[j] = in[r * src_rs + (e - r * dof)];
       }
+      if ((per & 1) == 0 && e0 + 8 <= per) {  // 16-byte stores (rows of even length are 16-byte aligned)
 #pragma unroll
-      for (int j = 0; j < 8; j++)
-        if (e0 + j < per) out[e0 + j] = v[j];
+        for (int j = 0; j < 8; j += 2) *(double2*)(out + e0 + j) = make_double2(v[j], v[j + 1]);
+      } else {
+#pragma unroll
+        for (int j = 0; j < 8; j++)
+          if (e0 + j < per) out[e0 + j] = v[j];
+      }
+    }
+  }
+}
+
+// Target values leave in two passes so that both sides of the permutation
+// move whole lines: k_rows_to_slots transposes the leaf-order values
+// [r][slot*dof + d] into slot-major rows tmp[slot][r*dof + d] through LDS
+// (64 slots x 16 right-hand sides per block), then k_slots_to_caller takes one
+// caller-order point per thread, reads its slot's row (128 bytes for 16 RHS)
+// and writes the values across the right-hand sides, coalesced over the
+// threads for each one.  The one-pass form writes 8 bytes to a different line
+// per (point, RHS).
+constexpr int ROWS_TS = 64;
+
+__global__ __launch_bounds__(256) void k_rows_to_slots(int n, int dof, int nr, const double* __restrict__ src,
+                                                       long src_rs, double* __restrict__ tmp)
+{
+  __shared__ double s_v[16][ROWS_TS * 3 + 1];  // dof <= 3
+  const int k0 = blockIdx.x * ROWS_TS, r0 = blockIdx.y * 16;
+  const int nk = min(ROWS_TS, n - k0), nrr = min(16, nr - r0);
+  const int len = nk * dof;
+  for (int e = threadIdx.x; e < nrr * len; e += 256) {
+    const int rr = e / len, j = e - rr * len;
+    s_v[rr][j] = src[(long)(r0 + rr) * src_rs + (long)k0 * dof + j];
+  }
+  __syncthreads();
+  const int per = nrr * dof;
+  for (int e = threadIdx.x; e < nk * per; e += 256) {
+    const int kk = e / per, q = e - kk * per, rr = q / dof, d = q - rr * dof;
+    tmp[((long)(k0 + kk) * nr + r0 + rr) * dof + d] = s_v[rr][kk * dof + d];
+  }
+}
+
+__global__ void k_slots_to_caller(int n, int dof, int nr, const int* __restrict__ iperm, const double* __restrict__ tmp,
+                                  double* __restrict__ dst, long dst_rs)
+{
+  const int per = nr * dof;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const double* row = tmp + (long)iperm[i] * per;
+    for (int e0 = 0; e0 < per; e0 += 8) {  // 8 loads in flight, then 8 stores
+      double v[8];
+#pragma unroll
+      for (int j = 0; j < 8; j++) v[j] = row[min(e0 + j, per - 1)];
+#pragma unroll
+      for (int j = 0; j < 8; j++) {
+        const int e = e0 + j, r = e / dof;
+        if (e < per) dst[r * dst_rs + i * dof + (e - r * dof)] = v[j];
+      }
     }
   }
 }
@@ -325,6 +378,15 @@ void scatter(hipStream_t s, int n, int dof, int nr, const int* perm, const doubl
   long tot = (long)n * dof * nr;
   if (tot == 0) return;
   hipLaunchKernelGGL(k_scatter, dim3(grid_for(tot, 256)), dim3(256), 0, s, n, dof, nr, perm, src, src_rs, dst, dst_rs);
+}
+
+void scatter_rows(hipStream_t s, int n, int dof, int nr, const int* iperm, const double* src, long src_rs,
+                  double* tmp, double* dst, long dst_rs)
+{
+  if ((long)n * dof * nr == 0 || dof > 3) return;  // dof <= 3: the caller checks
+  hipLaunchKernelGGL(k_rows_to_slots, dim3((unsigned)((n + ROWS_TS - 1) / ROWS_TS), (unsigned)((nr + 15) / 16)),
+                     dim3(256), 0, s, n, dof, nr, src, src_rs, tmp);
+  hipLaunchKernelGGL(k_slots_to_caller, dim3(grid_for(n, 256)), dim3(256), 0, s, n, dof, nr, iperm, tmp, dst, dst_rs);
 }
 
 // ---------------------------------------------------------------------------
@@ -788,7 +850,7 @@ template <int KT, int NR>
 __global__ __launch_bounds__(P2S_BS) void k_p2s(const int* __restrict__ out_node, const double4* __restrict__ geom,
                                                 const int* __restrict__ eptr, const int2* __restrict__ ent,
                                                 const double* __restrict__ unit, int npt, Points src,
-                                                double* __restrict__ out, int Rc, int pln, int roff, KC kc)
+                                                double* __restrict__ out, int Rc, int pln, int roff, KC kc, int accum)
 {
   using K = Knl<KT>;
   constexpr int NPOS = K::NRM ? 6 : 3;
@@ -842,7 +904,7 @@ __global__ __launch_bounds__(P2S_BS) void k_p2s(const int* __restrict__ out_node
     for (int q = 0; q < NR; q++) {
       double* o = out + (node * Rc + r0 + q) * pln + (long)p * K::T;
 #pragma unroll
-      for (int d = 0; d < K::T; d++) o[d] += u[q][d];
+      for (int d = 0; d < K::T; d++) o[d] = u[q][d] + (accum ? o[d] : 0.0);
     }
   }
 }
@@ -965,7 +1027,8 @@ __global__ __launch_bounds__(P2P_BS) void k_p2s_mfma111(int nitem, int npc, cons
                                                         const double4* __restrict__ geom,
                                                         const int* __restrict__ eptr, const int2* __restrict__ ent,
                                                         const double* __restrict__ unit, int npt, Points src,
-                                                        double* __restrict__ out, int Rc, int pln, int roff, KC kc)
+                                                        double* __restrict__ out, int Rc, int pln, int roff, KC kc,
+                                                        int accum)
 {
   constexpr int NR = 16;
   __shared__ double s_pos[3][TILE];
@@ -1025,40 +1088,40 @@ __global__ __launch_bounds__(P2P_BS) void k_p2s_mfma111(int nitem, int npc, cons
     if (tt >= tn) continue;
     double* o = out + (node * Rc + r0 + kk) * pln + pb + tt;  // item v: right-hand side kk + 4v
 #pragma unroll
-    for (int v = 0; v < 4; v++) o[(long)(4 * v) * pln] += kc.a * acc[i][v];
+    for (int v = 0; v < 4; v++) o[(long)(4 * v) * pln] = kc.a * acc[i][v] + (accum ? o[(long)(4 * v) * pln] : 0.0);
   }
 }
 
 template <int KT>
 static void p2s_kt(hipStream_t s, int nitem, const int* out_node, const double4* geom, const int* eptr,
                    const int2* ent, const double* unit, int npt, Points src, double* out, int Rc, int pln, int nr,
-                   KC kc, int variant)
+                   KC kc, int variant, int accum)
 {
   rhs_blocks(nr, [&](int NR, int nb, int roff) {
     dim3 grid(nitem, (npt + P2S_BS - 1) / P2S_BS, nb);
     if (KT == 111 && NR == 16 && variant >= 2 && src.den_rs == 1) {
       const int npc = (npt + P2P_BS - 1) / P2P_BS;
       hipLaunchKernelGGL(k_p2s_mfma111, dim3(xcd_grid(nitem * npc), nb), dim3(P2P_BS), 0, s, nitem, npc, out_node,
-                         geom, eptr, ent, unit, npt, src, out, Rc, pln, roff, kc);
+                         geom, eptr, ent, unit, npt, src, out, Rc, pln, roff, kc, accum);
     } else if (NR == 16)
       hipLaunchKernelGGL((k_p2s<KT, 16>), grid, dim3(P2S_BS), 0, s, out_node, geom, eptr, ent, unit, npt, src, out,
-                         Rc, pln, roff, kc);
+                         Rc, pln, roff, kc, accum);
     else if (NR == 4)
       hipLaunchKernelGGL((k_p2s<KT, 4>), grid, dim3(P2S_BS), 0, s, out_node, geom, eptr, ent, unit, npt, src, out,
-                         Rc, pln, roff, kc);
+                         Rc, pln, roff, kc, accum);
     else
       hipLaunchKernelGGL((k_p2s<KT, 1>), grid, dim3(P2S_BS), 0, s, out_node, geom, eptr, ent, unit, npt, src, out,
-                         Rc, pln, roff, kc);
+                         Rc, pln, roff, kc, accum);
   });
 }
 
 void p2s(hipStream_t s, int kt, int nitem, const int* out_node, const double4* geom, const int* eptr, const int2* ent,
-         const double* unit, int npt, Points src, double* out, int Rc, int pln, int nr, KC kc, int variant)
+         const double* unit, int npt, Points src, double* out, int Rc, int pln, int nr, KC kc, int variant, int accum)
 {
   if (nitem <= 0) return;
   switch (kt) {
 #define KIFMM_P2S(KT) \
-  case KT: p2s_kt<KT>(s, nitem, out_node, geom, eptr, ent, unit, npt, src, out, Rc, pln, nr, kc, variant); break;
+  case KT: p2s_kt<KT>(s, nitem, out_node, geom, eptr, ent, unit, npt, src, out, Rc, pln, nr, kc, variant, accum); break;
   KIFMM_P2S(111) KIFMM_P2S(121) KIFMM_P2S(311) KIFMM_P2S(312) KIFMM_P2S(321) KIFMM_P2S(322) KIFMM_P2S(511)
   KIFMM_P2S(521)
 #undef KIFMM_P2S

@@ -18,6 +18,10 @@ void copy(hipStream_t s, const double* src, double* dst, long n);
 // Source-major gather through the inverse permutation (iperm[perm[k]] = k):
 // dst[(iperm[i]*nr + r)*dof + d] = src[r*src_rs + i*dof + d] for i < n
 void gather(hipStream_t s, int n, int dof, int nr, const int* iperm, const double* src, long src_rs, double* dst);
+// The same through slot-major rows (tmp holds n * nr * dof doubles; iperm is
+// the inverse of perm; dof <= 3): a transpose, then one caller point per thread.
+void scatter_rows(hipStream_t s, int n, int dof, int nr, const int* iperm, const double* src, long src_rs,
+                  double* tmp, double* dst, long dst_rs);
 // dst[r*dst_rs + perm[k]*dof + d] = src[r*src_rs + k*dof + d]
 void scatter(hipStream_t s, int n, int dof, int nr, const int* perm, const double* src, long src_rs, double* dst,
              long dst_rs);
@@ -53,10 +57,11 @@ void direct(hipStream_t s, int kt, int tdof, int nchk, const double* cx, const d
             int nsrc, int nseg, Points src, double* part, double* out, int nr, KC kc);
 
 // Points to check surface (S2M onto UC, S2L onto DC): out[(node*Rc + r)*pln + p*tdof + t] +=
-// sum over the source leaves of item i of K(ctr_i + rad_i * unit_p <- y) den(y).
+// sum over the source leaves of item i of K(ctr_i + rad_i * unit_p <- y) den(y);
+// with accum = 0 the sum overwrites out (items own distinct nodes).
 void p2s(hipStream_t s, int kt, int nitem, const int* out_node, const double4* geom, const int* eptr,
          const int2* ent, const double* unit, int npt, Points src, double* out, int Rc, int pln, int nr, KC kc,
-         int variant);
+         int variant, int accum = 1);
 
 // Equivalent surfaces to targets with the mm kernel (L2T from DE, M2T from UE).
 // Entry e: node enode[e] of buffer ekind[e] (0 = dneden, 1 = upeden), surface

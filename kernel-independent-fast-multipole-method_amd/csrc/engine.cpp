@@ -187,6 +187,11 @@ void Engine::setup(const double* spos, const double* snor, int ns, const double*
     siperm_.upload(inv);
   }
   tperm_.upload(tree_.tperm);
+  {
+    std::vector<int> inv(tree_.tperm.size());
+    for (size_t k = 0; k < inv.size(); k++) inv[tree_.tperm[k]] = (int)k;
+    tiperm_.upload(inv);
+  }
   upload_static();
   HIPCHK(hipDeviceSynchronize());
   lap("upload");
@@ -705,13 +710,13 @@ int Engine::choose_rhs_chunk(int nrhs)
 {
   const Operators& O = *ops_;
   const Tree& T = tree_;
-  double per = 8.0 * ((double)T.sextcnt * user_.sdof + (double)T.textcnt * user_.tdof +
+  double per = 8.0 * ((double)T.sextcnt * user_.sdof + 2.0 * T.textcnt * user_.tdof +
                       (double)T.sndecnt * (O.pln[SURF_UC] + O.pln[SURF_UE]) +
                       (double)T.tndecnt * (O.pln[SURF_DC] + O.pln[SURF_DE])) +
                16.0 * O.C * ((double)plan_.max_vsrc * mm_.sdof + (double)plan_.max_vtrg * mm_.tdof);
   size_t fr = 0, tot = 0;
   HIPCHK(hipMemGetInfo(&fr, &tot));
-  double budget = 0.6 * (double)fr + 8.0 * (double)(sden_.n + tval_.n + upcval_.n + upeden_.n + dncval_.n + dneden_.n) +
+  double budget = 0.6 * (double)fr + 8.0 * (double)(sden_.n + tval_.n + tvalT_.n + upcval_.n + upeden_.n + dncval_.n + dneden_.n) +
                   8.0 * (double)(effden_.n + effval_.n);
   int rc = std::min((int)std::max(1.0, std::floor(budget / std::max(per, 1.0))), nrhs);
   if (rc >= 16) rc -= rc % 16;  // chunks of 16k right-hand sides take the matrix-core kernels
@@ -724,6 +729,7 @@ void Engine::ensure_work(int rc)
   const Tree& T = tree_;
   sden_.ensure((size_t)rc * T.sextcnt * user_.sdof);
   tval_.ensure((size_t)rc * T.textcnt * user_.tdof);
+  tvalT_.ensure((size_t)rc * T.textcnt * user_.tdof);
   upcval_.ensure((size_t)rc * T.sndecnt * O.pln[SURF_UC]);
   upeden_.ensure((size_t)rc * T.sndecnt * O.pln[SURF_UE]);
   dncval_.ensure((size_t)rc * T.tndecnt * O.pln[SURF_DC]);
@@ -930,10 +936,11 @@ void Engine::run_chunk(const double* srcden, double* trgval, int rc, int mode, h
   auto timed = [&](int cls, auto&& fn) { timed_on(s, cls, fn); };
 
   mark(0);
-  // zero the accumulated buffers (fmm3d_eval.cpp:33-42)
+  // zero the accumulated buffers (fmm3d_eval.cpp:33-42); upward check values
+  // at depth >= 2 are overwritten by S2M (leaves) and M2M (inner nodes)
   dev::zero(s, tval_.p, (long)rc * trs);
-  dev::zero(s, upcval_.p, (long)rc * T.sndecnt * pUC);
   dev::zero(s, dncval_.p, (long)rc * T.tndecnt * pDC);
+  if (P.nlow_s) dev::zero(s, upcval_.p, (long)rc * P.nlow_s * pUC);
   if (P.nlow_s) dev::zero(s, upeden_.p, (long)rc * P.nlow_s * pUE);
   if (P.nlow_t) dev::zero(s, dneden_.p, (long)rc * P.nlow_t * pDE);
   // load (fmm3d_eval.cpp:50-64)
@@ -943,7 +950,7 @@ void Engine::run_chunk(const double* srcden, double* trgval, int rc, int mode, h
   timed(KIFMM3D_K_S2M, [&] {
     dev::p2s(s, user_.kt, P.s2m.n, I + P.s2m.out, (const double4*)(D + P.s2m.geom), I + P.s2m.eptr,
              (const int2*)(I + P.s2m.ent), unit_[SURF_UC].p, O.nsp[SURF_UC], src, upcval_.p, rc, pUC, rc, kcu,
-             std::min(p2p_variant_, 2));  // FP64 in both forms: the check values feed the pseudoinverse
+             std::min(p2p_variant_, 2), 0);  // FP64 in both forms: the check values feed the pseudoinverse
   });
   for (const GemmOp& op : mixed ? P.up_mixed : P.up) timed(KIFMM3D_K_GEMM_UP, [&] { run_gemm(op); });
   mark(2);
@@ -1011,7 +1018,12 @@ void Engine::run_chunk(const double* srcden, double* trgval, int rc, int mode, h
   });
   mark(7);
   // save (fmm3d_eval.cpp:221-235)
-  timed(KIFMM3D_K_SCATTER, [&] { dev::scatter(s, T.textcnt, tdof, rc, tperm_.p, tval_.p, trs, trgval, (long)nt_ * tdof); });
+  timed(KIFMM3D_K_SCATTER, [&] {
+    if (tdof <= 3)
+      dev::scatter_rows(s, T.textcnt, tdof, rc, tiperm_.p, tval_.p, trs, tvalT_.p, trgval, (long)nt_ * tdof);
+    else
+      dev::scatter(s, T.textcnt, tdof, rc, tperm_.p, tval_.p, trs, trgval, (long)nt_ * tdof);
+  });
   HIPCHK(hipGetLastError());
   mark(8);
   if (ev) nev_used_ = nev;

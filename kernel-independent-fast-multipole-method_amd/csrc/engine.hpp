@@ -176,6 +176,8 @@ class Engine {
   DVec<double> sx_, sy_, sz_, snx_, sny_, snz_, tx_, ty_, tz_;
   DVec<int> tperm_, s2c_;
   DVec<int> siperm_;  // inverse of the source leaf permutation: caller order -> leaf order
+  DVec<int> tiperm_;  // inverse of the target leaf permutation
+  DVec<double> tvalT_;  // target values as slot-major rows, on their way out
   DVec<short> c2s_;
   DVec<double> unit_[4];
   DVec<double> uc2ue_, ue2uc_, dc2de_, de2dc_, m2l_;
