@@ -560,7 +560,11 @@ __device__ __forceinline__ void p2p_tile64(int cnt4, int kk, int col, const doub
 
 constexpr int TILE32 = 128;  // sources per staged tile of the FP32 kernel (measured 2 % faster than 64)
 
-template <int NG>
+// CHK: pairs with r^2 < 1e-16 contribute zero (the reference's r < 1e-8
+// rule), needed where a source and a target can coincide -- only in the
+// target's own leaf, since equal points always fall in the same leaf.
+// Elsewhere 1/r is taken of max(r^2, 1e-16), one instruction instead of two.
+template <int NG, bool CHK = true>
 __device__ __forceinline__ void p2p_tile32(int cnt4, int kk, int col, const float (*s_pos)[TILE32],
                                            const float (*s_den)[16], const float* tx, const float* ty,
                                            const float* tz, double (*acc)[4])
@@ -575,7 +579,7 @@ __device__ __forceinline__ void p2p_tile32(int cnt4, int kk, int col, const floa
     for (int i = 0; i < NG; i++) {
       const float dx = tx[i] - sx, dy = ty[i] - sy, dz = tz[i] - sz;
       const float r2 = dx * dx + dy * dy + dz * dz;
-      const float gv = r2 < 1e-16f ? 0.0f : __builtin_amdgcn_rsqf(r2);
+      const float gv = CHK ? (r2 < 1e-16f ? 0.0f : __builtin_amdgcn_rsqf(r2)) : __builtin_amdgcn_rsqf(fmaxf(r2, 1e-16f));
       part[i] = __builtin_amdgcn_mfma_f32_16x16x4f32(f, gv, part[i], 0, 0, 0);
     }
   }
@@ -656,10 +660,12 @@ __global__ __launch_bounds__(P2P_BS) void k_p2p_mfma111(int nchunk, const int* _
 // relative to the chunk's first target in FP64 and then rounded, so FP32
 // keeps the differences of nearby points to ~1e-7 of the leaf size; 1/r
 // comes from v_rsq_f32.  The FP32 products of one staged source tile (at most
-// 128 sources) are summed into FP64 accumulators.
+// 128 sources) are summed into FP64 accumulators.  selfe[c] is the entry of
+// the chunk's own leaf (or -1): only its pairs take the r < 1e-8 exclusion.
 __global__ __launch_bounds__(P2P_BS) void k_p2p_mfma111_f32(int nchunk, const int* __restrict__ tbeg,
                                                             const int* __restrict__ tnum, const int* __restrict__ eptr,
-                                                            const int2* __restrict__ ent, Points src, Targets trg,
+                                                            const int2* __restrict__ ent,
+                                                            const int* __restrict__ selfe, Points src, Targets trg,
                                                             int roff, KC kc)
 {
   constexpr int NR = 16;
@@ -688,8 +694,10 @@ __global__ __launch_bounds__(P2P_BS) void k_p2p_mfma111_f32(int nchunk, const in
 #pragma unroll
     for (int v = 0; v < 4; v++) acc[i][v] = 0.0;
   const double* den = src.den + (long)r0 * src.den_rs;
+  const int eself = selfe[c];
   for (int e = eptr[c]; e < eptr[c + 1]; e++) {
     const int2 se = ent[e];
+    const bool chk = e == eself;  // block-uniform
     for (int b0 = 0; b0 < se.y; b0 += TILE32) {
       const int cnt = min(TILE32, se.y - b0), cnt4 = (cnt + 3) & ~3;
       const int j0 = se.x + b0;
@@ -705,14 +713,25 @@ __global__ __launch_bounds__(P2P_BS) void k_p2p_mfma111_f32(int nchunk, const in
         s_den[jj][q] = jj < cnt ? (float)den[(long)(j0 + jj) * src.den_js + (long)q * src.den_rs] : 0.0f;
       }
       __syncthreads();
-      if (ng == 4)
-        p2p_tile32<4>(cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc);
-      else if (ng == 3)
-        p2p_tile32<3>(cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc);
-      else if (ng == 2)
-        p2p_tile32<2>(cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc);
-      else if (ng == 1)
-        p2p_tile32<1>(cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc);
+      if (chk) {
+        if (ng == 4)
+          p2p_tile32<4, true>(cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc);
+        else if (ng == 3)
+          p2p_tile32<3, true>(cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc);
+        else if (ng == 2)
+          p2p_tile32<2, true>(cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc);
+        else if (ng == 1)
+          p2p_tile32<1, true>(cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc);
+      } else {
+        if (ng == 4)
+          p2p_tile32<4, false>(cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc);
+        else if (ng == 3)
+          p2p_tile32<3, false>(cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc);
+        else if (ng == 2)
+          p2p_tile32<2, false>(cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc);
+        else if (ng == 1)
+          p2p_tile32<1, false>(cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc);
+      }
     }
   }
   if (!wact) return;
@@ -975,11 +994,11 @@ __global__ __launch_bounds__(P2P_BS) void k_s2p(const int* __restrict__ tbeg, co
 
 template <int KT, int NR>
 static void p2p_launch(hipStream_t s, int variant, int nchunk, int nb, const int* tbeg, const int* tnum,
-                       const int* eptr, const int2* ent, Points src, Targets trg, int roff, KC kc)
+                       const int* eptr, const int2* ent, const int* selfe, Points src, Targets trg, int roff, KC kc)
 {
-  if (variant == 3 && KT == 111 && NR == 16)
+  if (variant == 3 && KT == 111 && NR == 16 && selfe)
     hipLaunchKernelGGL(k_p2p_mfma111_f32, dim3(xcd_grid(nchunk), nb), dim3(P2P_BS), 0, s, nchunk, tbeg, tnum, eptr,
-                       ent, src, trg, roff, kc);
+                       ent, selfe, src, trg, roff, kc);
   else if (variant >= 2 && KT == 111 && NR == 16 && src.den_rs == 1)
     hipLaunchKernelGGL(k_p2p_mfma111, dim3(xcd_grid(nchunk), nb), dim3(P2P_BS), 0, s, nchunk, tbeg, tnum, eptr, ent,
                        src, trg, roff, kc);
@@ -993,25 +1012,25 @@ static void p2p_launch(hipStream_t s, int variant, int nchunk, int nb, const int
 
 template <int KT>
 static void p2p_kt(hipStream_t s, int nchunk, const int* tbeg, const int* tnum, const int* eptr, const int2* ent,
-                   Points src, Targets trg, int nr, KC kc, int variant)
+                   const int* selfe, Points src, Targets trg, int nr, KC kc, int variant)
 {
   rhs_blocks(nr, [&](int NR, int nb, int roff) {
     if (NR == 16)
-      p2p_launch<KT, 16>(s, variant, nchunk, nb, tbeg, tnum, eptr, ent, src, trg, roff, kc);
+      p2p_launch<KT, 16>(s, variant, nchunk, nb, tbeg, tnum, eptr, ent, selfe, src, trg, roff, kc);
     else if (NR == 4)
-      p2p_launch<KT, 4>(s, variant, nchunk, nb, tbeg, tnum, eptr, ent, src, trg, roff, kc);
+      p2p_launch<KT, 4>(s, variant, nchunk, nb, tbeg, tnum, eptr, ent, selfe, src, trg, roff, kc);
     else
-      p2p_launch<KT, 1>(s, variant, nchunk, nb, tbeg, tnum, eptr, ent, src, trg, roff, kc);
+      p2p_launch<KT, 1>(s, variant, nchunk, nb, tbeg, tnum, eptr, ent, selfe, src, trg, roff, kc);
   });
 }
 
 void p2p(hipStream_t s, int kt, int nchunk, const int* tbeg, const int* tnum, const int* eptr, const int2* ent,
-         Points src, Targets trg, int nr, KC kc, int variant)
+         const int* selfe, Points src, Targets trg, int nr, KC kc, int variant)
 {
   if (nchunk <= 0) return;
   switch (kt) {
 #define KIFMM_P2P(KT) \
-  case KT: p2p_kt<KT>(s, nchunk, tbeg, tnum, eptr, ent, src, trg, nr, kc, variant); break;
+  case KT: p2p_kt<KT>(s, nchunk, tbeg, tnum, eptr, ent, selfe, src, trg, nr, kc, variant); break;
   KIFMM_P2P(111) KIFMM_P2P(121) KIFMM_P2P(311) KIFMM_P2P(312) KIFMM_P2P(321) KIFMM_P2P(322) KIFMM_P2P(511)
   KIFMM_P2P(521)
 #undef KIFMM_P2P
@@ -2151,13 +2170,18 @@ __global__ __launch_bounds__(64 * R) void k_fft_inv_ri(int nslot, const int* __r
   }
 }
 
+// Right-hand sides per block of the FP32-spectrum transforms (FP64: 4).
+// Measured at 4M points / 16 RHS: forward 4 -> 2.00 ms, 8 -> 2.15 ms;
+// inverse 8 -> 2.24 ms, 4 -> 2.29 ms, 16 -> 3.16 ms.
+constexpr int FFT_RW32_FWD = 4, FFT_RW32_INV = 8;
+
 template <class CT>
 static void fft_fwd_t(hipStream_t s, int np, int nslot, const int* slot_node, int Rc, const double* upeden, int pln,
                       int sm, const double* dscale, const short* cube2surf, CT* effden, const double2* tw, int ri)
 {
   if (nslot <= 0) return;
   if (ri && sm == 1) {
-    constexpr int RW = sizeof(CT) == 8 ? 8 : 4;  // 64-byte runs of right-hand sides
+    constexpr int RW = sizeof(CT) == 8 ? FFT_RW32_FWD : 4;
     switch (np) {
     case 4: hipLaunchKernelGGL((k_fft_fwd_ri<4, CT, RW>), dim3(xcd_grid(nslot * (Rc / RW))), dim3(64 * RW), 0, s, nslot, slot_node, Rc, upeden, pln, dscale, cube2surf, effden, tw); break;
     case 6: hipLaunchKernelGGL((k_fft_fwd_ri<6, CT, RW>), dim3(xcd_grid(nslot * (Rc / RW))), dim3(64 * RW), 0, s, nslot, slot_node, Rc, upeden, pln, dscale, cube2surf, effden, tw); break;
@@ -2179,7 +2203,7 @@ static void fft_inv_t(hipStream_t s, int np, int nslot, const int* slot_node, in
 {
   if (nslot <= 0) return;
   if (ri && tm == 1) {
-    constexpr int RW = sizeof(CT) == 8 ? 8 : 4;
+    constexpr int RW = sizeof(CT) == 8 ? FFT_RW32_INV : 4;
     switch (np) {
     case 4: hipLaunchKernelGGL((k_fft_inv_ri<4, CT, RW>), dim3(xcd_grid(nslot * (Rc / RW))), dim3(64 * RW), 0, s, nslot, slot_node, Rc, effval, surf2cube, nsp, dncval, pln, tw); break;
     case 6: hipLaunchKernelGGL((k_fft_inv_ri<6, CT, RW>), dim3(xcd_grid(nslot * (Rc / RW))), dim3(64 * RW), 0, s, nslot, slot_node, Rc, effval, surf2cube, nsp, dncval, pln, tw); break;

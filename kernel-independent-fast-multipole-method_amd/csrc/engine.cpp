@@ -363,7 +363,7 @@ void Engine::build_plan()
   }
   // near field, S2L items and L2T/M2T chunks, per target leaf
   {
-    std::vector<int> tbeg, tnum, eptr(1, 0), ent;
+    std::vector<int> tbeg, tnum, eptr(1, 0), ent, eself;
     std::vector<int> ftbeg, ftnum, feptr(1, 0), fnode, fkind;
     std::vector<double> fgeom;
     std::vector<int> xout, xeptr(1, 0), xent;
@@ -388,8 +388,10 @@ void Engine::build_plan()
       if (!leaf) continue;
       dent.clear(); sent.clear(); sknd.clear(); sgeom.clear();
       long npair = 0, fpair = 0;
+      int dself = -1;
       auto U = list(LIST_U, g);
       for (auto it = U.first; it != U.second; ++it) {
+        if (*it == g) dself = (int)dent.size() / 2;
         dent.push_back(T.sextbeg[*it]); dent.push_back(T.sextnum[*it]);
         npair += T.sextnum[*it];
       }
@@ -418,6 +420,7 @@ void Engine::build_plan()
       for (int off = 0; off < T.textnum[g]; off += 128) {
         int n = std::min(128, T.textnum[g] - off);
         tbeg.push_back(T.textbeg[g] + off); tnum.push_back(n);
+        eself.push_back(dself >= 0 ? (int)ent.size() / 2 + dself : -1);
         ent.insert(ent.end(), dent.begin(), dent.end());
         eptr.push_back((int)ent.size() / 2);
         ftbeg.push_back(T.textbeg[g] + off); ftnum.push_back(n);
@@ -429,7 +432,7 @@ void Engine::build_plan()
     }
     P.near.n = (int)tbeg.size();
     P.near.tbeg = pool_i(P.I, tbeg); P.near.tnum = pool_i(P.I, tnum);
-    P.near.eptr = pool_i(P.I, eptr); P.near.ent = pool_i(P.I, ent);
+    P.near.eptr = pool_i(P.I, eptr); P.near.ent = pool_i(P.I, ent); P.near.self = pool_i(P.I, eself);
     P.far.n = (int)ftbeg.size();
     P.far.tbeg = pool_i(P.I, ftbeg); P.far.tnum = pool_i(P.I, ftnum);
     P.far.eptr = pool_i(P.I, feptr); P.far.enode = pool_i(P.I, fnode);
@@ -964,7 +967,7 @@ void Engine::run_chunk(const double* srcden, double* trgval, int rc, int mode, h
   }
   timed_on(side ? side_ : s, KIFMM3D_K_P2P, [&] {
     dev::p2p(side ? side_ : s, user_.kt, P.near.n, I + P.near.tbeg, I + P.near.tnum, I + P.near.eptr,
-             (const int2*)(I + P.near.ent), src, trg, rc, kcu, mixed ? 3 : p2p_variant_);
+             (const int2*)(I + P.near.ent), I + P.near.self, src, trg, rc, kcu, mixed ? 3 : p2p_variant_);
   });
   if (side) HIPCHK(hipEventRecord(join_ev_, side_));
   mark(3);

@@ -51,6 +51,7 @@ struct ChunkList {  // chunks of <= 128 leaf targets
   int n = 0;
   size_t tbeg = 0, tnum = 0, eptr = 0;
   size_t ent = 0;                          // near field: int2 {sextbeg, sextnum}
+  size_t self = 0;                         // near field: entry of the chunk's own leaf, or -1
   size_t enode = 0, ekind = 0, egeom = 0;  // surfaces: node, kind, (cx, cy, cz, rad)
 };
 
