@@ -1657,10 +1657,11 @@ __global__ __launch_bounds__(256) void k_gemm_mx(GemmArgs p, int ks)
   const double* bp[G];
   double bm[G];
   auto seg_ptrs = [&](int seg) {
+    const double* Bb = (seg > 0 && p.B2) ? p.B2 : p.B;
 #pragma unroll
     for (int g = 0; g < G; g++) {
       const int bn = cok[g] ? p.bnode[(long)seg * p.ncol + cn[g]] : -1;
-      bp[g] = p.B + (bn >= 0 ? ((long)bn * p.Rc + cr[g]) * p.Kseg : 0) + 2 * lk;
+      bp[g] = Bb + (bn >= 0 ? ((long)bn * p.Rc + cr[g]) * p.Kseg : 0) + 2 * lk;
       bm[g] = bn >= 0 ? 1.0 : 0.0;
     }
   };
@@ -1673,7 +1674,7 @@ __global__ __launch_bounds__(256) void k_gemm_mx(GemmArgs p, int ks)
       double v = 0.0;
       if (e < 8 * MR && rb + row < p.M) {
         v = A[(long)gk * p.M + rb + row];
-        if (p.bscale) v *= p.bscale[gk - (gk / p.Kseg) * p.Kseg];  // power of two: exact
+        if (p.bscale && gk >= p.bscale_seg * p.Kseg) v *= p.bscale[gk - (gk / p.Kseg) * p.Kseg];  // power of two: exact
       }
       ar[i] = v;
     }
@@ -1799,7 +1800,8 @@ void gemm(hipStream_t s, const GemmArgs& a)
   // columns alone give fewer than 2 blocks per CU; 64 x 64 tiles without a
   // workspace
   if (!a.compensated && a.mfma == 2 && a.Kseg % 8 == 0) {
-    const int ks = gemm_split(a);
+    int ks = gemm_split(a);
+    if (a.B2 && !(a.work && (long)ks * a.ctot * a.M <= a.work_n)) ks = 1;  // two B buffers: wide tiles only
     if (ks == 1 || (a.work && (long)ks * a.ctot * a.M <= a.work_n)) {
       if (a.M <= 64) {
         dim3 grid((unsigned)((cols + 255) / 256), 1, a.ngroup * ks);

@@ -97,6 +97,8 @@ struct GemmArgs {
   double* work;     // split-K partials, gemm_split(a) x ctot x M doubles (null or too small: no split)
   long work_n;      // capacity of work in doubles
   int split_blocks; // blocks the split-K aims at (0: 1024); lower when other streams share the GPU
+  const double* B2; // segments >= 1 read this buffer instead of B (wide-tile plain form only)
+  int bscale_seg;   // bscale applies to segments >= bscale_seg
 };
 void gemm(hipStream_t s, const GemmArgs& a);
 // K splits the wide-tile product takes for these sizes (1: none).

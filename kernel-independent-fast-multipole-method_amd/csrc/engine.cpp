@@ -247,6 +247,23 @@ void Engine::upload_static()
     dev::gemm(stream_, a);
     HIPCHK(hipGetLastError());
   }
+  if (!plan_.down_mixed.empty()) {  // l2de_ = [DC2DE | DC2DE x DE2DC_c], c = 0..7
+    const int pDE = O.pln[SURF_DE], pDC = O.pln[SURF_DC];
+    l2de_.alloc((size_t)8 * pDE * 2 * pDC);
+    for (int k = 0; k < 8; k++)
+      HIPCHK(hipMemcpyAsync(l2de_.p + (size_t)k * pDE * 2 * pDC, dc2de_.p, sizeof(double) * pDE * pDC,
+                            hipMemcpyDeviceToDevice, stream_));
+    dev::GemmArgs a{};
+    a.M = pDE; a.Kseg = pDC; a.nseg = 1; a.Rc = 1; a.ngroup = 1; a.maxcols = a.ncol = 8 * pDE;
+    a.A = dc2de_.p;
+    a.B = de2dc_.p; a.bnode = I_.p + plan_.fdn_b;
+    a.C = l2de_.p; a.cnode = I_.p + plan_.fdn_c; a.gbeg = I_.p + plan_.fdn_g;
+    a.compensated = 1;
+    a.mfma = gemm_mfma_;
+    a.ctot = a.ncol;
+    dev::gemm(stream_, a);
+    HIPCHK(hipGetLastError());
+  }
 }
 
 void Engine::build_plan()
@@ -662,6 +679,7 @@ void Engine::build_plan()
     }
   }
   // down pass, top level first (fmm3d_eval.cpp:195-209)
+  const bool fuse_down = pDC == pDE && pDC % 8 == 0 && gemm_mfma_ == 2;
   for (int L = 2; L < T.nlevel; L++) {
     std::vector<int> all;
     for (int g : by_depth[L])
@@ -692,6 +710,17 @@ void Engine::build_plan()
       op.bscale = (long)pool_d(P.D, scales(L - 1 + rootlvl_, +1, pDE));  // parent level (fmm3d_eval.cpp:203)
       P.down_flops += 2.0 * op.M * op.Kseg * op.ncol;
       P.down.push_back(op);
+      if (fuse_down) {
+        GemmOp f = op;  // [own check values; parent density] -> own density
+        f.M = pDE; f.nseg = 2;
+        f.a_which = 5; f.a_gstride = (long)pDE * 2 * pDC;
+        f.b_buf = 2; f.b2_buf = 3; f.bscale_seg = 1; f.c_buf = 3; f.accumulate = 0;
+        std::vector<int> b2(cnode);
+        b2.insert(b2.end(), bnode.begin(), bnode.end());
+        f.bnode = pool_i(P.I, b2);
+        f.cscale = (long)pool_d(P.D, scales(L + rootlvl_, -1, pDE));
+        P.down_mixed.push_back(f);
+      }
     }
     GemmOp op;
     op.M = pDE; op.Kseg = pDC; op.nseg = 1; op.ngroup = 1;
@@ -704,8 +733,19 @@ void Engine::build_plan()
     op.cscale = (long)pool_d(P.D, scales(L + rootlvl_, -1, pDE));
     P.down_flops += 2.0 * op.M * op.Kseg * op.ncol;
     P.down.push_back(op);
+    if (fuse_down && L < 3) P.down_mixed.push_back(op);
   }
-  (void)pDC;
+  if (fuse_down) {
+    std::vector<int> b(8 * (size_t)pDE), c(8 * (size_t)pDE);
+    for (int k = 0; k < 8; k++)
+      for (int j = 0; j < pDE; j++) {
+        b[(size_t)k * pDE + j] = k * pDE + j;                // column j of DE2DC_k
+        c[(size_t)k * pDE + j] = k * 2 * pDE + pDE + j;      // column pDE + j of block k
+      }
+    P.fdn_b = pool_i(P.I, b);
+    P.fdn_c = pool_i(P.I, c);
+    P.fdn_g = pool_i(P.I, {0, 8 * pDE});
+  }
   plan_ = std::move(P);
 }
 
@@ -742,7 +782,7 @@ void Engine::ensure_work(int rc)
   // split-K partials of the matrix-core GEMMs with few columns, for every
   // chunk size up to rc (the last chunk of an evaluate may be smaller)
   size_t gw = 0;
-  for (const std::vector<GemmOp>* ops : {&plan_.up, &plan_.up_mixed, &plan_.down})
+  for (const std::vector<GemmOp>* ops : {&plan_.up, &plan_.up_mixed, &plan_.down, &plan_.down_mixed})
     for (const GemmOp& op : *ops)
       for (int r = 1; r <= rc; r++) {
         dev::GemmArgs a{};
@@ -888,7 +928,7 @@ void Engine::run_chunk(const double* srcden, double* trgval, int rc, int mode, h
   const int pUE = O.pln[SURF_UE], pUC = O.pln[SURF_UC], pDE = O.pln[SURF_DE], pDC = O.pln[SURF_DC];
   const dev::KC kcu{kcu_.a, kcu_.b, kcu_.c}, kcm{kcm_.a, kcm_.b, kcm_.c};
   double* bufs[4] = {upcval_.p, upeden_.p, dncval_.p, dneden_.p};
-  const double* opsA[5] = {uc2ue_.p, ue2uc_.p, dc2de_.p, de2dc_.p, m2me_.p};
+  const double* opsA[6] = {uc2ue_.p, ue2uc_.p, dc2de_.p, de2dc_.p, m2me_.p, l2de_.p};
   const long trs = (long)T.textcnt * tdof;
   dev::Points src{sx_.p, sy_.p, sz_.p, snx_.p, sny_.p, snz_.p, sden_.p, (long)rc * sdof, sdof};
   dev::Targets trg{tx_.p, ty_.p, tz_.p, tval_.p, trs};
@@ -898,6 +938,7 @@ void Engine::run_chunk(const double* srcden, double* trgval, int rc, int mode, h
     a.M = op.M; a.Kseg = op.Kseg; a.nseg = op.nseg; a.Rc = rc; a.ngroup = op.ngroup; a.maxcols = op.maxcols;
     a.A = opsA[op.a_which]; a.a_gstride = op.a_gstride;
     a.B = bufs[op.b_buf]; a.bnode = I + op.bnode; a.ncol = op.ncol;
+    a.B2 = op.b2_buf >= 0 ? bufs[op.b2_buf] : nullptr; a.bscale_seg = op.bscale_seg;
     a.C = bufs[op.c_buf]; a.cnode = I + op.cnode; a.gbeg = I + op.gbeg;
     a.bscale = op.bscale >= 0 ? D + op.bscale : nullptr;
     a.cscale = op.cscale >= 0 ? D + op.cscale : nullptr;
@@ -1010,7 +1051,8 @@ void Engine::run_chunk(const double* srcden, double* trgval, int rc, int mode, h
   });
   mark(5);
   // down: L2L + DC2DE level by level
-  for (const GemmOp& op : P.down) timed(KIFMM3D_K_GEMM_DOWN, [&] { run_gemm(op); });
+  for (const GemmOp& op : mixed && !P.down_mixed.empty() ? P.down_mixed : P.down)
+    timed(KIFMM3D_K_GEMM_DOWN, [&] { run_gemm(op); });
   mark(6);
   if (side) HIPCHK(hipStreamWaitEvent(s, join_ev_, 0));
   // L2T + M2T

@@ -33,9 +33,12 @@ struct DVec {
 // A GEMM step of the up or down pass (see dev::GemmArgs).
 struct GemmOp {
   int M = 0, Kseg = 0, nseg = 1, ngroup = 1, maxcols = 0, ncol = 0;
-  int a_which = 0;  // 0 uc2ue, 1 ue2uc, 2 dc2de, 3 de2dc, 4 uc2ue x ue2uc (fused M2M + UC2UE)
+  int a_which = 0;  // 0 uc2ue, 1 ue2uc, 2 dc2de, 3 de2dc, 4 uc2ue x ue2uc (fused M2M + UC2UE),
+                    // 5 [dc2de | dc2de x de2dc_c] (fused L2L + DC2DE)
   long a_gstride = 0;
   int b_buf = 0, c_buf = 0;  // 0 upcval, 1 upeden, 2 dncval, 3 dneden
+  int b2_buf = -1;            // buffer of segments >= 1 (-1: b_buf)
+  int bscale_seg = 0;         // bscale applies to segments >= bscale_seg
   size_t bnode = 0, cnode = 0, gbeg = 0;
   long bscale = -1, cscale = -1;
   int accumulate = 0;
@@ -77,6 +80,11 @@ struct Plan {
   std::vector<GemmOp> up_mixed;
   size_t fuse_idx = 0;         // identity index 0 .. 8 plnUE - 1 (builds the fused operator)
   size_t fuse_idx_g = 0;       // its group bounds {0, 8 plnUE}
+  // Mixed form's down pass for plnDC == plnDE: a node's downward equivalent
+  // density from its own check values and its parent's density in one
+  // product, [DC2DE | DC2DE x DE2DC_c] (L2L into the check values skipped).
+  std::vector<GemmOp> down_mixed;
+  size_t fdn_b = 0, fdn_c = 0, fdn_g = 0;  // index arrays that build the fused operator
   std::vector<VLevel> vlev;
   int max_vsrc = 0, max_vtrg = 0;
   bool m2l_pairs = false;      // per-pair M2L kernel instead of the parent-grouped one
@@ -183,6 +191,7 @@ class Engine {
   DVec<double> unit_[4];
   DVec<double> uc2ue_, ue2uc_, dc2de_, de2dc_, m2l_;
   DVec<double> m2me_;  // 8 blocks of plnUE x plnUE: UC2UE x UE2UC_c (Dot2 on the device at setup)
+  DVec<double> l2de_;  // 8 blocks of plnDE x 2 plnDC: [DC2DE | DC2DE x DE2DC_c] (plnDC == plnDE)
   DVec<double> opsT_;  // M2L operators as [C][343][sdof][tdof] complex (FP64)
   DVec<float> opsT32_; // the same in FP32
   DVec<double> tw_;
