@@ -2482,7 +2482,7 @@ struct Mt<float> {
 };
 
 constexpr int M2LT_BS = 512;  // 8 waves: one per target parent of the tile
-constexpr int M2LT_FPB = 16;  // frequencies per block, visited in a software-pipelined loop
+constexpr int M2LT_FPB = 16;  // frequencies per block, visited in a software-pipelined loop (8: +1 %, 32: same, 63: +5 %)
 constexpr int NOPS = 343;     // operator table entries ((o+3) in [0,7)^3)
 constexpr int NAENT = 26 * 32;  // A-operand table entries: direction x (source k, target child c)
 // FP32 form: directions summed on the matrix cores in FP32 before their sum
@@ -2580,6 +2580,7 @@ __global__ __launch_bounds__(M2LT_BS, sizeof(R) == 4 ? 4 : 2) void k_m2l_tile(
     const int h = hc + (m / 9 - 1) * 16 + ((m / 3) % 3 - 1) * 4 + (m % 3 - 1);
     if (s_hm[h]) dmask |= 1u << n;
   }
+  dmask = __builtin_amdgcn_readfirstlane(dmask);  // uniform per wave (one parent): scalar branches
   for (int f = f0; f < f1; f++) {
     __syncthreads();  // the previous frequency's products are done with xs and s_A
 #pragma unroll
@@ -2596,10 +2597,7 @@ __global__ __launch_bounds__(M2LT_BS, sizeof(R) == 4 ? 4 : 2) void k_m2l_tile(
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
     V4 d0 = {0, 0, 0, 0}, d1 = {0, 0, 0, 0};  // two chains for matrix-core overlap
     int nacc = 0;
-    // fully unrolled (measured fastest in both precisions: 26 > 8 > 4 > 2)
-#pragma unroll
-    for (int n = 0; n < 26; n++) {
-      if (!(dmask >> n & 1u)) continue;
+    auto direction = [&](int n) {
       const int m = n + (n >= 13);
       const int h = hc + (m / 9 - 1) * 16 + ((m / 3) % 3 - 1) * 4 + (m % 3 - 1);
       const C2 m0 = s_A[n * 32 + aent][0], m1 = s_A[n * 32 + aent][1];
@@ -2620,6 +2618,39 @@ __global__ __launch_bounds__(M2LT_BS, sizeof(R) == 4 ? 4 : 2) void k_m2l_tile(
         }
         nacc = 0;
       }
+    };
+    // fully unrolled (measured fastest in both precisions: 26 > 8 > 4 > 2).
+    // A parent with all 26 neighbour directions takes a branch-free sequence
+    // whose LDS operand loads run one direction ahead of the matrix cores.
+    if (dmask == 0x3ffffffu) {
+      auto hof = [&](int n) {
+        const int m = n + (n >= 13);
+        return hc + (m / 9 - 1) * 16 + ((m / 3) % 3 - 1) * 4 + (m % 3 - 1);
+      };
+      C2 p0 = s_A[aent][0], p1 = s_A[aent][1];
+      C2 q0 = xs[(hof(0) * 8 + k) * 16 + i], q1 = xs[(hof(0) * 8 + 4 + k) * 16 + i];
+#pragma unroll
+      for (int n = 0; n < 26; n++) {
+        const C2 m0 = p0, m1 = p1, x0 = q0, x1 = q1;
+        if (n + 1 < 26) {
+          const int h = hof(n + 1);
+          p0 = s_A[(n + 1) * 32 + aent][0];
+          p1 = s_A[(n + 1) * 32 + aent][1];
+          q0 = xs[(h * 8 + k) * 16 + i];
+          q1 = xs[(h * 8 + 4 + k) * 16 + i];
+        }
+        const R a0 = pim ? m0.y : m0.x, a1 = pim ? m1.y : m1.x;
+        const R a2 = pim ? m0.x : -m0.y, a3 = pim ? m1.x : -m1.y;
+        d0 = Mt<R>::mfma(a0, x0.x, d0);
+        d1 = Mt<R>::mfma(a2, x0.y, d1);
+        d0 = Mt<R>::mfma(a1, x1.x, d0);
+        d1 = Mt<R>::mfma(a3, x1.y, d1);
+        asm volatile("" ::: "memory");  // keep the loads one direction ahead, no further
+      }
+    } else {
+#pragma unroll
+      for (int n = 0; n < 26; n++)
+        if (dmask >> n & 1u) direction(n);
     }
 #pragma unroll
     for (int v = 0; v < 4; v++) acc[v] += (double)d0[v] + (double)d1[v];
