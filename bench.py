@@ -250,8 +250,10 @@ def kernel_rates(f, c, stats, nrhs, mixed):
     work = {  # (algorithmic amount per step, unit, bound, peak)
         "p2p": (stats["near_pairs"] * PAIR_FLOPS[c["kt"]](nrhs), "flop", "mfma",
                 FP32_PEAK_TF if mfma32 else FP64_PEAK_TF),
-        "s2m": (stats["s2m_pairs"] * PAIR_FLOPS[c["kt"]](nrhs), "flop", "mfma", FP64_PEAK_TF),
-        "s2l": (stats["s2l_pairs"] * PAIR_FLOPS[c["kt"]](nrhs), "flop", "mfma", FP64_PEAK_TF),
+        "s2m": (stats["s2m_pairs"] * PAIR_FLOPS[c["kt"]](nrhs), "flop", "mfma",
+                FP32_PEAK_TF if mfma32 else FP64_PEAK_TF),
+        "s2l": (stats["s2l_pairs"] * PAIR_FLOPS[c["kt"]](nrhs), "flop", "mfma",
+                FP32_PEAK_TF if mfma32 else FP64_PEAK_TF),
         "s2p": (stats["far_pairs"] * PAIR_FLOPS[kt_mm](nrhs), "flop", "mfma", FP64_PEAK_TF),
         "m2l": (stats["v_pairs"] * nrhs * stats["C"] * 8.0 * sm * tm, "flop", "mfma",
                 FP32_PEAK_TF if mfma32 else FP64_PEAK_TF),

@@ -1111,6 +1111,84 @@ __global__ __launch_bounds__(P2P_BS) void k_p2s_mfma111(int nitem, int npc, cons
   }
 }
 
+// Mixed-precision S2M / S2L: the layout of k_p2s_mfma111 on the FP32 matrix
+// cores, coordinates relative to the item's box centre rounded to FP32 (the
+// surface points exactly, c + r u - c = r u), FP32 products of one source
+// tile summed in FP64.  Check surfaces never touch the item's sources, so
+// 1/r is taken of max(r^2, 1e-16) without the exclusion test.
+__global__ __launch_bounds__(P2P_BS) void k_p2s_mfma111_f32(int nitem, int npc, const int* __restrict__ out_node,
+                                                            const double4* __restrict__ geom,
+                                                            const int* __restrict__ eptr, const int2* __restrict__ ent,
+                                                            const double* __restrict__ unit, int npt, Points src,
+                                                            double* __restrict__ out, int Rc, int pln, int roff, KC kc,
+                                                            int accum)
+{
+  constexpr int NR = 16;
+  __shared__ float s_pos[3][TILE32];
+  __shared__ float s_den[TILE32][NR];
+  const int nb = nitem * npc, b = xcd_item(blockIdx.x, nb);
+  if (b >= nb) return;  // block-uniform
+  const int it = b / npc, pc = b - it * npc;
+  const int t = threadIdx.x, w = t >> 6, l = t & 63, col = l & 15, kk = l >> 4;
+  const int r0 = roff + blockIdx.y * NR;
+  const int pb = pc * P2P_BS, tn = min(P2P_BS, npt - pb);  // surface points of this block
+  const bool wact = 64 * w < tn;                           // wave-uniform
+  const int ng = wact ? min(4, (tn - 64 * w + 15) >> 4) : 0;
+  const double4 g = geom[it];
+  float tx[4], ty[4], tz[4];
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    int tt = 64 * w + 16 * i + col;
+    tt = pb + (tt < tn ? tt : 0);
+    tx[i] = (float)(g.w * unit[3 * tt]);
+    ty[i] = (float)(g.w * unit[3 * tt + 1]);
+    tz[i] = (float)(g.w * unit[3 * tt + 2]);
+  }
+  double acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int v = 0; v < 4; v++) acc[i][v] = 0.0;
+  const double* den = src.den + (long)r0 * src.den_rs;
+  for (int e = eptr[it]; e < eptr[it + 1]; e++) {
+    const int2 se = ent[e];
+    for (int b0 = 0; b0 < se.y; b0 += TILE32) {
+      const int cnt = min(TILE32, se.y - b0), cnt4 = (cnt + 3) & ~3;
+      const int j0 = se.x + b0;
+      __syncthreads();
+      for (int i = t; i < cnt4; i += P2P_BS) {
+        const bool in = i < cnt;
+        s_pos[0][i] = in ? (float)(src.x[j0 + i] - g.x) : 0.0f;
+        s_pos[1][i] = in ? (float)(src.y[j0 + i] - g.y) : 0.0f;
+        s_pos[2][i] = in ? (float)(src.z[j0 + i] - g.z) : 0.0f;
+      }
+      for (int i = t; i < cnt4 * NR; i += P2P_BS) {
+        const int jj = i / NR, q = i - jj * NR;
+        s_den[jj][q] = jj < cnt ? (float)den[(long)(j0 + jj) * src.den_js + (long)q * src.den_rs] : 0.0f;
+      }
+      __syncthreads();
+      if (ng == 4)
+        p2p_tile32<4, false>(cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc);
+      else if (ng == 3)
+        p2p_tile32<3, false>(cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc);
+      else if (ng == 2)
+        p2p_tile32<2, false>(cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc);
+      else if (ng == 1)
+        p2p_tile32<1, false>(cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc);
+    }
+  }
+  if (!wact) return;
+  const long node = out_node[it];
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const int tt = 64 * w + 16 * i + col;
+    if (tt >= tn) continue;
+    double* o = out + (node * Rc + r0 + 4 * kk) * pln + pb + tt;  // item v: right-hand side 4 kk + v
+#pragma unroll
+    for (int v = 0; v < 4; v++) o[(long)v * pln] = kc.a * acc[i][v] + (accum ? o[(long)v * pln] : 0.0);
+  }
+}
+
 template <int KT>
 static void p2s_kt(hipStream_t s, int nitem, const int* out_node, const double4* geom, const int* eptr,
                    const int2* ent, const double* unit, int npt, Points src, double* out, int Rc, int pln, int nr,
@@ -1118,7 +1196,11 @@ static void p2s_kt(hipStream_t s, int nitem, const int* out_node, const double4*
 {
   rhs_blocks(nr, [&](int NR, int nb, int roff) {
     dim3 grid(nitem, (npt + P2S_BS - 1) / P2S_BS, nb);
-    if (KT == 111 && NR == 16 && variant >= 2 && src.den_rs == 1) {
+    if (KT == 111 && NR == 16 && variant == 3) {
+      const int npc = (npt + P2P_BS - 1) / P2P_BS;
+      hipLaunchKernelGGL(k_p2s_mfma111_f32, dim3(xcd_grid(nitem * npc), nb), dim3(P2P_BS), 0, s, nitem, npc,
+                         out_node, geom, eptr, ent, unit, npt, src, out, Rc, pln, roff, kc, accum);
+    } else if (KT == 111 && NR == 16 && variant >= 2 && src.den_rs == 1) {
       const int npc = (npt + P2P_BS - 1) / P2P_BS;
       hipLaunchKernelGGL(k_p2s_mfma111, dim3(xcd_grid(nitem * npc), nb), dim3(P2P_BS), 0, s, nitem, npc, out_node,
                          geom, eptr, ent, unit, npt, src, out, Rc, pln, roff, kc, accum);

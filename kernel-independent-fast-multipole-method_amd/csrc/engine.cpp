@@ -104,6 +104,7 @@ Engine::Engine(int kt, const double* coefs, int ncoefs, int np, int ptsmax, int 
   }
   if (const char* gph = std::getenv("KIFMM3D_GRAPH")) use_graph_ = std::atoi(gph) != 0;
   if (const char* ov = std::getenv("KIFMM3D_OVERLAP")) overlap_ = std::atoi(ov) != 0;
+  if (const char* v = std::getenv("KIFMM3D_S2M32")) s2m32_ = std::atoi(v) != 0;
 }
 
 Engine::~Engine()
@@ -994,7 +995,7 @@ void Engine::run_chunk(const double* srcden, double* trgval, int rc, int mode, h
   timed(KIFMM3D_K_S2M, [&] {
     dev::p2s(s, user_.kt, P.s2m.n, I + P.s2m.out, (const double4*)(D + P.s2m.geom), I + P.s2m.eptr,
              (const int2*)(I + P.s2m.ent), unit_[SURF_UC].p, O.nsp[SURF_UC], src, upcval_.p, rc, pUC, rc, kcu,
-             std::min(p2p_variant_, 2), 0);  // FP64 in both forms: the check values feed the pseudoinverse
+             mixed && s2m32_ ? 3 : std::min(p2p_variant_, 2), 0);
   });
   for (const GemmOp& op : mixed ? P.up_mixed : P.up) timed(KIFMM3D_K_GEMM_UP, [&] { run_gemm(op); });
   mark(2);
@@ -1047,7 +1048,7 @@ void Engine::run_chunk(const double* srcden, double* trgval, int rc, int mode, h
   timed(KIFMM3D_K_S2L, [&] {
     dev::p2s(s, user_.kt, P.s2l.n, I + P.s2l.out, (const double4*)(D + P.s2l.geom), I + P.s2l.eptr,
              (const int2*)(I + P.s2l.ent), unit_[SURF_DC].p, O.nsp[SURF_DC], src, dncval_.p, rc, pDC, rc, kcu,
-             std::min(p2p_variant_, 2));
+             mixed && s2m32_ ? 3 : std::min(p2p_variant_, 2));
   });
   mark(5);
   // down: L2L + DC2DE level by level

@@ -173,6 +173,7 @@ class Engine {
   hipEvent_t fork_ev_ = nullptr, join_ev_ = nullptr;
   bool overlap_ = false;        // KIFMM3D_OVERLAP=1 (measured: -0.5 ms of 51 at 4M/16 RHS; off so that kernel times stay uncontended)
   bool use_graph_ = false;  // KIFMM3D_GRAPH
+  bool s2m32_ = true;       // mixed form: S2M / S2L on the FP32 matrix cores (KIFMM3D_S2M32=0: FP64)
   hipGraphExec_t gexec_ = nullptr;
   const double* gsrc_ = nullptr;
   double* gtrg_ = nullptr;
