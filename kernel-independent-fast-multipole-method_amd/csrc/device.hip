@@ -248,11 +248,12 @@ struct Knl<521> {
 // the point's contiguous destination row (a full line for 16 RHS), instead of
 // gathering 8 bytes from a different line per (point, RHS).
 __global__ void k_gather(int n, int dof, int nr, const int* __restrict__ iperm, const double* __restrict__ src,
-                         long src_rs, double* __restrict__ dst)
+                         long src_rs, double* __restrict__ dst, float* __restrict__ dst32)
 {
   const int per = nr * dof;
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-    double* out = dst + (long)iperm[i] * per;
+    double* out = dst ? dst + (long)iperm[i] * per : nullptr;
+    float* out32 = dst32 ? dst32 + (long)iperm[i] * per : nullptr;
     const double* in = src + i * dof;
     for (int e0 = 0; e0 < per; e0 += 8) {  // 8 loads in flight, then 8 stores
       double v[8];
@@ -261,13 +262,24 @@ __global__ void k_gather(int n, int dof, int nr, const int* __restrict__ iperm, 
         const int e = min(e0 + j, per - 1), r = e / dof;
         v[j] = in[r * src_rs + (e - r * dof)];
       }
-      if ((per & 1) == 0 && e0 + 8 <= per) {  // 16-byte stores (rows of even length are 16-byte aligned)
+      if (!out) {
+      } else if ((per & 1) == 0 && e0 + 8 <= per) {  // 16-byte stores (rows of even length are 16-byte aligned)
 #pragma unroll
         for (int j = 0; j < 8; j += 2) *(double2*)(out + e0 + j) = make_double2(v[j], v[j + 1]);
       } else {
 #pragma unroll
         for (int j = 0; j < 8; j++)
           if (e0 + j < per) out[e0 + j] = v[j];
+      }
+      if (out32) {
+        if ((per & 3) == 0 && e0 + 8 <= per) {  // rows of a multiple of 4 are 16-byte aligned
+          *(float4*)(out32 + e0) = make_float4((float)v[0], (float)v[1], (float)v[2], (float)v[3]);
+          *(float4*)(out32 + e0 + 4) = make_float4((float)v[4], (float)v[5], (float)v[6], (float)v[7]);
+        } else {
+#pragma unroll
+          for (int j = 0; j < 8; j++)
+            if (e0 + j < per) out32[e0 + j] = (float)v[j];
+        }
       }
     }
   }
@@ -366,10 +378,11 @@ void copy(hipStream_t s, const double* src, double* dst, long n)
   hipLaunchKernelGGL(k_copy, dim3(grid_for(n, 256)), dim3(256), 0, s, n, src, dst);
 }
 
-void gather(hipStream_t s, int n, int dof, int nr, const int* iperm, const double* src, long src_rs, double* dst)
+void gather(hipStream_t s, int n, int dof, int nr, const int* iperm, const double* src, long src_rs, double* dst,
+            float* dst32)
 {
   if ((long)n * dof * nr == 0) return;
-  hipLaunchKernelGGL(k_gather, dim3(grid_for(n, 256)), dim3(256), 0, s, n, dof, nr, iperm, src, src_rs, dst);
+  hipLaunchKernelGGL(k_gather, dim3(grid_for(n, 256)), dim3(256), 0, s, n, dof, nr, iperm, src, src_rs, dst, dst32);
 }
 
 void scatter(hipStream_t s, int n, int dof, int nr, const int* perm, const double* src, long src_rs, double* dst,
@@ -694,6 +707,7 @@ __global__ __launch_bounds__(P2P_BS) void k_p2p_mfma111_f32(int nchunk, const in
 #pragma unroll
     for (int v = 0; v < 4; v++) acc[i][v] = 0.0;
   const double* den = src.den + (long)r0 * src.den_rs;
+  const float* den32 = src.den32 ? src.den32 + (long)r0 * src.den_rs : nullptr;
   const int eself = selfe[c];
   for (int e = eptr[c]; e < eptr[c + 1]; e++) {
     const int2 se = ent[e];
@@ -710,7 +724,8 @@ __global__ __launch_bounds__(P2P_BS) void k_p2p_mfma111_f32(int nchunk, const in
       }
       for (int i = t; i < cnt4 * NR; i += P2P_BS) {
         const int jj = i / NR, q = i - jj * NR;
-        s_den[jj][q] = jj < cnt ? (float)den[(long)(j0 + jj) * src.den_js + (long)q * src.den_rs] : 0.0f;
+        const long at = (long)(j0 + jj) * src.den_js + (long)q * src.den_rs;
+        s_den[jj][q] = jj < cnt ? (den32 ? den32[at] : (float)den[at]) : 0.0f;
       }
       __syncthreads();
       if (chk) {
@@ -1150,6 +1165,7 @@ __global__ __launch_bounds__(P2P_BS) void k_p2s_mfma111_f32(int nitem, int npc, 
 #pragma unroll
     for (int v = 0; v < 4; v++) acc[i][v] = 0.0;
   const double* den = src.den + (long)r0 * src.den_rs;
+  const float* den32 = src.den32 ? src.den32 + (long)r0 * src.den_rs : nullptr;
   for (int e = eptr[it]; e < eptr[it + 1]; e++) {
     const int2 se = ent[e];
     for (int b0 = 0; b0 < se.y; b0 += TILE32) {
@@ -1164,7 +1180,8 @@ __global__ __launch_bounds__(P2P_BS) void k_p2s_mfma111_f32(int nitem, int npc, 
       }
       for (int i = t; i < cnt4 * NR; i += P2P_BS) {
         const int jj = i / NR, q = i - jj * NR;
-        s_den[jj][q] = jj < cnt ? (float)den[(long)(j0 + jj) * src.den_js + (long)q * src.den_rs] : 0.0f;
+        const long at = (long)(j0 + jj) * src.den_js + (long)q * src.den_rs;
+        s_den[jj][q] = jj < cnt ? (den32 ? den32[at] : (float)den[at]) : 0.0f;
       }
       __syncthreads();
       if (ng == 4)

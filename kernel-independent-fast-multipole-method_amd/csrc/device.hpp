@@ -17,7 +17,8 @@ void zero(hipStream_t s, double* p, long n);
 void copy(hipStream_t s, const double* src, double* dst, long n);
 // Source-major gather through the inverse permutation (iperm[perm[k]] = k):
 // dst[(iperm[i]*nr + r)*dof + d] = src[r*src_rs + i*dof + d] for i < n
-void gather(hipStream_t s, int n, int dof, int nr, const int* iperm, const double* src, long src_rs, double* dst);
+void gather(hipStream_t s, int n, int dof, int nr, const int* iperm, const double* src, long src_rs, double* dst,
+            float* dst32 = nullptr);  // dst32: also (dst null: only) an FP32 copy
 // The same through slot-major rows (tmp holds n * nr * dof doubles; iperm is
 // the inverse of perm; dof <= 3): a transpose, then one caller point per thread.
 void scatter_rows(hipStream_t s, int n, int dof, int nr, const int* iperm, const double* src, long src_rs,
@@ -34,6 +35,7 @@ struct Points {
   const double *x, *y, *z, *nx, *ny, *nz;
   const double* den;
   long den_js, den_rs;
+  const float* den32 = nullptr;  // FP32 copy of den (same layout), read by the FP32 matrix-core kernels
 };
 struct Targets {  // leaf-ordered targets (SoA) and values [rhs][n*tdof]
   const double *x, *y, *z;

@@ -754,13 +754,14 @@ int Engine::choose_rhs_chunk(int nrhs)
 {
   const Operators& O = *ops_;
   const Tree& T = tree_;
-  double per = 8.0 * ((double)T.sextcnt * user_.sdof + 2.0 * T.textcnt * user_.tdof +
+  double per = 8.0 * ((double)T.sextcnt * user_.sdof * 1.5 + 2.0 * T.textcnt * user_.tdof +
                       (double)T.sndecnt * (O.pln[SURF_UC] + O.pln[SURF_UE]) +
                       (double)T.tndecnt * (O.pln[SURF_DC] + O.pln[SURF_DE])) +
                16.0 * O.C * ((double)plan_.max_vsrc * mm_.sdof + (double)plan_.max_vtrg * mm_.tdof);
   size_t fr = 0, tot = 0;
   HIPCHK(hipMemGetInfo(&fr, &tot));
-  double budget = 0.6 * (double)fr + 8.0 * (double)(sden_.n + tval_.n + tvalT_.n + upcval_.n + upeden_.n + dncval_.n + dneden_.n) +
+  double budget = 0.6 * (double)fr + 4.0 * (double)sden32_.n +
+                  8.0 * (double)(sden_.n + tval_.n + tvalT_.n + upcval_.n + upeden_.n + dncval_.n + dneden_.n) +
                   8.0 * (double)(effden_.n + effval_.n);
   int rc = std::min((int)std::max(1.0, std::floor(budget / std::max(per, 1.0))), nrhs);
   if (rc >= 16) rc -= rc % 16;  // chunks of 16k right-hand sides take the matrix-core kernels
@@ -772,6 +773,7 @@ void Engine::ensure_work(int rc)
   const Operators& O = *ops_;
   const Tree& T = tree_;
   sden_.ensure((size_t)rc * T.sextcnt * user_.sdof);
+  if (user_.kt == 111) sden32_.ensure((size_t)rc * T.sextcnt);
   tval_.ensure((size_t)rc * T.textcnt * user_.tdof);
   tvalT_.ensure((size_t)rc * T.textcnt * user_.tdof);
   upcval_.ensure((size_t)rc * T.sndecnt * O.pln[SURF_UC]);
@@ -932,6 +934,11 @@ void Engine::run_chunk(const double* srcden, double* trgval, int rc, int mode, h
   const double* opsA[6] = {uc2ue_.p, ue2uc_.p, dc2de_.p, de2dc_.p, m2me_.p, l2de_.p};
   const long trs = (long)T.textcnt * tdof;
   dev::Points src{sx_.p, sy_.p, sz_.p, snx_.p, sny_.p, snz_.p, sden_.p, (long)rc * sdof, sdof};
+  // the mixed form's Laplace kernels stage FP32 densities (half the bytes, no conversions);
+  // with every RHS block of 16 no kernel of the pass reads the FP64 copy
+  float* den32 = mixed && user_.kt == 111 ? sden32_.p : nullptr;
+  src.den32 = den32;
+  double* den64 = den32 && rc % 16 == 0 && s2m32_ ? nullptr : sden_.p;
   dev::Targets trg{tx_.p, ty_.p, tz_.p, tval_.p, trs};
 
   auto run_gemm = [&](const GemmOp& op) {
@@ -989,7 +996,8 @@ void Engine::run_chunk(const double* srcden, double* trgval, int rc, int mode, h
   if (P.nlow_s) dev::zero(s, upeden_.p, (long)rc * P.nlow_s * pUE);
   if (P.nlow_t) dev::zero(s, dneden_.p, (long)rc * P.nlow_t * pDE);
   // load (fmm3d_eval.cpp:50-64)
-  timed(KIFMM3D_K_GATHER, [&] { dev::gather(s, T.sextcnt, sdof, rc, siperm_.p, srcden, (long)ns_ * sdof, sden_.p); });
+  timed(KIFMM3D_K_GATHER,
+        [&] { dev::gather(s, T.sextcnt, sdof, rc, siperm_.p, srcden, (long)ns_ * sdof, den64, den32); });
   mark(1);
   // up: S2M, then M2M + UC2UE level by level
   timed(KIFMM3D_K_S2M, [&] {

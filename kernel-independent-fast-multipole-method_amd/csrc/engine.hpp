@@ -188,6 +188,7 @@ class Engine {
   DVec<int> siperm_;  // inverse of the source leaf permutation: caller order -> leaf order
   DVec<int> tiperm_;  // inverse of the target leaf permutation
   DVec<double> tvalT_;  // target values as slot-major rows, on their way out
+  DVec<float> sden32_;  // FP32 copy of the gathered densities (mixed form, Laplace)
   DVec<short> c2s_;
   DVec<double> unit_[4];
   DVec<double> uc2ue_, ue2uc_, dc2de_, de2dc_, m2l_;
