@@ -679,7 +679,7 @@ __global__ __launch_bounds__(P2P_BS) void k_p2p_mfma111_f32(int nchunk, const in
                                                             const int* __restrict__ tnum, const int* __restrict__ eptr,
                                                             const int2* __restrict__ ent,
                                                             const int* __restrict__ selfe, Points src, Targets trg,
-                                                            int roff, KC kc)
+                                                            int roff, KC kc, int nrv)
 {
   constexpr int NR = 16;
   __shared__ float s_pos[3][TILE32];
@@ -725,7 +725,7 @@ __global__ __launch_bounds__(P2P_BS) void k_p2p_mfma111_f32(int nchunk, const in
       for (int i = t; i < cnt4 * NR; i += P2P_BS) {
         const int jj = i / NR, q = i - jj * NR;
         const long at = (long)(j0 + jj) * src.den_js + (long)q * src.den_rs;
-        s_den[jj][q] = jj < cnt ? (den32 ? den32[at] : (float)den[at]) : 0.0f;
+        s_den[jj][q] = jj < cnt && q < nrv ? (den32 ? den32[at] : (float)den[at]) : 0.0f;
       }
       __syncthreads();
       if (chk) {
@@ -756,9 +756,20 @@ __global__ __launch_bounds__(P2P_BS) void k_p2p_mfma111_f32(int nchunk, const in
     if (tt < tn) {
       double* val = trg.val + (long)(r0 + 4 * kk) * trg.val_rs + tb + tt;
 #pragma unroll
-      for (int v = 0; v < 4; v++) val[(long)v * trg.val_rs] += kc.a * acc[i][v];
+      for (int v = 0; v < 4; v++)
+        if (4 * kk + v < nrv) val[(long)v * trg.val_rs] += kc.a * acc[i][v];
     }
   }
+}
+
+// Launches of the FP32 matrix-core kernels: nr / 16 full blocks of 16
+// right-hand sides, then the remainder as one block of 16 of which nrv are
+// real (zero densities staged, nothing stored for the rest); fn(nb, roff, nrv).
+template <class F>
+static void rhs_blocks16(int nr, F&& fn)
+{
+  if (nr >= 16) fn(nr / 16, 0, 16);
+  if (nr % 16) fn(1, nr - nr % 16, nr % 16);
 }
 
 // Split nr right-hand sides into launches of NR in {16, 4, 1}; fn(NR, nblocks, roff).
@@ -1011,10 +1022,7 @@ template <int KT, int NR>
 static void p2p_launch(hipStream_t s, int variant, int nchunk, int nb, const int* tbeg, const int* tnum,
                        const int* eptr, const int2* ent, const int* selfe, Points src, Targets trg, int roff, KC kc)
 {
-  if (variant == 3 && KT == 111 && NR == 16 && selfe)
-    hipLaunchKernelGGL(k_p2p_mfma111_f32, dim3(xcd_grid(nchunk), nb), dim3(P2P_BS), 0, s, nchunk, tbeg, tnum, eptr,
-                       ent, selfe, src, trg, roff, kc);
-  else if (variant >= 2 && KT == 111 && NR == 16 && src.den_rs == 1)
+  if (variant >= 2 && KT == 111 && NR == 16 && src.den_rs == 1)
     hipLaunchKernelGGL(k_p2p_mfma111, dim3(xcd_grid(nchunk), nb), dim3(P2P_BS), 0, s, nchunk, tbeg, tnum, eptr, ent,
                        src, trg, roff, kc);
   else if (variant == 1)
@@ -1029,6 +1037,13 @@ template <int KT>
 static void p2p_kt(hipStream_t s, int nchunk, const int* tbeg, const int* tnum, const int* eptr, const int2* ent,
                    const int* selfe, Points src, Targets trg, int nr, KC kc, int variant)
 {
+  if (KT == 111 && variant == 3 && selfe) {
+    rhs_blocks16(nr, [&](int nb, int roff, int nrv) {
+      hipLaunchKernelGGL(k_p2p_mfma111_f32, dim3(xcd_grid(nchunk), nb), dim3(P2P_BS), 0, s, nchunk, tbeg, tnum, eptr,
+                         ent, selfe, src, trg, roff, kc, nrv);
+    });
+    return;
+  }
   rhs_blocks(nr, [&](int NR, int nb, int roff) {
     if (NR == 16)
       p2p_launch<KT, 16>(s, variant, nchunk, nb, tbeg, tnum, eptr, ent, selfe, src, trg, roff, kc);
@@ -1136,7 +1151,7 @@ __global__ __launch_bounds__(P2P_BS) void k_p2s_mfma111_f32(int nitem, int npc, 
                                                             const int* __restrict__ eptr, const int2* __restrict__ ent,
                                                             const double* __restrict__ unit, int npt, Points src,
                                                             double* __restrict__ out, int Rc, int pln, int roff, KC kc,
-                                                            int accum)
+                                                            int accum, int nrv)
 {
   constexpr int NR = 16;
   __shared__ float s_pos[3][TILE32];
@@ -1181,7 +1196,7 @@ __global__ __launch_bounds__(P2P_BS) void k_p2s_mfma111_f32(int nitem, int npc, 
       for (int i = t; i < cnt4 * NR; i += P2P_BS) {
         const int jj = i / NR, q = i - jj * NR;
         const long at = (long)(j0 + jj) * src.den_js + (long)q * src.den_rs;
-        s_den[jj][q] = jj < cnt ? (den32 ? den32[at] : (float)den[at]) : 0.0f;
+        s_den[jj][q] = jj < cnt && q < nrv ? (den32 ? den32[at] : (float)den[at]) : 0.0f;
       }
       __syncthreads();
       if (ng == 4)
@@ -1202,7 +1217,8 @@ __global__ __launch_bounds__(P2P_BS) void k_p2s_mfma111_f32(int nitem, int npc, 
     if (tt >= tn) continue;
     double* o = out + (node * Rc + r0 + 4 * kk) * pln + pb + tt;  // item v: right-hand side 4 kk + v
 #pragma unroll
-    for (int v = 0; v < 4; v++) o[(long)v * pln] = kc.a * acc[i][v] + (accum ? o[(long)v * pln] : 0.0);
+    for (int v = 0; v < 4; v++)
+      if (4 * kk + v < nrv) o[(long)v * pln] = kc.a * acc[i][v] + (accum ? o[(long)v * pln] : 0.0);
   }
 }
 
@@ -1211,13 +1227,17 @@ static void p2s_kt(hipStream_t s, int nitem, const int* out_node, const double4*
                    const int2* ent, const double* unit, int npt, Points src, double* out, int Rc, int pln, int nr,
                    KC kc, int variant, int accum)
 {
+  if (KT == 111 && variant == 3) {
+    const int npc = (npt + P2P_BS - 1) / P2P_BS;
+    rhs_blocks16(nr, [&](int nb, int roff, int nrv) {
+      hipLaunchKernelGGL(k_p2s_mfma111_f32, dim3(xcd_grid(nitem * npc), nb), dim3(P2P_BS), 0, s, nitem, npc,
+                         out_node, geom, eptr, ent, unit, npt, src, out, Rc, pln, roff, kc, accum, nrv);
+    });
+    return;
+  }
   rhs_blocks(nr, [&](int NR, int nb, int roff) {
     dim3 grid(nitem, (npt + P2S_BS - 1) / P2S_BS, nb);
-    if (KT == 111 && NR == 16 && variant == 3) {
-      const int npc = (npt + P2P_BS - 1) / P2P_BS;
-      hipLaunchKernelGGL(k_p2s_mfma111_f32, dim3(xcd_grid(nitem * npc), nb), dim3(P2P_BS), 0, s, nitem, npc,
-                         out_node, geom, eptr, ent, unit, npt, src, out, Rc, pln, roff, kc, accum);
-    } else if (KT == 111 && NR == 16 && variant >= 2 && src.den_rs == 1) {
+    if (KT == 111 && NR == 16 && variant >= 2 && src.den_rs == 1) {
       const int npc = (npt + P2P_BS - 1) / P2P_BS;
       hipLaunchKernelGGL(k_p2s_mfma111, dim3(xcd_grid(nitem * npc), nb), dim3(P2P_BS), 0, s, nitem, npc, out_node,
                          geom, eptr, ent, unit, npt, src, out, Rc, pln, roff, kc, accum);
@@ -1260,7 +1280,7 @@ __global__ __launch_bounds__(P2P_BS) void k_s2p_mfma111(int nchunk, const int* _
                                                         const double* __restrict__ unit_ue, int nsp,
                                                         const double* __restrict__ dneden,
                                                         const double* __restrict__ upeden, int Rc, int pln,
-                                                        Targets trg, int roff, KC kc)
+                                                        Targets trg, int roff, KC kc, int nrv)
 {
   typedef typename std::conditional<F32, float, double>::type R;
   constexpr int NR = 16, TL = F32 ? TILE32 : TILE;
@@ -1309,7 +1329,7 @@ __global__ __launch_bounds__(P2P_BS) void k_s2p_mfma111(int nchunk, const int* _
       }
       for (int i = t; i < cnt4 * NR; i += P2P_BS) {
         const int q = i / cnt4, jj = i - q * cnt4;
-        s_den[jj][q] = jj < cnt ? (R)buf[(long)q * pln + b0 + jj] : (R)0;
+        s_den[jj][q] = jj < cnt && q < nrv ? (R)buf[(long)q * pln + b0 + jj] : (R)0;
       }
       __syncthreads();
       if constexpr (F32) {
@@ -1349,11 +1369,13 @@ __global__ __launch_bounds__(P2P_BS) void k_s2p_mfma111(int nchunk, const int* _
     if constexpr (F32) {  // FP32 item v of lane l: right-hand side 4*(l/16) + v
       double* val = trg.val + (long)(r0 + 4 * kk) * trg.val_rs + tb + tt;
 #pragma unroll
-      for (int v = 0; v < 4; v++) val[(long)v * trg.val_rs] += kc.a * acc64[i][v];
+      for (int v = 0; v < 4; v++)
+        if (4 * kk + v < nrv) val[(long)v * trg.val_rs] += kc.a * acc64[i][v];
     } else {  // FP64 item v of lane l: right-hand side l/16 + 4v
       double* val = trg.val + (long)(r0 + kk) * trg.val_rs + tb + tt;
 #pragma unroll
-      for (int v = 0; v < 4; v++) val[(long)(4 * v) * trg.val_rs] += kc.a * accm[i][v];
+      for (int v = 0; v < 4; v++)
+        if (kk + 4 * v < nrv) val[(long)(4 * v) * trg.val_rs] += kc.a * accm[i][v];
     }
   }
 }
@@ -1364,14 +1386,18 @@ static void s2p_kt(hipStream_t s, int nchunk, const int* tbeg, const int* tnum, 
                    const double* dneden, const double* upeden, int Rc, int pln, Targets trg, int nr, KC kc,
                    int variant)
 {
+  if (KT == 111 && variant == 3) {
+    rhs_blocks16(nr, [&](int nb, int roff, int nrv) {
+      hipLaunchKernelGGL((k_s2p_mfma111<true>), dim3(xcd_grid(nchunk), nb), dim3(P2P_BS), 0, s, nchunk, tbeg, tnum,
+                         eptr, enode, ekind, egeom, unit_de, unit_ue, nsp, dneden, upeden, Rc, pln, trg, roff, kc, nrv);
+    });
+    return;
+  }
   rhs_blocks(nr, [&](int NR, int nb, int roff) {
     dim3 grid(nchunk, nb);
-    if (KT == 111 && NR == 16 && variant == 3)
-      hipLaunchKernelGGL((k_s2p_mfma111<true>), dim3(xcd_grid(nchunk), nb), dim3(P2P_BS), 0, s, nchunk, tbeg, tnum,
-                         eptr, enode, ekind, egeom, unit_de, unit_ue, nsp, dneden, upeden, Rc, pln, trg, roff, kc);
-    else if (KT == 111 && NR == 16 && variant == 2)
+    if (KT == 111 && NR == 16 && variant == 2)
       hipLaunchKernelGGL((k_s2p_mfma111<false>), dim3(xcd_grid(nchunk), nb), dim3(P2P_BS), 0, s, nchunk, tbeg, tnum,
-                         eptr, enode, ekind, egeom, unit_de, unit_ue, nsp, dneden, upeden, Rc, pln, trg, roff, kc);
+                         eptr, enode, ekind, egeom, unit_de, unit_ue, nsp, dneden, upeden, Rc, pln, trg, roff, kc, 16);
     else if (NR == 16)
       hipLaunchKernelGGL((k_s2p<KT, 16>), grid, dim3(P2P_BS), 0, s, tbeg, tnum, eptr, enode, ekind, egeom, unit_de,
                          unit_ue, nsp, dneden, upeden, Rc, pln, trg, roff, kc);
