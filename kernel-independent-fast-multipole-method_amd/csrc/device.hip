@@ -763,13 +763,17 @@ __global__ __launch_bounds__(P2P_BS) void k_p2p_mfma111_f32(int nchunk, const in
 }
 
 // Launches of the FP32 matrix-core kernels: nr / 16 full blocks of 16
-// right-hand sides, then the remainder as one block of 16 of which nrv are
-// real (zero densities staged, nothing stored for the rest); fn(nb, roff, nrv).
+// right-hand sides, then a remainder of at least PAD16_MIN as one block of 16
+// of which nrv are real (zero densities staged, nothing stored for the rest);
+// fn(nb, roff, nrv).  Returns the right-hand sides left for the vector
+// kernels (a remainder below PAD16_MIN: measured faster there at 1 RHS).
 template <class F>
-static void rhs_blocks16(int nr, F&& fn)
+static int rhs_blocks16(int nr, F&& fn)
 {
   if (nr >= 16) fn(nr / 16, 0, 16);
-  if (nr % 16) fn(1, nr - nr % 16, nr % 16);
+  const int rem = nr % 16;
+  if (rem >= PAD16_MIN) fn(1, nr - rem, rem);
+  return rem >= PAD16_MIN ? 0 : rem;
 }
 
 // Split nr right-hand sides into launches of NR in {16, 4, 1}; fn(NR, nblocks, roff).
@@ -1037,20 +1041,21 @@ template <int KT>
 static void p2p_kt(hipStream_t s, int nchunk, const int* tbeg, const int* tnum, const int* eptr, const int2* ent,
                    const int* selfe, Points src, Targets trg, int nr, KC kc, int variant)
 {
+  int r0 = 0;
   if (KT == 111 && variant == 3 && selfe) {
-    rhs_blocks16(nr, [&](int nb, int roff, int nrv) {
+    const int left = rhs_blocks16(nr, [&](int nb, int roff, int nrv) {
       hipLaunchKernelGGL(k_p2p_mfma111_f32, dim3(xcd_grid(nchunk), nb), dim3(P2P_BS), 0, s, nchunk, tbeg, tnum, eptr,
                          ent, selfe, src, trg, roff, kc, nrv);
     });
-    return;
+    r0 = nr - left;
   }
-  rhs_blocks(nr, [&](int NR, int nb, int roff) {
+  rhs_blocks(nr - r0, [&](int NR, int nb, int roff) {
     if (NR == 16)
-      p2p_launch<KT, 16>(s, variant, nchunk, nb, tbeg, tnum, eptr, ent, selfe, src, trg, roff, kc);
+      p2p_launch<KT, 16>(s, variant, nchunk, nb, tbeg, tnum, eptr, ent, selfe, src, trg, r0 + roff, kc);
     else if (NR == 4)
-      p2p_launch<KT, 4>(s, variant, nchunk, nb, tbeg, tnum, eptr, ent, selfe, src, trg, roff, kc);
+      p2p_launch<KT, 4>(s, variant, nchunk, nb, tbeg, tnum, eptr, ent, selfe, src, trg, r0 + roff, kc);
     else
-      p2p_launch<KT, 1>(s, variant, nchunk, nb, tbeg, tnum, eptr, ent, selfe, src, trg, roff, kc);
+      p2p_launch<KT, 1>(s, variant, nchunk, nb, tbeg, tnum, eptr, ent, selfe, src, trg, r0 + roff, kc);
   });
 }
 
@@ -1227,15 +1232,16 @@ static void p2s_kt(hipStream_t s, int nitem, const int* out_node, const double4*
                    const int2* ent, const double* unit, int npt, Points src, double* out, int Rc, int pln, int nr,
                    KC kc, int variant, int accum)
 {
+  int r0 = 0;
   if (KT == 111 && variant == 3) {
     const int npc = (npt + P2P_BS - 1) / P2P_BS;
-    rhs_blocks16(nr, [&](int nb, int roff, int nrv) {
+    r0 = nr - rhs_blocks16(nr, [&](int nb, int roff, int nrv) {
       hipLaunchKernelGGL(k_p2s_mfma111_f32, dim3(xcd_grid(nitem * npc), nb), dim3(P2P_BS), 0, s, nitem, npc,
                          out_node, geom, eptr, ent, unit, npt, src, out, Rc, pln, roff, kc, accum, nrv);
     });
-    return;
   }
-  rhs_blocks(nr, [&](int NR, int nb, int roff) {
+  rhs_blocks(nr - r0, [&](int NR, int nb, int roff0) {
+    const int roff = r0 + roff0;
     dim3 grid(nitem, (npt + P2S_BS - 1) / P2S_BS, nb);
     if (KT == 111 && NR == 16 && variant >= 2 && src.den_rs == 1) {
       const int npc = (npt + P2P_BS - 1) / P2P_BS;
@@ -1386,14 +1392,15 @@ static void s2p_kt(hipStream_t s, int nchunk, const int* tbeg, const int* tnum, 
                    const double* dneden, const double* upeden, int Rc, int pln, Targets trg, int nr, KC kc,
                    int variant)
 {
+  int r0 = 0;
   if (KT == 111 && variant == 3) {
-    rhs_blocks16(nr, [&](int nb, int roff, int nrv) {
+    r0 = nr - rhs_blocks16(nr, [&](int nb, int roff, int nrv) {
       hipLaunchKernelGGL((k_s2p_mfma111<true>), dim3(xcd_grid(nchunk), nb), dim3(P2P_BS), 0, s, nchunk, tbeg, tnum,
                          eptr, enode, ekind, egeom, unit_de, unit_ue, nsp, dneden, upeden, Rc, pln, trg, roff, kc, nrv);
     });
-    return;
   }
-  rhs_blocks(nr, [&](int NR, int nb, int roff) {
+  rhs_blocks(nr - r0, [&](int NR, int nb, int roff0) {
+    const int roff = r0 + roff0;
     dim3 grid(nchunk, nb);
     if (KT == 111 && NR == 16 && variant == 2)
       hipLaunchKernelGGL((k_s2p_mfma111<false>), dim3(xcd_grid(nchunk), nb), dim3(P2P_BS), 0, s, nchunk, tbeg, tnum,

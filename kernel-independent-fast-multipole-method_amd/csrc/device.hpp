@@ -31,6 +31,11 @@ void scatter(hipStream_t s, int n, int dof, int nr, const int* perm, const doubl
 // right-hand side r is den[j*den_js + r*den_rs + d].  The evaluate pass keeps
 // them source-major (den_js = Rc*sdof, den_rs = sdof) so that the densities of
 // one source for a block of right-hand sides are contiguous.
+// The FP32 matrix-core Laplace kernels run a last block of fewer than 16
+// right-hand sides zero-padded to 16 when it has at least this many; smaller
+// remainders take the FP64 vector kernels.
+constexpr int PAD16_MIN = 4;
+
 struct Points {
   const double *x, *y, *z, *nx, *ny, *nz;
   const double* den;

@@ -935,10 +935,11 @@ void Engine::run_chunk(const double* srcden, double* trgval, int rc, int mode, h
   const long trs = (long)T.textcnt * tdof;
   dev::Points src{sx_.p, sy_.p, sz_.p, snx_.p, sny_.p, snz_.p, sden_.p, (long)rc * sdof, sdof};
   // the mixed form's Laplace kernels stage FP32 densities (half the bytes, no conversions);
-  // with S2M / S2L in FP32 too no kernel of the pass reads the FP64 copy
+  // with S2M / S2L in FP32 too and every RHS block on those kernels (a last
+  // block of 1-3 RHS takes the FP64 vector kernels) nothing reads the FP64 copy
   float* den32 = mixed && user_.kt == 111 ? sden32_.p : nullptr;
   src.den32 = den32;
-  double* den64 = den32 && s2m32_ ? nullptr : sden_.p;
+  double* den64 = den32 && s2m32_ && (rc % 16 == 0 || rc % 16 >= dev::PAD16_MIN) ? nullptr : sden_.p;
   dev::Targets trg{tx_.p, ty_.p, tz_.p, tval_.p, trs};
 
   auto run_gemm = [&](const GemmOp& op) {
