@@ -4,9 +4,11 @@ The mixed form has no counterpart in the reference: it is bounded against the
 plain FP64 form on the same tree (the FP32 products must not move the result
 by more than MIXED_VS_PLAIN relative L2) and against the FP64 direct sum (the
 FMM error band of np).  Blocks of 16 right-hand sides take the FP32
-matrix-core near field and tiled M2L; other right-hand-side counts and the
-vector kernels take the FP32 parent-grouped matrix-core M2L with FP64 near
-fields.
+matrix-core near field and tiled M2L; a remainder of 4 to 15 right-hand sides
+runs zero-padded on the same FP32 matrix-core kernels (near field, S2M/S2L,
+L2T/M2T) with the parent-grouped M2L, and one of 1 to 3 on the FP64 vector
+kernels; the vector kernels take the FP32 parent-grouped matrix-core M2L with
+FP64 near fields.
 """
 import numpy as np
 import pytest
@@ -57,6 +59,15 @@ def test_mixed_laplace_16rhs_np6():
 def test_mixed_laplace_21rhs_adaptive():
     p = clusters(8000, 22)
     run(111, 4, 60, p, None, (0.5, 0.5, 0.5), 1, 21, 22)
+
+
+@pytest.mark.parametrize("nrhs", [1, 3, 8, 35])
+def test_mixed_laplace_rhs_remainders(nrhs):
+    """Right-hand-side counts that leave no full block (1, 3, 8) or a short
+    last chunk (35 = 16 + 16 + 3): padded FP32 matrix-core blocks and the FP64
+    vector-kernel fallback, whose densities come from the FP64 copy."""
+    p = clusters(6000, 30 + nrhs)
+    run(111, 4, 50, p, None, (0.5, 0.5, 0.5), 1, nrhs, 30 + nrhs)
 
 
 def test_mixed_stokes_dl_fp32_m2l():
