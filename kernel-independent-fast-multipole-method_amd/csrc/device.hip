@@ -2655,11 +2655,15 @@ __global__ __launch_bounds__(M2LT_BS, sizeof(R) == 4 ? 4 : 2) void k_m2l_tile(
     s_hm[t] = m;
   }
   // staging: thread t moves entries e = it*512 + t, i.e. halo child row
-  // it*32 + t/16 and right-hand side t%16, and A-table entries t and t + 512
-  int hsl[16];
+  // it*32 + t/16 and right-hand side t%16 (FP64), or the 16-byte pair of
+  // right-hand sides 2(t%8), 2(t%8)+1 of halo child row it*64 + t/8 (FP32:
+  // half the loads, stores and slot registers), and A-table entries t and t + 512
+  constexpr bool P2 = sizeof(R) == 4;
+  constexpr int NIT = P2 ? 8 : 16;
+  int hsl[NIT];
 #pragma unroll
-  for (int it = 0; it < 16; it++) hsl[it] = hs[it * 32 + (t >> 4)];
-  const C2* eb = effden + r0 + (t & 15);
+  for (int it = 0; it < NIT; it++) hsl[it] = P2 ? hs[it * 64 + (t >> 3)] : hs[it * 32 + (t >> 4)];
+  const C2* eb = effden + r0 + (P2 ? 2 * (t & 7) : (t & 15));
   int aop[2][2];
 #pragma unroll
   for (int j = 0; j < 2; j++) {
@@ -2667,13 +2671,22 @@ __global__ __launch_bounds__(M2LT_BS, sizeof(R) == 4 ? 4 : 2) void k_m2l_tile(
     aop[j][0] = e < NAENT ? optab[(n * 8 + k) * 8 + c] : -1;
     aop[j][1] = e < NAENT ? optab[(n * 8 + 4 + k) * 8 + c] : -1;
   }
-  C2 pre[16], pa[2][2];
+  C2 pre[16], pa[2][2];  // FP32: pre[2 it], pre[2 it + 1] hold the pair of iteration it
   auto fetch = [&](int f) {
 #pragma unroll
-    for (int it = 0; it < 16; it++) {
-      pre[it].x = 0;
-      pre[it].y = 0;
-      if (hsl[it] >= 0) pre[it] = eb[((long)hsl[it] * C + f) * Rc];
+    for (int it = 0; it < NIT; it++) {
+      if constexpr (P2) {
+        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (hsl[it] >= 0) v = *(const float4*)(eb + ((long)hsl[it] * C + f) * Rc);
+        pre[2 * it].x = v.x;
+        pre[2 * it].y = v.y;
+        pre[2 * it + 1].x = v.z;
+        pre[2 * it + 1].y = v.w;
+      } else {
+        pre[it].x = 0;
+        pre[it].y = 0;
+        if (hsl[it] >= 0) pre[it] = eb[((long)hsl[it] * C + f) * Rc];
+      }
     }
 #pragma unroll
     for (int j = 0; j < 2; j++)
@@ -2716,7 +2729,12 @@ __global__ __launch_bounds__(M2LT_BS, sizeof(R) == 4 ? 4 : 2) void k_m2l_tile(
   for (int f = f0; f < f1; f++) {
     __syncthreads();  // the previous frequency's products are done with xs and s_A
 #pragma unroll
-    for (int it = 0; it < 16; it++) xs[it * M2LT_BS + t] = pre[it];
+    for (int it = 0; it < NIT; it++) {
+      if constexpr (P2)
+        *(float4*)&xs[it * 1024 + 2 * t] = make_float4(pre[2 * it].x, pre[2 * it].y, pre[2 * it + 1].x, pre[2 * it + 1].y);
+      else
+        xs[it * M2LT_BS + t] = pre[it];
+    }
 #pragma unroll
     for (int j = 0; j < 2; j++)
       if (t + j * M2LT_BS < NAENT) {
