@@ -2624,16 +2624,17 @@ constexpr int NAENT = 26 * 32;  // A-operand table entries: direction x (source 
 #endif
 
 // The A operands depend only on (direction, frequency, lane), not on the
-// target parent, so the block builds them once per frequency: entry
+// target parent, so the block stages them once per frequency: entry
 // n*32 + k*8 + c holds (T_{optab[n][k][c]}, T_{optab[n][4+k][c]}) (zero where
-// the child pair is not in a V list) and every wave reads its lane's entry
-// with one LDS load.  (FP32: two blocks per CU fit the LDS, so registers are
+// the child pair is not in a V list), read contiguously from a table expanded
+// at setup ([C][NAENT][2]), and every wave reads its lane's entry with one LDS
+// load.  (FP32: two blocks per CU fit the LDS, so registers are
 // capped to match: the second launch bound is waves per SIMD.)
 template <class R>
 __global__ __launch_bounds__(M2LT_BS, sizeof(R) == 4 ? 4 : 2) void k_m2l_tile(
     int C, int Rc, int ntile, const int* __restrict__ ttile, const int* __restrict__ htile,
-    const int* __restrict__ optab, const typename Mt<R>::C2* __restrict__ opsT,
-    const typename Mt<R>::C2* __restrict__ effden, typename Mt<R>::C2* __restrict__ effval, int roff)
+    const typename Mt<R>::C2* __restrict__ atab, const typename Mt<R>::C2* __restrict__ effden,
+    typename Mt<R>::C2* __restrict__ effval, int roff)
 {
   using C2 = typename Mt<R>::C2;
   using V4 = typename Mt<R>::V4;
@@ -2664,14 +2665,8 @@ __global__ __launch_bounds__(M2LT_BS, sizeof(R) == 4 ? 4 : 2) void k_m2l_tile(
 #pragma unroll
   for (int it = 0; it < NIT; it++) hsl[it] = P2 ? hs[it * 64 + (t >> 3)] : hs[it * 32 + (t >> 4)];
   const C2* eb = effden + r0 + (P2 ? 2 * (t & 7) : (t & 15));
-  int aop[2][2];
-#pragma unroll
-  for (int j = 0; j < 2; j++) {
-    const int e = t + j * M2LT_BS, n = e >> 5, k = (e >> 3) & 3, c = e & 7;
-    aop[j][0] = e < NAENT ? optab[(n * 8 + k) * 8 + c] : -1;
-    aop[j][1] = e < NAENT ? optab[(n * 8 + 4 + k) * 8 + c] : -1;
-  }
-  C2 pre[16], pa[2][2];  // FP32: pre[2 it], pre[2 it + 1] hold the pair of iteration it
+  C2 pre[16];  // FP32: pre[2 it], pre[2 it + 1] hold the pair of iteration it
+  C2 pa00, pa01, pa10, pa11;  // A-table entries t and t + 512 (named: no local array)
   auto fetch = [&](int f) {
 #pragma unroll
     for (int it = 0; it < NIT; it++) {
@@ -2688,14 +2683,14 @@ __global__ __launch_bounds__(M2LT_BS, sizeof(R) == 4 ? 4 : 2) void k_m2l_tile(
         if (hsl[it] >= 0) pre[it] = eb[((long)hsl[it] * C + f) * Rc];
       }
     }
-#pragma unroll
-    for (int j = 0; j < 2; j++)
-#pragma unroll
-      for (int q = 0; q < 2; q++) {
-        pa[j][q].x = 0;
-        pa[j][q].y = 0;
-        if (aop[j][q] >= 0) pa[j][q] = opsT[(long)f * NOPS + aop[j][q]];
-      }
+    // A table of frequency f: a contiguous [NAENT][2] block of the expanded table
+    const C2* at = atab + (long)f * NAENT * 2;
+    pa00 = at[2 * t];
+    pa01 = at[2 * t + 1];
+    if (t + M2LT_BS < NAENT) {  // (measured 1.5 % faster than a clamped unconditional load)
+      pa10 = at[2 * (t + M2LT_BS)];
+      pa11 = at[2 * (t + M2LT_BS) + 1];
+    }
   };
   fetch(f0);
   const int* ts = ttile + ((long)tile * 8 + w) * 8;
@@ -2735,12 +2730,12 @@ __global__ __launch_bounds__(M2LT_BS, sizeof(R) == 4 ? 4 : 2) void k_m2l_tile(
       else
         xs[it * M2LT_BS + t] = pre[it];
     }
-#pragma unroll
-    for (int j = 0; j < 2; j++)
-      if (t + j * M2LT_BS < NAENT) {
-        s_A[t + j * M2LT_BS][0] = pa[j][0];
-        s_A[t + j * M2LT_BS][1] = pa[j][1];
-      }
+    s_A[t][0] = pa00;  // t < 512 < NAENT
+    s_A[t][1] = pa01;
+    if (t + M2LT_BS < NAENT) {
+      s_A[t + M2LT_BS][0] = pa10;
+      s_A[t + M2LT_BS][1] = pa11;
+    }
     __syncthreads();
     if (f + 1 < f1) fetch(f + 1);  // in flight while the matrix cores work on f
     if (!live) continue;
@@ -2817,18 +2812,18 @@ __global__ __launch_bounds__(M2LT_BS, sizeof(R) == 4 ? 4 : 2) void k_m2l_tile(
   }
 }
 
-void m2l_tiled(hipStream_t s, int C, int ntile, int Rc, const int* ttile, const int* htile, const int* optab,
-               const void* opsT, const void* effden, void* effval, int single)
+void m2l_tiled(hipStream_t s, int C, int ntile, int Rc, const int* ttile, const int* htile, const void* atab,
+               const void* effden, void* effval, int single)
 {
   if (ntile <= 0 || Rc < 16) return;
   const int per = (ntile + 7) / 8;
   dim3 grid(8 * per, (C + M2LT_FPB - 1) / M2LT_FPB, Rc / 16);
   if (single)
-    hipLaunchKernelGGL(k_m2l_tile<float>, grid, dim3(M2LT_BS), 0, s, C, Rc, ntile, ttile, htile, optab,
-                       (const float2*)opsT, (const float2*)effden, (float2*)effval, 0);
+    hipLaunchKernelGGL(k_m2l_tile<float>, grid, dim3(M2LT_BS), 0, s, C, Rc, ntile, ttile, htile,
+                       (const float2*)atab, (const float2*)effden, (float2*)effval, 0);
   else
-    hipLaunchKernelGGL(k_m2l_tile<double>, grid, dim3(M2LT_BS), 0, s, C, Rc, ntile, ttile, htile, optab,
-                       (const double2*)opsT, (const double2*)effden, (double2*)effval, 0);
+    hipLaunchKernelGGL(k_m2l_tile<double>, grid, dim3(M2LT_BS), 0, s, C, Rc, ntile, ttile, htile,
+                       (const double2*)atab, (const double2*)effden, (double2*)effval, 0);
 }
 
 // ---------------------------------------------------------------------------

@@ -144,11 +144,12 @@ void fft_inv(hipStream_t s, int np, int nslot, const int* slot_node, int Rc, con
 // right-hand sides innermost and Rc a multiple of 16: for every tile (2x2x2
 // target parents, ttile [ntile][8][8] child slots) and its 4x4x4 halo of
 // neighbour parents (htile [ntile][64][8] child slots), effval of each target
-// child = sum over its V list of T_o effden.  opsT is [C][343] complex;
-// single selects FP32 spectra and operators (products summed in FP64 per
-// neighbour direction).
-void m2l_tiled(hipStream_t s, int C, int ntile, int Rc, const int* ttile, const int* htile, const int* optab,
-               const void* opsT, const void* effden, void* effval, int single);
+// child = sum over its V list of T_o effden.  atab is the A-operand table
+// [C][26 directions x 32][2] complex: entry n*32 + 8k + c holds the operators
+// of (source child k, target child c) and (4 + k, c) in direction n (zero for
+// pairs outside the V list); single selects FP32 spectra and operators.
+void m2l_tiled(hipStream_t s, int C, int ntile, int Rc, const int* ttile, const int* htile, const void* atab,
+               const void* effden, void* effval, int single);
 
 }  // namespace dev
 }  // namespace kifmm

@@ -219,6 +219,22 @@ void Engine::upload_static()
     opsT_.upload(t);
     std::vector<float> t32(t.begin(), t.end());
     opsT32_.upload(t32);
+    if (st == 1) {  // tiled form: A-operand tables [C][26 x 32][2], expanded from the optab of the plan
+      const int* optab = plan_.I.data() + plan_.optab;
+      std::vector<double> a((size_t)O.C * 832 * 2 * 2, 0.0);
+      for (int f = 0; f < O.C; f++)
+        for (int e = 0; e < 832; e++)
+          for (int q = 0; q < 2; q++) {
+            const int n = e >> 5, k = (e >> 3) & 3, c = e & 7;
+            const int o = optab[(n * 8 + (q ? 4 + k : k)) * 8 + c];
+            if (o < 0) continue;
+            for (int z = 0; z < 2; z++)
+              a[(((size_t)f * 832 + e) * 2 + q) * 2 + z] = t[((size_t)f * 343 + o) * 2 + z];
+          }
+      atab_.upload(a);
+      std::vector<float> a32(a.begin(), a.end());
+      atab32_.upload(a32);
+    }
   }
   c2s_.upload(O.cube2surf);
   std::vector<int> s2c(O.nsp[SURF_DC], 0);
@@ -1035,8 +1051,8 @@ void Engine::run_chunk(const double* srcden, double* trgval, int rc, int mode, h
     });
     timed(KIFMM3D_K_M2L, [&] {
       if (tiled)
-        dev::m2l_tiled(s, O.C, vl.ntile, rc, I + vl.ttile, I + vl.htile, I + P.optab,
-                       single ? (const void*)opsT32_.p : (const void*)opsT_.p, effden_.p, effval_.p, single);
+        dev::m2l_tiled(s, O.C, vl.ntile, rc, I + vl.ttile, I + vl.htile,
+                       single ? (const void*)atab32_.p : (const void*)atab_.p, effden_.p, effval_.p, single);
       else if (P.m2l_pairs)
         dev::m2l(s, sm, tm, O.C, vl.ntrg, rc, I + vl.vptr, I + vl.vsrc, I + vl.vop, (const double2*)m2l_.p,
                  (const double2*)effden_.p, (double2*)effval_.p);

@@ -195,6 +195,8 @@ class Engine {
   DVec<double> m2me_;  // 8 blocks of plnUE x plnUE: UC2UE x UE2UC_c (Dot2 on the device at setup)
   DVec<double> l2de_;  // 8 blocks of plnDE x 2 plnDC: [DC2DE | DC2DE x DE2DC_c] (plnDC == plnDE)
   DVec<double> opsT_;  // M2L operators as [C][343][sdof][tdof] complex (FP64)
+  DVec<double> atab_;  // tiled M2L A-operand tables [C][26 x 32][2] complex (scalar kernels)
+  DVec<float> atab32_;  // the same in FP32
   DVec<float> opsT32_; // the same in FP32
   DVec<double> tw_;
   DVec<int> I_;
