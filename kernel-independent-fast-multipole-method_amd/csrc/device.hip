@@ -2615,7 +2615,6 @@ struct Mt<float> {
 
 constexpr int M2LT_BS = 512;  // 8 waves: one per target parent of the tile
 constexpr int M2LT_FPB = 16;  // frequencies per block, visited in a software-pipelined loop (8: +1 %, 32: same, 63: +5 %)
-constexpr int NOPS = 343;     // operator table entries ((o+3) in [0,7)^3)
 constexpr int NAENT = 26 * 32;  // A-operand table entries: direction x (source k, target child c)
 // FP32 form: directions summed on the matrix cores in FP32 before their sum
 // is added to the FP64 accumulators (0: all 26 in FP32)
