@@ -166,6 +166,20 @@ def run_batch(args, K):
     t_setup = time.perf_counter() - t0
     ntot = sum(pr["n"] for pr in probs) * nrhs
     log(f"batch: {len(probs)} problems, {ntot // nrhs} points, setup {t_setup:.1f} s")
+    tuning = None
+    if not args.no_tune:
+        # each problem's near-field kernel and per-level M2L forms, timed on its own tree
+        t0 = time.perf_counter()
+        near, m2l = {}, {}
+        for f in fmms:
+            st = f.tune(nrhs, mode)
+            near[st["near"]] = near.get(st["near"], 0) + 1
+            for v in st["m2l"]:
+                m2l[v] = m2l.get(v, 0) + 1
+        tuning = {"s": round(time.perf_counter() - t0, 2), "near_kernels": {str(k): v for k, v in sorted(near.items())},
+                  "m2l_forms_per_level": {str(k): v for k, v in sorted(m2l.items())}}
+        log(f"batch: autotuned in {tuning['s']:.1f} s: near-field kernels {tuning['near_kernels']}, "
+            f"M2L forms over levels {tuning['m2l_forms_per_level']}")
     # individual eager evaluations (warm), for comparison
     for f, s_, t_ in zip(fmms, srcs, trgs):
         f.eval_device(s_, t_, nrhs, mode)
@@ -203,7 +217,7 @@ def run_batch(args, K):
         "dtype": "f32+f64" if mode == K.MODE_MIXED else "f64", "data": "synthetic",
         "config": {"workload": "batch-64", "form": "mixed FP32/FP64" if mode == K.MODE_MIXED else "plain FP64",
                    "problems": len(probs), "points": ntot // nrhs, "nrhs": nrhs, "one_graph_ms": round(ms, 3),
-                   "one_by_one_eager_ms": round(eager_ms, 3), "setup_s": round(t_setup, 2),
+                   "one_by_one_eager_ms": round(eager_ms, 3), "setup_s": round(t_setup, 2), "tuning": tuning,
                    "worst_rel_err_vs_direct": {str(k): v for k, v in errs.items()} if not args.no_check else None},
         "roofline": None, "cpu_baseline": None,
     }
@@ -298,6 +312,8 @@ def main():
                     help="both: time the plain FP64 form and the mixed form; value is the mixed form's")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-check", action="store_true", help="skip the direct-sum error check")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip the autotuner (near-field kernel and per-level M2L form, timed on the live tree)")
     ap.add_argument("--calibrate", action="store_true",
                     help="only run five 1 GiB streaming copies (reference for memory-traffic counters)")
     args = ap.parse_args()
@@ -368,6 +384,13 @@ def main():
 
     res = {}
     for name, mode in forms:
+        tuned = None
+        if not args.no_tune:
+            t0 = time.perf_counter()
+            st = f.tune(nrhs, mode)
+            tuned = {"near": st["near"], "m2l": st["m2l"], "s": round(time.perf_counter() - t0, 3)}
+            log(f"{name}: autotuned in {tuned['s']:.2f} s ({st['evaluates']} evaluates): near-field kernel "
+                f"{st['near']}, M2L form per V-list level {st['m2l']}")
         ms, stages = timed_steps(f, K, d_src, d_trg, nrhs, mode, args.steps, args.warmup, name)
         value = n * nrhs / (ms * 1e-3)
         log(f"{name}: {args.steps} steps, {ms:.2f} ms/step, {value:.4e} targets/s")
@@ -376,7 +399,7 @@ def main():
         if not args.no_check:
             err = direct_error(f, d_src, d_trg, c, nrhs, n, tdof, seed)
             log(f"{name}: rel L2 error vs FP64 direct sum on {c['nchk']} targets x {nrhs} RHS: {err:.3e}")
-        res[name] = dict(ms=ms, value=value, stages=stages, rates=rates, err=err)
+        res[name] = dict(ms=ms, value=value, stages=stages, rates=rates, err=err, tuned=tuned)
 
     chosen = forms[-1][0]
     r = res[chosen]
@@ -403,10 +426,12 @@ def main():
            "stage_ms": {k: round(v, 3) for k, v in r["stages"].items()}}
     if tune:
         cfg["ptsmax_tuning_ms"] = tune
+    if r["tuned"]:
+        cfg["tuned"] = r["tuned"]
     if "plain" in res and chosen != "plain":
         p = res["plain"]
         cfg["plain_fp64"] = {"value": p["value"], "ms_per_step": round(p["ms"], 3), "rel_err_vs_direct": p["err"],
-                             "stage_ms": {k: round(v, 3) for k, v in p["stages"].items()}}
+                             "stage_ms": {k: round(v, 3) for k, v in p["stages"].items()}, "tuned": p["tuned"]}
         cfg["speedup_vs_plain"] = round(r["value"] / p["value"], 3)
     out = {
         "metric": METRIC,

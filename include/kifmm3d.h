@@ -176,6 +176,32 @@ int kifmm3d_release_operators(void);
  * launches it.  Stage and kernel times then report the total only. */
 int kifmm3d_set_graph(kifmm3d* h, int on);
 
+/* Autotuner (no reference counterpart; the reference has one kernel per
+ * stage).  Times, on this handle's tree and for (nrhs, mode), the candidate
+ * near-field kernels and, per V-list level, the candidate M2L forms over
+ * warm evaluates of zero densities (one warm-up and two timed evaluates per
+ * round; rounds = the larger candidate count, at most 3), and keeps the
+ * fastest of each for later evaluates of that mode.  Every candidate computes
+ * the same sums (the plain form's candidates agree to rounding, 1e-12).
+ * settings (may be NULL) receives up to nsettings entries: [0] near-field
+ * kernel (0 LDS broadcast, 1 scalar-cache sources, 2 FP64 matrix cores,
+ * 3 FP32 matrix cores), [1 + l] M2L form of V-list level l, top level first
+ * (0 tiled matrix-core, 1 parent-grouped matrix-core, 2 grouped VALU,
+ * 3 per pair); nsettings = 17 always suffices.  *nrun (may be NULL) receives
+ * the number of evaluates run.  Requires setup; allocates scratch densities
+ * and values of nrhs vectors for its duration. */
+int kifmm3d_tune(kifmm3d* h, int nrhs, int mode, int* settings, int nsettings, int* nrun);
+/* The settings in force for a mode (same layout as kifmm3d_tune; the tiled
+ * M2L form applies to chunks of 16k right-hand sides, judged on the last
+ * evaluate's chunk).  Returns the number of entries (1 + V-list levels) or a
+ * negative status. */
+int kifmm3d_settings(const kifmm3d* h, int mode, int* settings, int nsettings);
+/* Pin settings for a mode (same layout; -1 keeps an entry's default), e.g.
+ * the ones a previous process tuned for the same problem.  EINVAL for a
+ * kernel or form the problem cannot take (kernel 3 outside the mixed Laplace
+ * form, the tiled M2L for vector kernels).  Requires setup. */
+int kifmm3d_set_settings(kifmm3d* h, int mode, const int* settings, int nsettings);
+
 /* Batched evaluation of independent problems (no reference counterpart; the
  * reference evaluates one FMM3d at a time).  A batch groups set-up handles;
  * kifmm3d_batch_evaluate_device evaluates handle i on d_srcden[i] ->

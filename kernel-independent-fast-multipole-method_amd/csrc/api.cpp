@@ -224,6 +224,30 @@ int kifmm3d_set_graph(kifmm3d* h, int on)
   return guarded(h, [&] { h->eng->set_graph(on != 0); });
 }
 
+int kifmm3d_tune(kifmm3d* h, int nrhs, int mode, int* settings, int nsettings, int* nrun)
+{
+  if (!h) return KIFMM3D_EINVAL;
+  return guarded(h, [&] {
+    const int runs = h->eng->tune(nrhs, mode);
+    if (nrun) *nrun = runs;
+    if (settings && nsettings > 0) h->eng->settings(mode, settings, nsettings);
+  });
+}
+
+int kifmm3d_settings(const kifmm3d* h, int mode, int* settings, int nsettings)
+{
+  if (!h) return KIFMM3D_EINVAL;
+  if (mode != KIFMM3D_MODE_PLAIN && mode != KIFMM3D_MODE_MIXED) return KIFMM3D_EINVAL;
+  if (!h->eng->host_ready()) return KIFMM3D_ESTATE;
+  return h->eng->settings(mode, settings, nsettings);
+}
+
+int kifmm3d_set_settings(kifmm3d* h, int mode, const int* settings, int nsettings)
+{
+  if (!h) return KIFMM3D_EINVAL;
+  return guarded(h, [&] { h->eng->set_settings(mode, settings, nsettings); });
+}
+
 int kifmm3d_batch_create(kifmm3d_batch** out, kifmm3d* const* handles, int n)
 {
   if (!out) return KIFMM3D_EINVAL;

@@ -892,6 +892,7 @@ void Engine::evaluate_device(const double* d_srcden, double* d_trgval, int nrhs,
     last_rc_ = std::min(rc, nrhs - (nrhs - 1) / rc * rc);
     return;
   }
+  vlev_ms_.assign(plan_.vlev.size(), 0.0);
   hipEvent_t ev[9];
   for (auto& e : ev) HIPCHK(hipEventCreate(&e));
   try {
@@ -910,6 +911,7 @@ void Engine::evaluate_device(const double* d_srcden, double* d_trgval, int nrhs,
         HIPCHK(hipEventElapsedTime(&ms, evpool_[e], evpool_[e + 1]));
         kernel_ms[evcls_[e / 2]] += ms;
         kernel_calls[evcls_[e / 2]] += 1;
+        if (evlvl_[e / 2] >= 0) vlev_ms_[evlvl_[e / 2]] += ms;
       }
       last_rc_ = n;
     }
@@ -935,6 +937,133 @@ void Engine::evaluate_host(const double* srcden, double* trgval, int nrhs, int m
   if (nt) HIPCHK(hipMemcpy(trgval, htrg_.p, nt * sizeof(double), hipMemcpyDeviceToHost));
 }
 
+int Engine::near_variant(bool mixed) const
+{
+  const int t = p2p_tuned_[mixed ? 1 : 0];
+  const int v = t >= 0 ? t : mixed ? 3 : p2p_variant_;
+  // the matrix-core near-field kernels are Laplace's; the other kernel types
+  // run kernels 2 and 3 as the LDS-broadcast form 0
+  return user_.kt != 111 && v >= 2 ? 0 : v;
+}
+
+// M2L form of V-list level `level` for a chunk of rc right-hand sides: the
+// tuned one where set and applicable (the tiled form needs a scalar
+// translation kernel and 16k right-hand sides), else tiled, parent-grouped
+// matrix-core or grouped VALU as the switches allow; the per-pair form when
+// the plan was built for it.
+int Engine::m2l_form(bool mixed, size_t level, int rc) const
+{
+  if (plan_.m2l_pairs) return 3;
+  const bool tile_ok = mm_.sdof == 1 && mm_.tdof == 1 && rc % 16 == 0;
+  const std::vector<int>& tf = m2l_form_[mixed ? 1 : 0];
+  if (level < tf.size() && tf[level] >= 0 && (tf[level] != 0 || tile_ok)) return tf[level];
+  return m2l_tile_ && tile_ok ? 0 : m2l_matrix_ ? 1 : 2;
+}
+
+int Engine::tune(int nrhs, int mode)
+{
+  if (!setup_done_) throw Error(KIFMM3D_ESTATE, "tune called before setup");
+  if (nrhs < 1) throw Error(KIFMM3D_EINVAL, "nrhs must be >= 1");
+  if (mode != KIFMM3D_MODE_PLAIN && mode != KIFMM3D_MODE_MIXED) throw Error(KIFMM3D_EINVAL, "unsupported evaluation mode");
+  if (nt_ == 0) return 0;
+  const bool mixed = mode == KIFMM3D_MODE_MIXED;
+  const int mi = mixed ? 1 : 0;
+  const size_t nlev = plan_.vlev.size();
+  const int rc = choose_rhs_chunk(nrhs);
+  // candidates: the mixed form keeps the Laplace near field on the FP32
+  // matrix cores (its precision choice, not a schedule); the other kernel
+  // types have the LDS-broadcast and scalar-cache forms
+  std::vector<int> nf, mf;
+  if (user_.kt == 111)
+    nf = mixed ? std::vector<int>{3} : std::vector<int>{2, 0, 1};
+  else
+    nf = {0, 1};
+  if (!plan_.m2l_pairs) {
+    if (mm_.sdof == 1 && mm_.tdof == 1 && rc % 16 == 0) mf.push_back(0);
+    mf.push_back(1);
+    mf.push_back(2);
+  }
+  DVec<double> den, val;
+  den.ensure((size_t)nrhs * ns_ * user_.sdof);
+  val.ensure((size_t)nrhs * nt_ * user_.tdof);
+  if (den.n) HIPCHK(hipMemset(den.p, 0, den.n * sizeof(double)));
+  const bool graph = use_graph_;
+  use_graph_ = false;
+  drop_graph();
+  const size_t rounds = std::max(nf.size(), std::max(mf.size(), (size_t)1));
+  std::vector<double> best_lvl(nlev, 1e300);
+  std::vector<int> best_form(nlev, -1);
+  double best_near = 1e300;
+  int best_v = -1, runs = 0;
+  try {
+    for (size_t r = 0; r < rounds; r++) {
+      p2p_tuned_[mi] = nf[std::min(r, nf.size() - 1)];
+      const int f = mf.empty() ? -1 : mf[std::min(r, mf.size() - 1)];
+      m2l_form_[mi].assign(nlev, f);
+      std::vector<double> lvl(nlev, 1e300);
+      double near = 1e300;
+      for (int k = 0; k < 3; k++) {  // one warm-up, the faster of two timed
+        evaluate_device(den.p, val.p, nrhs, mode);
+        runs++;
+        if (k == 0) continue;
+        near = std::min(near, kernel_ms[KIFMM3D_K_P2P]);
+        for (size_t l = 0; l < nlev; l++) lvl[l] = std::min(lvl[l], vlev_ms_[l]);
+      }
+      if (near < best_near) {
+        best_near = near;
+        best_v = p2p_tuned_[mi];
+      }
+      for (size_t l = 0; l < nlev; l++)
+        if (lvl[l] < best_lvl[l]) {
+          best_lvl[l] = lvl[l];
+          best_form[l] = f;
+        }
+    }
+  } catch (...) {
+    p2p_tuned_[mi] = -1;
+    m2l_form_[mi].clear();
+    use_graph_ = graph;
+    throw;
+  }
+  p2p_tuned_[mi] = best_v;
+  m2l_form_[mi] = best_form;
+  use_graph_ = graph;
+  return runs;
+}
+
+void Engine::set_settings(int mode, const int* in, int n)
+{
+  if (!host_done_) throw Error(KIFMM3D_ESTATE, "settings before setup");
+  if (mode != KIFMM3D_MODE_PLAIN && mode != KIFMM3D_MODE_MIXED) throw Error(KIFMM3D_EINVAL, "unsupported evaluation mode");
+  if (!in || n < 1) throw Error(KIFMM3D_EINVAL, "settings: empty array");
+  const bool mixed = mode == KIFMM3D_MODE_MIXED;
+  const int v = in[0];
+  // kernel 3 (FP32 matrix cores) is the mixed form's Laplace near field only
+  if (v < -1 || v > 3 || (v == 3 && !(mixed && user_.kt == 111)))
+    throw Error(KIFMM3D_EINVAL, "settings: near-field kernel " + std::to_string(v) + " not available here");
+  const size_t nlev = plan_.vlev.size();
+  std::vector<int> forms(nlev, -1);
+  for (size_t l = 0; l < nlev && (long)l + 1 < n; l++) {
+    const int f = in[l + 1];
+    if (f < -1 || f > 2 || (f == 0 && !(mm_.sdof == 1 && mm_.tdof == 1)))
+      throw Error(KIFMM3D_EINVAL, "settings: M2L form " + std::to_string(f) + " not available here");
+    forms[l] = f;
+  }
+  p2p_tuned_[mixed ? 1 : 0] = v;
+  m2l_form_[mixed ? 1 : 0] = forms;
+  drop_graph();
+}
+
+int Engine::settings(int mode, int* out, int n) const
+{
+  const bool mixed = mode == KIFMM3D_MODE_MIXED;
+  const int rc = last_rc_ > 0 ? last_rc_ : 16;
+  const int cnt = 1 + (int)plan_.vlev.size();
+  if (out && n > 0) out[0] = near_variant(mixed);
+  for (int l = 0; out && l + 1 < std::min(n, cnt); l++) out[l + 1] = m2l_form(mixed, (size_t)l, rc);
+  return cnt;
+}
+
 void Engine::run_chunk(const double* srcden, double* trgval, int rc, int mode, hipStream_t s, hipEvent_t* ev)
 {
   const bool mixed = mode == KIFMM3D_MODE_MIXED;
@@ -951,11 +1080,14 @@ void Engine::run_chunk(const double* srcden, double* trgval, int rc, int mode, h
   const long trs = (long)T.textcnt * tdof;
   dev::Points src{sx_.p, sy_.p, sz_.p, snx_.p, sny_.p, snz_.p, sden_.p, (long)rc * sdof, sdof};
   // the mixed form's Laplace kernels stage FP32 densities (half the bytes, no conversions);
-  // with S2M / S2L in FP32 too and every RHS block on those kernels (a last
-  // block of 1-3 RHS takes the FP64 vector kernels) nothing reads the FP64 copy
+  // with S2M / S2L and the near field in FP32 too and every RHS block on those
+  // kernels (a last block of 1-3 RHS takes the FP64 vector kernels) nothing
+  // reads the FP64 copy
   float* den32 = mixed && user_.kt == 111 ? sden32_.p : nullptr;
   src.den32 = den32;
-  double* den64 = den32 && s2m32_ && (rc % 16 == 0 || rc % 16 >= dev::PAD16_MIN) ? nullptr : sden_.p;
+  double* den64 = den32 && s2m32_ && near_variant(mixed) == 3 && (rc % 16 == 0 || rc % 16 >= dev::PAD16_MIN)
+                      ? nullptr
+                      : sden_.p;
   dev::Targets trg{tx_.p, ty_.p, tz_.p, tval_.p, trs};
 
   auto run_gemm = [&](const GemmOp& op) {
@@ -983,6 +1115,7 @@ void Engine::run_chunk(const double* srcden, double* trgval, int rc, int mode, h
     if (ev) HIPCHK(hipEventRecord(ev[k], s));
   };
   size_t nev = 0;
+  int cur_lvl = -1;  // V-list level being enqueued (for the per-level times)
   auto timed_on = [&](hipStream_t st, int cls, auto&& fn) {
     if (!ev) {
       fn();
@@ -1000,6 +1133,8 @@ void Engine::run_chunk(const double* srcden, double* trgval, int rc, int mode, h
     HIPCHK(hipEventRecord(evpool_[nev + 1], st));
     evcls_.resize(nev / 2 + 1);
     evcls_[nev / 2] = cls;
+    evlvl_.resize(nev / 2 + 1);
+    evlvl_[nev / 2] = cur_lvl;
     nev += 2;
   };
   auto timed = [&](int cls, auto&& fn) { timed_on(s, cls, fn); };
@@ -1034,17 +1169,19 @@ void Engine::run_chunk(const double* srcden, double* trgval, int rc, int mode, h
   }
   timed_on(side ? side_ : s, KIFMM3D_K_P2P, [&] {
     dev::p2p(side ? side_ : s, user_.kt, P.near.n, I + P.near.tbeg, I + P.near.tnum, I + P.near.eptr,
-             (const int2*)(I + P.near.ent), I + P.near.self, src, trg, rc, kcu, mixed ? 3 : p2p_variant_);
+             (const int2*)(I + P.near.ent), I + P.near.self, src, trg, rc, kcu, near_variant(mixed));
   });
   if (side) HIPCHK(hipEventRecord(join_ev_, side_));
   mark(3);
   // V list: tiled matrix-core M2L for scalar kernels on chunks of 16k
   // right-hand sides, the parent-grouped matrix-core form otherwise (VALU
   // grouped and per-pair forms for comparison)
-  const bool tiled = m2l_tile_ && !P.m2l_pairs && sm == 1 && tm == 1 && rc % 16 == 0;
-  const bool matrix = !tiled && m2l_matrix_ && !P.m2l_pairs;
-  const int ri = tiled ? 1 : 0, single = mixed && (tiled || matrix) ? 1 : 0;
-  for (const VLevel& vl : P.vlev) {
+  for (size_t li = 0; li < P.vlev.size(); li++) {
+    const VLevel& vl = P.vlev[li];
+    const int form = m2l_form(mixed, li, rc);
+    const bool tiled = form == 0;
+    const int ri = tiled ? 1 : 0, single = mixed && form <= 1 ? 1 : 0;
+    cur_lvl = (int)li;
     timed(KIFMM3D_K_FFT_FWD, [&] {
       dev::fft_fwd(s, np_, vl.nsrc, I + vl.src_node, rc, upeden_.p, pUE, sm, D + vl.dscale, c2s_.p, effden_.p, single,
                    ri, (const double2*)tw_.p);
@@ -1053,10 +1190,10 @@ void Engine::run_chunk(const double* srcden, double* trgval, int rc, int mode, h
       if (tiled)
         dev::m2l_tiled(s, O.C, vl.ntile, rc, I + vl.ttile, I + vl.htile,
                        single ? (const void*)atab32_.p : (const void*)atab_.p, effden_.p, effval_.p, single);
-      else if (P.m2l_pairs)
+      else if (form == 3)
         dev::m2l(s, sm, tm, O.C, vl.ntrg, rc, I + vl.vptr, I + vl.vsrc, I + vl.vop, (const double2*)m2l_.p,
                  (const double2*)effden_.p, (double2*)effval_.p);
-      else if (m2l_matrix_)
+      else if (form == 1)
         dev::m2l_matrix(s, sm, tm, O.C, vl.npar, rc, I + vl.tslot, I + vl.sslot, I + vl.pmask, I + P.aidx[single],
                         single ? (const void*)opsT32_.p : (const void*)opsT_.p, effden_.p, effval_.p, single);
       else
@@ -1067,6 +1204,7 @@ void Engine::run_chunk(const double* srcden, double* trgval, int rc, int mode, h
       dev::fft_inv(s, np_, vl.ntrg, I + vl.trg_node, rc, effval_.p, single, ri, tm, s2c_.p, O.nsp[SURF_DC],
                    dncval_.p, pDC, (const double2*)tw_.p);
     });
+    cur_lvl = -1;
   }
   mark(4);
   // X list S2L

@@ -112,6 +112,23 @@ class Engine {
   void setup(const double* spos, const double* snor, int ns, const double* tpos, int nt);
   void evaluate_device(const double* d_srcden, double* d_trgval, int nrhs, int mode);
   void evaluate_host(const double* srcden, double* trgval, int nrhs, int mode);
+  // Autotuning: times the candidate near-field kernels and, per V-list level,
+  // the candidate M2L forms on this tree for (nrhs, mode) -- one warm-up and
+  // two timed evaluates of zero densities per round, rounds pairing the
+  // near-field and M2L candidates -- and keeps the fastest of each for later
+  // evaluates of that mode.  Returns the number of evaluates run.
+  int tune(int nrhs, int mode);
+  // Settings in force for a mode: [0] near-field kernel (0 LDS broadcast,
+  // 1 scalar-cache sources, 2 FP64 matrix cores, 3 FP32 matrix cores),
+  // [1 + l] M2L form of V-list level l (0 tiled matrix-core, 1 parent-grouped
+  // matrix-core, 2 grouped VALU, 3 per pair).  Returns the count (1 + levels).
+  int settings(int mode, int* out, int n) const;
+  // Pins settings for a mode (same layout; -1 entries keep the default), e.g.
+  // ones a previous process tuned.  Rejects kernels and forms the problem
+  // cannot take.
+  void set_settings(int mode, const int* in, int n);
+  // FFTs + M2L of each V-list level in the last timed evaluate (ms)
+  const std::vector<double>& level_ms() const { return vlev_ms_; }
   // Graph replay: with it on, evaluate_device captures the whole pass into a
   // hipGraph on first use for a given (buffers, nrhs, mode) and then only
   // launches it (stage_times then reports the total only).
@@ -149,6 +166,8 @@ class Engine {
   void ensure_work(int rc);
   void run_chunk(const double* srcden, double* trgval, int rc, int mode, hipStream_t s, hipEvent_t* ev);
   void drop_graph();
+  int near_variant(bool mixed) const;
+  int m2l_form(bool mixed, size_t level, int rc) const;
 
   KernelInfo user_, mm_;
   KernelConsts kcu_, kcm_;
@@ -180,6 +199,10 @@ class Engine {
   int gnrhs_ = 0, gmode_ = -1, grc_ = 0;
   std::vector<hipEvent_t> evpool_;
   std::vector<int> evcls_;
+  std::vector<int> evlvl_;        // V-list level of each timed launch (-1 outside the V list)
+  std::vector<double> vlev_ms_;   // per V-list level: FFTs + M2L of the last timed evaluate
+  int p2p_tuned_[2] = {-1, -1};   // per mode: tuned near-field kernel (-1: default)
+  std::vector<int> m2l_form_[2];  // per mode, per V-list level: tuned M2L form (empty: default)
   size_t nev_used_ = 0;
 
   // device data

@@ -63,6 +63,9 @@ SYMBOLS = [
     ("kifmm3d_operator_cache_stats", ctypes.c_int, [_I, _I]),
     ("kifmm3d_release_operators", ctypes.c_int, []),
     ("kifmm3d_set_graph", ctypes.c_int, [_VP, ctypes.c_int]),
+    ("kifmm3d_tune", ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int, _I, ctypes.c_int, _I]),
+    ("kifmm3d_settings", ctypes.c_int, [_VP, ctypes.c_int, _I, ctypes.c_int]),
+    ("kifmm3d_set_settings", ctypes.c_int, [_VP, ctypes.c_int, _I, ctypes.c_int]),
     ("kifmm3d_batch_create", ctypes.c_int, [ctypes.POINTER(_VP), ctypes.POINTER(_VP), ctypes.c_int]),
     ("kifmm3d_batch_evaluate_device", ctypes.c_int, [_VP, ctypes.POINTER(_VP), ctypes.POINTER(_VP), _I, ctypes.c_int]),
     ("kifmm3d_batch_time", ctypes.c_int, [_VP, _D]),
@@ -288,6 +291,32 @@ class FMM3d:
     def set_graph(self, on: bool = True):
         """Capture the evaluate pass into a hipGraph and replay it on repeat calls."""
         self._check(lib().kifmm3d_set_graph(self.h, 1 if on else 0))
+
+    def tune(self, nrhs: int = 1, mode: int = MODE_PLAIN) -> dict:
+        """Time the candidate near-field kernels and per-level M2L forms on this
+        tree for (nrhs, mode) and keep the fastest; returns the settings chosen."""
+        st = np.full(17, -1, dtype=np.int32)
+        nrun = ctypes.c_int(0)
+        self._check(lib().kifmm3d_tune(self.h, nrhs, mode, st.ctypes.data_as(_I), len(st), ctypes.byref(nrun)))
+        out = self.settings(mode)
+        out["evaluates"] = nrun.value
+        return out
+
+    def settings(self, mode: int = MODE_PLAIN) -> dict:
+        """{'near': kernel, 'm2l': [form per V-list level]} in force for a mode
+        (near: 0 LDS broadcast, 1 scalar-cache sources, 2 FP64 matrix cores,
+        3 FP32 matrix cores; m2l: 0 tiled, 1 parent-grouped matrix-core,
+        2 grouped VALU, 3 per pair)."""
+        st = np.full(17, -1, dtype=np.int32)
+        n = lib().kifmm3d_settings(self.h, mode, st.ctypes.data_as(_I), len(st))
+        if n < 0:
+            self._check(n)
+        return {"near": int(st[0]), "m2l": [int(v) for v in st[1:min(n, len(st))]]}
+
+    def set_settings(self, mode: int = MODE_PLAIN, near: int = -1, m2l=()):
+        """Pin the near-field kernel and per-level M2L forms for a mode (-1: default)."""
+        st = np.array([near] + list(m2l), dtype=np.int32)
+        self._check(lib().kifmm3d_set_settings(self.h, mode, st.ctypes.data_as(_I), len(st)))
 
     def destroy(self):
         if getattr(self, "h", None):

@@ -186,3 +186,36 @@ def test_operator_disk_cache_roundtrip_and_corruption(tmp_path):
         assert files[0].read_bytes() != bytes(data)  # rewritten after the rebuild
     finally:
         K.set_operator_cache(None)
+
+
+def test_settings_pin_and_validate():
+    """Autotuner settings: defaults, pinning, and the kernels / forms a problem
+    cannot take (host side; tuning itself times evaluates on the device)."""
+    f, p = setup_host(111, 4, ptsmax=30, n=3000)
+    st = f.settings(K.MODE_PLAIN)
+    nlev = len(st["m2l"])
+    assert nlev >= 1
+    assert st["near"] == 2 and st["m2l"] == [0] * nlev
+    assert f.settings(K.MODE_MIXED)["near"] == 3
+    f.set_settings(K.MODE_PLAIN, 0, [1, 2] * nlev)
+    assert f.settings(K.MODE_PLAIN) == {"near": 0, "m2l": ([1, 2] * nlev)[:nlev]}
+    assert f.settings(K.MODE_MIXED)["near"] == 3  # per mode
+    f.set_settings(K.MODE_PLAIN, -1, [-1] * nlev)
+    assert f.settings(K.MODE_PLAIN) == st
+    for bad in ((K.MODE_PLAIN, 3, []), (K.MODE_PLAIN, 4, []), (K.MODE_PLAIN, 0, [3]), (7, 0, [])):
+        with pytest.raises(K.KifmmError) as e:
+            f.set_settings(*bad)
+        assert e.value.code == K.EINVAL
+    with pytest.raises(K.KifmmError) as e:
+        f.tune(1, K.MODE_PLAIN)
+    assert e.value.code == K.ESTATE
+    g, _ = setup_host(511, 4)
+    assert g.settings(K.MODE_PLAIN)["near"] == 0  # the matrix-core near field is Laplace's
+    with pytest.raises(K.KifmmError) as e:
+        g.set_settings(K.MODE_MIXED, 3)  # FP32 matrix-core near field: Laplace only
+    assert e.value.code == K.EINVAL
+    with pytest.raises(K.KifmmError) as e:
+        g.set_settings(K.MODE_PLAIN, 0, [0])  # tiled M2L: scalar kernels only
+    assert e.value.code == K.EINVAL
+    g.set_settings(K.MODE_PLAIN, 1, [1])
+    assert g.settings(K.MODE_PLAIN)["near"] == 1
