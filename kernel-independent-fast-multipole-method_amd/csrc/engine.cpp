@@ -849,6 +849,7 @@ void Engine::drop_graph()
 void Engine::evaluate_device(const double* d_srcden, double* d_trgval, int nrhs, int mode)
 {
   check_eval_args(d_srcden, d_trgval, nrhs, mode);
+  last_nrhs_ = nrhs;
   for (double& m : stage_ms) m = 0.0;
   for (double& m : kernel_ms) m = 0.0;
   for (int& c : kernel_calls) c = 0;
@@ -1294,6 +1295,7 @@ void Engine::direct(const double* srcden, bool on_device, int nrhs, const int* i
 
 BatchRunner::BatchRunner(const std::vector<Engine*>& engines) : eng_(engines)
 {
+  if (const char* b = std::getenv("KIFMM3D_BATCH_BALANCE")) balance_ = std::atoi(b);
   const int nb = (int)std::min<size_t>(8, eng_.size());
   br_.resize(nb, nullptr);
   for (auto& s : br_) HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
@@ -1336,8 +1338,26 @@ void BatchRunner::evaluate(const double* const* d_srcden, double* const* d_trgva
       const int nb = (int)br_.size();
       HIPCHK(hipEventRecord(fork_join_[0], s0));
       for (int k = 1; k < nb; k++) HIPCHK(hipStreamWaitEvent(br_[k], fork_join_[0], 0));
-      for (size_t i = 0; i < n; i++)
-        eng_[i]->record(d_srcden[i], d_trgval[i], nrhs[i], rc[i], mode, br_[i % nb]);
+      // branches balanced by cost, largest problem first onto the least loaded
+      // branch; the cost is the problem's last timed evaluate per right-hand
+      // side, or its target count when it has none
+      std::vector<double> cost(n);
+      for (size_t i = 0; i < n; i++) {
+        const double ms = eng_[i]->stage_ms[KIFMM3D_STAGE_TOTAL], lr = eng_[i]->last_nrhs();
+        cost[i] = ms > 0 && lr > 0 ? ms / lr * nrhs[i] : 1e-6 * eng_[i]->tree().textcnt * nrhs[i];
+      }
+      std::vector<size_t> order(n);
+      for (size_t i = 0; i < n; i++) order[i] = i;
+      const bool rr = balance_ == 0;
+      if (!rr) std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return cost[a] > cost[b]; });
+      std::vector<double> load(nb, 0.0);
+      for (size_t k = 0; k < n; k++) {
+        const size_t i = order[k];
+        int br = (int)(k % nb);
+        if (!rr) br = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        load[br] += cost[i];
+        eng_[i]->record(d_srcden[i], d_trgval[i], nrhs[i], rc[i], mode, br_[br]);
+      }
       for (int k = 1; k < nb; k++) {
         HIPCHK(hipEventRecord(fork_join_[nb + k], br_[k]));
         HIPCHK(hipStreamWaitEvent(s0, fork_join_[nb + k], 0));

@@ -149,6 +149,7 @@ class Engine {
   const Operators& ops() const { return *ops_; }
   const KernelInfo& user() const { return user_; }
   bool is_setup() const { return setup_done_; }
+  int last_nrhs() const { return last_nrhs_; }
   bool host_ready() const { return host_done_; }
   double stage_ms[KIFMM3D_NSTAGE] = {0};
   double kernel_ms[KIFMM3D_NKCLASS] = {0};
@@ -180,6 +181,7 @@ class Engine {
   bool setup_done_ = false;
   bool host_done_ = false;
   int last_rc_ = 0;
+  int last_nrhs_ = 0;  // right-hand sides of the last evaluate_device
   int m2l_rb_ = 0;   // right-hand sides per lane in the grouped M2L (0 = default; KIFMM3D_M2L_RB)
   int m2l_bs_ = 64;  // frequencies per block in the grouped M2L (64 or 256; KIFMM3D_M2L_BS)
   int p2p_variant_ = 2;  // near field: 0 LDS broadcast, 1 scalar-cache sources, 2 matrix cores (KIFMM3D_P2P)
@@ -254,6 +256,7 @@ class BatchRunner {
   std::vector<int> knrhs_, krc_;
   int kmode_ = -1;
   double last_ms_ = 0.0;
+  int balance_ = 1;  // 1: branches balanced by problem cost, 0: round robin (KIFMM3D_BATCH_BALANCE)
 };
 
 }  // namespace kifmm
