@@ -2123,9 +2123,57 @@ __device__ inline void load_twiddles(double2 (*s_wt)[2 * NP], const double2* tw,
   }
 }
 
-// Intermediate spectra in LDS: FP64 for FP64 spectra, FP32 for FP32 spectra
-// (arithmetic stays FP64); per-RHS rows padded by one element so the R rows
-// of a wave's accesses fall in different banks.
+// Arithmetic of the right-hand-side-innermost transforms: FP64 for FP64
+// spectra, FP32 for FP32 spectra (the spectra are rounded to FP32 for the M2L
+// products either way).
+template <class CT> struct FftMath { using A = double; using A2 = double2; };
+template <> struct FftMath<float2> { using A = float; using A2 = float2; };
+template <class T2> __device__ inline T2 cz()
+{
+  T2 v;
+  v.x = 0;
+  v.y = 0;
+  return v;
+}
+template <class T2> __device__ inline T2 cmul2(T2 a, T2 b)
+{
+  T2 v;
+  v.x = a.x * b.x - a.y * b.y;
+  v.y = a.x * b.y + a.y * b.x;
+  return v;
+}
+template <class T2> __device__ inline T2 cmulc(T2 a, T2 b)  // a * conj(b)
+{
+  T2 v;
+  v.x = a.x * b.x + a.y * b.y;
+  v.y = a.y * b.x - a.x * b.y;
+  return v;
+}
+template <class D, class S> __device__ inline D cget(const S& s)
+{
+  D d;
+  d.x = s.x;
+  d.y = s.y;
+  return d;
+}
+template <class D, class S> __device__ inline void cset(D& d, S s)
+{
+  d.x = s.x;
+  d.y = s.y;
+}
+template <int NP, class A2>
+__device__ inline void load_twiddles_t(A2 (*s_wt)[2 * NP], const double2* tw, int t, int nt)
+{
+  constexpr int N = 2 * NP;
+  for (int e = t; e < NP * N; e += nt) {
+    const int i = e / N, m = e - i * N;
+    s_wt[i][m] = cget<A2>(tw[((i + NP / 2) * m) % N]);
+  }
+}
+
+// Intermediate spectra in LDS: FP64 for FP64 spectra, FP32 for FP32 spectra;
+// per-RHS rows padded by one element so the R rows of a wave's accesses fall
+// in different banks.
 __device__ inline double2 lds_get(const double2& v) { return v; }
 __device__ inline double2 lds_get(const float2& v) { return make_double2(v.x, v.y); }
 __device__ inline void lds_put(double2& d, double2 v) { d = v; }
@@ -2150,34 +2198,36 @@ __global__ __launch_bounds__(64 * R) void k_fft_fwd_ri(int nslot, const int* __r
   constexpr int N = 2 * NP, H = NP + 1, C = N * N * H, NC = NP * NP * NP, BS = 64 * R;
   constexpr int NSP = NP * NP * NP - (NP - 2) * (NP - 2) * (NP - 2);
   typedef CT LT;
-  __shared__ double s_den[R][NSP];
+  using A = typename FftMath<CT>::A;
+  using A2 = typename FftMath<CT>::A2;
+  __shared__ A s_den[R][NSP];
   __shared__ short s_c2s[NC];
-  __shared__ double2 s_wt[NP][N];
+  __shared__ A2 s_wt[NP][N];
   __shared__ LT X1[R][NP * NP * H + 1];  // [k][j][kx]
   __shared__ LT X2[R][NP * N * H + 1];   // [k][ky][kx]
   int slot, r0;
   if (!fft_block(nslot, Rc / R, R, slot, r0)) return;  // block-uniform
   const int t = threadIdx.x;
   for (int i = t; i < NC; i += BS) s_c2s[i] = c2s[i];
-  load_twiddles<NP>(s_wt, tw, t, BS);
+  load_twiddles_t<NP>(s_wt, tw, t, BS);
   const double ds = dscale[0];
   const double* den = upeden + ((long)slot_node[slot] * Rc + r0) * pln;
   for (int e = t; e < R * NSP; e += BS) {
     const int rr = e / NSP, i = e - rr * NSP;
-    s_den[rr][i] = den[(long)rr * pln + i] * ds;
+    s_den[rr][i] = (A)(den[(long)rr * pln + i] * ds);
   }
   __syncthreads();
   for (int e = t; e < R * NP * NP; e += BS) {  // along x: line (k, j)
     asm volatile("" ::: "memory");  // keep the twiddle loads inside the loop (registers)
     const int rr = e % R, col = e / R, k = col / NP, j = col - k * NP;
-    double2 acc[H];
+    A2 acc[H];
 #pragma unroll
-    for (int kx = 0; kx < H; kx++) acc[kx] = make_double2(0, 0);
+    for (int kx = 0; kx < H; kx++) acc[kx] = cz<A2>();
 #pragma unroll
     for (int i = 0; i < NP; i++) {
       const int si = s_c2s[(i * NP + j) * NP + k];
       if (si >= 0) {
-        const double v = s_den[rr][si];
+        const A v = s_den[rr][si];
 #pragma unroll
         for (int kx = 0; kx < H; kx++) {
           acc[kx].x += v * s_wt[i][kx].x;
@@ -2186,25 +2236,25 @@ __global__ __launch_bounds__(64 * R) void k_fft_fwd_ri(int nslot, const int* __r
       }
     }
 #pragma unroll
-    for (int kx = 0; kx < H; kx++) lds_put(X1[rr][(k * NP + j) * H + kx], acc[kx]);
+    for (int kx = 0; kx < H; kx++) cset(X1[rr][(k * NP + j) * H + kx], acc[kx]);
   }
   __syncthreads();
   for (int e = t; e < R * NP * H; e += BS) {  // along y: line (k, kx)
     asm volatile("" ::: "memory");  // keep the twiddle loads inside the loop (registers)
     const int rr = e % R, col = e / R, k = col / H, kx = col - k * H;
-    double2 x[NP];
+    A2 x[NP];
 #pragma unroll
-    for (int j = 0; j < NP; j++) x[j] = lds_get(X1[rr][(k * NP + j) * H + kx]);
+    for (int j = 0; j < NP; j++) x[j] = cget<A2>(X1[rr][(k * NP + j) * H + kx]);
 #pragma unroll
     for (int ky = 0; ky < N; ky++) {
-      double2 acc = make_double2(0, 0);
+      A2 acc = cz<A2>();
 #pragma unroll
       for (int j = 0; j < NP; j++) {
-        const double2 v = cmul(x[j], s_wt[j][ky]);
+        const A2 v = cmul2(x[j], s_wt[j][ky]);
         acc.x += v.x;
         acc.y += v.y;
       }
-      lds_put(X2[rr][(k * N + ky) * H + kx], acc);
+      cset(X2[rr][(k * N + ky) * H + kx], acc);
     }
   }
   __syncthreads();
@@ -2212,19 +2262,19 @@ __global__ __launch_bounds__(64 * R) void k_fft_fwd_ri(int nslot, const int* __r
   for (int e = t; e < R * N * H; e += BS) {  // along z: line (ky, kx), right-hand sides innermost
     asm volatile("" ::: "memory");  // keep the twiddle loads inside the loop (registers)
     const int rr = e % R, col = e / R;
-    double2 x[NP];
+    A2 x[NP];
 #pragma unroll
-    for (int k = 0; k < NP; k++) x[k] = lds_get(X2[rr][k * N * H + col]);
+    for (int k = 0; k < NP; k++) x[k] = cget<A2>(X2[rr][k * N * H + col]);
 #pragma unroll
     for (int kz = 0; kz < N; kz++) {
-      double2 acc = make_double2(0, 0);
+      A2 acc = cz<A2>();
 #pragma unroll
       for (int k = 0; k < NP; k++) {
-        const double2 v = cmul(x[k], s_wt[k][kz]);
+        const A2 v = cmul2(x[k], s_wt[k][kz]);
         acc.x += v.x;
         acc.y += v.y;
       }
-      cstore(out + (long)(kz * N * H + col) * Rc + rr, acc);
+      cset(out[(long)(kz * N * H + col) * Rc + rr], acc);
     }
   }
 }
@@ -2237,52 +2287,52 @@ __global__ __launch_bounds__(64 * R) void k_fft_inv_ri(int nslot, const int* __r
 {
   constexpr int N = 2 * NP, H = NP + 1, C = N * N * H, BS = 64 * R;
   typedef CT LT;
-  __shared__ double2 s_wt[NP][N];
+  using A = typename FftMath<CT>::A;
+  using A2 = typename FftMath<CT>::A2;
+  __shared__ A2 s_wt[NP][N];
   __shared__ LT Y1[R][NP * N * H + 1];   // [k][ky][kx], only the cube's planes
   __shared__ LT Y2[R][NP * NP * H + 1];  // [k][j][kx], only the cube's rows
   int slot, r0;
   if (!fft_block(nslot, Rc / R, R, slot, r0)) return;  // block-uniform
   const int t = threadIdx.x;
-  load_twiddles<NP>(s_wt, tw, t, BS);
+  load_twiddles_t<NP>(s_wt, tw, t, BS);
   __syncthreads();
   const CT* in = effval + (long)slot * C * Rc + r0;
   for (int e = t; e < R * N * H; e += BS) {  // along z: line (ky, kx), spectra read once from HBM
     asm volatile("" ::: "memory");  // keep the twiddle loads inside the loop (registers)
     const int rr = e % R, col = e / R;
-    double2 x[N];
+    A2 x[N];
 #pragma unroll
-    for (int kz = 0; kz < N; kz++) x[kz] = cload(in + (long)(kz * N * H + col) * Rc + rr);
+    for (int kz = 0; kz < N; kz++) x[kz] = cget<A2>(in[(long)(kz * N * H + col) * Rc + rr]);
 #pragma unroll
     for (int k = 0; k < NP; k++) {
-      double2 acc = make_double2(0, 0);
+      A2 acc = cz<A2>();
 #pragma unroll
       for (int kz = 0; kz < N; kz++) {
-        const double2 w = s_wt[k][kz];
-        const double2 v = cmul(x[kz], make_double2(w.x, -w.y));
+        const A2 v = cmulc(x[kz], s_wt[k][kz]);
         acc.x += v.x;
         acc.y += v.y;
       }
-      lds_put(Y1[rr][k * N * H + col], acc);
+      cset(Y1[rr][k * N * H + col], acc);
     }
   }
   __syncthreads();
   for (int e = t; e < R * NP * H; e += BS) {  // along y: line (k, kx)
     asm volatile("" ::: "memory");  // keep the twiddle loads inside the loop (registers)
     const int rr = e % R, col = e / R, k = col / H, kx = col - k * H;
-    double2 y[N];
+    A2 y[N];
 #pragma unroll
-    for (int ky = 0; ky < N; ky++) y[ky] = lds_get(Y1[rr][(k * N + ky) * H + kx]);
+    for (int ky = 0; ky < N; ky++) y[ky] = cget<A2>(Y1[rr][(k * N + ky) * H + kx]);
 #pragma unroll
     for (int j = 0; j < NP; j++) {
-      double2 acc = make_double2(0, 0);
+      A2 acc = cz<A2>();
 #pragma unroll
       for (int ky = 0; ky < N; ky++) {
-        const double2 w = s_wt[j][ky];
-        const double2 v = cmul(y[ky], make_double2(w.x, -w.y));
+        const A2 v = cmulc(y[ky], s_wt[j][ky]);
         acc.x += v.x;
         acc.y += v.y;
       }
-      lds_put(Y2[rr][(k * NP + j) * H + kx], acc);
+      cset(Y2[rr][(k * NP + j) * H + kx], acc);
     }
   }
   __syncthreads();
@@ -2294,11 +2344,11 @@ __global__ __launch_bounds__(64 * R) void k_fft_inv_ri(int nslot, const int* __r
     const int i = cc / (NP * NP), j = (cc / NP) % NP, k = cc % NP;
     const LT* y = &Y2[rr][(k * NP + j) * H];
     const int xi = i + NP / 2;
-    double v = lds_get(y[0]).x + ((xi & 1) ? -lds_get(y[NP]).x : lds_get(y[NP]).x);
+    A v = cget<A2>(y[0]).x + ((xi & 1) ? -cget<A2>(y[NP]).x : cget<A2>(y[NP]).x);
 #pragma unroll
     for (int kx = 1; kx < NP; kx++) {
-      const double2 w = s_wt[i][kx], yk = lds_get(y[kx]);
-      v += 2.0 * (yk.x * w.x + yk.y * w.y);
+      const A2 w = s_wt[i][kx], yk = cget<A2>(y[kx]);
+      v += (A)2 * (yk.x * w.x + yk.y * w.y);
     }
     out[(long)rr * pln + p] += v;
   }
