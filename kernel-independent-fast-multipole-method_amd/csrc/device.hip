@@ -1969,163 +1969,9 @@ void gemm(hipStream_t s, const GemmArgs& a)
 // Spectra are stored per slot either right-hand-side major ([slot][r][f][C],
 // rhs_inner = 0) or right-hand-side innermost ([slot][f][C][Rc], rhs_inner =
 // 1, the layout of the tiled matrix-core M2L); CT is double2 or float2.
-__device__ inline double2 cmul(double2 a, double2 b) { return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
 
-__device__ inline void cstore(double2* p, double2 v) { *p = v; }
-__device__ inline void cstore(float2* p, double2 v) { *p = make_float2((float)v.x, (float)v.y); }
-__device__ inline double2 cload(const double2* p) { return *p; }
-__device__ inline double2 cload(const float2* p)
-{
-  float2 v = *p;
-  return make_double2(v.x, v.y);
-}
-
-template <int NP, class CT>
-__global__ __launch_bounds__(256) void k_fft_fwd(const int* __restrict__ slot_node, int Rc,
-                                                 const double* __restrict__ upeden, int pln, int sm,
-                                                 const double* __restrict__ dscale, const short* __restrict__ c2s,
-                                                 CT* __restrict__ effden, const double2* __restrict__ tw, int rhs_inner)
-{
-  constexpr int N = 2 * NP, H = NP + 1, C = N * N * H, NC = NP * NP * NP;
-  constexpr int NSP = NP * NP * NP - (NP - 2) * (NP - 2) * (NP - 2);
-  __shared__ double s_den[NSP * 4];
-  __shared__ short s_c2s[NC];
-  __shared__ double2 s_w[N];
-  __shared__ double2 X1[NP * NP * H];
-  __shared__ double2 X2[NP * N * H];
-  const int slot = blockIdx.x, r = blockIdx.y, t = threadIdx.x;
-  const double* den = upeden + ((long)slot_node[slot] * Rc + r) * pln;
-  for (int i = t; i < pln; i += 256) s_den[i] = den[i] * dscale[i % sm];
-  for (int i = t; i < NC; i += 256) s_c2s[i] = c2s[i];
-  if (t < N) s_w[t] = tw[t];
-  __syncthreads();
-  const long os = rhs_inner ? Rc : 1;
-  CT* out = effden + (long)slot * Rc * sm * C + (rhs_inner ? (long)r : (long)r * sm * C);
-  for (int f = 0; f < sm; f++) {
-    for (int o = t; o < NP * NP * H; o += 256) {  // along x: X1[k][j][kx]
-      int kx = o % H, j = (o / H) % NP, k = o / (H * NP);
-      double re = 0, im = 0;
-#pragma unroll
-      for (int i = 0; i < NP; i++) {
-        int si = s_c2s[(i * NP + j) * NP + k];
-        if (si >= 0) {
-          double v = s_den[si * sm + f];
-          double2 w = s_w[((i + NP / 2) * kx) % N];
-          re += v * w.x;
-          im += v * w.y;
-        }
-      }
-      X1[o] = make_double2(re, im);
-    }
-    __syncthreads();
-    for (int o = t; o < NP * N * H; o += 256) {  // along y: X2[k][ky][kx]
-      int kx = o % H, ky = (o / H) % N, k = o / (H * N);
-      double2 acc = make_double2(0, 0);
-#pragma unroll
-      for (int j = 0; j < NP; j++) {
-        double2 v = cmul(X1[(k * NP + j) * H + kx], s_w[((j + NP / 2) * ky) % N]);
-        acc.x += v.x;
-        acc.y += v.y;
-      }
-      X2[o] = acc;
-    }
-    __syncthreads();
-    for (int o = t; o < C; o += 256) {  // along z
-      int kx = o % H, ky = (o / H) % N, kz = o / (H * N);
-      double2 acc = make_double2(0, 0);
-#pragma unroll
-      for (int k = 0; k < NP; k++) {
-        double2 v = cmul(X2[(k * N + ky) * H + kx], s_w[((k + NP / 2) * kz) % N]);
-        acc.x += v.x;
-        acc.y += v.y;
-      }
-      cstore(out + ((long)f * C + o) * os, acc);
-    }
-    __syncthreads();
-  }
-}
-
-template <int NP, class CT>
-__global__ __launch_bounds__(256) void k_fft_inv(const int* __restrict__ slot_node, int Rc,
-                                                 const CT* __restrict__ effval, int tm,
-                                                 const int* __restrict__ s2c, int nsp, double* __restrict__ dncval,
-                                                 int pln, const double2* __restrict__ tw, int rhs_inner)
-{
-  constexpr int N = 2 * NP, H = NP + 1, C = N * N * H;
-  __shared__ double2 s_x[C];           // spectrum, later Y2[k][j][kx]
-  __shared__ double2 Y1[NP * N * H];   // [k][ky][kx]
-  __shared__ double2 s_w[N];
-  const int slot = blockIdx.x, r = blockIdx.y, t = threadIdx.x;
-  if (t < N) s_w[t] = tw[t];
-  double* out = dncval + ((long)slot_node[slot] * Rc + r) * pln;
-  const long os = rhs_inner ? Rc : 1;
-  const CT* in = effval + (long)slot * Rc * tm * C + (rhs_inner ? (long)r : (long)r * tm * C);
-  for (int f = 0; f < tm; f++) {
-    __syncthreads();
-    for (int o = t; o < C; o += 256) s_x[o] = cload(in + ((long)f * C + o) * os);
-    __syncthreads();
-    for (int o = t; o < NP * N * H; o += 256) {  // along z, only the cube's planes
-      int kx = o % H, ky = (o / H) % N, k = o / (H * N);
-      double2 acc = make_double2(0, 0);
-#pragma unroll
-      for (int kz = 0; kz < N; kz++) {
-        double2 w = s_w[((k + NP / 2) * kz) % N];
-        double2 v = cmul(s_x[(kz * N + ky) * H + kx], make_double2(w.x, -w.y));
-        acc.x += v.x;
-        acc.y += v.y;
-      }
-      Y1[o] = acc;
-    }
-    __syncthreads();
-    for (int o = t; o < NP * NP * H; o += 256) {  // along y, only the cube's rows
-      int kx = o % H, j = (o / H) % NP, k = o / (H * NP);
-      double2 acc = make_double2(0, 0);
-#pragma unroll
-      for (int ky = 0; ky < N; ky++) {
-        double2 w = s_w[((j + NP / 2) * ky) % N];
-        double2 v = cmul(Y1[(k * N + ky) * H + kx], make_double2(w.x, -w.y));
-        acc.x += v.x;
-        acc.y += v.y;
-      }
-      s_x[o] = acc;
-    }
-    __syncthreads();
-    for (int p = t; p < nsp; p += 256) {  // along x (c2r) at the surface points
-      int cc = s2c[p];
-      int i = cc / (NP * NP), j = (cc / NP) % NP, k = cc % NP;
-      const double2* y = &s_x[(k * NP + j) * H];
-      int xi = i + NP / 2;
-      double v = y[0].x + ((xi & 1) ? -y[NP].x : y[NP].x);
-#pragma unroll
-      for (int kx = 1; kx < NP; kx++) {
-        double2 w = s_w[(xi * kx) % N];
-        v += 2.0 * (y[kx].x * w.x + y[kx].y * w.y);
-      }
-      out[p * tm + f] += v;
-    }
-  }
-}
-
-// Right-hand-side-innermost forms for scalar kernels (sm = tm = 1).  A block
-// of 64 R threads transforms R right-hand sides of one slot.  Every stage of
-// the separable DFT gives a thread one (rhs, line) pair: it loads the line's
-// inputs into registers once and produces all of the line's outputs, with the
-// twiddles w((i + np/2) m) read from a small table as LDS broadcasts, so LDS
-// traffic is one read and one write per intermediate value.  Global accesses
-// put the R right-hand sides of a frequency next to each other.
-template <int NP>
-__device__ inline void load_twiddles(double2 (*s_wt)[2 * NP], const double2* tw, int t, int nt)
-{
-  constexpr int N = 2 * NP;
-  for (int e = t; e < NP * N; e += nt) {
-    const int i = e / N, m = e - i * N;
-    s_wt[i][m] = tw[((i + NP / 2) * m) % N];
-  }
-}
-
-// Arithmetic of the right-hand-side-innermost transforms: FP64 for FP64
-// spectra, FP32 for FP32 spectra (the spectra are rounded to FP32 for the M2L
-// products either way).
+// Arithmetic of the transforms: FP64 for FP64 spectra, FP32 for FP32 spectra
+// (the spectra are rounded to FP32 for the M2L products either way).
 template <class CT> struct FftMath { using A = double; using A2 = double2; };
 template <> struct FftMath<float2> { using A = float; using A2 = float2; };
 template <class T2> __device__ inline T2 cz()
@@ -2171,13 +2017,145 @@ __device__ inline void load_twiddles_t(A2 (*s_wt)[2 * NP], const double2* tw, in
   }
 }
 
-// Intermediate spectra in LDS: FP64 for FP64 spectra, FP32 for FP32 spectra;
-// per-RHS rows padded by one element so the R rows of a wave's accesses fall
-// in different banks.
-__device__ inline double2 lds_get(const double2& v) { return v; }
-__device__ inline double2 lds_get(const float2& v) { return make_double2(v.x, v.y); }
-__device__ inline void lds_put(double2& d, double2 v) { d = v; }
-__device__ inline void lds_put(float2& d, double2 v) { d = make_float2((float)v.x, (float)v.y); }
+
+template <int NP, class CT>
+__global__ __launch_bounds__(256) void k_fft_fwd(const int* __restrict__ slot_node, int Rc,
+                                                 const double* __restrict__ upeden, int pln, int sm,
+                                                 const double* __restrict__ dscale, const short* __restrict__ c2s,
+                                                 CT* __restrict__ effden, const double2* __restrict__ tw, int rhs_inner)
+{
+  constexpr int N = 2 * NP, H = NP + 1, C = N * N * H, NC = NP * NP * NP;
+  constexpr int NSP = NP * NP * NP - (NP - 2) * (NP - 2) * (NP - 2);
+  using A = typename FftMath<CT>::A;
+  using A2 = typename FftMath<CT>::A2;
+  __shared__ A s_den[NSP * 4];
+  __shared__ short s_c2s[NC];
+  __shared__ A2 s_w[N];
+  __shared__ A2 X1[NP * NP * H];
+  __shared__ A2 X2[NP * N * H];
+  const int slot = blockIdx.x, r = blockIdx.y, t = threadIdx.x;
+  const double* den = upeden + ((long)slot_node[slot] * Rc + r) * pln;
+  for (int i = t; i < pln; i += 256) s_den[i] = (A)(den[i] * dscale[i % sm]);
+  for (int i = t; i < NC; i += 256) s_c2s[i] = c2s[i];
+  if (t < N) s_w[t] = cget<A2>(tw[t]);
+  __syncthreads();
+  const long os = rhs_inner ? Rc : 1;
+  CT* out = effden + (long)slot * Rc * sm * C + (rhs_inner ? (long)r : (long)r * sm * C);
+  for (int f = 0; f < sm; f++) {
+    for (int o = t; o < NP * NP * H; o += 256) {  // along x: X1[k][j][kx]
+      int kx = o % H, j = (o / H) % NP, k = o / (H * NP);
+      A2 acc = cz<A2>();
+#pragma unroll
+      for (int i = 0; i < NP; i++) {
+        int si = s_c2s[(i * NP + j) * NP + k];
+        if (si >= 0) {
+          const A v = s_den[si * sm + f];
+          const A2 w = s_w[((i + NP / 2) * kx) % N];
+          acc.x += v * w.x;
+          acc.y += v * w.y;
+        }
+      }
+      X1[o] = acc;
+    }
+    __syncthreads();
+    for (int o = t; o < NP * N * H; o += 256) {  // along y: X2[k][ky][kx]
+      int kx = o % H, ky = (o / H) % N, k = o / (H * N);
+      A2 acc = cz<A2>();
+#pragma unroll
+      for (int j = 0; j < NP; j++) {
+        const A2 v = cmul2(X1[(k * NP + j) * H + kx], s_w[((j + NP / 2) * ky) % N]);
+        acc.x += v.x;
+        acc.y += v.y;
+      }
+      X2[o] = acc;
+    }
+    __syncthreads();
+    for (int o = t; o < C; o += 256) {  // along z
+      int kx = o % H, ky = (o / H) % N, kz = o / (H * N);
+      A2 acc = cz<A2>();
+#pragma unroll
+      for (int k = 0; k < NP; k++) {
+        const A2 v = cmul2(X2[(k * N + ky) * H + kx], s_w[((k + NP / 2) * kz) % N]);
+        acc.x += v.x;
+        acc.y += v.y;
+      }
+      cset(out[((long)f * C + o) * os], acc);
+    }
+    __syncthreads();
+  }
+}
+
+template <int NP, class CT>
+__global__ __launch_bounds__(256) void k_fft_inv(const int* __restrict__ slot_node, int Rc,
+                                                 const CT* __restrict__ effval, int tm,
+                                                 const int* __restrict__ s2c, int nsp, double* __restrict__ dncval,
+                                                 int pln, const double2* __restrict__ tw, int rhs_inner)
+{
+  constexpr int N = 2 * NP, H = NP + 1, C = N * N * H;
+  using A = typename FftMath<CT>::A;
+  using A2 = typename FftMath<CT>::A2;
+  __shared__ A2 s_x[C];           // spectrum, later Y2[k][j][kx]
+  __shared__ A2 Y1[NP * N * H];   // [k][ky][kx]
+  __shared__ A2 s_w[N];
+  const int slot = blockIdx.x, r = blockIdx.y, t = threadIdx.x;
+  if (t < N) s_w[t] = cget<A2>(tw[t]);
+  double* out = dncval + ((long)slot_node[slot] * Rc + r) * pln;
+  const long os = rhs_inner ? Rc : 1;
+  const CT* in = effval + (long)slot * Rc * tm * C + (rhs_inner ? (long)r : (long)r * tm * C);
+  for (int f = 0; f < tm; f++) {
+    __syncthreads();
+    for (int o = t; o < C; o += 256) s_x[o] = cget<A2>(in[((long)f * C + o) * os]);
+    __syncthreads();
+    for (int o = t; o < NP * N * H; o += 256) {  // along z, only the cube's planes
+      int kx = o % H, ky = (o / H) % N, k = o / (H * N);
+      A2 acc = cz<A2>();
+#pragma unroll
+      for (int kz = 0; kz < N; kz++) {
+        const A2 v = cmulc(s_x[(kz * N + ky) * H + kx], s_w[((k + NP / 2) * kz) % N]);
+        acc.x += v.x;
+        acc.y += v.y;
+      }
+      Y1[o] = acc;
+    }
+    __syncthreads();
+    for (int o = t; o < NP * NP * H; o += 256) {  // along y, only the cube's rows
+      int kx = o % H, j = (o / H) % NP, k = o / (H * NP);
+      A2 acc = cz<A2>();
+#pragma unroll
+      for (int ky = 0; ky < N; ky++) {
+        const A2 v = cmulc(Y1[(k * N + ky) * H + kx], s_w[((j + NP / 2) * ky) % N]);
+        acc.x += v.x;
+        acc.y += v.y;
+      }
+      s_x[o] = acc;
+    }
+    __syncthreads();
+    for (int p = t; p < nsp; p += 256) {  // along x (c2r) at the surface points
+      int cc = s2c[p];
+      int i = cc / (NP * NP), j = (cc / NP) % NP, k = cc % NP;
+      const A2* y = &s_x[(k * NP + j) * H];
+      int xi = i + NP / 2;
+      A v = y[0].x + ((xi & 1) ? -y[NP].x : y[NP].x);
+#pragma unroll
+      for (int kx = 1; kx < NP; kx++) {
+        const A2 w = s_w[(xi * kx) % N];
+        v += (A)2 * (y[kx].x * w.x + y[kx].y * w.y);
+      }
+      out[p * tm + f] += v;
+    }
+  }
+}
+
+// Right-hand-side-innermost forms for scalar kernels (sm = tm = 1).  A block
+// of 64 R threads transforms R right-hand sides of one slot.  Every stage of
+// the separable DFT gives a thread one (rhs, line) pair: it loads the line's
+// inputs into registers once and produces all of the line's outputs, with the
+// twiddles w((i + np/2) m) read from a small table as LDS broadcasts, so LDS
+// traffic is one read and one write per intermediate value.  Global accesses
+// put the R right-hand sides of a frequency next to each other.  Intermediate
+// spectra in LDS are FP64 for FP64 spectra, FP32 for FP32 spectra; per-RHS rows
+// are padded by one element so the R rows of a wave's accesses fall in
+// different banks.
 
 // Blocks (slot, group of R right-hand sides): the groups of one slot share
 // the 128-byte lines of its spectra, so they are dealt to one XCD side by side.
