@@ -44,7 +44,7 @@ DEFAULT = "laplace-uniform-4m-16rhs"
 # kernel class, measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE; the commands and the
 # calibration are in the named profile.  Keyed by (workload, form, kernel class).
 TRAFFIC = {
-    ("laplace-uniform-4m-16rhs", "mixed", "m2l"): (16.06e9, "profiles/laplace-uniform-4m-16rhs_m2l-tile-f32_traffic.md"),
+    ("laplace-uniform-4m-16rhs", "mixed", "m2l"): (16.08e9, "profiles/laplace-uniform-4m-16rhs_m2l-tile-f32_traffic.md"),
 }
 
 # flops per source-target pair for nrhs right-hand sides (FMA = 2, sqrt/div = 1)

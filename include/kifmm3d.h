@@ -37,11 +37,11 @@ typedef struct kifmm3d kifmm3d;
 
 /* Evaluation forms */
 #define KIFMM3D_MODE_PLAIN 0  /* FP64 throughout: the drop-in form */
-#define KIFMM3D_MODE_MIXED 1  /* mixed precision: FP32 near field (FP32 matrix-core products summed in FP64 per
-                                 64-source tile) and FP32 M2L spectra and products (summed in FP64 per neighbour
-                                 direction); FP64 pseudoinverse applies without compensation; FP64 elsewhere.
-                                 No reference counterpart; error against the FP64 direct sum is reported by the
-                                 benchmark and bounded in the tests. */
+#define KIFMM3D_MODE_MIXED 1  /* mixed precision: Laplace near field, S2M / S2L and L2T / M2T as FP32 matrix-core
+                                 products summed in FP64 per 128-source tile; FP32 M2L spectra, FFT arithmetic,
+                                 operators and products; FP64 pseudoinverse applies without compensation; FP64
+                                 elsewhere.  No reference counterpart; error against the FP64 direct sum is
+                                 reported by the benchmark and bounded in the tests. */
 
 /* Per-stage timer slots filled by kifmm3d_evaluate*() (milliseconds, HIP events) */
 enum {
