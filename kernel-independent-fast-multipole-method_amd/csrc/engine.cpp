@@ -1339,13 +1339,14 @@ void BatchRunner::evaluate(const double* const* d_srcden, double* const* d_trgva
       HIPCHK(hipEventRecord(fork_join_[0], s0));
       for (int k = 1; k < nb; k++) HIPCHK(hipStreamWaitEvent(br_[k], fork_join_[0], 0));
       // branches balanced by cost, largest problem first onto the least loaded
-      // branch; the cost is the problem's last timed evaluate per right-hand
-      // side, or its target count when it has none
+      // branch; the cost is each problem's last timed evaluate per right-hand
+      // side, or, unless every problem has one, its target count
       std::vector<double> cost(n);
-      for (size_t i = 0; i < n; i++) {
-        const double ms = eng_[i]->stage_ms[KIFMM3D_STAGE_TOTAL], lr = eng_[i]->last_nrhs();
-        cost[i] = ms > 0 && lr > 0 ? ms / lr * nrhs[i] : 1e-6 * eng_[i]->tree().textcnt * nrhs[i];
-      }
+      bool timed = true;
+      for (size_t i = 0; i < n; i++) timed = timed && eng_[i]->stage_ms[KIFMM3D_STAGE_TOTAL] > 0 && eng_[i]->last_nrhs() > 0;
+      for (size_t i = 0; i < n; i++)
+        cost[i] = timed ? eng_[i]->stage_ms[KIFMM3D_STAGE_TOTAL] / eng_[i]->last_nrhs() * nrhs[i]
+                        : (double)eng_[i]->tree().textcnt * nrhs[i];
       std::vector<size_t> order(n);
       for (size_t i = 0; i < n; i++) order[i] = i;
       const bool rr = balance_ == 0;
