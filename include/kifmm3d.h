@@ -38,7 +38,9 @@ typedef struct kifmm3d kifmm3d;
 /* Evaluation forms */
 #define KIFMM3D_MODE_PLAIN 0  /* FP64 throughout: the drop-in form */
 #define KIFMM3D_MODE_MIXED 1  /* mixed precision: Laplace near field, S2M / S2L and L2T / M2T as FP32 matrix-core
-                                 products summed in FP64 per 128-source tile; FP32 M2L spectra, FFT arithmetic,
+                                 products summed in FP64 per 128-source tile; the near field of the other
+                                 kernels without normals in FP32 summed in FP64 per 64-source tile; FP32 M2L
+                                 spectra, FFT arithmetic,
                                  operators and products; FP64 pseudoinverse applies without compensation; FP64
                                  elsewhere.  No reference counterpart; error against the FP64 direct sum is
                                  reported by the benchmark and bounded in the tests. */
@@ -185,7 +187,7 @@ int kifmm3d_set_graph(kifmm3d* h, int on);
  * the same sums (the plain form's candidates agree to rounding, 1e-12).
  * settings (may be NULL) receives up to nsettings entries: [0] near-field
  * kernel (0 LDS broadcast, 1 scalar-cache sources, 2 FP64 matrix cores,
- * 3 FP32 matrix cores), [1 + l] M2L form of V-list level l, top level first
+ * 3 FP32 matrix cores, 4 FP32 LDS broadcast), [1 + l] M2L form of V-list level l, top level first
  * (0 tiled matrix-core, 1 parent-grouped matrix-core, 2 grouped VALU,
  * 3 per pair); nsettings = 17 always suffices.  *nrun (may be NULL) receives
  * the number of evaluates run.  Requires setup; allocates scratch densities

@@ -57,6 +57,24 @@ __device__ inline double rdotn(double dx, double dy, double dz, const double* n)
   return (dx * n[0] + dy * n[1]) + dz * n[2];
 }
 
+// FP32 forms (mixed-precision near field of the vector kernels): 1/r from
+// v_rsq_f32 with the same exclusion, r.n unfused.
+__device__ inline float rinv(float r2) { return r2 < 1e-16f ? 0.0f : __builtin_amdgcn_rsqf(r2); }
+__device__ inline float rdotn(float dx, float dy, float dz, const float* n)
+{
+#pragma clang fp contract(off)
+  return (dx * n[0] + dy * n[1]) + dz * n[2];
+}
+
+// the normal pointer may be null (kernels without normals): F comes from the
+// coordinates
+template <class F>
+struct NoDeduceT {
+  using type = F;
+};
+template <class F>
+using NoDeduce = typename NoDeduceT<F>::type;
+
 template <int KT>
 struct Knl;
 
@@ -64,11 +82,10 @@ template <>
 struct Knl<111> {
   static constexpr int S = 1, T = 1;
   static constexpr bool NRM = false;
-  template <int NR>
-  __device__ static inline void add(double dx, double dy, double dz, const double*, const double* f, double (&u)[NR][T],
-                                    const KC& k)
+  template <int NR, class F>
+  __device__ static inline void add(F dx, F dy, F dz, const NoDeduce<F>*, const F* f, F (&u)[NR][T], const KC& k)
   {
-    double g = k.a * rinv(dx * dx + dy * dy + dz * dz);
+    F g = (F)k.a * rinv(dx * dx + dy * dy + dz * dz);
 #pragma unroll
     for (int q = 0; q < NR; q++) u[q][0] += g * f[q];
   }
@@ -78,17 +95,16 @@ template <>
 struct Knl<301> {
   static constexpr int S = 4, T = 3;
   static constexpr bool NRM = false;
-  template <int NR>
-  __device__ static inline void add(double dx, double dy, double dz, const double*, const double* f, double (&u)[NR][T],
-                                    const KC& k)
+  template <int NR, class F>
+  __device__ static inline void add(F dx, F dy, F dz, const NoDeduce<F>*, const F* f, F (&u)[NR][T], const KC& k)
   {
-    double ri = rinv(dx * dx + dy * dy + dz * dz), ri3 = ri * ri * ri;
-    double G = k.a * ri, H = k.a * ri3, A = k.b * ri3;
+    F ri = rinv(dx * dx + dy * dy + dz * dz), ri3 = ri * ri * ri;
+    F G = (F)k.a * ri, H = (F)k.a * ri3, A = (F)k.b * ri3;
 #pragma unroll
     for (int q = 0; q < NR; q++) {
-      const double* g = f + 4 * q;
-      double df = dx * g[0] + dy * g[1] + dz * g[2];
-      double h = H * df + A * g[3];
+      const F* g = f + 4 * q;
+      F df = dx * g[0] + dy * g[1] + dz * g[2];
+      F h = H * df + A * g[3];
       u[q][0] += G * g[0] + h * dx;
       u[q][1] += G * g[1] + h * dy;
       u[q][2] += G * g[2] + h * dz;
@@ -100,17 +116,16 @@ template <>
 struct Knl<321> {
   static constexpr int S = 3, T = 3;
   static constexpr bool NRM = true;
-  template <int NR>
-  __device__ static inline void add(double dx, double dy, double dz, const double* n, const double* f, double (&u)[NR][T],
-                                    const KC& k)
+  template <int NR, class F>
+  __device__ static inline void add(F dx, F dy, F dz, const NoDeduce<F>* n, const F* f, F (&u)[NR][T], const KC& k)
   {
-    double rn = rdotn(dx, dy, dz, n);
-    double ri = rinv(dx * dx + dy * dy + dz * dz), ri2 = ri * ri;
-    double c = (k.a * (ri2 * ri2 * ri)) * rn;
+    F rn = rdotn(dx, dy, dz, n);
+    F ri = rinv(dx * dx + dy * dy + dz * dz), ri2 = ri * ri;
+    F c = ((F)k.a * (ri2 * ri2 * ri)) * rn;
 #pragma unroll
     for (int q = 0; q < NR; q++) {
-      const double* g = f + 3 * q;
-      double w = c * (dx * g[0] + dy * g[1] + dz * g[2]);
+      const F* g = f + 3 * q;
+      F w = c * (dx * g[0] + dy * g[1] + dz * g[2]);
       u[q][0] += w * dx;
       u[q][1] += w * dy;
       u[q][2] += w * dz;
@@ -122,19 +137,18 @@ template <>
 struct Knl<511> {
   static constexpr int S = 3, T = 3;
   static constexpr bool NRM = false;
-  template <int NR>
-  __device__ static inline void add(double dx, double dy, double dz, const double*, const double* f, double (&u)[NR][T],
-                                    const KC& k)
+  template <int NR, class F>
+  __device__ static inline void add(F dx, F dy, F dz, const NoDeduce<F>*, const F* f, F (&u)[NR][T], const KC& k)
   {
-    double ri = rinv(dx * dx + dy * dy + dz * dz);
-    double G = k.a * ri, H = k.b * (ri * ri * ri);
+    F ri = rinv(dx * dx + dy * dy + dz * dz);
+    F G = (F)k.a * ri, H = (F)k.b * (ri * ri * ri);
 #pragma unroll
     for (int q = 0; q < NR; q++) {
-      const double* g = f + 3 * q;
-      double h = H * (dx * g[0] + dy * g[1] + dz * g[2]);
-      u[q][0] += k.c * (G * g[0] + h * dx);
-      u[q][1] += k.c * (G * g[1] + h * dy);
-      u[q][2] += k.c * (G * g[2] + h * dz);
+      const F* g = f + 3 * q;
+      F h = H * (dx * g[0] + dy * g[1] + dz * g[2]);
+      u[q][0] += (F)k.c * (G * g[0] + h * dx);
+      u[q][1] += (F)k.c * (G * g[1] + h * dy);
+      u[q][2] += (F)k.c * (G * g[2] + h * dz);
     }
   }
 };
@@ -144,12 +158,11 @@ template <>
 struct Knl<121> {
   static constexpr int S = 1, T = 1;
   static constexpr bool NRM = true;
-  template <int NR>
-  __device__ static inline void add(double dx, double dy, double dz, const double* n, const double* f, double (&u)[NR][T],
-                                    const KC& k)
+  template <int NR, class F>
+  __device__ static inline void add(F dx, F dy, F dz, const NoDeduce<F>* n, const F* f, F (&u)[NR][T], const KC& k)
   {
-    double ri = rinv(dx * dx + dy * dy + dz * dz);
-    double g = k.a * (ri * ri * ri) * rdotn(dx, dy, dz, n);
+    F ri = rinv(dx * dx + dy * dy + dz * dz);
+    F g = (F)k.a * (ri * ri * ri) * rdotn(dx, dy, dz, n);
 #pragma unroll
     for (int q = 0; q < NR; q++) u[q][0] += g * f[q];
   }
@@ -160,16 +173,15 @@ template <>
 struct Knl<311> {
   static constexpr int S = 3, T = 3;
   static constexpr bool NRM = false;
-  template <int NR>
-  __device__ static inline void add(double dx, double dy, double dz, const double*, const double* f, double (&u)[NR][T],
-                                    const KC& k)
+  template <int NR, class F>
+  __device__ static inline void add(F dx, F dy, F dz, const NoDeduce<F>*, const F* f, F (&u)[NR][T], const KC& k)
   {
-    double ri = rinv(dx * dx + dy * dy + dz * dz);
-    double G = k.a * ri, H = k.a * (ri * ri * ri);
+    F ri = rinv(dx * dx + dy * dy + dz * dz);
+    F G = (F)k.a * ri, H = (F)k.a * (ri * ri * ri);
 #pragma unroll
     for (int q = 0; q < NR; q++) {
-      const double* g = f + 3 * q;
-      double h = H * (dx * g[0] + dy * g[1] + dz * g[2]);
+      const F* g = f + 3 * q;
+      F h = H * (dx * g[0] + dy * g[1] + dz * g[2]);
       u[q][0] += G * g[0] + h * dx;
       u[q][1] += G * g[1] + h * dy;
       u[q][2] += G * g[2] + h * dz;
@@ -182,15 +194,14 @@ template <>
 struct Knl<312> {
   static constexpr int S = 3, T = 1;
   static constexpr bool NRM = false;
-  template <int NR>
-  __device__ static inline void add(double dx, double dy, double dz, const double*, const double* f, double (&u)[NR][T],
-                                    const KC& k)
+  template <int NR, class F>
+  __device__ static inline void add(F dx, F dy, F dz, const NoDeduce<F>*, const F* f, F (&u)[NR][T], const KC& k)
   {
-    double ri = rinv(dx * dx + dy * dy + dz * dz);
-    double H = k.a * (ri * ri * ri);
+    F ri = rinv(dx * dx + dy * dy + dz * dz);
+    F H = (F)k.a * (ri * ri * ri);
 #pragma unroll
     for (int q = 0; q < NR; q++) {
-      const double* g = f + 3 * q;
+      const F* g = f + 3 * q;
       u[q][0] += H * (dx * g[0] + dy * g[1] + dz * g[2]);
     }
   }
@@ -201,16 +212,15 @@ template <>
 struct Knl<322> {
   static constexpr int S = 3, T = 1;
   static constexpr bool NRM = true;
-  template <int NR>
-  __device__ static inline void add(double dx, double dy, double dz, const double* n, const double* f, double (&u)[NR][T],
-                                    const KC& k)
+  template <int NR, class F>
+  __device__ static inline void add(F dx, F dy, F dz, const NoDeduce<F>* n, const F* f, F (&u)[NR][T], const KC& k)
   {
-    double ri = rinv(dx * dx + dy * dy + dz * dz), ri2 = ri * ri, ri3 = ri2 * ri;
-    double rn = rdotn(dx, dy, dz, n);
-    double A = k.a * ri3, B = -3.0 * k.a * (ri3 * ri2) * rn;
+    F ri = rinv(dx * dx + dy * dy + dz * dz), ri2 = ri * ri, ri3 = ri2 * ri;
+    F rn = rdotn(dx, dy, dz, n);
+    F A = (F)k.a * ri3, B = (F)-3 * (F)k.a * (ri3 * ri2) * rn;
 #pragma unroll
     for (int q = 0; q < NR; q++) {
-      const double* g = f + 3 * q;
+      const F* g = f + 3 * q;
       u[q][0] += A * (n[0] * g[0] + n[1] * g[1] + n[2] * g[2]) + B * (dx * g[0] + dy * g[1] + dz * g[2]);
     }
   }
@@ -222,18 +232,17 @@ template <>
 struct Knl<521> {
   static constexpr int S = 3, T = 3;
   static constexpr bool NRM = true;
-  template <int NR>
-  __device__ static inline void add(double dx, double dy, double dz, const double* n, const double* f, double (&u)[NR][T],
-                                    const KC& k)
+  template <int NR, class F>
+  __device__ static inline void add(F dx, F dy, F dz, const NoDeduce<F>* n, const F* f, F (&u)[NR][T], const KC& k)
   {
-    double ri = rinv(dx * dx + dy * dy + dz * dz), ri2 = ri * ri, ri3 = ri2 * ri;
-    double rn = rdotn(dx, dy, dz, n);
-    double A = k.a * ri3, B = k.b * ri3, C = k.c * (ri3 * ri2) * rn;
+    F ri = rinv(dx * dx + dy * dy + dz * dz), ri2 = ri * ri, ri3 = ri2 * ri;
+    F rn = rdotn(dx, dy, dz, n);
+    F A = (F)k.a * ri3, B = (F)k.b * ri3, C = (F)k.c * (ri3 * ri2) * rn;
 #pragma unroll
     for (int q = 0; q < NR; q++) {
-      const double* g = f + 3 * q;
-      double df = dx * g[0] + dy * g[1] + dz * g[2], nf = n[0] * g[0] + n[1] * g[1] + n[2] * g[2];
-      double ar = A * rn, an = A * df, bt = B * nf + C * df;
+      const F* g = f + 3 * q;
+      F df = dx * g[0] + dy * g[1] + dz * g[2], nf = n[0] * g[0] + n[1] * g[1] + n[2] * g[2];
+      F ar = A * rn, an = A * df, bt = B * nf + C * df;
       u[q][0] += ar * g[0] + an * n[0] + bt * dx;
       u[q][1] += ar * g[1] + an * n[1] + bt * dy;
       u[q][2] += ar * g[2] + an * n[2] + bt * dz;
@@ -419,14 +428,14 @@ static inline unsigned xcd_grid(int n) { return 8u * (unsigned)((n + 7) / 8); }
 
 // Densities of sources j0 .. j0+cnt for NR right-hand sides into LDS,
 // s_den[jj][q*S + d]; den points at right-hand side r0 of the block.
-template <int S, int NR, int BS>
-__device__ inline void stage_den(double (*s_den)[NR * S], const double* den, const Points& src, int j0, int cnt, int t)
+template <int S, int NR, int BS, class E>
+__device__ inline void stage_den(E (*s_den)[NR * S], const double* den, const Points& src, int j0, int cnt, int t)
 {
   constexpr int ND = NR * S;
   for (int i = t; i < cnt * ND; i += BS) {
     int jj = i / ND, rem = i - jj * ND;
     int q = rem / S, d = rem - q * S;
-    s_den[jj][rem] = den[(long)(j0 + jj) * src.den_js + (long)q * src.den_rs + d];
+    s_den[jj][rem] = (E)den[(long)(j0 + jj) * src.den_js + (long)q * src.den_rs + d];
   }
 }
 
@@ -478,6 +487,80 @@ __global__ __launch_bounds__(P2P_BS) void k_p2p(const int* __restrict__ tbeg, co
         for (int jj = 0; jj < cnt; jj++)
           K::template add<NR>(x - s_pos[jj][0], y - s_pos[jj][1], z - s_pos[jj][2], &s_pos[jj][NPOS - 3], s_den[jj], u,
                               kc);
+    }
+  }
+  if (act) {
+    double* val = trg.val + (long)r0 * trg.val_rs + (long)(tb + t) * K::T;
+#pragma unroll
+    for (int q = 0; q < NR; q++)
+#pragma unroll
+      for (int d = 0; d < K::T; d++) val[(long)q * trg.val_rs + d] += u[q][d];
+  }
+}
+
+// Mixed-precision form of k_p2p for the kernels without a matrix-core form:
+// coordinates relative to the chunk's first target computed in FP64 and
+// rounded, normals and densities rounded, the kernel evaluated in FP32 and the
+// sums of each staged tile (at most TILE sources) added in FP64.
+template <int KT, int NR>
+__global__ __launch_bounds__(P2P_BS) void k_p2p32(const int* __restrict__ tbeg, const int* __restrict__ tnum,
+                                                  const int* __restrict__ eptr, const int2* __restrict__ ent, Points src,
+                                                  Targets trg, int roff, KC kc)
+{
+  using K = Knl<KT>;
+  constexpr int NPOS = K::NRM ? 6 : 3;
+  constexpr int ND = NR * K::S;
+  __shared__ float s_pos[TILE][NPOS];
+  __shared__ float s_den[TILE][ND];
+  const int c = blockIdx.x, t = threadIdx.x;
+  const int r0 = roff + blockIdx.y * NR;
+  const int tb = tbeg[c], tn = tnum[c];
+  const bool act = t < tn;
+  const double ox = trg.x[tb], oy = trg.y[tb], oz = trg.z[tb];
+  float x = 0, y = 0, z = 0;
+  if (act) {
+    x = (float)(trg.x[tb + t] - ox);
+    y = (float)(trg.y[tb + t] - oy);
+    z = (float)(trg.z[tb + t] - oz);
+  }
+  double u[NR][K::T];
+#pragma unroll
+  for (int q = 0; q < NR; q++)
+#pragma unroll
+    for (int d = 0; d < K::T; d++) u[q][d] = 0.0;
+  const double* den = src.den + (long)r0 * src.den_rs;
+  for (int e = eptr[c]; e < eptr[c + 1]; e++) {
+    const int2 se = ent[e];
+    for (int b0 = 0; b0 < se.y; b0 += TILE) {
+      const int cnt = min(TILE, se.y - b0);
+      const int j0 = se.x + b0;
+      __syncthreads();
+      for (int i = t; i < cnt; i += P2P_BS) {
+        s_pos[i][0] = (float)(src.x[j0 + i] - ox);
+        s_pos[i][1] = (float)(src.y[j0 + i] - oy);
+        s_pos[i][2] = (float)(src.z[j0 + i] - oz);
+        if (K::NRM) {
+          s_pos[i][NPOS - 3] = (float)src.nx[j0 + i];
+          s_pos[i][NPOS - 2] = (float)src.ny[j0 + i];
+          s_pos[i][NPOS - 1] = (float)src.nz[j0 + i];
+        }
+      }
+      stage_den<K::S, NR, P2P_BS>(s_den, den, src, j0, cnt, t);
+      __syncthreads();
+      if (act) {
+        float p[NR][K::T];
+#pragma unroll
+        for (int q = 0; q < NR; q++)
+#pragma unroll
+          for (int d = 0; d < K::T; d++) p[q][d] = 0.0f;
+        for (int jj = 0; jj < cnt; jj++)
+          K::template add<NR>(x - s_pos[jj][0], y - s_pos[jj][1], z - s_pos[jj][2], &s_pos[jj][NPOS - 3], s_den[jj], p,
+                              kc);
+#pragma unroll
+        for (int q = 0; q < NR; q++)
+#pragma unroll
+          for (int d = 0; d < K::T; d++) u[q][d] += (double)p[q][d];
+      }
     }
   }
   if (act) {
@@ -1031,6 +1114,9 @@ static void p2p_launch(hipStream_t s, int variant, int nchunk, int nb, const int
                        src, trg, roff, kc);
   else if (variant == 1)
     hipLaunchKernelGGL((k_p2p_s<KT, NR>), dim3(2 * nchunk, nb), dim3(64), 0, s, tbeg, tnum, eptr, ent, src, trg, roff,
+                       kc);
+  else if (variant == 4)
+    hipLaunchKernelGGL((k_p2p32<KT, NR>), dim3(nchunk, nb), dim3(P2P_BS), 0, s, tbeg, tnum, eptr, ent, src, trg, roff,
                        kc);
   else
     hipLaunchKernelGGL((k_p2p<KT, NR>), dim3(nchunk, nb), dim3(P2P_BS), 0, s, tbeg, tnum, eptr, ent, src, trg, roff,

@@ -941,10 +941,13 @@ void Engine::evaluate_host(const double* srcden, double* trgval, int nrhs, int m
 int Engine::near_variant(bool mixed) const
 {
   const int t = p2p_tuned_[mixed ? 1 : 0];
-  const int v = t >= 0 ? t : mixed ? 3 : p2p_variant_;
+  // the mixed form: FP32 matrix cores for Laplace, FP32 vector kernels for the
+  // kernels without normals (with normals, r.n of nearly tangent pairs cancels
+  // beyond what FP32 coordinates resolve: FP64 there)
+  const int v = t >= 0 ? t : mixed ? (user_.kt == 111 ? 3 : user_.needs_normal ? 0 : 4) : p2p_variant_;
   // the matrix-core near-field kernels are Laplace's; the other kernel types
   // run kernels 2 and 3 as the LDS-broadcast form 0
-  return user_.kt != 111 && v >= 2 ? 0 : v;
+  return user_.kt != 111 && (v == 2 || v == 3) ? 0 : v;
 }
 
 // M2L form of V-list level `level` for a chunk of rc right-hand sides: the
@@ -973,12 +976,13 @@ int Engine::tune(int nrhs, int mode)
   const int rc = choose_rhs_chunk(nrhs);
   // candidates: the mixed form keeps the Laplace near field on the FP32
   // matrix cores (its precision choice, not a schedule); the other kernel
-  // types have the LDS-broadcast and scalar-cache forms
+  // types have the LDS-broadcast and scalar-cache forms, and in the mixed
+  // form the FP32 LDS-broadcast one
   std::vector<int> nf, mf;
   if (user_.kt == 111)
     nf = mixed ? std::vector<int>{3} : std::vector<int>{2, 0, 1};
   else
-    nf = {0, 1};
+    nf = mixed && !user_.needs_normal ? std::vector<int>{4, 0, 1} : std::vector<int>{0, 1};
   if (!plan_.m2l_pairs) {
     if (mm_.sdof == 1 && mm_.tdof == 1 && rc % 16 == 0) mf.push_back(0);
     mf.push_back(1);
@@ -1039,8 +1043,9 @@ void Engine::set_settings(int mode, const int* in, int n)
   if (!in || n < 1) throw Error(KIFMM3D_EINVAL, "settings: empty array");
   const bool mixed = mode == KIFMM3D_MODE_MIXED;
   const int v = in[0];
-  // kernel 3 (FP32 matrix cores) is the mixed form's Laplace near field only
-  if (v < -1 || v > 3 || (v == 3 && !(mixed && user_.kt == 111)))
+  // kernel 3 (FP32 matrix cores) is the mixed form's Laplace near field
+  // only, kernel 4 (FP32 vector) the mixed form's for kernels without normals
+  if (v < -1 || v > 4 || (v == 3 && !(mixed && user_.kt == 111)) || (v == 4 && !(mixed && !user_.needs_normal)))
     throw Error(KIFMM3D_EINVAL, "settings: near-field kernel " + std::to_string(v) + " not available here");
   const size_t nlev = plan_.vlev.size();
   std::vector<int> forms(nlev, -1);

@@ -119,7 +119,8 @@ class Engine {
   // evaluates of that mode.  Returns the number of evaluates run.
   int tune(int nrhs, int mode);
   // Settings in force for a mode: [0] near-field kernel (0 LDS broadcast,
-  // 1 scalar-cache sources, 2 FP64 matrix cores, 3 FP32 matrix cores),
+  // 1 scalar-cache sources, 2 FP64 matrix cores, 3 FP32 matrix cores, 4 FP32
+  // LDS broadcast),
   // [1 + l] M2L form of V-list level l (0 tiled matrix-core, 1 parent-grouped
   // matrix-core, 2 grouped VALU, 3 per pair).  Returns the count (1 + levels).
   int settings(int mode, int* out, int n) const;

@@ -305,7 +305,7 @@ class FMM3d:
     def settings(self, mode: int = MODE_PLAIN) -> dict:
         """{'near': kernel, 'm2l': [form per V-list level]} in force for a mode
         (near: 0 LDS broadcast, 1 scalar-cache sources, 2 FP64 matrix cores,
-        3 FP32 matrix cores; m2l: 0 tiled, 1 parent-grouped matrix-core,
+        3 FP32 matrix cores, 4 FP32 LDS broadcast; m2l: 0 tiled, 1 parent-grouped matrix-core,
         2 grouped VALU, 3 per pair)."""
         st = np.full(17, -1, dtype=np.int32)
         n = lib().kifmm3d_settings(self.h, mode, st.ctypes.data_as(_I), len(st))

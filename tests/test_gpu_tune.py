@@ -116,3 +116,26 @@ def test_tune_and_pin_mixed_laplace():
     # the plain form's settings are untouched by the mixed form's
     again = f.eval(den, nrhs=16, mode=K.MODE_PLAIN).reshape(16, -1)
     assert np.array_equal(again, plain)
+
+
+@pytest.mark.parametrize("kt", [311, 511])
+def test_mixed_fp32_vector_near_field(kt):
+    """Mixed form of kernels without normals: the FP32 vector near field (the
+    default), the FP64 ones pinned, and the tuned choice, against the plain form."""
+    print(f"\n[case] mixed vector near field kt={kt}", flush=True)
+    p, nrm, ctr, lvl = problem(kt, 4000, 90 + kt % 7)
+    f = make(kt, 6, 60, p, nrm, ctr, lvl)
+    den = densities(len(p), 3, 4, 91)
+    plain = f.eval(den, nrhs=4, mode=K.MODE_PLAIN).reshape(4, -1).copy()
+    assert f.settings(K.MODE_MIXED)["near"] == 4
+    for near in (None, 0, 1, "tune"):
+        if near == "tune":
+            st = f.tune(4, K.MODE_MIXED)
+            assert st["near"] in (4, 0, 1)
+        elif near is not None:
+            f.set_settings(K.MODE_MIXED, near)
+        mixed = f.eval(den, nrhs=4, mode=K.MODE_MIXED).reshape(4, -1)
+        assert np.all(np.isfinite(mixed))
+        worst = max(relerr(mixed[r], plain[r]) for r in range(4))
+        print(f"  settings {f.settings(K.MODE_MIXED)}: mixed-vs-plain {worst:.3e}", flush=True)
+        assert worst <= MIXED_VS_PLAIN

@@ -219,3 +219,13 @@ def test_settings_pin_and_validate():
     assert e.value.code == K.EINVAL
     g.set_settings(K.MODE_PLAIN, 1, [1])
     assert g.settings(K.MODE_PLAIN)["near"] == 1
+    # FP32 vector near field: the mixed form's default for kernels without normals
+    assert g.settings(K.MODE_MIXED)["near"] == 4
+    with pytest.raises(K.KifmmError) as e:
+        g.set_settings(K.MODE_PLAIN, 4)
+    assert e.value.code == K.EINVAL
+    h, _ = setup_host(121, 4)  # with normals: FP64 near field in both forms
+    assert h.settings(K.MODE_MIXED)["near"] == 0
+    with pytest.raises(K.KifmmError) as e:
+        h.set_settings(K.MODE_MIXED, 4)
+    assert e.value.code == K.EINVAL
