@@ -516,7 +516,8 @@ __global__ __launch_bounds__(P2P_BS) void k_p2p32(const int* __restrict__ tbeg, 
   const int r0 = roff + blockIdx.y * NR;
   const int tb = tbeg[c], tn = tnum[c];
   const bool act = t < tn;
-  const double ox = trg.x[tb], oy = trg.y[tb], oz = trg.z[tb];
+  const int tb0 = tn > 0 ? tb : 0;  // origin of the relative coordinates
+  const double ox = trg.x[tb0], oy = trg.y[tb0], oz = trg.z[tb0];
   float x = 0, y = 0, z = 0;
   if (act) {
     x = (float)(trg.x[tb + t] - ox);
