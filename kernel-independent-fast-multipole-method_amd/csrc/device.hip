@@ -2420,9 +2420,9 @@ __global__ __launch_bounds__(64 * R) void k_fft_inv_ri(int nslot, const int* __r
 }
 
 // Right-hand sides per block of the FP32-spectrum transforms (FP64: 4).
-// Measured at 4M points / 16 RHS: forward 4 -> 2.00 ms, 8 -> 2.15 ms;
-// inverse 8 -> 2.24 ms, 4 -> 2.29 ms, 16 -> 3.16 ms.
-constexpr int FFT_RW32_FWD = 4, FFT_RW32_INV = 8;
+// Measured at 4M points / 16 RHS with FP32 arithmetic: forward 8 -> 1.74 ms,
+// 4 -> 1.92 ms; inverse 8 -> 1.35 ms, 16 -> 1.63 ms.
+constexpr int FFT_RW32_FWD = 8, FFT_RW32_INV = 8;
 
 template <class CT>
 static void fft_fwd_t(hipStream_t s, int np, int nslot, const int* slot_node, int Rc, const double* upeden, int pln,
