@@ -36,7 +36,7 @@ CONFIGS = {
                                gen="clusters32", ctr=(0.5, 0.5, 0.5), rootlvl=1, nchk=200),
     "laplace-uniform-4m-16rhs": dict(cfg=4, kt=111, coefs=[1.0], n=4194304, np=6, ptsmax="auto", nrhs=16,
                                      gen="cube", ctr=(0.5, 0.5, 0.5), rootlvl=1, nchk=1000,
-                                     ptsmax_cands=(64, 128, 256)),
+                                     ptsmax_cands=(32, 64, 96, 128, 160, 192, 256), gate_np=8),
 }
 DEFAULT = "laplace-uniform-4m-16rhs"
 
@@ -89,10 +89,31 @@ def make_density(c, sdof, seed):
     return rng.uniform(-1, 1, (c["nrhs"], c["n"] * sdof))
 
 
-def cpu_baseline(c, seconds_target=15.0):
+def host_cpu():
+    """(CPU model, logical CPUs of the host, CPUs this process may run on)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        avail = os.cpu_count() or 1
+    return model, os.cpu_count() or 1, avail
+
+
+def cpu_baseline(c):
     """Time the CPU oracle (plain FP64 restatement of the reference's eval, one
-    thread) on a bounded sample of the workload: the same generator, kernel, np
-    and ptsmax with fewer points, one RHS.  Returns (targets/s, sample text)."""
+    thread, as the reference is serial) on a bounded sample of the workload:
+    the same generator, kernel, np and ptsmax with fewer points, one RHS.  One
+    warm-up eval (the reference builds its operators lazily in the first eval
+    and its driver runs three, fmm3d/tt.cpp:97-101), then the median of three
+    warm evals.  Returns the baseline object of the JSON line."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import ctypes
 
@@ -107,12 +128,19 @@ def cpu_baseline(c, seconds_target=15.0):
     o = O.OracleFMM(c["kt"], c["coefs"], c["np"], c["ptsmax"], 10, c["ctr"], c["rootlvl"])
     o.setup(pos, nor, pos)
     den = make_density(s, O.kernel_dofs(c["kt"])[0], 0)[0]
-    o.eval(den)  # first eval builds the lazy operators, as in the reference
-    t0 = time.perf_counter()
-    o.eval(den)
-    dt = time.perf_counter() - t0
+    o.eval(den)  # warm-up: builds the lazy operators
+    times = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        o.eval(den)
+        times.append(time.perf_counter() - t0)
     lib.orc_set_accurate(1)
-    return n / dt, f"{c['gen']} N={n}, np={c['np']}, ptsmax={c['ptsmax']}, 1 RHS, warm eval {dt:.2f} s"
+    dt = float(np.median(times))
+    model, ncpu, avail = host_cpu()
+    return {"value": n / dt, "unit": "targets/s", "cores": 1, "kind": "port",
+            "sample": f"{c['gen']} N={n}, np={c['np']}, ptsmax={c['ptsmax']}, 1 RHS; median of 3 warm evals "
+                      f"after one warm-up: {dt:.2f} s ({', '.join(f'{t:.2f}' for t in times)})",
+            "cpu_model": model, "nproc": ncpu, "cpus_available": avail, "threads": 1}
 
 
 def batch_problems(seed=0x4B1F0005):
@@ -141,6 +169,14 @@ def batch_problems(seed=0x4B1F0005):
     return out
 
 
+def batch_density(pr, sdof, nrhs):
+    """The nrhs density vectors of a batch problem (N(0,1) for Navier, U[-1,1) otherwise)."""
+    rng = np.random.default_rng(pr["seed"])
+    if pr["kt"] == 511:
+        return rng.standard_normal((nrhs, pr["n"] * sdof))
+    return rng.uniform(-1, 1, (nrhs, pr["n"] * sdof))
+
+
 def run_batch(args, K):
     """The 64-problem batch in one graph: aggregate targets/s of the chosen
     form, with the sum of the problems' individual eager evaluations beside it."""
@@ -155,10 +191,7 @@ def run_batch(args, K):
         f.ctr, f.rootlvl = pr["ctr"], pr["rootlvl"]
         f.knl = K.Kernel3d(pr["kt"], pr["coefs"])
         f.setup({"-fmm3d_np": str(pr["np"]), "-fmm3d_let3d_ptsmax": "100", "-fmm3d_let3d_maxlevel": "10"})
-        rng = np.random.default_rng(pr["seed"])
-        sdof = f.knl.sdof()
-        den = rng.standard_normal((nrhs, pr["n"] * sdof)) if pr["kt"] == 511 else \
-            rng.uniform(-1, 1, (nrhs, pr["n"] * sdof))
+        den = batch_density(pr, f.knl.sdof(), nrhs)
         d_src = K.DeviceBuffer(den.nbytes)
         d_src.upload(den)
         d_trg = K.DeviceBuffer(nrhs * pr["n"] * f.knl.tdof() * 8)
@@ -242,17 +275,22 @@ def timed_steps(f, K, d_src, d_trg, nrhs, mode, steps, warmup, label):
     return 1000.0 * dt / steps, {k: v / steps for k, v in stage_sum.items()}
 
 
-def direct_error(f, d_src, d_trg, c, nrhs, n, tdof, seed, idx_cache={}):
+def direct_error(f, d_src, d_trg, c, nrhs, n, tdof, seed, cache):
+    """Relative L2 error of the values in d_trg at c["nchk"] seeded targets x
+    all right-hand sides against the GPU FP64 direct sum: (||u - u_dir|| /
+    ||u_dir||, ||u - u_dir|| / ||u||), the second being the reference's printed
+    variant (fmm3d/fmm3d_check.cpp:61-69).  The direct sums depend only on the
+    points and densities, so they are computed once per (points, nrhs)."""
     val = np.empty((nrhs, n * tdof))
     d_trg.download(val)
     rng = np.random.default_rng(seed + 2)
     idx = np.sort(rng.choice(n, c["nchk"], replace=False)).astype(np.int32)
-    key = (id(f), nrhs)
-    if key not in idx_cache:
-        idx_cache[key] = f.direct(d_src, idx, nrhs)
-    d = idx_cache[key]
+    if nrhs not in cache:
+        cache[nrhs] = f.direct(d_src, idx, nrhs)
+    d = cache[nrhs]
     v = val.reshape(nrhs, n, tdof)[:, idx, :].reshape(nrhs, -1)
-    return float(np.linalg.norm(v - d) / np.linalg.norm(d))
+    e = float(np.linalg.norm(v - d))
+    return e / float(np.linalg.norm(d)), e / float(np.linalg.norm(v))
 
 
 def kernel_rates(f, c, stats, nrhs, mixed):
@@ -284,20 +322,56 @@ def kernel_rates(f, c, stats, nrhs, mixed):
         ms, calls = kt_times[k]
         if ms > 0 and calls > 0:
             rate = amt / (ms * 1e-3) / (1e12 if unit == "flop" else 1e9)
-            rates[k] = (ms, calls, rate, unit, bound, peak)
+            rates[k] = (ms, calls, rate, unit, bound, peak, amt)
             log(f"  kernel {k:9s}: {ms:9.3f} ms in {calls:3d} launch groups, "
                 f"{rate:9.3f} {'TFLOP/s' if unit == 'flop' else 'GB/s'} ({100 * rate / peak:.1f}% of peak)")
     return rates
 
 
-def setup_fmm(K, c, pos, nor, ptsmax):
+def setup_fmm(K, c, pos, nor, ptsmax, np_=None):
     f = K.FMM3d("fmm3d_")
     f.srcpos, f.srcnor, f.trgpos = pos, nor, pos
     f.ctr, f.rootlvl = c["ctr"], c["rootlvl"]
     f.knl = K.Kernel3d(c["kt"], c["coefs"])
     t0 = time.perf_counter()
-    f.setup({"-fmm3d_np": str(c["np"]), "-fmm3d_let3d_ptsmax": str(ptsmax), "-fmm3d_let3d_maxlevel": "10"})
+    f.setup({"-fmm3d_np": str(np_ or c["np"]), "-fmm3d_let3d_ptsmax": str(ptsmax), "-fmm3d_let3d_maxlevel": "10"})
     return f, time.perf_counter() - t0
+
+
+def run_forms(K, f, forms, c, args, d_src, d_trg, nrhs, n, tdof, seed, stats, cache, label=""):
+    """Tune, time and check each form on one set-up handle; cache holds the
+    direct sums of these points and densities."""
+    res = {}
+    for name, mode in forms:
+        tag = f"{label}{name}"
+        tuned = None
+        if not args.no_tune:
+            t0 = time.perf_counter()
+            st = f.tune(nrhs, mode)
+            tuned = {"near": st["near"], "m2l": st["m2l"], "s": round(time.perf_counter() - t0, 3)}
+            log(f"{tag}: autotuned in {tuned['s']:.2f} s ({st['evaluates']} evaluates): near-field kernel "
+                f"{st['near']}, M2L form per V-list level {st['m2l']}")
+        ms, stages = timed_steps(f, K, d_src, d_trg, nrhs, mode, args.steps, args.warmup, tag)
+        value = n * nrhs / (ms * 1e-3)
+        log(f"{tag}: {args.steps} steps, {ms:.2f} ms/step, {value:.4e} targets/s")
+        rates = kernel_rates(f, c, stats, nrhs, mode == K.MODE_MIXED)
+        err = errf = None
+        if not args.no_check:
+            err, errf = direct_error(f, d_src, d_trg, c, nrhs, n, tdof, seed, cache)
+            log(f"{tag}: rel L2 error vs FP64 direct sum on {c['nchk']} targets x {nrhs} RHS: {err:.3e} "
+                f"(FMM-norm variant {errf:.3e})")
+        res[name] = dict(ms=ms, value=value, stages=stages, rates=rates, err=err, errf=errf, tuned=tuned)
+    return res
+
+
+def kernel_table(rates):
+    """Per kernel class of one form: time, algorithmic rate and its fraction of the class's peak."""
+    out = {}
+    for k, (ms, calls, rate, unit, bound, peak, amt) in rates.items():
+        out[k] = {"ms": round(ms, 3), "launch_groups": calls, "bound": bound,
+                  "achieved": round(rate, 3), "unit": "TFLOP/s" if unit == "flop" else "GB/s",
+                  "peak": peak, "frac": round(rate / peak, 4)}
+    return out
 
 
 def main():
@@ -312,6 +386,8 @@ def main():
                     help="both: time the plain FP64 form and the mixed form; value is the mixed form's")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-check", action="store_true", help="skip the direct-sum error check")
+    ap.add_argument("--no-gate", action="store_true",
+                    help="skip the higher-np leg timed beside the workload's np (for the 1e-6 error gate)")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the autotuner (near-field kernel and per-level M2L form, timed on the live tree)")
     ap.add_argument("--calibrate", action="store_true",
@@ -353,77 +429,72 @@ def main():
 
     ptsmax = args.ptsmax or c["ptsmax"]
     tune = None
+    sdof, tdof = K.Kernel3d(c["kt"]).sdof(), K.Kernel3d(c["kt"]).tdof()
+    den = make_density(c, sdof, seed)
+    d_src = K.DeviceBuffer(den.nbytes)
+    d_src.upload(den)
+    d_trg = K.DeviceBuffer(nrhs * n * tdof * 8)
     if str(ptsmax) == "auto":
         # pick the leaf size by timing one warm evaluate of the chosen form per candidate
         tune = {}
         best = None
         for p in c["ptsmax_cands"]:
             f, ts = setup_fmm(K, c, pos, nor, p)
-            sdof, tdof = f.knl.sdof(), f.knl.tdof()
-            den = make_density(c, sdof, seed)
-            d_src = K.DeviceBuffer(den.nbytes)
-            d_src.upload(den)
-            d_trg = K.DeviceBuffer(nrhs * n * tdof * 8)
             ms, _ = timed_steps(f, K, d_src, d_trg, nrhs, chosen_mode, 1, 1, f"tune ptsmax={p}")
             tune[p] = round(ms, 2)
             log(f"tune ptsmax={p}: setup {ts:.2f} s, {ms:.2f} ms/step")
             if best is None or ms < best[1]:
                 best = (p, ms)
-            del f, d_src, d_trg
+            del f
         ptsmax = best[0]
     ptsmax = int(ptsmax)
     f, t_setup = setup_fmm(K, c, pos, nor, ptsmax)
     info, stats = f.info(), f.plan_stats()
     log(f"{args.config}: ptsmax {ptsmax}, setup {t_setup:.2f} s, {info['nnode']} nodes, {info['nlevel']} levels, "
         f"near pairs {stats['near_pairs']:.3e}, V pairs {stats['v_pairs']:.3e}")
-    sdof, tdof = f.knl.sdof(), f.knl.tdof()
-    den = make_density(c, sdof, seed)
-    d_src = K.DeviceBuffer(den.nbytes)
-    d_src.upload(den)
-    d_trg = K.DeviceBuffer(nrhs * n * tdof * 8)
-
-    res = {}
-    for name, mode in forms:
-        tuned = None
-        if not args.no_tune:
-            t0 = time.perf_counter()
-            st = f.tune(nrhs, mode)
-            tuned = {"near": st["near"], "m2l": st["m2l"], "s": round(time.perf_counter() - t0, 3)}
-            log(f"{name}: autotuned in {tuned['s']:.2f} s ({st['evaluates']} evaluates): near-field kernel "
-                f"{st['near']}, M2L form per V-list level {st['m2l']}")
-        ms, stages = timed_steps(f, K, d_src, d_trg, nrhs, mode, args.steps, args.warmup, name)
-        value = n * nrhs / (ms * 1e-3)
-        log(f"{name}: {args.steps} steps, {ms:.2f} ms/step, {value:.4e} targets/s")
-        rates = kernel_rates(f, c, stats, nrhs, mode == K.MODE_MIXED)
-        err = None
-        if not args.no_check:
-            err = direct_error(f, d_src, d_trg, c, nrhs, n, tdof, seed)
-            log(f"{name}: rel L2 error vs FP64 direct sum on {c['nchk']} targets x {nrhs} RHS: {err:.3e}")
-        res[name] = dict(ms=ms, value=value, stages=stages, rates=rates, err=err, tuned=tuned)
+    cache = {}
+    res = run_forms(K, f, forms, c, args, d_src, d_trg, nrhs, n, tdof, seed, stats, cache)
 
     chosen = forms[-1][0]
     r = res[chosen]
     # roofline of the chosen form's dominant kernel class, timed live with HIP
     # events on the library's stream during its last timed step
     dom = max(r["rates"], key=lambda k: r["rates"][k][0])
-    ms, calls, rate, unit, bound, peak = r["rates"][dom]
+    ms, calls, rate, unit, bound, peak, amt = r["rates"][dom]
     tr = TRAFFIC.get((args.config, chosen, dom))
     roofline = {"kernel": dom, "bound": bound, "achieved": round(rate, 4), "peak": peak,
                 "unit": "TFLOP/s" if unit == "flop" else "GB/s", "frac": round(rate / peak, 5),
-                "traffic": tr[0] if tr else None, "ms_per_step": round(ms, 3), "launch_groups": calls}
+                "traffic": tr[0] if tr else None, "ms_per_step": round(ms, 3), "launch_groups": calls,
+                "algorithmic_per_step": amt}
     if tr:
         roofline["traffic_note"] = f"bytes per evaluate over the {calls} launches, from {tr[1]}"
 
+    # the metric's gate (error <= 1e-6 against the direct sum) is below the
+    # truncation error of np 6 on this workload; the same points at the
+    # higher np are timed beside it, both forms, with the same checks
+    gate = None
+    if c.get("gate_np") and not args.no_gate:
+        del f
+        g, t_g = setup_fmm(K, c, pos, nor, ptsmax, np_=c["gate_np"])
+        gst = g.plan_stats()
+        gres = run_forms(K, g, forms, dict(c, np=c["gate_np"]), args, d_src, d_trg, nrhs, n, tdof, seed, gst,
+                         cache, label=f"np{c['gate_np']} ")
+        gate = {"np": c["gate_np"], "ptsmax": ptsmax, "setup_s": round(t_g, 3)}
+        for name, q in gres.items():
+            gate[name] = {"value": q["value"], "ms_per_step": round(q["ms"], 3), "rel_err_vs_direct": q["err"],
+                          "rel_err_fmm_norm": q["errf"], "tuned": q["tuned"]}
+        del g
+
     cpu = None
     if not args.no_cpu:
-        cv, sample = cpu_baseline(dict(c, ptsmax=ptsmax))
-        cpu = {"value": cv, "unit": "targets/s", "cores": 1, "kind": "port", "sample": sample}
-        log(f"cpu baseline: {cv:.4e} targets/s ({sample})")
+        cpu = cpu_baseline(dict(c, ptsmax=ptsmax))
+        log(f"cpu baseline: {cpu['value']:.4e} targets/s ({cpu['sample']}; {cpu['cpu_model']}, "
+            f"nproc {cpu['nproc']}, threads {cpu['threads']})")
 
     cfg = {"workload": args.config, "form": "mixed FP32/FP64" if chosen == "mixed" else "plain FP64",
            "kernel": c["kt"], "n_points": n, "nrhs": nrhs, "np": c["np"], "ptsmax": ptsmax,
-           "rel_err_vs_direct": r["err"], "setup_s": round(t_setup, 3),
-           "stage_ms": {k: round(v, 3) for k, v in r["stages"].items()}}
+           "rel_err_vs_direct": r["err"], "rel_err_fmm_norm": r["errf"], "setup_s": round(t_setup, 3),
+           "stage_ms": {k: round(v, 3) for k, v in r["stages"].items()}, "kernels": kernel_table(r["rates"])}
     if tune:
         cfg["ptsmax_tuning_ms"] = tune
     if r["tuned"]:
@@ -431,8 +502,12 @@ def main():
     if "plain" in res and chosen != "plain":
         p = res["plain"]
         cfg["plain_fp64"] = {"value": p["value"], "ms_per_step": round(p["ms"], 3), "rel_err_vs_direct": p["err"],
-                             "stage_ms": {k: round(v, 3) for k, v in p["stages"].items()}, "tuned": p["tuned"]}
+                             "rel_err_fmm_norm": p["errf"],
+                             "stage_ms": {k: round(v, 3) for k, v in p["stages"].items()}, "tuned": p["tuned"],
+                             "kernels": kernel_table(p["rates"])}
         cfg["speedup_vs_plain"] = round(r["value"] / p["value"], 3)
+    if gate:
+        cfg["at_error_gate"] = gate
     out = {
         "metric": METRIC,
         "value": r["value"],

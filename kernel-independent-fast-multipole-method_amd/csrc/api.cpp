@@ -255,8 +255,17 @@ int kifmm3d_batch_create(kifmm3d_batch** out, kifmm3d* const* handles, int n)
   if (n < 1 || !handles) return KIFMM3D_EINVAL;
   try {
     auto b = std::make_unique<kifmm3d_batch>();
+    // one engine has one set of work buffers: recorded twice, its two
+    // problems would run concurrently on different graph branches over them
     for (int i = 0; i < n; i++) {
       if (!handles[i]) return KIFMM3D_EINVAL;
+      for (int j = 0; j < i; j++)
+        if (handles[j] == handles[i]) {
+          g_create_error = "batch handles " + std::to_string(j) + " and " + std::to_string(i) + " are the same handle";
+          return KIFMM3D_EINVAL;
+        }
+    }
+    for (int i = 0; i < n; i++) {
       if (!handles[i]->eng->is_setup()) {
         g_create_error = "batch handle " + std::to_string(i) + " is not set up";
         return KIFMM3D_ESTATE;

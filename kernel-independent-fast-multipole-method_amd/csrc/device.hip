@@ -501,11 +501,15 @@ __global__ __launch_bounds__(P2P_BS) void k_p2p(const int* __restrict__ tbeg, co
 // Mixed-precision form of k_p2p for the kernels without a matrix-core form:
 // coordinates relative to the chunk's first target computed in FP64 and
 // rounded, normals and densities rounded, the kernel evaluated in FP32 and the
-// sums of each staged tile (at most TILE sources) added in FP64.
+// sums of each staged tile (at most TILE sources) added in FP64.  Entries
+// flagged 2 in echk (a pair may be closer than the relative FP32 coordinates
+// resolve, so the r < 1e-8 rule needs the FP64 positions) take each pair's
+// displacement as the FP64 difference rounded to FP32 instead.
 template <int KT, int NR>
 __global__ __launch_bounds__(P2P_BS) void k_p2p32(const int* __restrict__ tbeg, const int* __restrict__ tnum,
-                                                  const int* __restrict__ eptr, const int2* __restrict__ ent, Points src,
-                                                  Targets trg, int roff, KC kc)
+                                                  const int* __restrict__ eptr, const int2* __restrict__ ent,
+                                                  const int* __restrict__ echk, Points src, Targets trg, int roff,
+                                                  KC kc)
 {
   using K = Knl<KT>;
   constexpr int NPOS = K::NRM ? 6 : 3;
@@ -519,10 +523,14 @@ __global__ __launch_bounds__(P2P_BS) void k_p2p32(const int* __restrict__ tbeg, 
   const int tb0 = tn > 0 ? tb : 0;  // origin of the relative coordinates
   const double ox = trg.x[tb0], oy = trg.y[tb0], oz = trg.z[tb0];
   float x = 0, y = 0, z = 0;
+  double x64 = 0, y64 = 0, z64 = 0;
   if (act) {
-    x = (float)(trg.x[tb + t] - ox);
-    y = (float)(trg.y[tb + t] - oy);
-    z = (float)(trg.z[tb + t] - oz);
+    x64 = trg.x[tb + t];
+    y64 = trg.y[tb + t];
+    z64 = trg.z[tb + t];
+    x = (float)(x64 - ox);
+    y = (float)(y64 - oy);
+    z = (float)(z64 - oz);
   }
   double u[NR][K::T];
 #pragma unroll
@@ -532,6 +540,7 @@ __global__ __launch_bounds__(P2P_BS) void k_p2p32(const int* __restrict__ tbeg, 
   const double* den = src.den + (long)r0 * src.den_rs;
   for (int e = eptr[c]; e < eptr[c + 1]; e++) {
     const int2 se = ent[e];
+    const bool exact = echk && echk[e] == 2;  // block-uniform
     for (int b0 = 0; b0 < se.y; b0 += TILE) {
       const int cnt = min(TILE, se.y - b0);
       const int j0 = se.x + b0;
@@ -554,9 +563,14 @@ __global__ __launch_bounds__(P2P_BS) void k_p2p32(const int* __restrict__ tbeg, 
         for (int q = 0; q < NR; q++)
 #pragma unroll
           for (int d = 0; d < K::T; d++) p[q][d] = 0.0f;
-        for (int jj = 0; jj < cnt; jj++)
-          K::template add<NR>(x - s_pos[jj][0], y - s_pos[jj][1], z - s_pos[jj][2], &s_pos[jj][NPOS - 3], s_den[jj], p,
-                              kc);
+        if (exact)
+          for (int jj = 0; jj < cnt; jj++)
+            K::template add<NR>((float)(x64 - src.x[j0 + jj]), (float)(y64 - src.y[j0 + jj]),
+                                (float)(z64 - src.z[j0 + jj]), &s_pos[jj][NPOS - 3], s_den[jj], p, kc);
+        else
+          for (int jj = 0; jj < cnt; jj++)
+            K::template add<NR>(x - s_pos[jj][0], y - s_pos[jj][1], z - s_pos[jj][2], &s_pos[jj][NPOS - 3], s_den[jj],
+                                p, kc);
 #pragma unroll
         for (int q = 0; q < NR; q++)
 #pragma unroll
@@ -657,14 +671,25 @@ __device__ __forceinline__ void p2p_tile64(int cnt4, int kk, int col, const doub
 
 constexpr int TILE32 = 128;  // sources per staged tile of the FP32 kernel (measured 2 % faster than 64)
 
-// CHK: pairs with r^2 < 1e-16 contribute zero (the reference's r < 1e-8
-// rule), needed where a source and a target can coincide -- only in the
-// target's own leaf, since equal points always fall in the same leaf.
-// Elsewhere 1/r is taken of max(r^2, 1e-16), one instruction instead of two.
-template <int NG, bool CHK = true>
-__device__ __forceinline__ void p2p_tile32(int cnt4, int kk, int col, const float (*s_pos)[TILE32],
+// The reference's r < 1e-8 rule (kernel3d.cpp:21, :132) by entry, as the
+// set-up flagged it (echk):
+//   CHK 0: no pair of the entry is closer than a few 1e-8 (two different
+//          leaves without points next to their shared boundary): 1/r of
+//          max(r^2, 1e-16), one instruction instead of two;
+//   CHK 1: the target's own leaf with no two distinct points that close:
+//          equal points (equal FP32 offsets) have r^2 = 0 and drop out;
+//   CHK 2: a pair can be closer than the FP32 offsets resolve (leaves with
+//          points next to a shared boundary, or a leaf holding two distinct
+//          near-coincident points): the pair's 1/r and the rule are taken
+//          from the FP64 coordinates (x64: the entry's sources, t64: the
+//          lane's targets), so near-coincident pairs get the plain form's
+//          values.
+template <int NG, int CHK>
+__device__ __forceinline__ void p2p_tile32(int cnt, int cnt4, int kk, int col, const float (*s_pos)[TILE32],
                                            const float (*s_den)[16], const float* tx, const float* ty,
-                                           const float* tz, double (*acc)[4])
+                                           const float* tz, double (*acc)[4], const double* x64 = nullptr,
+                                           const double* y64 = nullptr, const double* z64 = nullptr,
+                                           const double (*t64)[3] = nullptr)
 {
   v4f part[NG];
 #pragma unroll
@@ -672,11 +697,24 @@ __device__ __forceinline__ void p2p_tile32(int cnt4, int kk, int col, const floa
   for (int g = 0; g < cnt4; g += 4) {
     const float sx = s_pos[0][g + kk], sy = s_pos[1][g + kk], sz = s_pos[2][g + kk];
     const float f = s_den[g + kk][col];
+    double qx = 0, qy = 0, qz = 0;
+    if (CHK == 2) {
+      const int j = g + kk < cnt ? g + kk : 0;  // padded sources have zero density
+      qx = x64[j];
+      qy = y64[j];
+      qz = z64[j];
+    }
 #pragma unroll
     for (int i = 0; i < NG; i++) {
-      const float dx = tx[i] - sx, dy = ty[i] - sy, dz = tz[i] - sz;
-      const float r2 = dx * dx + dy * dy + dz * dz;
-      const float gv = CHK ? (r2 < 1e-16f ? 0.0f : __builtin_amdgcn_rsqf(r2)) : __builtin_amdgcn_rsqf(fmaxf(r2, 1e-16f));
+      float gv;
+      if (CHK == 2) {
+        const double ex = t64[i][0] - qx, ey = t64[i][1] - qy, ez = t64[i][2] - qz;
+        gv = (float)rinv(ex * ex + ey * ey + ez * ez);
+      } else {
+        const float dx = tx[i] - sx, dy = ty[i] - sy, dz = tz[i] - sz;
+        const float r2 = dx * dx + dy * dy + dz * dz;
+        gv = CHK == 1 ? (r2 < 1e-16f ? 0.0f : __builtin_amdgcn_rsqf(r2)) : __builtin_amdgcn_rsqf(fmaxf(r2, 1e-16f));
+      }
       part[i] = __builtin_amdgcn_mfma_f32_16x16x4f32(f, gv, part[i], 0, 0, 0);
     }
   }
@@ -757,12 +795,12 @@ __global__ __launch_bounds__(P2P_BS) void k_p2p_mfma111(int nchunk, const int* _
 // relative to the chunk's first target in FP64 and then rounded, so FP32
 // keeps the differences of nearby points to ~1e-7 of the leaf size; 1/r
 // comes from v_rsq_f32.  The FP32 products of one staged source tile (at most
-// 128 sources) are summed into FP64 accumulators.  selfe[c] is the entry of
-// the chunk's own leaf (or -1): only its pairs take the r < 1e-8 exclusion.
+// 128 sources) are summed into FP64 accumulators.  echk[e] selects how the
+// pairs of entry e take the r < 1e-8 exclusion (p2p_tile32).
 __global__ __launch_bounds__(P2P_BS) void k_p2p_mfma111_f32(int nchunk, const int* __restrict__ tbeg,
                                                             const int* __restrict__ tnum, const int* __restrict__ eptr,
                                                             const int2* __restrict__ ent,
-                                                            const int* __restrict__ selfe, Points src, Targets trg,
+                                                            const int* __restrict__ echk, Points src, Targets trg,
                                                             int roff, KC kc, int nrv)
 {
   constexpr int NR = 16;
@@ -792,10 +830,9 @@ __global__ __launch_bounds__(P2P_BS) void k_p2p_mfma111_f32(int nchunk, const in
     for (int v = 0; v < 4; v++) acc[i][v] = 0.0;
   const double* den = src.den + (long)r0 * src.den_rs;
   const float* den32 = src.den32 ? src.den32 + (long)r0 * src.den_rs : nullptr;
-  const int eself = selfe[c];
   for (int e = eptr[c]; e < eptr[c + 1]; e++) {
     const int2 se = ent[e];
-    const bool chk = e == eself;  // block-uniform
+    const int chk = echk[e];  // block-uniform
     for (int b0 = 0; b0 < se.y; b0 += TILE32) {
       const int cnt = min(TILE32, se.y - b0), cnt4 = (cnt + 3) & ~3;
       const int j0 = se.x + b0;
@@ -812,24 +849,44 @@ __global__ __launch_bounds__(P2P_BS) void k_p2p_mfma111_f32(int nchunk, const in
         s_den[jj][q] = jj < cnt && q < nrv ? (den32 ? den32[at] : (float)den[at]) : 0.0f;
       }
       __syncthreads();
-      if (chk) {
+      if (chk == 2) {
+        // rare: FP64 coordinates of the lane's targets (clamped like tx) and of the tile's sources
+        double t64[4][3];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          int tt = 64 * w + 16 * i + col;
+          tt = tt < tn ? tt : 0;
+          t64[i][0] = trg.x[tb + tt];
+          t64[i][1] = trg.y[tb + tt];
+          t64[i][2] = trg.z[tb + tt];
+        }
+        const double *qx = src.x + j0, *qy = src.y + j0, *qz = src.z + j0;
         if (ng == 4)
-          p2p_tile32<4, true>(cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc);
+          p2p_tile32<4, 2>(cnt, cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc, qx, qy, qz, t64);
         else if (ng == 3)
-          p2p_tile32<3, true>(cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc);
+          p2p_tile32<3, 2>(cnt, cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc, qx, qy, qz, t64);
         else if (ng == 2)
-          p2p_tile32<2, true>(cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc);
+          p2p_tile32<2, 2>(cnt, cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc, qx, qy, qz, t64);
         else if (ng == 1)
-          p2p_tile32<1, true>(cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc);
+          p2p_tile32<1, 2>(cnt, cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc, qx, qy, qz, t64);
+      } else if (chk == 1) {
+        if (ng == 4)
+          p2p_tile32<4, 1>(cnt, cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc);
+        else if (ng == 3)
+          p2p_tile32<3, 1>(cnt, cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc);
+        else if (ng == 2)
+          p2p_tile32<2, 1>(cnt, cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc);
+        else if (ng == 1)
+          p2p_tile32<1, 1>(cnt, cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc);
       } else {
         if (ng == 4)
-          p2p_tile32<4, false>(cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc);
+          p2p_tile32<4, 0>(cnt, cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc);
         else if (ng == 3)
-          p2p_tile32<3, false>(cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc);
+          p2p_tile32<3, 0>(cnt, cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc);
         else if (ng == 2)
-          p2p_tile32<2, false>(cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc);
+          p2p_tile32<2, 0>(cnt, cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc);
         else if (ng == 1)
-          p2p_tile32<1, false>(cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc);
+          p2p_tile32<1, 0>(cnt, cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc);
       }
     }
   }
@@ -1108,7 +1165,7 @@ __global__ __launch_bounds__(P2P_BS) void k_s2p(const int* __restrict__ tbeg, co
 
 template <int KT, int NR>
 static void p2p_launch(hipStream_t s, int variant, int nchunk, int nb, const int* tbeg, const int* tnum,
-                       const int* eptr, const int2* ent, const int* selfe, Points src, Targets trg, int roff, KC kc)
+                       const int* eptr, const int2* ent, const int* echk, Points src, Targets trg, int roff, KC kc)
 {
   if (variant >= 2 && KT == 111 && NR == 16 && src.den_rs == 1)
     hipLaunchKernelGGL(k_p2p_mfma111, dim3(xcd_grid(nchunk), nb), dim3(P2P_BS), 0, s, nchunk, tbeg, tnum, eptr, ent,
@@ -1117,8 +1174,8 @@ static void p2p_launch(hipStream_t s, int variant, int nchunk, int nb, const int
     hipLaunchKernelGGL((k_p2p_s<KT, NR>), dim3(2 * nchunk, nb), dim3(64), 0, s, tbeg, tnum, eptr, ent, src, trg, roff,
                        kc);
   else if (variant == 4)
-    hipLaunchKernelGGL((k_p2p32<KT, NR>), dim3(nchunk, nb), dim3(P2P_BS), 0, s, tbeg, tnum, eptr, ent, src, trg, roff,
-                       kc);
+    hipLaunchKernelGGL((k_p2p32<KT, NR>), dim3(nchunk, nb), dim3(P2P_BS), 0, s, tbeg, tnum, eptr, ent, echk, src, trg,
+                       roff, kc);
   else
     hipLaunchKernelGGL((k_p2p<KT, NR>), dim3(nchunk, nb), dim3(P2P_BS), 0, s, tbeg, tnum, eptr, ent, src, trg, roff,
                        kc);
@@ -1126,33 +1183,33 @@ static void p2p_launch(hipStream_t s, int variant, int nchunk, int nb, const int
 
 template <int KT>
 static void p2p_kt(hipStream_t s, int nchunk, const int* tbeg, const int* tnum, const int* eptr, const int2* ent,
-                   const int* selfe, Points src, Targets trg, int nr, KC kc, int variant)
+                   const int* echk, Points src, Targets trg, int nr, KC kc, int variant)
 {
   int r0 = 0;
-  if (KT == 111 && variant == 3 && selfe) {
+  if (KT == 111 && variant == 3 && echk) {
     const int left = rhs_blocks16(nr, [&](int nb, int roff, int nrv) {
       hipLaunchKernelGGL(k_p2p_mfma111_f32, dim3(xcd_grid(nchunk), nb), dim3(P2P_BS), 0, s, nchunk, tbeg, tnum, eptr,
-                         ent, selfe, src, trg, roff, kc, nrv);
+                         ent, echk, src, trg, roff, kc, nrv);
     });
     r0 = nr - left;
   }
   rhs_blocks(nr - r0, [&](int NR, int nb, int roff) {
     if (NR == 16)
-      p2p_launch<KT, 16>(s, variant, nchunk, nb, tbeg, tnum, eptr, ent, selfe, src, trg, r0 + roff, kc);
+      p2p_launch<KT, 16>(s, variant, nchunk, nb, tbeg, tnum, eptr, ent, echk, src, trg, r0 + roff, kc);
     else if (NR == 4)
-      p2p_launch<KT, 4>(s, variant, nchunk, nb, tbeg, tnum, eptr, ent, selfe, src, trg, r0 + roff, kc);
+      p2p_launch<KT, 4>(s, variant, nchunk, nb, tbeg, tnum, eptr, ent, echk, src, trg, r0 + roff, kc);
     else
-      p2p_launch<KT, 1>(s, variant, nchunk, nb, tbeg, tnum, eptr, ent, selfe, src, trg, r0 + roff, kc);
+      p2p_launch<KT, 1>(s, variant, nchunk, nb, tbeg, tnum, eptr, ent, echk, src, trg, r0 + roff, kc);
   });
 }
 
 void p2p(hipStream_t s, int kt, int nchunk, const int* tbeg, const int* tnum, const int* eptr, const int2* ent,
-         const int* selfe, Points src, Targets trg, int nr, KC kc, int variant)
+         const int* echk, Points src, Targets trg, int nr, KC kc, int variant)
 {
   if (nchunk <= 0) return;
   switch (kt) {
 #define KIFMM_P2P(KT) \
-  case KT: p2p_kt<KT>(s, nchunk, tbeg, tnum, eptr, ent, selfe, src, trg, nr, kc, variant); break;
+  case KT: p2p_kt<KT>(s, nchunk, tbeg, tnum, eptr, ent, echk, src, trg, nr, kc, variant); break;
   KIFMM_P2P(111) KIFMM_P2P(121) KIFMM_P2P(311) KIFMM_P2P(312) KIFMM_P2P(321) KIFMM_P2P(322) KIFMM_P2P(511)
   KIFMM_P2P(521)
 #undef KIFMM_P2P
@@ -1292,13 +1349,13 @@ __global__ __launch_bounds__(P2P_BS) void k_p2s_mfma111_f32(int nitem, int npc, 
       }
       __syncthreads();
       if (ng == 4)
-        p2p_tile32<4, false>(cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc);
+        p2p_tile32<4, 0>(cnt, cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc);
       else if (ng == 3)
-        p2p_tile32<3, false>(cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc);
+        p2p_tile32<3, 0>(cnt, cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc);
       else if (ng == 2)
-        p2p_tile32<2, false>(cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc);
+        p2p_tile32<2, 0>(cnt, cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc);
       else if (ng == 1)
-        p2p_tile32<1, false>(cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc);
+        p2p_tile32<1, 0>(cnt, cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc);
     }
   }
   if (!wact) return;
@@ -1427,16 +1484,16 @@ __global__ __launch_bounds__(P2P_BS) void k_s2p_mfma111(int nchunk, const int* _
       __syncthreads();
       if constexpr (F32) {
         if (ng == 4)
-          p2p_tile32<4>(cnt4, kk, col, (const float(*)[TILE32])s_pos, (const float(*)[16])s_den, (const float*)tx,
+          p2p_tile32<4, 1>(cnt, cnt4, kk, col, (const float(*)[TILE32])s_pos, (const float(*)[16])s_den, (const float*)tx,
                         (const float*)ty, (const float*)tz, acc64);
         else if (ng == 3)
-          p2p_tile32<3>(cnt4, kk, col, (const float(*)[TILE32])s_pos, (const float(*)[16])s_den, (const float*)tx,
+          p2p_tile32<3, 1>(cnt, cnt4, kk, col, (const float(*)[TILE32])s_pos, (const float(*)[16])s_den, (const float*)tx,
                         (const float*)ty, (const float*)tz, acc64);
         else if (ng == 2)
-          p2p_tile32<2>(cnt4, kk, col, (const float(*)[TILE32])s_pos, (const float(*)[16])s_den, (const float*)tx,
+          p2p_tile32<2, 1>(cnt, cnt4, kk, col, (const float(*)[TILE32])s_pos, (const float(*)[16])s_den, (const float*)tx,
                         (const float*)ty, (const float*)tz, acc64);
         else if (ng == 1)
-          p2p_tile32<1>(cnt4, kk, col, (const float(*)[TILE32])s_pos, (const float(*)[16])s_den, (const float*)tx,
+          p2p_tile32<1, 1>(cnt, cnt4, kk, col, (const float(*)[TILE32])s_pos, (const float(*)[16])s_den, (const float*)tx,
                         (const float*)ty, (const float*)tz, acc64);
       } else {
         if (ng == 4)

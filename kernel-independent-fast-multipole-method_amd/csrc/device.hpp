@@ -55,10 +55,11 @@ struct Targets {  // leaf-ordered targets (SoA) and values [rhs][n*tdof]
 // variant 2: Laplace blocks of 16 right-hand sides on the FP64 matrix cores;
 // variant 3: the same on the FP32 matrix cores (mixed precision).  Variants
 // 2 and 3 fall back to variant 0 for other kernels and smaller blocks.
-// selfe[c]: entry of chunk c's own leaf or -1 (variant 3 needs it; without
-// it variant 3 takes the FP64 matrix-core kernel).
+// echk[e]: 1 where a pair of entry e can be closer than 1e-8 and takes the
+// exclusion explicitly (variant 3 needs it; without it variant 3 takes the
+// FP64 matrix-core kernel).
 void p2p(hipStream_t s, int kt, int nchunk, const int* tbeg, const int* tnum, const int* eptr, const int2* ent,
-         const int* selfe, Points src, Targets trg, int nr, KC kc, int variant);
+         const int* echk, Points src, Targets trg, int nr, KC kc, int variant);
 
 // FP64 direct sum of the user kernel over all nsrc sources at nchk points
 // (cx, cy, cz): out[r][k*tdof + t]; part holds nr * nseg * nchk * tdof doubles.

@@ -47,9 +47,11 @@ void DVec<T>::upload(const std::vector<T>& v)
   if (!v.empty()) HIPCHK(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
 }
 template <class T>
-void DVec<T>::ensure(size_t count)
+bool DVec<T>::ensure(size_t count)
 {
-  if (count > n || p == nullptr) alloc(count);
+  if (count <= n && p != nullptr) return false;
+  alloc(count);
+  return true;
 }
 template struct DVec<double>;
 template struct DVec<float>;
@@ -121,6 +123,10 @@ void Engine::setup(const double* spos, const double* snor, int ns, const double*
 {
   if (ns < 0 || nt < 0 || (ns > 0 && !spos) || (nt > 0 && !tpos)) throw Error(KIFMM3D_EINVAL, "bad point arrays");
   if (user_.needs_normal && ns > 0 && !snor) throw Error(KIFMM3D_EINVAL, "this kernel needs source normals");
+  // every device array below is replaced: a graph of the previous set-up
+  // must never be launched again
+  drop_graph();
+  gen_++;
   setup_done_ = false;
   host_done_ = false;
   const bool trace = std::getenv("KIFMM3D_TRACE") != nullptr;
@@ -150,6 +156,7 @@ void Engine::setup(const double* spos, const double* snor, int ns, const double*
   ns_ = ns;
   nt_ = nt;
   tpos_host_.assign(tpos, tpos + 3 * (size_t)nt);
+  mark_edge_leaves(spos, tpos);
   lap("tree");
   build_plan();
   lap("plan");
@@ -196,7 +203,66 @@ void Engine::setup(const double* spos, const double* snor, int ns, const double*
   upload_static();
   HIPCHK(hipDeviceSynchronize());
   lap("upload");
+  gen_++;
   setup_done_ = true;
+}
+
+// Which near-field entries can hold a pair that the reference's r < 1e-8
+// exclusion (kernel3d.cpp:21, :132) applies to, for the mixed form's FP32
+// near field (whose relative coordinates resolve ~1e-7 of a leaf's width):
+// * a pair of points of two different leaves closer than d has both points
+//   within d of their own leaf's box boundary (the segment between them
+//   leaves the box), so a cross-leaf entry needs that only when both leaves
+//   hold such a point (sedge_, tedge_);
+// * inside one leaf, equal points have equal FP32 offsets and drop out; two
+//   distinct points closer than d are looked for directly (close_).
+// The tolerance has a margin over 1e-8.
+constexpr double EDGE_TOL = 4e-8;
+
+void Engine::mark_edge_leaves(const double* spos, const double* tpos)
+{
+  const Tree& T = tree_;
+  sedge_.assign(T.nnode, 0);
+  tedge_.assign(T.nnode, 0);
+  close_.assign(T.nnode, 0);
+  auto near_edge = [&](int g, const double* p) {
+    double c[3];
+    T.center(g, c);
+    const double r = T.radius(g);
+    for (int d = 0; d < 3; d++)
+      if (p[d] - (c[d] - r) < EDGE_TOL || (c[d] + r) - p[d] < EDGE_TOL) return true;
+    return false;
+  };
+  std::vector<std::pair<double, int>> xs;
+  for (int g = 0; g < T.nnode; g++) {
+    if (!T.terminal(g)) continue;
+    if (T.tag[g] & NODE_SRC)
+      for (int k = T.sextbeg[g]; k < T.sextbeg[g] + T.sextnum[g] && !sedge_[g]; k++)
+        sedge_[g] = near_edge(g, spos + 3 * (size_t)T.sperm[k]);
+    if (T.tag[g] & NODE_TRG)
+      for (int k = T.textbeg[g]; k < T.textbeg[g] + T.textnum[g] && !tedge_[g]; k++)
+        tedge_[g] = near_edge(g, tpos + 3 * (size_t)T.tperm[k]);
+    if ((T.tag[g] & NODE_SRC) && (T.tag[g] & NODE_TRG)) {
+      // sources sorted by x; each target scans the sources within EDGE_TOL in x
+      xs.clear();
+      for (int k = T.sextbeg[g]; k < T.sextbeg[g] + T.sextnum[g]; k++)
+        xs.emplace_back(spos[3 * (size_t)T.sperm[k]], T.sperm[k]);
+      std::sort(xs.begin(), xs.end());
+      for (int k = T.textbeg[g]; k < T.textbeg[g] + T.textnum[g] && !close_[g]; k++) {
+        const double* q = tpos + 3 * (size_t)T.tperm[k];
+        auto it = std::lower_bound(xs.begin(), xs.end(), std::make_pair(q[0] - EDGE_TOL, -1));
+        for (; it != xs.end() && it->first <= q[0] + EDGE_TOL; ++it) {
+          const double* p = spos + 3 * (size_t)it->second;
+          const double dx = q[0] - p[0], dy = q[1] - p[1], dz = q[2] - p[2];
+          const double r2 = dx * dx + dy * dy + dz * dz;
+          if (r2 > 0.0 && r2 < EDGE_TOL * EDGE_TOL) {
+            close_[g] = 1;
+            break;
+          }
+        }
+      }
+    }
+  }
 }
 
 void Engine::upload_static()
@@ -397,12 +463,12 @@ void Engine::build_plan()
   }
   // near field, S2L items and L2T/M2T chunks, per target leaf
   {
-    std::vector<int> tbeg, tnum, eptr(1, 0), ent, eself;
+    std::vector<int> tbeg, tnum, eptr(1, 0), ent, echk;
     std::vector<int> ftbeg, ftnum, feptr(1, 0), fnode, fkind;
     std::vector<double> fgeom;
     std::vector<int> xout, xeptr(1, 0), xent;
     std::vector<double> xgeom;
-    std::vector<int> dent, sent, sknd;
+    std::vector<int> dent, dchk, sent, sknd;
     std::vector<double> sgeom;
     for (int g = 0; g < T.nnode; g++) {
       if (!(T.tag[g] & NODE_TRG)) continue;
@@ -420,12 +486,14 @@ void Engine::build_plan()
         xeptr.push_back((int)xent.size() / 2);
       }
       if (!leaf) continue;
-      dent.clear(); sent.clear(); sknd.clear(); sgeom.clear();
+      dent.clear(); dchk.clear(); sent.clear(); sknd.clear(); sgeom.clear();
       long npair = 0, fpair = 0;
-      int dself = -1;
       auto U = list(LIST_U, g);
       for (auto it = U.first; it != U.second; ++it) {
-        if (*it == g) dself = (int)dent.size() / 2;
+        // 2: the pairs' exclusion decided in FP64 (cross-leaf entries whose
+        // leaves both hold points next to their boundaries, the own leaf when
+        // it holds distinct near-coincident points); 1: own leaf otherwise
+        dchk.push_back(*it == g ? (close_[g] ? 2 : 1) : (sedge_[*it] && tedge_[g]) ? 2 : 0);
         dent.push_back(T.sextbeg[*it]); dent.push_back(T.sextnum[*it]);
         npair += T.sextnum[*it];
       }
@@ -433,6 +501,7 @@ void Engine::build_plan()
       for (auto it = W.first; it != W.second; ++it) {
         int v = *it;
         if (T.terminal(v) && T.sextnum[v] * sdof < pUE) {  // fmm3d_eval.cpp:161
+          dchk.push_back(0);  // W and X boxes are not adjacent to the target leaf
           dent.push_back(T.sextbeg[v]); dent.push_back(T.sextnum[v]);
           npair += T.sextnum[v];
         } else {
@@ -442,6 +511,7 @@ void Engine::build_plan()
       }
       if (xdirect)
         for (auto it = X.first; it != X.second; ++it) {
+          dchk.push_back(0);
           dent.push_back(T.sextbeg[*it]); dent.push_back(T.sextnum[*it]);
           npair += T.sextnum[*it];
         }
@@ -454,8 +524,8 @@ void Engine::build_plan()
       for (int off = 0; off < T.textnum[g]; off += 128) {
         int n = std::min(128, T.textnum[g] - off);
         tbeg.push_back(T.textbeg[g] + off); tnum.push_back(n);
-        eself.push_back(dself >= 0 ? (int)ent.size() / 2 + dself : -1);
         ent.insert(ent.end(), dent.begin(), dent.end());
+        echk.insert(echk.end(), dchk.begin(), dchk.end());
         eptr.push_back((int)ent.size() / 2);
         ftbeg.push_back(T.textbeg[g] + off); ftnum.push_back(n);
         fnode.insert(fnode.end(), sent.begin(), sent.end());
@@ -466,7 +536,7 @@ void Engine::build_plan()
     }
     P.near.n = (int)tbeg.size();
     P.near.tbeg = pool_i(P.I, tbeg); P.near.tnum = pool_i(P.I, tnum);
-    P.near.eptr = pool_i(P.I, eptr); P.near.ent = pool_i(P.I, ent); P.near.self = pool_i(P.I, eself);
+    P.near.eptr = pool_i(P.I, eptr); P.near.ent = pool_i(P.I, ent); P.near.chk = pool_i(P.I, echk);
     P.far.n = (int)ftbeg.size();
     P.far.tbeg = pool_i(P.I, ftbeg); P.far.tnum = pool_i(P.I, ftnum);
     P.far.eptr = pool_i(P.I, feptr); P.far.enode = pool_i(P.I, fnode);
@@ -788,16 +858,17 @@ void Engine::ensure_work(int rc)
 {
   const Operators& O = *ops_;
   const Tree& T = tree_;
-  sden_.ensure((size_t)rc * T.sextcnt * user_.sdof);
-  if (user_.kt == 111) sden32_.ensure((size_t)rc * T.sextcnt);
-  tval_.ensure((size_t)rc * T.textcnt * user_.tdof);
-  tvalT_.ensure((size_t)rc * T.textcnt * user_.tdof);
-  upcval_.ensure((size_t)rc * T.sndecnt * O.pln[SURF_UC]);
-  upeden_.ensure((size_t)rc * T.sndecnt * O.pln[SURF_UE]);
-  dncval_.ensure((size_t)rc * T.tndecnt * O.pln[SURF_DC]);
-  dneden_.ensure((size_t)rc * T.tndecnt * O.pln[SURF_DE]);
-  effden_.ensure((size_t)2 * rc * plan_.max_vsrc * mm_.sdof * O.C);
-  effval_.ensure((size_t)2 * rc * plan_.max_vtrg * mm_.tdof * O.C);
+  bool grew = false;
+  grew |= sden_.ensure((size_t)rc * T.sextcnt * user_.sdof);
+  if (user_.kt == 111) grew |= sden32_.ensure((size_t)rc * T.sextcnt);
+  grew |= tval_.ensure((size_t)rc * T.textcnt * user_.tdof);
+  grew |= tvalT_.ensure((size_t)rc * T.textcnt * user_.tdof);
+  grew |= upcval_.ensure((size_t)rc * T.sndecnt * O.pln[SURF_UC]);
+  grew |= upeden_.ensure((size_t)rc * T.sndecnt * O.pln[SURF_UE]);
+  grew |= dncval_.ensure((size_t)rc * T.tndecnt * O.pln[SURF_DC]);
+  grew |= dneden_.ensure((size_t)rc * T.tndecnt * O.pln[SURF_DE]);
+  grew |= effden_.ensure((size_t)2 * rc * plan_.max_vsrc * mm_.sdof * O.C);
+  grew |= effval_.ensure((size_t)2 * rc * plan_.max_vtrg * mm_.tdof * O.C);
   // split-K partials of the matrix-core GEMMs with few columns, for every
   // chunk size up to rc (the last chunk of an evaluate may be smaller)
   size_t gw = 0;
@@ -812,7 +883,13 @@ void Engine::ensure_work(int rc)
         const int ks = dev::gemm_split(a);
         if (ks > 1) gw = std::max(gw, (size_t)ks * op.ncol * r * op.M);
       }
-  gwork_.ensure(gw);
+  grew |= gwork_.ensure(gw);
+  // graphs captured before hold the freed buffers: they are dropped here and
+  // the batch graphs that recorded this engine see the new generation
+  if (grew) {
+    drop_graph();
+    gen_++;
+  }
 }
 
 void Engine::check_eval_args(const double* d_srcden, const double* d_trgval, int nrhs, int mode) const
@@ -857,7 +934,8 @@ void Engine::evaluate_device(const double* d_srcden, double* d_trgval, int nrhs,
   int rc = prepare(nrhs);
   if (use_graph_) {
     // one captured graph per (buffers, nrhs, mode); replayed while they stay the same
-    if (!gexec_ || gsrc_ != d_srcden || gtrg_ != d_trgval || gnrhs_ != nrhs || gmode_ != mode || grc_ != rc) {
+    if (!gexec_ || gsrc_ != d_srcden || gtrg_ != d_trgval || gnrhs_ != nrhs || gmode_ != mode || grc_ != rc ||
+        ggen_ != gen_) {
       drop_graph();
       HIPCHK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
       hipGraph_t g = nullptr;
@@ -875,7 +953,7 @@ void Engine::evaluate_device(const double* d_srcden, double* d_trgval, int nrhs,
         gexec_ = nullptr;
         throw Error(KIFMM3D_EDEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
       }
-      gsrc_ = d_srcden; gtrg_ = d_trgval; gnrhs_ = nrhs; gmode_ = mode; grc_ = rc;
+      gsrc_ = d_srcden; gtrg_ = d_trgval; gnrhs_ = nrhs; gmode_ = mode; grc_ = rc; ggen_ = gen_;
     }
     hipEvent_t a, b;
     HIPCHK(hipEventCreate(&a));
@@ -1033,6 +1111,7 @@ int Engine::tune(int nrhs, int mode)
   p2p_tuned_[mi] = best_v;
   m2l_form_[mi] = best_form;
   use_graph_ = graph;
+  gen_++;  // batch graphs recorded with the previous settings are recaptured
   return runs;
 }
 
@@ -1058,6 +1137,7 @@ void Engine::set_settings(int mode, const int* in, int n)
   p2p_tuned_[mixed ? 1 : 0] = v;
   m2l_form_[mixed ? 1 : 0] = forms;
   drop_graph();
+  gen_++;
 }
 
 int Engine::settings(int mode, int* out, int n) const
@@ -1175,7 +1255,7 @@ void Engine::run_chunk(const double* srcden, double* trgval, int rc, int mode, h
   }
   timed_on(side ? side_ : s, KIFMM3D_K_P2P, [&] {
     dev::p2p(side ? side_ : s, user_.kt, P.near.n, I + P.near.tbeg, I + P.near.tnum, I + P.near.eptr,
-             (const int2*)(I + P.near.ent), I + P.near.self, src, trg, rc, kcu, near_variant(mixed));
+             (const int2*)(I + P.near.ent), I + P.near.chk, src, trg, rc, kcu, near_variant(mixed));
   });
   if (side) HIPCHK(hipEventRecord(join_ev_, side_));
   mark(3);
@@ -1331,9 +1411,12 @@ void BatchRunner::evaluate(const double* const* d_srcden, double* const* d_trgva
     eng_[i]->check_eval_args(d_srcden[i], d_trgval[i], nrhs[i], mode);
     rc[i] = eng_[i]->prepare(nrhs[i]);  // allocations happen here, outside the capture
   }
+  // the graph holds every engine's work buffers and settings as they were at
+  // capture: it is replayed only while none of them changed
   bool same = exec_ && kmode_ == mode;
   for (size_t i = 0; same && i < n; i++)
-    same = ksrc_[i] == d_srcden[i] && ktrg_[i] == d_trgval[i] && knrhs_[i] == nrhs[i] && krc_[i] == rc[i];
+    same = ksrc_[i] == d_srcden[i] && ktrg_[i] == d_trgval[i] && knrhs_[i] == nrhs[i] && krc_[i] == rc[i] &&
+           kgen_[i] == eng_[i]->generation();
   hipStream_t s0 = br_[0];
   if (!same) {
     drop();
@@ -1384,6 +1467,8 @@ void BatchRunner::evaluate(const double* const* d_srcden, double* const* d_trgva
     ktrg_.assign(d_trgval, d_trgval + n);
     knrhs_.assign(nrhs, nrhs + n);
     krc_ = rc;
+    kgen_.resize(n);
+    for (size_t i = 0; i < n; i++) kgen_[i] = eng_[i]->generation();
     kmode_ = mode;
   }
   hipEvent_t a, b;

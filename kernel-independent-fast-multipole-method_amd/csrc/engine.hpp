@@ -27,7 +27,7 @@ struct DVec {
   void reset();
   void alloc(size_t count);                // uninitialised
   void upload(const std::vector<T>& v);    // alloc + copy
-  void ensure(size_t count);               // grow only
+  bool ensure(size_t count);               // grow only; true when it reallocated
 };
 
 // A GEMM step of the up or down pass (see dev::GemmArgs).
@@ -54,7 +54,8 @@ struct ChunkList {  // chunks of <= 128 leaf targets
   int n = 0;
   size_t tbeg = 0, tnum = 0, eptr = 0;
   size_t ent = 0;                          // near field: int2 {sextbeg, sextnum}
-  size_t self = 0;                         // near field: entry of the chunk's own leaf, or -1
+  size_t chk = 0;                          // near field: per entry, how the FP32 kernel applies the
+                                           // r < 1e-8 exclusion (0 none needed, 1 FP32, 2 FP64 decision)
   size_t enode = 0, ekind = 0, egeom = 0;  // surfaces: node, kind, (cx, cy, cz, rad)
 };
 
@@ -150,6 +151,10 @@ class Engine {
   const Operators& ops() const { return *ops_; }
   const KernelInfo& user() const { return user_; }
   bool is_setup() const { return setup_done_; }
+  // Bumped whenever setup or a work-buffer growth replaces device memory that
+  // a captured graph may hold: a graph recorded at another generation must
+  // not be replayed.
+  unsigned long generation() const { return gen_; }
   int last_nrhs() const { return last_nrhs_; }
   bool host_ready() const { return host_done_; }
   double stage_ms[KIFMM3D_NSTAGE] = {0};
@@ -162,6 +167,7 @@ class Engine {
   const Plan& plan() const { return plan_; }
 
  private:
+  void mark_edge_leaves(const double* spos, const double* tpos);
   void build_plan();
   void upload_static();
   int choose_rhs_chunk(int nrhs);
@@ -182,6 +188,7 @@ class Engine {
   bool setup_done_ = false;
   bool host_done_ = false;
   int last_rc_ = 0;
+  unsigned long gen_ = 0;
   int last_nrhs_ = 0;  // right-hand sides of the last evaluate_device
   int m2l_rb_ = 0;   // right-hand sides per lane in the grouped M2L (0 = default; KIFMM3D_M2L_RB)
   int m2l_bs_ = 64;  // frequencies per block in the grouped M2L (64 or 256; KIFMM3D_M2L_BS)
@@ -200,6 +207,7 @@ class Engine {
   const double* gsrc_ = nullptr;
   double* gtrg_ = nullptr;
   int gnrhs_ = 0, gmode_ = -1, grc_ = 0;
+  unsigned long ggen_ = 0;
   std::vector<hipEvent_t> evpool_;
   std::vector<int> evcls_;
   std::vector<int> evlvl_;        // V-list level of each timed launch (-1 outside the V list)
@@ -233,6 +241,10 @@ class Engine {
   DVec<double> chk_, part_, dout_;  // direct-sum scratch
   DVec<double> gwork_;              // split-K partials of the GEMMs
   std::vector<double> tpos_host_;  // caller-order target positions (for direct sums)
+  // per node: a source (target) point of the leaf lies within EDGE_TOL of the
+  // leaf's box boundary, so it can be closer than 1e-8 to a point of an
+  // adjacent leaf; the leaf holds a distinct source-target pair that close
+  std::vector<char> sedge_, tedge_, close_;
 };
 
 // Several set-up engines evaluated in one captured graph: problem i runs on
@@ -255,6 +267,7 @@ class BatchRunner {
   std::vector<const double*> ksrc_;
   std::vector<double*> ktrg_;
   std::vector<int> knrhs_, krc_;
+  std::vector<unsigned long> kgen_;
   int kmode_ = -1;
   double last_ms_ = 0.0;
   int balance_ = 1;  // 1: branches balanced by problem cost, 0: round robin (KIFMM3D_BATCH_BALANCE)

@@ -1,6 +1,10 @@
 // operators.cpp -- build and cache the translation operators.
 #include "operators.hpp"
 
+#include <fcntl.h>
+#include <unistd.h>
+
+#include <atomic>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -232,9 +236,18 @@ std::string cache_path(const std::string& dir, const KernelInfo& mm, int np, dou
 
 void save(const std::string& path, const Operators& o, double c0, double c1)
 {
-  const std::string tmp = path + ".tmp" + std::to_string((unsigned long)std::hash<std::string>{}(path) ^ (unsigned long)(uintptr_t)&o);
-  std::FILE* f = std::fopen(tmp.c_str(), "wb");
-  if (!f) return;  // an unwritable cache directory only costs the rebuild
+  // a temporary name unique per process and call, created exclusively: two
+  // writers never share one file, and the rename publishes a whole file
+  static std::atomic<unsigned long> serial{0};
+  const std::string tmp = path + ".tmp" + std::to_string((long)getpid()) + "_" + std::to_string(serial++);
+  const int fd = ::open(tmp.c_str(), O_WRONLY | O_CREAT | O_EXCL, 0644);
+  if (fd < 0) return;  // an unwritable cache directory only costs the rebuild
+  std::FILE* f = ::fdopen(fd, "wb");
+  if (!f) {
+    ::close(fd);
+    std::remove(tmp.c_str());
+    return;
+  }
   Writer w{f};
   w.put("KIFMMOP1", 8);
   const int32_t hdr[6] = {OPS_VERSION, o.mm.kt, o.np, o.mm.sdof, o.mm.tdof, 0};

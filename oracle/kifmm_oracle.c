@@ -30,6 +30,9 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #define OK 0
 #define ERR (-1)
@@ -945,7 +948,23 @@ static void dgemv_acc(int m, int n, const double* A, const double* x, double* y)
   }
   long double* t = (long double*)malloc(sizeof(long double) * (size_t)m);
   for (int i = 0; i < m; i++) t[i] = y[i];
-  for (int jj = 0; jj < n; jj++) {
+  /* four columns per pass over t (each t[i] still takes them in order) */
+  int jj = 0;
+  for (; jj + 4 <= n; jj += 4) {
+    int j0 = (g_order & 2) ? n - 1 - jj : jj, dj = (g_order & 2) ? -1 : 1;
+    long double x0 = x[j0], x1 = x[j0 + dj], x2 = x[j0 + 2 * dj], x3 = x[j0 + 3 * dj];
+    const double *a0 = A + (size_t)j0 * m, *a1 = A + (size_t)(j0 + dj) * m, *a2 = A + (size_t)(j0 + 2 * dj) * m,
+                 *a3 = A + (size_t)(j0 + 3 * dj) * m;
+    for (int i = 0; i < m; i++) {
+      long double u = t[i];
+      u += (long double)a0[i] * x0;
+      u += (long double)a1[i] * x1;
+      u += (long double)a2[i] * x2;
+      u += (long double)a3[i] * x3;
+      t[i] = u;
+    }
+  }
+  for (; jj < n; jj++) {
     int j = (g_order & 2) ? n - 1 - jj : jj;
     long double xj = x[j];
     const double* a = A + (size_t)j * m;
@@ -1075,9 +1094,11 @@ static int ue2dc(orc_fmm* h, const int* idx, const double* effden, double* effva
   }
   double nrm = 1.0 / (double)G;
   const double* mat = h->ue2dc[o];
-  for (int i = 0; i < tm; i++)
-    for (int j = 0; j < sm; j++)
-      for (int q = 0; q < C; q++) {
+  /* per output (q, i) the terms are added in ascending j, as the reference's
+   * loop nest (i, j, q) does; q outermost walks the arrays contiguously */
+  for (int q = 0; q < C; q++)
+    for (int i = 0; i < tm; i++)
+      for (int j = 0; j < sm; j++) {
         const double* x = mat + 2 * ((size_t)q * sm * tm + j * tm + i);
         const double* y = effden + 2 * ((size_t)q * sm + j);
         double* z = effval + 2 * ((size_t)q * tm + i);
@@ -1101,11 +1122,75 @@ int orc_m2l_fft(orc_fmm* h, int l, const int* idx, const double* den, double* va
 
 /* ------------------------------------------------------------------ */
 /* FMM3d::eval, fmm3d_eval.cpp:24-243 (one right-hand side)             */
+/* Threads of the evaluation (OpenMP when the oracle is built with it).
+ * Every output value is summed by one thread in the order of the sequential
+ * restatement (per node, its list order), so the result does not depend on
+ * the thread count; n <= 0 restores the OpenMP default. */
+void orc_set_threads(int n)
+{
+#ifdef _OPENMP
+  omp_set_num_threads(n > 0 ? n : omp_get_num_procs());
+#else
+  (void)n;
+#endif
+}
+int orc_get_threads(void)
+{
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+/* The operators the sequential code builds lazily on first use (M2M, L2L and
+ * the M2L spectra of the 316 offsets, matmgnt3d.cpp:130-138, 195-203, 307-325)
+ * are built before the parallel stages, which then only read them. */
+static void prebuild_operators(orc_fmm* h)
+{
+  double* z = (double*)calloc((size_t)h->pln[UC] + h->pln[DC] + h->pln[UE] + h->pln[DE] + 2 * (size_t)h->C * (h->sm + h->tm), sizeof(double));
+  double* chk = z;
+  double* den = chk + h->pln[UC] + h->pln[DC];
+  double* ed = den + h->pln[UE] + h->pln[DE];
+  double* ev = ed + 2 * (size_t)h->C * h->sm;
+  for (int c = 0; c < 8; c++) {
+    int idx[3] = {c >> 2, (c >> 1) & 1, c & 1};
+    if (!h->ue2uc[c]) ue2uc(h, 0, idx, den, chk);
+    if (!h->de2dc[c]) de2dc(h, 0, idx, den, chk);
+  }
+  for (int o = 0; o < 343; o++) {
+    int idx[3] = {o / 49 - 3, (o / 7) % 7 - 3, o % 7 - 3};
+    int linf = 0;
+    for (int d = 0; d < 3; d++) if (abs(idx[d]) > linf) linf = abs(idx[d]);
+    if (linf >= 2 && !h->ue2dc[o]) ue2dc(h, idx, ed, ev);
+  }
+  free(z);
+}
+
+#include <time.h>
+static double wall_s(void)
+{
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return ts.tv_sec + 1e-9 * ts.tv_nsec;
+}
+/* ORC_TRACE in the environment: per-stage wall times of each eval on stderr */
+static void trace_stage(const char* what, double* t)
+{
+  static int on = -1;
+  if (on < 0) on = getenv("ORC_TRACE") != NULL;
+  double now = wall_s();
+  if (on) fprintf(stderr, "[oracle] %-8s %8.3f s\n", what, now - *t);
+  *t = now;
+}
+
 int orc_eval(orc_fmm* h, const double* srcden, double* trgval)
 {
   if (!h->uc2ue || !h->dc2de || !h->nd) return ERR;
+  double tt = wall_s();
   int sdof = h->sdof, tdof = h->tdof;
   int pUE = h->pln[UE], pUC = h->pln[UC], pDE = h->pln[DE], pDC = h->pln[DC];
+  prebuild_operators(h);
   /* 1. zero (:33-42) */
   memset(trgval, 0, sizeof(double) * (size_t)h->nt * tdof);
   memset(h->sextden, 0, sizeof(double) * (size_t)h->sextcnt * sdof);
@@ -1115,6 +1200,12 @@ int orc_eval(orc_fmm* h, const double* srcden, double* trgval)
   memset(h->tdneden, 0, sizeof(double) * (size_t)h->tndecnt * pDE);
   memset(h->tdncval, 0, sizeof(double) * (size_t)h->tndecnt * pDC);
   int nn = h->nnode;
+  /* nodes are in BFS order: depth d occupies [dbeg[d], dbeg[d+1]) */
+  int nd = 0;
+  for (int g = 0; g < nn; g++) if (h->nd[g].depth + 1 > nd) nd = h->nd[g].depth + 1;
+  int* dbeg = (int*)calloc((size_t)nd + 1, sizeof(int));
+  for (int g = 0; g < nn; g++) dbeg[h->nd[g].depth + 1] = g + 1;
+  for (int d = 1; d <= nd; d++) if (dbeg[d] < dbeg[d - 1]) dbeg[d] = dbeg[d - 1];
   /* 2. load (:50-64) */
   for (int g = nn - 1; g >= 0; g--) {
     onode* n = &h->nd[g];
@@ -1122,33 +1213,42 @@ int orc_eval(orc_fmm* h, const double* srcden, double* trgval)
       for (int k = 0; k < n->sown.n; k++)
         for (int d = 0; d < sdof; d++) h->sextden[(n->sextbeg + k) * sdof + d] = srcden[n->sown.a[k] * sdof + d];
   }
-  /* 3. up (:69-92) */
-  double* pos = (double*)malloc(sizeof(double) * 3 * (h->nsp[UC] > h->nsp[DC] ? h->nsp[UC] : h->nsp[DC]));
-  for (int g = nn - 1; g >= 0; g--) {
-    onode* n = &h->nd[g];
-    if (!(n->tag & SRCNODE) || n->depth < 2) continue;
-    double* cval = h->supcval + (size_t)n->sndeidx * pUC;
-    double* eden = h->supeden + (size_t)n->sndeidx * pUE;
-    if (n->chd == -1) {
-      double c[3];
-      center(h, g, c);
-      locpos(h, UC, c, radius(h, g), pos);
-      kapply(&h->knl, n->sextnum, h->sextpos + 3 * n->sextbeg, h->sextnor + 3 * n->sextbeg,
-             h->sextden + (size_t)n->sextbeg * sdof, h->nsp[UC], pos, cval);
-    } else {
-      for (int a = 0; a < 2; a++)
-        for (int b = 0; b < 2; b++)
-          for (int c = 0; c < 2; c++) {
-            int chi = child(h, g, a, b, c);
-            if (h->nd[chi].tag & SRCNODE) {
-              int idx[3] = {a, b, c};
-              ue2uc(h, h->nd[chi].depth + h->rootlvl, idx, h->supeden + (size_t)h->nd[chi].sndeidx * pUE, cval);
-            }
-          }
+  /* 3. up (:69-92), deepest level first; the nodes of one level are independent */
+  for (int dep = nd - 1; dep >= 2; dep--) {
+#pragma omp parallel
+    {
+      double* pos = (double*)malloc(sizeof(double) * 3 * h->nsp[UC]);
+#pragma omp for schedule(dynamic, 4)
+      for (int g = dbeg[dep]; g < dbeg[dep + 1]; g++) {
+        onode* n = &h->nd[g];
+        if (!(n->tag & SRCNODE)) continue;
+        double* cval = h->supcval + (size_t)n->sndeidx * pUC;
+        double* eden = h->supeden + (size_t)n->sndeidx * pUE;
+        if (n->chd == -1) {
+          double c[3];
+          center(h, g, c);
+          locpos(h, UC, c, radius(h, g), pos);
+          kapply(&h->knl, n->sextnum, h->sextpos + 3 * n->sextbeg, h->sextnor + 3 * n->sextbeg,
+                 h->sextden + (size_t)n->sextbeg * sdof, h->nsp[UC], pos, cval);
+        } else {
+          for (int a = 0; a < 2; a++)
+            for (int b = 0; b < 2; b++)
+              for (int c = 0; c < 2; c++) {
+                int chi = child(h, g, a, b, c);
+                if (h->nd[chi].tag & SRCNODE) {
+                  int idx[3] = {a, b, c};
+                  ue2uc(h, h->nd[chi].depth + h->rootlvl, idx, h->supeden + (size_t)h->nd[chi].sndeidx * pUE, cval);
+                }
+              }
+        }
+        c2e(h, h->uc2ue, pUE, pUC, h->nsp[UE], n->depth + h->rootlvl, cval, eden);
+      }
+      free(pos);
     }
-    c2e(h, h->uc2ue, pUE, pUC, h->nsp[UE], n->depth + h->rootlvl, cval, eden);
   }
-  /* U (:98-111) */
+  trace_stage("up", &tt);
+  /* U (:98-111): each target leaf sums its U list in list order */
+#pragma omp parallel for schedule(dynamic, 4)
   for (int g = 0; g < nn; g++) {
     onode* n = &h->nd[g];
     if (!(n->tag & TRGNODE) || n->chd != -1) continue;
@@ -1159,100 +1259,156 @@ int orc_eval(orc_fmm* h, const double* srcden, double* trgval)
              h->textval + (size_t)n->textbeg * tdof);
     }
   }
-  /* V (:115-150) */
+  trace_stage("U", &tt);
+  /* V (:115-150).  The reference transforms a source's density on first use
+   * and a target's spectrum after its last V source; here targets are taken
+   * in BFS chunks: the spectra their V lists need are formed first (a source
+   * keeps its spectrum until its last V target is done), then each target
+   * accumulates its products in its V-list order and is transformed back --
+   * the same sums in the same order per target. */
   size_t effn = 2 * (size_t)h->C;
-  for (int g = 0; g < nn; g++) {
-    onode* n = &h->nd[g];
-    if (!(n->tag & TRGNODE)) continue;
-    double gc[3];
-    center(h, g, gc);
-    double D = 2.0 * radius(h, g);
-    int l = n->depth + h->rootlvl;
-    for (int qq = 0; qq < n->V.n; qq++) {
-      int q = (g_order & 4) ? n->V.n - 1 - qq : qq;
-      int vi = n->V.a[q];
-      double vc[3];
-      center(h, vi, vc);
-      int idx[3];
-      for (int d = 0; d < 3; d++) idx[d] = (int)round((vc[d] - gc[d]) / D);
-      if (h->votcnt[vi] == 0) {
-        h->effden[vi] = (double*)calloc(effn * h->sm, sizeof(double));
-        plnden2effden(h, l, h->supeden + (size_t)h->nd[vi].sndeidx * pUE, h->effden[vi]);
+  int* vleft = (int*)malloc(sizeof(int) * (size_t)(nn ? nn : 1));
+  int* need = (int*)malloc(sizeof(int) * (size_t)(nn ? nn : 1));
+  memcpy(vleft, h->votnum, sizeof(int) * (size_t)nn);
+  int verr = 0;
+  const int CH = 1024;
+  for (int c0 = 0; c0 < nn; c0 += CH) {
+    int c1 = c0 + CH < nn ? c0 + CH : nn, nneed = 0;
+    for (int g = c0; g < c1; g++) {
+      onode* n = &h->nd[g];
+      if (!(n->tag & TRGNODE)) continue;
+      for (int q = 0; q < n->V.n; q++) {
+        int vi = n->V.a[q];
+        if (!h->effden[vi]) {
+          h->effden[vi] = (double*)calloc(effn * h->sm, sizeof(double));
+          need[nneed++] = vi;
+        }
       }
-      if (h->vincnt[g] == 0) h->effval[g] = (double*)calloc(effn * h->tm, sizeof(double));
-      if (ue2dc(h, idx, h->effden[vi], h->effval[g]) != OK) { free(pos); return ERR; }
-      h->votcnt[vi]++;
-      h->vincnt[g]++;
-      if (h->votcnt[vi] == h->votnum[vi]) { free(h->effden[vi]); h->effden[vi] = NULL; h->votcnt[vi] = 0; }
-      if (h->vincnt[g] == h->vinnum[g]) {
-        effval2plnval(h, h->effval[g], h->tdncval + (size_t)n->tndeidx * pDC);
-        free(h->effval[g]); h->effval[g] = NULL; h->vincnt[g] = 0;
+    }
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int k = 0; k < nneed; k++) {
+      int vi = need[k];
+      plnden2effden(h, h->nd[vi].depth + h->rootlvl, h->supeden + (size_t)h->nd[vi].sndeidx * pUE, h->effden[vi]);
+    }
+#pragma omp parallel for schedule(dynamic, 4) reduction(| : verr)
+    for (int g = c0; g < c1; g++) {
+      onode* n = &h->nd[g];
+      if (!(n->tag & TRGNODE) || n->V.n == 0) continue;
+      double gc[3];
+      center(h, g, gc);
+      double D = 2.0 * radius(h, g);
+      double* ev = (double*)calloc(effn * h->tm, sizeof(double));
+      for (int qq = 0; qq < n->V.n; qq++) {
+        int q = (g_order & 4) ? n->V.n - 1 - qq : qq;
+        int vi = n->V.a[q];
+        double vc[3];
+        center(h, vi, vc);
+        int idx[3];
+        for (int d = 0; d < 3; d++) idx[d] = (int)round((vc[d] - gc[d]) / D);
+        if (ue2dc(h, idx, h->effden[vi], ev) != OK) verr = 1;
+      }
+      effval2plnval(h, ev, h->tdncval + (size_t)n->tndeidx * pDC);
+      free(ev);
+    }
+    for (int g = c0; g < c1; g++) {
+      onode* n = &h->nd[g];
+      if (!(n->tag & TRGNODE)) continue;
+      for (int q = 0; q < n->V.n; q++) {
+        int vi = n->V.a[q];
+        if (--vleft[vi] == 0) { free(h->effden[vi]); h->effden[vi] = NULL; }
       }
     }
   }
+  for (int g = 0; g < nn; g++)  /* none is left, but no spectrum may outlive the eval */
+    if (h->effden[g]) { free(h->effden[g]); h->effden[g] = NULL; }
+  free(vleft); free(need);
+  if (verr) { free(dbeg); return ERR; }
+  trace_stage("V", &tt);
   /* W (:154-172) */
-  double* uepos = (double*)malloc(sizeof(double) * 3 * h->nsp[UE]);
-  for (int g = 0; g < nn; g++) {
-    onode* n = &h->nd[g];
-    if (!(n->tag & TRGNODE) || n->chd != -1) continue;
-    for (int q = 0; q < n->W.n; q++) {
-      int vi = n->W.a[q];
-      onode* s = &h->nd[vi];
-      if (s->chd == -1 && s->sextnum * sdof < pUE) {
-        kapply(&h->knl, s->sextnum, h->sextpos + 3 * s->sextbeg, h->sextnor + 3 * s->sextbeg,
-               h->sextden + (size_t)s->sextbeg * sdof, n->textnum, h->textpos + 3 * n->textbeg,
-               h->textval + (size_t)n->textbeg * tdof);
-      } else {
-        double c[3];
-        center(h, vi, c);
-        locpos(h, UE, c, radius(h, vi), uepos);
-        kapply(&h->mm, h->nsp[UE], uepos, uepos, h->supeden + (size_t)s->sndeidx * pUE, n->textnum,
-               h->textpos + 3 * n->textbeg, h->textval + (size_t)n->textbeg * tdof);
+#pragma omp parallel
+  {
+    double* uepos = (double*)malloc(sizeof(double) * 3 * h->nsp[UE]);
+#pragma omp for schedule(dynamic, 4)
+    for (int g = 0; g < nn; g++) {
+      onode* n = &h->nd[g];
+      if (!(n->tag & TRGNODE) || n->chd != -1) continue;
+      for (int q = 0; q < n->W.n; q++) {
+        int vi = n->W.a[q];
+        onode* s = &h->nd[vi];
+        if (s->chd == -1 && s->sextnum * sdof < pUE) {
+          kapply(&h->knl, s->sextnum, h->sextpos + 3 * s->sextbeg, h->sextnor + 3 * s->sextbeg,
+                 h->sextden + (size_t)s->sextbeg * sdof, n->textnum, h->textpos + 3 * n->textbeg,
+                 h->textval + (size_t)n->textbeg * tdof);
+        } else {
+          double c[3];
+          center(h, vi, c);
+          locpos(h, UE, c, radius(h, vi), uepos);
+          kapply(&h->mm, h->nsp[UE], uepos, uepos, h->supeden + (size_t)s->sndeidx * pUE, n->textnum,
+                 h->textpos + 3 * n->textbeg, h->textval + (size_t)n->textbeg * tdof);
+        }
       }
     }
+    free(uepos);
   }
+  trace_stage("W", &tt);
   /* X (:176-191) */
-  for (int g = 0; g < nn; g++) {
-    onode* n = &h->nd[g];
-    if (!(n->tag & TRGNODE)) continue;
-    for (int q = 0; q < n->X.n; q++) {
-      onode* s = &h->nd[n->X.a[q]];
-      if (n->chd == -1 && n->textnum * tdof < pDC) {
-        kapply(&h->knl, s->sextnum, h->sextpos + 3 * s->sextbeg, h->sextnor + 3 * s->sextbeg,
-               h->sextden + (size_t)s->sextbeg * sdof, n->textnum, h->textpos + 3 * n->textbeg,
-               h->textval + (size_t)n->textbeg * tdof);
-      } else {
-        double c[3];
-        center(h, g, c);
-        locpos(h, DC, c, radius(h, g), pos);
-        kapply(&h->knl, s->sextnum, h->sextpos + 3 * s->sextbeg, h->sextnor + 3 * s->sextbeg,
-               h->sextden + (size_t)s->sextbeg * sdof, h->nsp[DC], pos, h->tdncval + (size_t)n->tndeidx * pDC);
+#pragma omp parallel
+  {
+    double* pos = (double*)malloc(sizeof(double) * 3 * h->nsp[DC]);
+#pragma omp for schedule(dynamic, 4)
+    for (int g = 0; g < nn; g++) {
+      onode* n = &h->nd[g];
+      if (!(n->tag & TRGNODE)) continue;
+      for (int q = 0; q < n->X.n; q++) {
+        onode* s = &h->nd[n->X.a[q]];
+        if (n->chd == -1 && n->textnum * tdof < pDC) {
+          kapply(&h->knl, s->sextnum, h->sextpos + 3 * s->sextbeg, h->sextnor + 3 * s->sextbeg,
+                 h->sextden + (size_t)s->sextbeg * sdof, n->textnum, h->textpos + 3 * n->textbeg,
+                 h->textval + (size_t)n->textbeg * tdof);
+        } else {
+          double c[3];
+          center(h, g, c);
+          locpos(h, DC, c, radius(h, g), pos);
+          kapply(&h->knl, s->sextnum, h->sextpos + 3 * s->sextbeg, h->sextnor + 3 * s->sextbeg,
+                 h->sextden + (size_t)s->sextbeg * sdof, h->nsp[DC], pos, h->tdncval + (size_t)n->tndeidx * pDC);
+        }
       }
     }
+    free(pos);
   }
-  /* combine (:195-216) */
-  double* depos = (double*)malloc(sizeof(double) * 3 * h->nsp[DE]);
-  for (int g = 0; g < nn; g++) {
-    onode* n = &h->nd[g];
-    if (!(n->tag & TRGNODE)) continue;
-    if (n->depth >= 3) {
-      int par = n->par;
-      int idx[3];
-      for (int d = 0; d < 3; d++) idx[d] = n->path[d] - 2 * h->nd[par].path[d];
-      de2dc(h, h->nd[par].depth + h->rootlvl, idx, h->tdneden + (size_t)h->nd[par].tndeidx * pDE,
-            h->tdncval + (size_t)n->tndeidx * pDC);
-    }
-    if (n->depth >= 2)
-      c2e(h, h->dc2de, pDE, pDC, h->nsp[DE], n->depth + h->rootlvl, h->tdncval + (size_t)n->tndeidx * pDC,
-          h->tdneden + (size_t)n->tndeidx * pDE);
-    if (n->chd == -1) {
-      double c[3];
-      center(h, g, c);
-      locpos(h, DE, c, radius(h, g), depos);
-      kapply(&h->mm, h->nsp[DE], depos, depos, h->tdneden + (size_t)n->tndeidx * pDE, n->textnum,
-             h->textpos + 3 * n->textbeg, h->textval + (size_t)n->textbeg * tdof);
+  trace_stage("X", &tt);
+  /* combine (:195-216), top level first; the nodes of one level are independent */
+  for (int dep = 0; dep < nd; dep++) {
+#pragma omp parallel
+    {
+      double* depos = (double*)malloc(sizeof(double) * 3 * h->nsp[DE]);
+#pragma omp for schedule(dynamic, 4)
+      for (int g = dbeg[dep]; g < dbeg[dep + 1]; g++) {
+        onode* n = &h->nd[g];
+        if (!(n->tag & TRGNODE)) continue;
+        if (n->depth >= 3) {
+          int par = n->par;
+          int idx[3];
+          for (int d = 0; d < 3; d++) idx[d] = n->path[d] - 2 * h->nd[par].path[d];
+          de2dc(h, h->nd[par].depth + h->rootlvl, idx, h->tdneden + (size_t)h->nd[par].tndeidx * pDE,
+                h->tdncval + (size_t)n->tndeidx * pDC);
+        }
+        if (n->depth >= 2)
+          c2e(h, h->dc2de, pDE, pDC, h->nsp[DE], n->depth + h->rootlvl, h->tdncval + (size_t)n->tndeidx * pDC,
+              h->tdneden + (size_t)n->tndeidx * pDE);
+        if (n->chd == -1) {
+          double c[3];
+          center(h, g, c);
+          locpos(h, DE, c, radius(h, g), depos);
+          kapply(&h->mm, h->nsp[DE], depos, depos, h->tdneden + (size_t)n->tndeidx * pDE, n->textnum,
+                 h->textpos + 3 * n->textbeg, h->textval + (size_t)n->textbeg * tdof);
+        }
+      }
+      free(depos);
     }
   }
+  trace_stage("down", &tt);
+  free(dbeg);
   /* save (:221-235) */
   for (int g = 0; g < nn; g++) {
     onode* n = &h->nd[g];
@@ -1260,7 +1416,6 @@ int orc_eval(orc_fmm* h, const double* srcden, double* trgval)
       for (int k = 0; k < n->town.n; k++)
         for (int d = 0; d < tdof; d++) trgval[n->town.a[k] * tdof + d] = h->textval[(n->textbeg + k) * tdof + d];
   }
-  free(pos); free(uepos); free(depos);
   return OK;
 }
 

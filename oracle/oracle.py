@@ -65,8 +65,20 @@ def lib():
         L.orc_regpos.argtypes = [ctypes.c_int, ctypes.c_double, _D]
         L.orc_rfft3.argtypes = [ctypes.c_int, ctypes.c_int, _D, _D]
         L.orc_irfft3.argtypes = [ctypes.c_int, ctypes.c_int, _D, _D]
+        L.orc_set_threads.argtypes = [ctypes.c_int]
+        L.orc_get_threads.restype = ctypes.c_int
         _lib = L
     return _lib
+
+
+def set_threads(n: int) -> None:
+    """Threads of OracleFMM.eval (n <= 0: all host cores).  Results are the
+    same for every thread count."""
+    lib().orc_set_threads(int(n))
+
+
+def get_threads() -> int:
+    return lib().orc_get_threads()
 
 
 def _dp(a):

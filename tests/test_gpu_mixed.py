@@ -82,3 +82,27 @@ def test_mixed_vector_kernels_one_rhs():
     run(321, 6, 60, p, n, (0.0, 0.0, 0.0), 0, 1, 24)
     c = clusters(8000, 25)
     run(511, 8, 80, c, None, (0.5, 0.5, 0.5), 1, 1, 25)
+
+
+@pytest.mark.parametrize("kt,nrhs", [(111, 16), (511, 4)])
+def test_mixed_near_coincident_pairs_across_leaves(kt, nrhs):
+    """Pairs of points 1e-10 to 2e-9 apart on either side of splitting planes
+    (multiples of 1/8, faces of every box of depth <= 3): the reference drops
+    such pairs (r < 1e-8, kernel3d.cpp:21, :132) whichever leaves hold them.
+    The FP32 offsets of the mixed near field cannot resolve these distances
+    (1/8 of the unit cube is ~1e-8 in FP32 ulps), so the set-up flags the leaf
+    pairs where this can happen and those pairs take FP64 displacements.
+    Without that, each pair would add ~1/(4 pi 1e-8) to both targets and the
+    mixed form would differ from the plain form by orders of magnitude."""
+    p = uniform_cube(60000, 26)
+    rng = np.random.default_rng(27)
+    k = 48
+    planes = rng.integers(1, 8, k) / 8.0
+    axis = rng.integers(0, 3, k)
+    base = rng.random((k, 3)) * 0.9 + 0.05
+    gap = 10.0 ** rng.uniform(-10, np.log10(2e-9), k)
+    a, b = base.copy(), base.copy()
+    a[np.arange(k), axis] = planes - gap / 2
+    b[np.arange(k), axis] = planes + gap / 2
+    pts = np.concatenate([p, a, b])
+    run(kt, 4, 150, pts, None, (0.5, 0.5, 0.5), 1, nrhs, 26)

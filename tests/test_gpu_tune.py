@@ -118,7 +118,7 @@ def test_tune_and_pin_mixed_laplace():
     assert np.array_equal(again, plain)
 
 
-@pytest.mark.parametrize("kt", [311, 511])
+@pytest.mark.parametrize("kt", [311, 312, 511])
 def test_mixed_fp32_vector_near_field(kt):
     """Mixed form of kernels without normals: the FP32 vector near field (the
     default), the FP64 ones pinned, and the tuned choice, against the plain form."""

@@ -123,6 +123,29 @@ def test_error_paths():
     assert L.kifmm3d_create(None, 111, cf.ctypes.data_as(D), 1, 6, 100, 10, ctr.ctypes.data_as(D), 0) == K.EINVAL
 
 
+def test_batch_rejects_repeated_handle():
+    """A handle owns one set of work buffers, so it may appear in a batch once:
+    twice, its two problems would run on concurrent graph branches over the
+    same buffers."""
+    L = K.lib()
+    D = ctypes.POINTER(ctypes.c_double)
+    ctr = np.zeros(3)
+    cf = np.array([1.0])
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert L.kifmm3d_create(ctypes.byref(a), 111, cf.ctypes.data_as(D), 1, 4, 50, 10, ctr.ctypes.data_as(D), 0) == K.OK
+    assert L.kifmm3d_create(ctypes.byref(b), 111, cf.ctypes.data_as(D), 1, 4, 50, 10, ctr.ctypes.data_as(D), 0) == K.OK
+    out = ctypes.c_void_p()
+    arr = (ctypes.c_void_p * 3)(a, b, a)
+    assert L.kifmm3d_batch_create(ctypes.byref(out), arr, 3) == K.EINVAL
+    assert b"same handle" in L.kifmm3d_batch_last_error(None)
+    assert not out.value
+    # distinct handles get past that check (and stop at the missing set-up)
+    arr = (ctypes.c_void_p * 2)(a, b)
+    assert L.kifmm3d_batch_create(ctypes.byref(out), arr, 2) == K.ESTATE
+    L.kifmm3d_destroy(a)
+    L.kifmm3d_destroy(b)
+
+
 def test_stokes_dl_needs_normals():
     p = uniform_cube(100, 3) * 2 - 1
     f = K.FMM3d()

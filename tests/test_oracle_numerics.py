@@ -147,3 +147,34 @@ def test_rounding_floor_of_the_algorithm():
     finally:
         L.orc_set_order(0)
         L.orc_set_accurate(1)
+
+
+@pytest.mark.parametrize("kt,gen", [(111, "clusters"), (321, "sphere"), (511, "clusters")])
+def test_threaded_eval_is_bitwise_sequential(kt, gen):
+    """The oracle's per-node loops run on several host threads; every value is
+    still summed by one thread in the sequential order, so any thread count
+    gives the same bits (also with reversed summation orders)."""
+    if gen == "sphere":
+        p, nrm = sphere(3000, 80)
+        ctr, lvl = (0.0, 0.0, 0.0), 0
+    else:
+        p, nrm = clusters(3000, 81), None
+        ctr, lvl = (0.5, 0.5, 0.5), 1
+    o = O.OracleFMM(kt, COEFS[kt], 4, 30, 10, ctr, lvl)
+    o.setup(p, nrm, p)
+    den = densities(len(p), o.sdof, 1, 82)[0]
+    old = O.get_threads()
+    lib = O.lib()
+    lib.orc_set_order.argtypes = [ctypes.c_int]
+    try:
+        for order in (0, 7):
+            lib.orc_set_order(order)
+            O.set_threads(1)
+            a = o.eval(den)
+            O.set_threads(4)
+            b = o.eval(den)
+            c = o.eval(den)
+            assert np.array_equal(a, b) and np.array_equal(b, c)
+    finally:
+        lib.orc_set_order(0)
+        O.set_threads(old)
