@@ -345,7 +345,15 @@ def run_forms(K, f, forms, c, args, d_src, d_trg, nrhs, n, tdof, seed, stats, ca
     for name, mode in forms:
         tag = f"{label}{name}"
         tuned = None
-        if not args.no_tune:
+        if args.pin and name == forms[-1][0]:
+            # settings pinned from an earlier tuned run (profiling runs keep
+            # the tuner's own evaluates out of the trace)
+            near, _, m2l = args.pin.partition(":")
+            f.set_settings(mode, int(near), [int(v) for v in m2l.split(",") if v])
+            st = f.settings(mode)
+            tuned = {"near": st["near"], "m2l": st["m2l"], "pinned": True}
+            log(f"{tag}: pinned settings: near-field kernel {st['near']}, M2L form per V-list level {st['m2l']}")
+        elif not args.no_tune:
             t0 = time.perf_counter()
             st = f.tune(nrhs, mode)
             tuned = {"near": st["near"], "m2l": st["m2l"], "s": round(time.perf_counter() - t0, 3)}
@@ -388,6 +396,8 @@ def main():
     ap.add_argument("--no-check", action="store_true", help="skip the direct-sum error check")
     ap.add_argument("--no-gate", action="store_true",
                     help="skip the higher-np leg timed beside the workload's np (for the 1e-6 error gate)")
+    ap.add_argument("--pin", default=None,
+                    help="NEAR:M2L,... settings of the chosen form (as the autotuner reports them) instead of tuning")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the autotuner (near-field kernel and per-level M2L form, timed on the live tree)")
     ap.add_argument("--calibrate", action="store_true",

@@ -286,7 +286,12 @@ int orc_direct(int kt, const double* coefs, int ncoefs, int ns, const double* sp
   if (orc_kernel_dofs(kt, &sdof, &tdof) != OK) return ERR;
   k.kt = kt; k.c[0] = ncoefs > 0 ? coefs[0] : 1.0; k.c[1] = ncoefs > 1 ? coefs[1] : 0.0;
   memset(tval, 0, sizeof(double) * (size_t)nt * tdof);
-  kapply(&k, ns, spos, snor, sden, nt, tpos, tval);
+  /* targets in blocks over the threads; each target sums its sources in order */
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int i0 = 0; i0 < nt; i0 += 16) {
+    int m = nt - i0 < 16 ? nt - i0 : 16;
+    kapply(&k, ns, spos, snor, sden, m, tpos + 3 * (size_t)i0, tval + (size_t)i0 * tdof);
+  }
   return OK;
 }
 

@@ -252,3 +252,21 @@ def test_settings_pin_and_validate():
     with pytest.raises(K.KifmmError) as e:
         h.set_settings(K.MODE_MIXED, 4)
     assert e.value.code == K.EINVAL
+
+
+def test_host_code_under_sanitizers(tmp_path):
+    """The host work of the library (tree, lists, plan, operators, disk cache,
+    argument checks) on uniform, clustered, sphere and on-plane point sets
+    with repeated points, through the C ABI, built with AddressSanitizer and
+    UndefinedBehaviorSanitizer (csrc/host_check.cpp, `make sanitize`): any
+    out-of-bounds access, overflow or misuse aborts it."""
+    csrc = os.path.join(os.path.dirname(K.LIBPATH), "csrc")
+    subprocess.run(["make", "-s", "-j", "8", "-C", csrc, "sanitize"], check=True)
+    exe = os.path.join(csrc, "_build", "host_check_san")
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0:abort_on_error=1",
+               UBSAN_OPTIONS="halt_on_error=1:print_stacktrace=1")
+    r = subprocess.run([exe, str(tmp_path)], capture_output=True, text=True, timeout=600, env=env)
+    print(r.stdout[-3000:], r.stderr[-3000:])
+    assert r.returncode == 0
+    assert "0 failed checks" in r.stdout
+    assert "ERROR: AddressSanitizer" not in r.stderr and "runtime error" not in r.stderr
