@@ -673,9 +673,10 @@ constexpr int TILE32 = 128;  // sources per staged tile of the FP32 kernel (meas
 
 // The reference's r < 1e-8 rule (kernel3d.cpp:21, :132) by entry, as the
 // set-up flagged it (echk):
-//   CHK 0: no pair of the entry is closer than a few 1e-8 (two different
-//          leaves without points next to their shared boundary): 1/r of
-//          max(r^2, 1e-16), one instruction instead of two;
+//   CHK 0: every pair of the entry is farther apart than its FP32 offsets
+//          can lose and than 1e-8 (two different leaves without points next
+//          to their shared boundary), so r^2 > 0: 1/r with no guard
+//          (padded sources sit far away with zero density);
 //   CHK 1: the target's own leaf with no two distinct points that close:
 //          equal points (equal FP32 offsets) have r^2 = 0 and drop out;
 //   CHK 2: a pair can be closer than the FP32 offsets resolve (leaves with
@@ -713,7 +714,7 @@ __device__ __forceinline__ void p2p_tile32(int cnt, int cnt4, int kk, int col, c
       } else {
         const float dx = tx[i] - sx, dy = ty[i] - sy, dz = tz[i] - sz;
         const float r2 = dx * dx + dy * dy + dz * dz;
-        gv = CHK == 1 ? (r2 < 1e-16f ? 0.0f : __builtin_amdgcn_rsqf(r2)) : __builtin_amdgcn_rsqf(fmaxf(r2, 1e-16f));
+        gv = CHK == 1 ? (r2 < 1e-16f ? 0.0f : __builtin_amdgcn_rsqf(r2)) : __builtin_amdgcn_rsqf(r2);
       }
       part[i] = __builtin_amdgcn_mfma_f32_16x16x4f32(f, gv, part[i], 0, 0, 0);
     }
@@ -839,7 +840,7 @@ __global__ __launch_bounds__(P2P_BS) void k_p2p_mfma111_f32(int nchunk, const in
       __syncthreads();
       for (int i = t; i < cnt4; i += P2P_BS) {
         const bool in = i < cnt;
-        s_pos[0][i] = in ? (float)(src.x[j0 + i] - ox) : 0.0f;
+        s_pos[0][i] = in ? (float)(src.x[j0 + i] - ox) : 1e6f;  // padding: far away, zero density
         s_pos[1][i] = in ? (float)(src.y[j0 + i] - oy) : 0.0f;
         s_pos[2][i] = in ? (float)(src.z[j0 + i] - oz) : 0.0f;
       }

@@ -207,16 +207,19 @@ void Engine::setup(const double* spos, const double* snor, int ns, const double*
   setup_done_ = true;
 }
 
-// Which near-field entries can hold a pair that the reference's r < 1e-8
-// exclusion (kernel3d.cpp:21, :132) applies to, for the mixed form's FP32
-// near field (whose relative coordinates resolve ~1e-7 of a leaf's width):
+// Which near-field entries can hold a pair closer than the mixed form's FP32
+// near field resolves -- offsets from the chunk's first target, which carry
+// an error of ~2^-24 of a leaf's width -- or closer than the reference's
+// r < 1e-8 exclusion (kernel3d.cpp:21, :132):
 // * a pair of points of two different leaves closer than d has both points
 //   within d of their own leaf's box boundary (the segment between them
 //   leaves the box), so a cross-leaf entry needs that only when both leaves
 //   hold such a point (sedge_, tedge_);
 // * inside one leaf, equal points have equal FP32 offsets and drop out; two
 //   distinct points closer than d are looked for directly (close_).
-// The tolerance has a margin over 1e-8.
+// d = max(4e-8, 1e-6 x the widest leaf): every pair of an entry that is not
+// flagged is at least d apart, more than the FP32 offsets can lose, so its
+// r^2 is positive and above the exclusion radius.
 constexpr double EDGE_TOL = 4e-8;
 
 void Engine::mark_edge_leaves(const double* spos, const double* tpos)
@@ -225,12 +228,16 @@ void Engine::mark_edge_leaves(const double* spos, const double* tpos)
   sedge_.assign(T.nnode, 0);
   tedge_.assign(T.nnode, 0);
   close_.assign(T.nnode, 0);
+  double wmax = 0.0;
+  for (int g = 0; g < T.nnode; g++)
+    if (T.terminal(g)) wmax = std::max(wmax, 2.0 * T.radius(g));
+  const double tol = std::max(EDGE_TOL, 1e-6 * wmax);
   auto near_edge = [&](int g, const double* p) {
     double c[3];
     T.center(g, c);
     const double r = T.radius(g);
     for (int d = 0; d < 3; d++)
-      if (p[d] - (c[d] - r) < EDGE_TOL || (c[d] + r) - p[d] < EDGE_TOL) return true;
+      if (p[d] - (c[d] - r) < tol || (c[d] + r) - p[d] < tol) return true;
     return false;
   };
   std::vector<std::pair<double, int>> xs;
@@ -250,12 +257,12 @@ void Engine::mark_edge_leaves(const double* spos, const double* tpos)
       std::sort(xs.begin(), xs.end());
       for (int k = T.textbeg[g]; k < T.textbeg[g] + T.textnum[g] && !close_[g]; k++) {
         const double* q = tpos + 3 * (size_t)T.tperm[k];
-        auto it = std::lower_bound(xs.begin(), xs.end(), std::make_pair(q[0] - EDGE_TOL, -1));
-        for (; it != xs.end() && it->first <= q[0] + EDGE_TOL; ++it) {
+        auto it = std::lower_bound(xs.begin(), xs.end(), std::make_pair(q[0] - tol, -1));
+        for (; it != xs.end() && it->first <= q[0] + tol; ++it) {
           const double* p = spos + 3 * (size_t)it->second;
           const double dx = q[0] - p[0], dy = q[1] - p[1], dz = q[2] - p[2];
           const double r2 = dx * dx + dy * dy + dz * dz;
-          if (r2 > 0.0 && r2 < EDGE_TOL * EDGE_TOL) {
+          if (r2 > 0.0 && r2 < tol * tol) {
             close_[g] = 1;
             break;
           }
