@@ -104,6 +104,24 @@ int kifmm3d_evaluate(kifmm3d* h, const double* srcden, double* trgval, int nrhs,
  * pointers from kifmm3d_device_alloc).  Returns when the work is complete. */
 int kifmm3d_evaluate_device(kifmm3d* h, const double* d_srcden, double* d_trgval, int nrhs, int mode);
 
+/* kifmm3d_evaluate_device that allocates no device memory.  No counterpart in
+ * the reference: for callers that cannot call hipMalloc (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid allocation).  Every work
+ * buffer of the pass (gathered densities, surface vectors, spectra, split-K
+ * partials, value staging) is carved out of one static device array of
+ * 256 MiB per GPU, which each GPU the library runs on holds until the process
+ * ends; the handle's set-up data and the caller's srcden / trgval are used as
+ * they are.  Right-hand sides run in chunks of kifmm3d_evaluate_device's size
+ * where they fit the array, else in the largest chunks that do; a problem
+ * whose work for one right-hand side exceeds 256 MiB fails with
+ * KIFMM3D_ENOMEM.  Every call on one GPU shares the array (each call
+ * overwrites what it needs before reading it), so calls that overlap on one
+ * GPU -- from several threads or handles at once -- race.  Launches are eager
+ * (no graph) and untimed (stage and kernel times read 0); kifmm3d_get_buffer
+ * reports KIFMM3D_ESTATE after such a call.  Returns when the work is
+ * complete. */
+int kifmm3d_evaluate_device_noalloc(kifmm3d* h, const double* d_srcden, double* d_trgval, int nrhs, int mode);
+
 /* Stage times (ms) of the last evaluate; n <= KIFMM3D_NSTAGE slots. */
 int kifmm3d_stage_times(const kifmm3d* h, double* ms, int n);
 

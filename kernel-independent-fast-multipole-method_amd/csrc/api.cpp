@@ -87,6 +87,12 @@ int kifmm3d_evaluate_device(kifmm3d* h, const double* d_srcden, double* d_trgval
   return guarded(h, [&] { h->eng->evaluate_device(d_srcden, d_trgval, nrhs, mode); });
 }
 
+int kifmm3d_evaluate_device_noalloc(kifmm3d* h, const double* d_srcden, double* d_trgval, int nrhs, int mode)
+{
+  if (!h) return KIFMM3D_EINVAL;
+  return guarded(h, [&] { h->eng->evaluate_device_noalloc(d_srcden, d_trgval, nrhs, mode); });
+}
+
 int kifmm3d_stage_times(const kifmm3d* h, double* ms, int n)
 {
   if (!h || !ms || n < 0) return KIFMM3D_EINVAL;

@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 #include <type_traits>
 
 #include "device.hpp"
@@ -360,6 +361,37 @@ static int grid_for(long tot, int bs)
   long g = (tot + bs - 1) / bs;
   if (g > 65536L * 8) g = 65536L * 8;
   return (int)(g > 0 ? g : 1);
+}
+
+// ---------------------------------------------------------------------------
+// Scratch array of the no-allocation evaluate.  In host code `scratch` names a
+// host-side stand-in, not device memory: kernels and copies get the address
+// that hipGetSymbolAddress returns for the current device (each GPU holds its
+// own copy at its own address), kept per device.
+static __device__ __align__(256) unsigned char scratch[SCRATCH_BYTES];
+
+namespace {
+constexpr int SCRATCH_MAXDEV = 64;
+std::mutex scratch_mu;
+void* scratch_dev[SCRATCH_MAXDEV];
+}  // namespace
+
+hipError_t scratch_ptr(void** out)
+{
+  int d = 0;
+  hipError_t e = hipGetDevice(&d);
+  if (e != hipSuccess) return e;
+  if (d < 0 || d >= SCRATCH_MAXDEV) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  if (!scratch_dev[d]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    if (!p) return hipErrorInvalidSymbol;
+    scratch_dev[d] = p;
+  }
+  *out = scratch_dev[d];
+  return hipSuccess;
 }
 
 // Zero fill by kernel (also inside captured graphs, where memset nodes are avoided)

@@ -11,6 +11,14 @@ struct KC {  // kernel constants (see kernel_consts in kernels_host.cpp)
   double a, b, c;
 };
 
+// Work memory of the no-allocation evaluate: one static device array of
+// SCRATCH_BYTES per GPU, which that GPU holds until the process ends.  Every
+// call on a GPU shares it, so calls that overlap on one GPU race.
+constexpr size_t SCRATCH_BYTES = size_t(256) << 20;
+// Device address of the array on the current device, resolved with
+// hipGetSymbolAddress on the first call made with that device current.
+hipError_t scratch_ptr(void** out);
+
 // p[0 .. n) = 0
 void zero(hipStream_t s, double* p, long n);
 // dst[0 .. n) = src[0 .. n)

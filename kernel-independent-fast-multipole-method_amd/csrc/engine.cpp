@@ -28,9 +28,10 @@ namespace kifmm {
 template <class T>
 void DVec<T>::reset()
 {
-  if (p) (void)hipFree(p);
+  if (p && own) (void)hipFree(p);
   p = nullptr;
   n = 0;
+  own = true;
 }
 template <class T>
 void DVec<T>::alloc(size_t count)
@@ -861,21 +862,19 @@ int Engine::choose_rhs_chunk(int nrhs)
   return rc;
 }
 
-void Engine::ensure_work(int rc)
+void Engine::work_counts(int rc, size_t cnt[NWORK]) const
 {
   const Operators& O = *ops_;
   const Tree& T = tree_;
-  bool grew = false;
-  grew |= sden_.ensure((size_t)rc * T.sextcnt * user_.sdof);
-  if (user_.kt == 111) grew |= sden32_.ensure((size_t)rc * T.sextcnt);
-  grew |= tval_.ensure((size_t)rc * T.textcnt * user_.tdof);
-  grew |= tvalT_.ensure((size_t)rc * T.textcnt * user_.tdof);
-  grew |= upcval_.ensure((size_t)rc * T.sndecnt * O.pln[SURF_UC]);
-  grew |= upeden_.ensure((size_t)rc * T.sndecnt * O.pln[SURF_UE]);
-  grew |= dncval_.ensure((size_t)rc * T.tndecnt * O.pln[SURF_DC]);
-  grew |= dneden_.ensure((size_t)rc * T.tndecnt * O.pln[SURF_DE]);
-  grew |= effden_.ensure((size_t)2 * rc * plan_.max_vsrc * mm_.sdof * O.C);
-  grew |= effval_.ensure((size_t)2 * rc * plan_.max_vtrg * mm_.tdof * O.C);
+  cnt[0] = (size_t)rc * T.sextcnt * user_.sdof;
+  cnt[1] = (size_t)rc * T.textcnt * user_.tdof;
+  cnt[2] = (size_t)rc * T.textcnt * user_.tdof;
+  cnt[3] = (size_t)rc * T.sndecnt * O.pln[SURF_UC];
+  cnt[4] = (size_t)rc * T.sndecnt * O.pln[SURF_UE];
+  cnt[5] = (size_t)rc * T.tndecnt * O.pln[SURF_DC];
+  cnt[6] = (size_t)rc * T.tndecnt * O.pln[SURF_DE];
+  cnt[7] = (size_t)2 * rc * plan_.max_vsrc * mm_.sdof * O.C;
+  cnt[8] = (size_t)2 * rc * plan_.max_vtrg * mm_.tdof * O.C;
   // split-K partials of the matrix-core GEMMs with few columns, for every
   // chunk size up to rc (the last chunk of an evaluate may be smaller)
   size_t gw = 0;
@@ -890,7 +889,19 @@ void Engine::ensure_work(int rc)
         const int ks = dev::gemm_split(a);
         if (ks > 1) gw = std::max(gw, (size_t)ks * op.ncol * r * op.M);
       }
-  grew |= gwork_.ensure(gw);
+  cnt[9] = gw;
+  cnt[10] = user_.kt == 111 ? (size_t)rc * T.sextcnt : 0;
+}
+
+void Engine::ensure_work(int rc)
+{
+  size_t cnt[NWORK];
+  work_counts(rc, cnt);
+  DVec<double>* bufs[NWORK - 1] = {&sden_,   &tval_,   &tvalT_, &upcval_, &upeden_,
+                                   &dncval_, &dneden_, &effden_, &effval_, &gwork_};
+  bool grew = false;
+  for (int b = 0; b < NWORK - 1; b++) grew |= bufs[b]->ensure(cnt[b]);
+  if (user_.kt == 111) grew |= sden32_.ensure(cnt[10]);
   // graphs captured before hold the freed buffers: they are dropped here and
   // the batch graphs that recorded this engine see the new generation
   if (grew) {
@@ -911,6 +922,7 @@ int Engine::prepare(int nrhs)
 {
   const int rc = choose_rhs_chunk(nrhs);
   ensure_work(rc);
+  last_in_scratch_ = false;
   return rc;
 }
 
@@ -1009,6 +1021,101 @@ void Engine::evaluate_device(const double* d_srcden, double* d_trgval, int nrhs,
   double tot = 0;
   for (int s = 0; s < 8; s++) tot += stage_ms[s];
   stage_ms[KIFMM3D_STAGE_TOTAL] = tot;
+}
+
+void Engine::evaluate_device_noalloc(const double* d_srcden, double* d_trgval, int nrhs, int mode)
+{
+  check_eval_args(d_srcden, d_trgval, nrhs, mode);
+  void* base = nullptr;
+  HIPCHK(dev::scratch_ptr(&base));
+  for (double& m : stage_ms) m = 0.0;
+  for (double& m : kernel_ms) m = 0.0;
+  for (int& c : kernel_calls) c = 0;
+  if (nt_ == 0) return;
+  // bytes of the work buffers for chunks of r right-hand sides, each piece
+  // starting on a 256-byte boundary of the array (debugging aid: with
+  // KIFMM3D_SCRATCH_GAP = g bytes, g-byte gaps filled with 0xff bytes follow
+  // every piece and are checked afterwards; KIFMM3D_SCRATCH_POISON = mask
+  // fills the pieces of the set bits with 0xff bytes (NaN) first)
+  const char* gap_env = std::getenv("KIFMM3D_SCRATCH_GAP");
+  const char* poison_env = std::getenv("KIFMM3D_SCRATCH_POISON");
+  const size_t gap = gap_env ? (size_t)std::atol(gap_env) / 256 * 256 : 0;
+  const unsigned poison = poison_env ? (unsigned)std::atoi(poison_env) : 0u;
+  size_t cnt[NWORK], off[NWORK], len[NWORK];
+  auto layout = [&](int r) {
+    work_counts(r, cnt);
+    size_t at = 0;
+    for (int b = 0; b < NWORK; b++) {
+      off[b] = at;
+      len[b] = (cnt[b] * (b == NWORK - 1 ? sizeof(float) : sizeof(double)) + 255) / 256 * 256;
+      at += len[b] + gap;
+    }
+    return at;
+  };
+  int rc = choose_rhs_chunk(nrhs);
+  while (rc > 1 && layout(rc) > dev::SCRATCH_BYTES) rc = rc > 16 ? rc - 16 : rc - 1;
+  const size_t need = layout(rc);
+  if (need > dev::SCRATCH_BYTES)
+    throw Error(KIFMM3D_ENOMEM, "the no-allocation evaluate needs " + std::to_string(need) +
+                                    " bytes of work memory for one right-hand side of this problem; its scratch "
+                                    "array holds " + std::to_string(dev::SCRATCH_BYTES));
+  // point the work buffers at their pieces of the array for this call, then
+  // give the handle its own buffers back
+  DVec<double>* bufs[NWORK - 1] = {&sden_,   &tval_,   &tvalT_, &upcval_, &upeden_,
+                                   &dncval_, &dneden_, &effden_, &effval_, &gwork_};
+  struct Saved {
+    double* p;
+    size_t n;
+    bool own;
+  } saved[NWORK - 1];
+  float* const p32 = sden32_.p;
+  const size_t n32 = sden32_.n;
+  const bool own32 = sden32_.own;
+  unsigned char* b0 = static_cast<unsigned char*>(base);
+  for (int b = 0; b < NWORK - 1; b++) {
+    saved[b] = {bufs[b]->p, bufs[b]->n, bufs[b]->own};
+    bufs[b]->p = reinterpret_cast<double*>(b0 + off[b]);
+    bufs[b]->n = cnt[b];
+    bufs[b]->own = false;
+  }
+  sden32_.p = reinterpret_cast<float*>(b0 + off[NWORK - 1]);
+  sden32_.n = cnt[NWORK - 1];
+  sden32_.own = false;
+  auto restore = [&] {
+    for (int b = 0; b < NWORK - 1; b++) {
+      bufs[b]->p = saved[b].p;
+      bufs[b]->n = saved[b].n;
+      bufs[b]->own = saved[b].own;
+    }
+    sden32_.p = p32;
+    sden32_.n = n32;
+    sden32_.own = own32;
+  };
+  try {
+    for (int b = 0; b < NWORK; b++) {
+      if (poison >> b & 1u) HIPCHK(hipMemset(b0 + off[b], 0xff, len[b]));
+      if (gap) HIPCHK(hipMemset(b0 + off[b] + len[b], 0xff, gap));
+    }
+    record(d_srcden, d_trgval, nrhs, rc, mode, stream_);
+    HIPCHK(hipStreamSynchronize(stream_));
+    if (gap) {
+      std::vector<unsigned char> g(gap);
+      for (int b = 0; b < NWORK; b++) {
+        HIPCHK(hipMemcpy(g.data(), b0 + off[b] + len[b], gap, hipMemcpyDeviceToHost));
+        for (size_t q = 0; q < gap; q++)
+          if (g[q] != 0xff)
+            throw Error(KIFMM3D_EDEVICE, "work buffer " + std::to_string(b) + " (" + std::to_string(cnt[b]) +
+                                             " elements) was written " + std::to_string(q) + " bytes past its end");
+      }
+    }
+  } catch (...) {
+    (void)hipStreamSynchronize(stream_);
+    restore();
+    throw;
+  }
+  restore();
+  last_nrhs_ = nrhs;
+  last_in_scratch_ = true;
 }
 
 void Engine::evaluate_host(const double* srcden, double* trgval, int nrhs, int mode)
@@ -1333,7 +1440,7 @@ void Engine::run_chunk(const double* srcden, double* trgval, int rc, int mode, h
 
 long Engine::get_buffer(int what, double* buf, long n)
 {
-  if (!setup_done_ || last_rc_ == 0) return KIFMM3D_ESTATE;
+  if (!setup_done_ || last_rc_ == 0 || last_in_scratch_) return KIFMM3D_ESTATE;
   const Operators& O = *ops_;
   const DVec<double>* src[4] = {&upeden_, &upcval_, &dneden_, &dncval_};
   const int pln[4] = {O.pln[SURF_UE], O.pln[SURF_UC], O.pln[SURF_DE], O.pln[SURF_DC]};

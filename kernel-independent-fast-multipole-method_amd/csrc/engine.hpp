@@ -15,11 +15,13 @@
 
 namespace kifmm {
 
-// Owning device array.
+// Owning device array (own = false: a view of memory held elsewhere, which
+// reset() leaves alone).
 template <class T>
 struct DVec {
   T* p = nullptr;
   size_t n = 0;
+  bool own = true;
   DVec() = default;
   DVec(const DVec&) = delete;
   DVec& operator=(const DVec&) = delete;
@@ -113,6 +115,13 @@ class Engine {
   void setup(const double* spos, const double* snor, int ns, const double* tpos, int nt);
   void evaluate_device(const double* d_srcden, double* d_trgval, int nrhs, int mode);
   void evaluate_host(const double* srcden, double* trgval, int nrhs, int mode);
+  // evaluate_device with every work buffer carved out of the library's static
+  // per-GPU scratch array (dev::SCRATCH_BYTES): no device memory is allocated
+  // or freed.  The chunk of right-hand sides is evaluate_device's where it
+  // fits the array, else the largest that does; a problem whose work buffers
+  // for one right-hand side exceed the array is rejected (KIFMM3D_ENOMEM).
+  // Kernel launches are eager (no graph) and untimed; returns when done.
+  void evaluate_device_noalloc(const double* d_srcden, double* d_trgval, int nrhs, int mode);
   // Autotuning: times the candidate near-field kernels and, per V-list level,
   // the candidate M2L forms on this tree for (nrhs, mode) -- one warm-up and
   // two timed evaluates of zero densities per round, rounds pairing the
@@ -172,6 +181,11 @@ class Engine {
   void upload_static();
   int choose_rhs_chunk(int nrhs);
   void ensure_work(int rc);
+  // Element counts of the work buffers for chunks of rc right-hand sides, in
+  // the order sden, tval, tvalT, upcval, upeden, dncval, dneden, effden,
+  // effval, gwork (doubles), sden32 (floats; 0 unless Laplace).
+  static constexpr int NWORK = 11;
+  void work_counts(int rc, size_t cnt[NWORK]) const;
   void run_chunk(const double* srcden, double* trgval, int rc, int mode, hipStream_t s, hipEvent_t* ev);
   void drop_graph();
   int near_variant(bool mixed) const;
@@ -188,6 +202,7 @@ class Engine {
   bool setup_done_ = false;
   bool host_done_ = false;
   int last_rc_ = 0;
+  bool last_in_scratch_ = false;  // the last evaluate ran in the shared scratch array
   unsigned long gen_ = 0;
   int last_nrhs_ = 0;  // right-hand sides of the last evaluate_device
   int m2l_rb_ = 0;   // right-hand sides per lane in the grouped M2L (0 = default; KIFMM3D_M2L_RB)

@@ -41,6 +41,7 @@ SYMBOLS = [
     ("kifmm3d_setup", ctypes.c_int, [_VP, _D, _D, ctypes.c_int, _D, ctypes.c_int]),
     ("kifmm3d_evaluate", ctypes.c_int, [_VP, _D, _D, ctypes.c_int, ctypes.c_int]),
     ("kifmm3d_evaluate_device", ctypes.c_int, [_VP, _VP, _VP, ctypes.c_int, ctypes.c_int]),
+    ("kifmm3d_evaluate_device_noalloc", ctypes.c_int, [_VP, _VP, _VP, ctypes.c_int, ctypes.c_int]),
     ("kifmm3d_stage_times", ctypes.c_int, [_VP, _D, ctypes.c_int]),
     ("kifmm3d_kernel_times", ctypes.c_int, [_VP, _D, _I, ctypes.c_int]),
     ("kifmm3d_info", ctypes.c_int, [_VP, _I, ctypes.c_int]),
@@ -287,6 +288,12 @@ class FMM3d:
     def eval_device(self, d_srcden: DeviceBuffer, d_trgval: DeviceBuffer, nrhs: int = 1, mode: int = MODE_PLAIN):
         """Evaluate with densities and values already resident in HBM."""
         self._check(lib().kifmm3d_evaluate_device(self.h, d_srcden.ptr, d_trgval.ptr, nrhs, mode))
+
+    def eval_device_noalloc(self, d_srcden: DeviceBuffer, d_trgval: DeviceBuffer, nrhs: int = 1,
+                            mode: int = MODE_PLAIN):
+        """eval_device with the work buffers in the library's static per-GPU
+        scratch array: no device allocation (256 MiB of work at most)."""
+        self._check(lib().kifmm3d_evaluate_device_noalloc(self.h, d_srcden.ptr, d_trgval.ptr, nrhs, mode))
 
     def set_graph(self, on: bool = True):
         """Capture the evaluate pass into a hipGraph and replay it on repeat calls."""
