@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <mutex>
 #include <type_traits>
+#include <utility>
 
 #include "device.hpp"
 
@@ -535,8 +536,8 @@ __global__ __launch_bounds__(P2P_BS) void k_p2p(const int* __restrict__ tbeg, co
 // rounded, normals and densities rounded, the kernel evaluated in FP32 and the
 // sums of each staged tile (at most TILE sources) added in FP64.  Entries
 // flagged 2 in echk (a pair may be closer than the relative FP32 coordinates
-// resolve, so the r < 1e-8 rule needs the FP64 positions) take each pair's
-// displacement as the FP64 difference rounded to FP32 instead.
+// resolve, so the r < 1e-8 rule needs the FP64 positions) are skipped: a
+// separate FP64 pass over those entries adds them.
 template <int KT, int NR>
 __global__ __launch_bounds__(P2P_BS) void k_p2p32(const int* __restrict__ tbeg, const int* __restrict__ tnum,
                                                   const int* __restrict__ eptr, const int2* __restrict__ ent,
@@ -555,14 +556,10 @@ __global__ __launch_bounds__(P2P_BS) void k_p2p32(const int* __restrict__ tbeg, 
   const int tb0 = tn > 0 ? tb : 0;  // origin of the relative coordinates
   const double ox = trg.x[tb0], oy = trg.y[tb0], oz = trg.z[tb0];
   float x = 0, y = 0, z = 0;
-  double x64 = 0, y64 = 0, z64 = 0;
   if (act) {
-    x64 = trg.x[tb + t];
-    y64 = trg.y[tb + t];
-    z64 = trg.z[tb + t];
-    x = (float)(x64 - ox);
-    y = (float)(y64 - oy);
-    z = (float)(z64 - oz);
+    x = (float)(trg.x[tb + t] - ox);
+    y = (float)(trg.y[tb + t] - oy);
+    z = (float)(trg.z[tb + t] - oz);
   }
   double u[NR][K::T];
 #pragma unroll
@@ -571,8 +568,8 @@ __global__ __launch_bounds__(P2P_BS) void k_p2p32(const int* __restrict__ tbeg, 
     for (int d = 0; d < K::T; d++) u[q][d] = 0.0;
   const double* den = src.den + (long)r0 * src.den_rs;
   for (int e = eptr[c]; e < eptr[c + 1]; e++) {
+    if (echk && echk[e] == 2) continue;  // block-uniform: the FP64 pass's
     const int2 se = ent[e];
-    const bool exact = echk && echk[e] == 2;  // block-uniform
     for (int b0 = 0; b0 < se.y; b0 += TILE) {
       const int cnt = min(TILE, se.y - b0);
       const int j0 = se.x + b0;
@@ -595,14 +592,9 @@ __global__ __launch_bounds__(P2P_BS) void k_p2p32(const int* __restrict__ tbeg, 
         for (int q = 0; q < NR; q++)
 #pragma unroll
           for (int d = 0; d < K::T; d++) p[q][d] = 0.0f;
-        if (exact)
-          for (int jj = 0; jj < cnt; jj++)
-            K::template add<NR>((float)(x64 - src.x[j0 + jj]), (float)(y64 - src.y[j0 + jj]),
-                                (float)(z64 - src.z[j0 + jj]), &s_pos[jj][NPOS - 3], s_den[jj], p, kc);
-        else
-          for (int jj = 0; jj < cnt; jj++)
-            K::template add<NR>(x - s_pos[jj][0], y - s_pos[jj][1], z - s_pos[jj][2], &s_pos[jj][NPOS - 3], s_den[jj],
-                                p, kc);
+        for (int jj = 0; jj < cnt; jj++)
+          K::template add<NR>(x - s_pos[jj][0], y - s_pos[jj][1], z - s_pos[jj][2], &s_pos[jj][NPOS - 3], s_den[jj],
+                              p, kc);
 #pragma unroll
         for (int q = 0; q < NR; q++)
 #pragma unroll
@@ -710,19 +702,14 @@ constexpr int TILE32 = 128;  // sources per staged tile of the FP32 kernel (meas
 //          to their shared boundary), so r^2 > 0: 1/r with no guard
 //          (padded sources sit far away with zero density);
 //   CHK 1: the target's own leaf with no two distinct points that close:
-//          equal points (equal FP32 offsets) have r^2 = 0 and drop out;
-//   CHK 2: a pair can be closer than the FP32 offsets resolve (leaves with
-//          points next to a shared boundary, or a leaf holding two distinct
-//          near-coincident points): the pair's 1/r and the rule are taken
-//          from the FP64 coordinates (x64: the entry's sources, t64: the
-//          lane's targets), so near-coincident pairs get the plain form's
-//          values.
+//          equal points (equal FP32 offsets) have r^2 = 0 and drop out.
+// (Flag 2 -- a pair can be closer than the FP32 offsets resolve: leaves with
+// points next to a shared boundary, or a leaf holding two distinct
+// near-coincident points -- is left to an FP64 pass over those entries.)
 template <int NG, int CHK>
-__device__ __forceinline__ void p2p_tile32(int cnt, int cnt4, int kk, int col, const float (*s_pos)[TILE32],
+__device__ __forceinline__ void p2p_tile32(int cnt4, int kk, int col, const float (*s_pos)[TILE32],
                                            const float (*s_den)[16], const float* tx, const float* ty,
-                                           const float* tz, double (*acc)[4], const double* x64 = nullptr,
-                                           const double* y64 = nullptr, const double* z64 = nullptr,
-                                           const double (*t64)[3] = nullptr)
+                                           const float* tz, double (*acc)[4])
 {
   v4f part[NG];
 #pragma unroll
@@ -730,24 +717,11 @@ __device__ __forceinline__ void p2p_tile32(int cnt, int cnt4, int kk, int col, c
   for (int g = 0; g < cnt4; g += 4) {
     const float sx = s_pos[0][g + kk], sy = s_pos[1][g + kk], sz = s_pos[2][g + kk];
     const float f = s_den[g + kk][col];
-    double qx = 0, qy = 0, qz = 0;
-    if (CHK == 2) {
-      const int j = g + kk < cnt ? g + kk : 0;  // padded sources have zero density
-      qx = x64[j];
-      qy = y64[j];
-      qz = z64[j];
-    }
 #pragma unroll
     for (int i = 0; i < NG; i++) {
-      float gv;
-      if (CHK == 2) {
-        const double ex = t64[i][0] - qx, ey = t64[i][1] - qy, ez = t64[i][2] - qz;
-        gv = (float)rinv(ex * ex + ey * ey + ez * ez);
-      } else {
-        const float dx = tx[i] - sx, dy = ty[i] - sy, dz = tz[i] - sz;
-        const float r2 = dx * dx + dy * dy + dz * dz;
-        gv = CHK == 1 ? (r2 < 1e-16f ? 0.0f : __builtin_amdgcn_rsqf(r2)) : __builtin_amdgcn_rsqf(r2);
-      }
+      const float dx = tx[i] - sx, dy = ty[i] - sy, dz = tz[i] - sz;
+      const float r2 = dx * dx + dy * dy + dz * dz;
+      const float gv = CHK == 1 ? (r2 < 1e-16f ? 0.0f : __builtin_amdgcn_rsqf(r2)) : __builtin_amdgcn_rsqf(r2);
       part[i] = __builtin_amdgcn_mfma_f32_16x16x4f32(f, gv, part[i], 0, 0, 0);
     }
   }
@@ -829,7 +803,8 @@ __global__ __launch_bounds__(P2P_BS) void k_p2p_mfma111(int nchunk, const int* _
 // keeps the differences of nearby points to ~1e-7 of the leaf size; 1/r
 // comes from v_rsq_f32.  The FP32 products of one staged source tile (at most
 // 128 sources) are summed into FP64 accumulators.  echk[e] selects how the
-// pairs of entry e take the r < 1e-8 exclusion (p2p_tile32).
+// pairs of entry e take the r < 1e-8 exclusion (p2p_tile32); entries flagged
+// 2 are skipped (an FP64 pass adds them).
 __global__ __launch_bounds__(P2P_BS) void k_p2p_mfma111_f32(int nchunk, const int* __restrict__ tbeg,
                                                             const int* __restrict__ tnum, const int* __restrict__ eptr,
                                                             const int2* __restrict__ ent,
@@ -864,8 +839,9 @@ __global__ __launch_bounds__(P2P_BS) void k_p2p_mfma111_f32(int nchunk, const in
   const double* den = src.den + (long)r0 * src.den_rs;
   const float* den32 = src.den32 ? src.den32 + (long)r0 * src.den_rs : nullptr;
   for (int e = eptr[c]; e < eptr[c + 1]; e++) {
-    const int2 se = ent[e];
     const int chk = echk[e];  // block-uniform
+    if (chk == 2) continue;   // the FP64 pass's
+    const int2 se = ent[e];
     for (int b0 = 0; b0 < se.y; b0 += TILE32) {
       const int cnt = min(TILE32, se.y - b0), cnt4 = (cnt + 3) & ~3;
       const int j0 = se.x + b0;
@@ -882,44 +858,24 @@ __global__ __launch_bounds__(P2P_BS) void k_p2p_mfma111_f32(int nchunk, const in
         s_den[jj][q] = jj < cnt && q < nrv ? (den32 ? den32[at] : (float)den[at]) : 0.0f;
       }
       __syncthreads();
-      if (chk == 2) {
-        // rare: FP64 coordinates of the lane's targets (clamped like tx) and of the tile's sources
-        double t64[4][3];
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-          int tt = 64 * w + 16 * i + col;
-          tt = tt < tn ? tt : 0;
-          t64[i][0] = trg.x[tb + tt];
-          t64[i][1] = trg.y[tb + tt];
-          t64[i][2] = trg.z[tb + tt];
-        }
-        const double *qx = src.x + j0, *qy = src.y + j0, *qz = src.z + j0;
+      if (chk == 1) {
         if (ng == 4)
-          p2p_tile32<4, 2>(cnt, cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc, qx, qy, qz, t64);
+          p2p_tile32<4, 1>(cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc);
         else if (ng == 3)
-          p2p_tile32<3, 2>(cnt, cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc, qx, qy, qz, t64);
+          p2p_tile32<3, 1>(cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc);
         else if (ng == 2)
-          p2p_tile32<2, 2>(cnt, cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc, qx, qy, qz, t64);
+          p2p_tile32<2, 1>(cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc);
         else if (ng == 1)
-          p2p_tile32<1, 2>(cnt, cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc, qx, qy, qz, t64);
-      } else if (chk == 1) {
-        if (ng == 4)
-          p2p_tile32<4, 1>(cnt, cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc);
-        else if (ng == 3)
-          p2p_tile32<3, 1>(cnt, cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc);
-        else if (ng == 2)
-          p2p_tile32<2, 1>(cnt, cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc);
-        else if (ng == 1)
-          p2p_tile32<1, 1>(cnt, cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc);
+          p2p_tile32<1, 1>(cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc);
       } else {
         if (ng == 4)
-          p2p_tile32<4, 0>(cnt, cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc);
+          p2p_tile32<4, 0>(cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc);
         else if (ng == 3)
-          p2p_tile32<3, 0>(cnt, cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc);
+          p2p_tile32<3, 0>(cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc);
         else if (ng == 2)
-          p2p_tile32<2, 0>(cnt, cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc);
+          p2p_tile32<2, 0>(cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc);
         else if (ng == 1)
-          p2p_tile32<1, 0>(cnt, cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc);
+          p2p_tile32<1, 0>(cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc);
       }
     }
   }
@@ -1215,8 +1171,8 @@ static void p2p_launch(hipStream_t s, int variant, int nchunk, int nb, const int
 }
 
 template <int KT>
-static void p2p_kt(hipStream_t s, int nchunk, const int* tbeg, const int* tnum, const int* eptr, const int2* ent,
-                   const int* echk, Points src, Targets trg, int nr, KC kc, int variant)
+static int p2p_kt(hipStream_t s, int nchunk, const int* tbeg, const int* tnum, const int* eptr, const int2* ent,
+                  const int* echk, Points src, Targets trg, int nr, KC kc, int variant)
 {
   int r0 = 0;
   if (KT == 111 && variant == 3 && echk) {
@@ -1234,19 +1190,22 @@ static void p2p_kt(hipStream_t s, int nchunk, const int* tbeg, const int* tnum, 
     else
       p2p_launch<KT, 1>(s, variant, nchunk, nb, tbeg, tnum, eptr, ent, echk, src, trg, r0 + roff, kc);
   });
+  // right-hand sides [0, n) went through a kernel that skips flagged entries
+  return variant == 4 && echk ? nr : r0;
 }
 
-void p2p(hipStream_t s, int kt, int nchunk, const int* tbeg, const int* tnum, const int* eptr, const int2* ent,
-         const int* echk, Points src, Targets trg, int nr, KC kc, int variant)
+int p2p(hipStream_t s, int kt, int nchunk, const int* tbeg, const int* tnum, const int* eptr, const int2* ent,
+        const int* echk, Points src, Targets trg, int nr, KC kc, int variant)
 {
-  if (nchunk <= 0) return;
+  if (nchunk <= 0) return 0;
   switch (kt) {
 #define KIFMM_P2P(KT) \
-  case KT: p2p_kt<KT>(s, nchunk, tbeg, tnum, eptr, ent, echk, src, trg, nr, kc, variant); break;
+  case KT: return p2p_kt<KT>(s, nchunk, tbeg, tnum, eptr, ent, echk, src, trg, nr, kc, variant);
   KIFMM_P2P(111) KIFMM_P2P(121) KIFMM_P2P(311) KIFMM_P2P(312) KIFMM_P2P(321) KIFMM_P2P(322) KIFMM_P2P(511)
   KIFMM_P2P(521)
 #undef KIFMM_P2P
   }
+  return 0;
 }
 
 // S2M / S2L for the Laplace kernel and 16-RHS blocks on the FP64 matrix
@@ -1382,13 +1341,13 @@ __global__ __launch_bounds__(P2P_BS) void k_p2s_mfma111_f32(int nitem, int npc, 
       }
       __syncthreads();
       if (ng == 4)
-        p2p_tile32<4, 0>(cnt, cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc);
+        p2p_tile32<4, 0>(cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc);
       else if (ng == 3)
-        p2p_tile32<3, 0>(cnt, cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc);
+        p2p_tile32<3, 0>(cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc);
       else if (ng == 2)
-        p2p_tile32<2, 0>(cnt, cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc);
+        p2p_tile32<2, 0>(cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc);
       else if (ng == 1)
-        p2p_tile32<1, 0>(cnt, cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc);
+        p2p_tile32<1, 0>(cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc);
     }
   }
   if (!wact) return;
@@ -1517,16 +1476,16 @@ __global__ __launch_bounds__(P2P_BS) void k_s2p_mfma111(int nchunk, const int* _
       __syncthreads();
       if constexpr (F32) {
         if (ng == 4)
-          p2p_tile32<4, 1>(cnt, cnt4, kk, col, (const float(*)[TILE32])s_pos, (const float(*)[16])s_den, (const float*)tx,
+          p2p_tile32<4, 1>(cnt4, kk, col, (const float(*)[TILE32])s_pos, (const float(*)[16])s_den, (const float*)tx,
                         (const float*)ty, (const float*)tz, acc64);
         else if (ng == 3)
-          p2p_tile32<3, 1>(cnt, cnt4, kk, col, (const float(*)[TILE32])s_pos, (const float(*)[16])s_den, (const float*)tx,
+          p2p_tile32<3, 1>(cnt4, kk, col, (const float(*)[TILE32])s_pos, (const float(*)[16])s_den, (const float*)tx,
                         (const float*)ty, (const float*)tz, acc64);
         else if (ng == 2)
-          p2p_tile32<2, 1>(cnt, cnt4, kk, col, (const float(*)[TILE32])s_pos, (const float(*)[16])s_den, (const float*)tx,
+          p2p_tile32<2, 1>(cnt4, kk, col, (const float(*)[TILE32])s_pos, (const float(*)[16])s_den, (const float*)tx,
                         (const float*)ty, (const float*)tz, acc64);
         else if (ng == 1)
-          p2p_tile32<1, 1>(cnt, cnt4, kk, col, (const float(*)[TILE32])s_pos, (const float(*)[16])s_den, (const float*)tx,
+          p2p_tile32<1, 1>(cnt4, kk, col, (const float(*)[TILE32])s_pos, (const float(*)[16])s_den, (const float*)tx,
                         (const float*)ty, (const float*)tz, acc64);
       } else {
         if (ng == 4)
@@ -3028,6 +2987,167 @@ void m2l_tiled(hipStream_t s, int C, int ntile, int Rc, const int* ttile, const 
   else
     hipLaunchKernelGGL(k_m2l_tile<double>, grid, dim3(M2LT_BS), 0, s, C, Rc, ntile, ttile, htile,
                        (const double2*)atab, (const double2*)effden, (double2*)effval, 0);
+}
+
+// ---------------------------------------------------------------------------
+// Tiled FP32 M2L, one wave per (tile, frequency): the products of
+// k_m2l_tile<float> (same A-operand table, lane layout and output rows) with
+// the work split the other way round.  A wave takes whole tiles at one
+// frequency f.  The block stages the A operands of all 26 directions at f
+// in LDS once, as the four values each lane feeds; a wave holds the
+// accumulators of all 8 target parents of its tile (4 per parent and lane)
+// and walks the 64 halo parents: each halo parent's children spectra are
+// loaded from memory once, straight into the B operands (D halo parents
+// ahead of the matrix cores), and applied to every target parent of the tile
+// that has the halo parent as a neighbour -- 1 to 8 of them, known at compile
+// time from the halo position -- with one 16-byte LDS read of A each.  One
+// block barrier, after the staging.  Tile slots travel in registers (lane
+// l holds entries q * 64 + l of the tile's halo table and entry l of its
+// target table) and are fetched with cross-lane reads.  Halo parents without
+// any source child are skipped; target parents without target children are
+// computed and not stored.  FP32 accumulation over the 26 directions, as in
+// k_m2l_tile<float>.
+//
+// Blocks (frequency f, group of 4 TPW tiles): the tile groups are split into
+// 8 contiguous ranges, one per XCD (block b runs on XCD b % 8), and each XCD
+// sweeps its range frequency by frequency, so the tiles that share a halo
+// spectrum line (slot, f) run side by side on one L2.
+constexpr int M2LW_BS = 256;
+constexpr int M2LW_TPW = 2;  // tiles per wave
+
+// Direction index (0 .. 25, dz fastest, centre skipped) in which halo parent
+// h (position (h / 16, h / 4 % 4, h % 4)) neighbours target parent p of the
+// tile (position (p / 4, p / 2 % 2, p % 2) + 1 in the halo), or -1.
+__host__ __device__ constexpr int m2lw_dir(int h, int p)
+{
+  const int dx = (h >> 4) - (p >> 2) - 1, dy = ((h >> 2) & 3) - ((p >> 1) & 1) - 1, dz = (h & 3) - (p & 1) - 1;
+  if (dx < -1 || dx > 1 || dy < -1 || dy > 1 || dz < -1 || dz > 1 || (dx == 0 && dy == 0 && dz == 0)) return -1;
+  const int m = (dx + 1) * 9 + (dy + 1) * 3 + (dz + 1);
+  return m - (m > 13);
+}
+
+// f(std::integral_constant<int, H>) for H = 0 .. 63 in order: every halo
+// position is a compile-time constant in the generated code
+template <class F, int... H>
+__device__ __forceinline__ void m2lw_each(std::integer_sequence<int, H...>, F&& f)
+{
+  (f(std::integral_constant<int, H>{}), ...);
+}
+
+template <int D>
+__global__ __launch_bounds__(M2LW_BS, 4) void k_m2l_wave(int C, int Rc, int ntile, int ngrp, int per_x,
+                                                        const int* __restrict__ ttile, const int* __restrict__ htile,
+                                                        const float2* __restrict__ atab,
+                                                        const float2* __restrict__ effden, float2* __restrict__ effval)
+{
+  // the A operands of frequency f, [direction][lane] as the four values each
+  // lane feeds: one 16-byte LDS read per (halo parent, target parent)
+  __shared__ float4 s_A4[26][64];
+  const int b = blockIdx.x, j = b >> 3;
+  const int f = j / per_x, g = (b & 7) * per_x + j % per_x;
+  if (f >= C || g >= ngrp) return;
+  const int r0 = blockIdx.y * 16;
+  const int t = threadIdx.x, w = t >> 6, l = t & 63;
+  const int i = l & 15, k = l >> 4;
+  // Lane (i, k) feeds row i of A, which holds output (child 2 kr + vr / 2,
+  // part vr % 2) (the FP32 accumulator item v of lane (i, k) is row 4k + v),
+  // in K slices 0 .. 3: with (m.x, m.y) the operator of source child k and
+  // (m.z, m.w) that of source child 4 + k, slice 0 meets the real part of
+  // child k's spectrum, 1 its imaginary part, 2 and 3 those of child 4 + k.
+  for (int e = t; e < 26 * 64; e += M2LW_BS) {
+    const int n = e >> 6, ln = e & 63, ii = ln & 15, kk = ln >> 4, kr = ii >> 2, vr = ii & 3;
+    const int ca = 2 * kr + (vr >> 1), pim = vr & 1;
+    const float4 m = *reinterpret_cast<const float4*>(atab + ((long)f * NAENT + n * 32 + kk * 8 + ca) * 2);
+    s_A4[n][ln] = pim ? make_float4(m.y, m.x, m.w, m.z) : make_float4(m.x, -m.y, m.z, -m.w);
+  }
+  __syncthreads();
+  const float2* eb = effden + r0 + i;
+  for (int tt = 0; tt < M2LW_TPW; tt++) {
+    const int tile = (g * 4 + w) * M2LW_TPW + tt;
+    if (tile >= ntile) break;  // wave-uniform
+    int hv[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) hv[q] = htile[(long)tile * 512 + q * 64 + l];
+    const int tv = ttile[(long)tile * 64 + l];
+    // halo parents with a source child: bit h (the entries 8h .. 8h+7 of
+    // halo parent h sit in lanes 8 (h % 8) .. 8 (h % 8) + 7 of hv[h / 8])
+    unsigned long long present = 0;
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      const unsigned long long bal = __ballot(hv[q] >= 0);
+#pragma unroll
+      for (int bb = 0; bb < 8; bb++)
+        if ((bal >> (8 * bb)) & 0xffull) present |= 1ull << (8 * q + bb);
+    }
+    // spectra of halo parent h, children k and 4 + k, for the lane's right-hand side
+    auto load = [&](int h, float2* x) {
+      const int s0 = __shfl(hv[h >> 3], 8 * (h & 7) + k), s1 = __shfl(hv[h >> 3], 8 * (h & 7) + 4 + k);
+      x[0] = make_float2(0.f, 0.f);
+      x[1] = make_float2(0.f, 0.f);
+      if (s0 >= 0) x[0] = eb[((long)s0 * C + f) * Rc];
+      if (s1 >= 0) x[1] = eb[((long)s1 * C + f) * Rc];
+    };
+    v4f acc[8];
+#pragma unroll
+    for (int p = 0; p < 8; p++) acc[p] = v4f{0.f, 0.f, 0.f, 0.f};
+    float2 ring[D][2];  // the spectra of the next D halo parents, in flight
+#pragma unroll
+    for (int d = 0; d < D; d++) load(d, ring[d]);
+    m2lw_each(std::make_integer_sequence<int, 64>{}, [&](auto hc) {
+      constexpr int h = decltype(hc)::value;
+      const float2 x0 = ring[h % D][0], x1 = ring[h % D][1];
+      if constexpr (h + D < 64) load(h + D, ring[h % D]);
+      if ((present >> h) & 1ull) {
+        // the target parents that have halo parent h as a neighbour; K
+        // slices in turn over them (independent accumulators between
+        // dependent products)
+        float4 av[8];
+#pragma unroll
+        for (int p = 0; p < 8; p++) {
+          const int n = m2lw_dir(h, p);
+          if (n >= 0) av[p] = s_A4[n][l];
+        }
+#pragma unroll
+        for (int ks = 0; ks < 4; ks++)
+#pragma unroll
+          for (int p = 0; p < 8; p++) {
+            if (m2lw_dir(h, p) < 0) continue;
+            const float xb = ks == 0 ? x0.x : ks == 1 ? x0.y : ks == 2 ? x1.x : x1.y;
+            const float ab = ks == 0 ? av[p].x : ks == 1 ? av[p].y : ks == 2 ? av[p].z : av[p].w;
+            acc[p] = __builtin_amdgcn_mfma_f32_16x16x4f32(ab, xb, acc[p], 0, 0, 0);
+          }
+      }
+      asm volatile("" ::: "memory");  // spectra loads D halo parents ahead, no further
+    });
+#pragma unroll
+    for (int p = 0; p < 8; p++)
+#pragma unroll
+      for (int h2 = 0; h2 < 2; h2++) {
+        const int sl = __shfl(tv, p * 8 + 2 * k + h2);
+        if (sl >= 0) effval[((long)sl * C + f) * Rc + r0 + i] = make_float2(acc[p][2 * h2], acc[p][2 * h2 + 1]);
+      }
+  }
+}
+
+void m2l_wave(hipStream_t s, int C, int ntile, int Rc, const int* ttile, const int* htile, const void* atab32,
+              const void* effden, void* effval, int depth)
+{
+  if (ntile <= 0 || Rc < 16) return;
+  const int ngrp = (ntile + 4 * M2LW_TPW - 1) / (4 * M2LW_TPW);
+  const int per_x = (ngrp + 7) / 8;
+  dim3 grid((unsigned)(8 * per_x * C), Rc / 16);
+#define KIFMM_M2LW(DD)                                                                                           \
+  hipLaunchKernelGGL(k_m2l_wave<DD>, grid, dim3(M2LW_BS), 0, s, C, Rc, ntile, ngrp, per_x, ttile, htile,        \
+                     (const float2*)atab32, (const float2*)effden, (float2*)effval)
+  if (depth <= 1)
+    KIFMM_M2LW(1);
+  else if (depth == 2)
+    KIFMM_M2LW(2);
+  else if (depth <= 4)
+    KIFMM_M2LW(4);
+  else
+    KIFMM_M2LW(6);
+#undef KIFMM_M2LW
 }
 
 // ---------------------------------------------------------------------------

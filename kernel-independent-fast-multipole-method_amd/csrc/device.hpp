@@ -63,11 +63,16 @@ struct Targets {  // leaf-ordered targets (SoA) and values [rhs][n*tdof]
 // variant 2: Laplace blocks of 16 right-hand sides on the FP64 matrix cores;
 // variant 3: the same on the FP32 matrix cores (mixed precision).  Variants
 // 2 and 3 fall back to variant 0 for other kernels and smaller blocks.
-// echk[e]: 1 where a pair of entry e can be closer than 1e-8 and takes the
-// exclusion explicitly (variant 3 needs it; without it variant 3 takes the
-// FP64 matrix-core kernel).
-void p2p(hipStream_t s, int kt, int nchunk, const int* tbeg, const int* tnum, const int* eptr, const int2* ent,
-         const int* echk, Points src, Targets trg, int nr, KC kc, int variant);
+// variant 4: the kernels without normals in FP32 (mixed precision).
+// echk[e] (variants 3 and 4; without it variant 3 takes the FP64 matrix-core
+// kernel): 0 no pair of entry e can be closer than 1e-8 or than its FP32
+// offsets resolve, 1 the target's own leaf (exclusion tested in FP32), 2 a
+// pair may be closer than the FP32 offsets resolve: the FP32 kernels skip such
+// entries.  Returns n: right-hand sides [0, n) skipped the entries flagged 2,
+// which a second call with variant 0 over those entries alone (and nr = n)
+// then adds in FP64.
+int p2p(hipStream_t s, int kt, int nchunk, const int* tbeg, const int* tnum, const int* eptr, const int2* ent,
+        const int* echk, Points src, Targets trg, int nr, KC kc, int variant);
 
 // FP64 direct sum of the user kernel over all nsrc sources at nchk points
 // (cx, cy, cz): out[r][k*tdof + t]; part holds nr * nseg * nchk * tdof doubles.
@@ -159,6 +164,12 @@ void fft_inv(hipStream_t s, int np, int nslot, const int* slot_node, int Rc, con
 // pairs outside the V list); single selects FP32 spectra and operators.
 void m2l_tiled(hipStream_t s, int C, int ntile, int Rc, const int* ttile, const int* htile, const void* atab,
                const void* effden, void* effval, int single);
+// The FP32 products of m2l_tiled with one wave per (tile, frequency): the A
+// operands of all directions in registers, halo spectra loaded straight into
+// the matrix-core operands (atab32: the FP32 table), depth halo parents
+// ahead (1, 2, 4 or 6).
+void m2l_wave(hipStream_t s, int C, int ntile, int Rc, const int* ttile, const int* htile, const void* atab32,
+              const void* effden, void* effval, int depth);
 
 }  // namespace dev
 }  // namespace kifmm

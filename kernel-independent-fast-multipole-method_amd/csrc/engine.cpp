@@ -104,7 +104,9 @@ Engine::Engine(int kt, const double* coefs, int ncoefs, int np, int ptsmax, int 
     const std::string v(m);
     m2l_tile_ = v != "group" && v != "pair" && v != "matrix";
     m2l_matrix_ = v != "group" && v != "pair";
+    m2l_wave_ = v == "wave";
   }
+  if (const char* d = std::getenv("KIFMM3D_M2L_DEPTH")) m2l_depth_ = std::atoi(d);
   if (const char* gph = std::getenv("KIFMM3D_GRAPH")) use_graph_ = std::atoi(gph) != 0;
   if (const char* ov = std::getenv("KIFMM3D_OVERLAP")) overlap_ = std::atoi(ov) != 0;
   if (const char* v = std::getenv("KIFMM3D_S2M32")) s2m32_ = std::atoi(v) != 0;
@@ -472,6 +474,7 @@ void Engine::build_plan()
   // near field, S2L items and L2T/M2T chunks, per target leaf
   {
     std::vector<int> tbeg, tnum, eptr(1, 0), ent, echk;
+    std::vector<int> xtbeg, xtnum, xfeptr(1, 0), xfent;  // flagged near-field entries
     std::vector<int> ftbeg, ftnum, feptr(1, 0), fnode, fkind;
     std::vector<double> fgeom;
     std::vector<int> xout, xeptr(1, 0), xent;
@@ -529,12 +532,23 @@ void Engine::build_plan()
       }
       P.near_pairs += npair * T.textnum[g];
       P.far_pairs += fpair * T.textnum[g];
+      bool flagged = false;
+      for (int v : dchk) flagged = flagged || v == 2;
       for (int off = 0; off < T.textnum[g]; off += 128) {
         int n = std::min(128, T.textnum[g] - off);
         tbeg.push_back(T.textbeg[g] + off); tnum.push_back(n);
         ent.insert(ent.end(), dent.begin(), dent.end());
         echk.insert(echk.end(), dchk.begin(), dchk.end());
         eptr.push_back((int)ent.size() / 2);
+        if (flagged) {  // the flagged entries again, for the FP64 pass of the mixed form
+          xtbeg.push_back(T.textbeg[g] + off); xtnum.push_back(n);
+          for (size_t e = 0; e < dchk.size(); e++)
+            if (dchk[e] == 2) {
+              xfent.push_back(dent[2 * e]);
+              xfent.push_back(dent[2 * e + 1]);
+            }
+          xfeptr.push_back((int)xfent.size() / 2);
+        }
         ftbeg.push_back(T.textbeg[g] + off); ftnum.push_back(n);
         fnode.insert(fnode.end(), sent.begin(), sent.end());
         fkind.insert(fkind.end(), sknd.begin(), sknd.end());
@@ -545,6 +559,9 @@ void Engine::build_plan()
     P.near.n = (int)tbeg.size();
     P.near.tbeg = pool_i(P.I, tbeg); P.near.tnum = pool_i(P.I, tnum);
     P.near.eptr = pool_i(P.I, eptr); P.near.ent = pool_i(P.I, ent); P.near.chk = pool_i(P.I, echk);
+    P.near_fix.n = (int)xtbeg.size();
+    P.near_fix.tbeg = pool_i(P.I, xtbeg); P.near_fix.tnum = pool_i(P.I, xtnum);
+    P.near_fix.eptr = pool_i(P.I, xfeptr); P.near_fix.ent = pool_i(P.I, xfent);
     P.far.n = (int)ftbeg.size();
     P.far.tbeg = pool_i(P.I, ftbeg); P.far.tnum = pool_i(P.I, ftnum);
     P.far.eptr = pool_i(P.I, feptr); P.far.enode = pool_i(P.I, fnode);
@@ -1136,7 +1153,7 @@ int Engine::near_variant(bool mixed) const
   // the mixed form: FP32 matrix cores for Laplace, FP32 vector kernels for the
   // kernels without normals (with normals, r.n of nearly tangent pairs cancels
   // beyond what FP32 coordinates resolve: FP64 there)
-  const int v = t >= 0 ? t : mixed ? (user_.kt == 111 ? 3 : user_.needs_normal ? 0 : 4) : p2p_variant_;
+  const int v = t >= 0 ? t : mixed ? (user_.kt == 111 ? 3 : near32_ok() ? 4 : 0) : p2p_variant_;
   // the matrix-core near-field kernels are Laplace's; the other kernel types
   // run kernels 2 and 3 as the LDS-broadcast form 0
   return user_.kt != 111 && (v == 2 || v == 3) ? 0 : v;
@@ -1152,8 +1169,9 @@ int Engine::m2l_form(bool mixed, size_t level, int rc) const
   if (plan_.m2l_pairs) return 3;
   const bool tile_ok = mm_.sdof == 1 && mm_.tdof == 1 && rc % 16 == 0;
   const std::vector<int>& tf = m2l_form_[mixed ? 1 : 0];
-  if (level < tf.size() && tf[level] >= 0 && (tf[level] != 0 || tile_ok)) return tf[level];
-  return m2l_tile_ && tile_ok ? 0 : m2l_matrix_ ? 1 : 2;
+  if (level < tf.size() && tf[level] >= 0 && ((tf[level] != 0 && tf[level] != 4) || tile_ok))
+    return tf[level] == 4 && !mixed ? 0 : tf[level];  // form 4 is FP32: the mixed form's
+  return m2l_tile_ && tile_ok ? (m2l_wave_ && mixed ? 4 : 0) : m2l_matrix_ ? 1 : 2;
 }
 
 int Engine::tune(int nrhs, int mode)
@@ -1174,9 +1192,12 @@ int Engine::tune(int nrhs, int mode)
   if (user_.kt == 111)
     nf = mixed ? std::vector<int>{3} : std::vector<int>{2, 0, 1};
   else
-    nf = mixed && !user_.needs_normal ? std::vector<int>{4, 0, 1} : std::vector<int>{0, 1};
+    nf = mixed && near32_ok() ? std::vector<int>{4, 0, 1} : std::vector<int>{0, 1};
   if (!plan_.m2l_pairs) {
-    if (mm_.sdof == 1 && mm_.tdof == 1 && rc % 16 == 0) mf.push_back(0);
+    if (mm_.sdof == 1 && mm_.tdof == 1 && rc % 16 == 0) {
+      mf.push_back(0);
+      if (mixed) mf.push_back(4);
+    }
     mf.push_back(1);
     mf.push_back(2);
   }
@@ -1238,13 +1259,13 @@ void Engine::set_settings(int mode, const int* in, int n)
   const int v = in[0];
   // kernel 3 (FP32 matrix cores) is the mixed form's Laplace near field
   // only, kernel 4 (FP32 vector) the mixed form's for kernels without normals
-  if (v < -1 || v > 4 || (v == 3 && !(mixed && user_.kt == 111)) || (v == 4 && !(mixed && !user_.needs_normal)))
+  if (v < -1 || v > 4 || (v == 3 && !(mixed && user_.kt == 111)) || (v == 4 && !(mixed && near32_ok())))
     throw Error(KIFMM3D_EINVAL, "settings: near-field kernel " + std::to_string(v) + " not available here");
   const size_t nlev = plan_.vlev.size();
   std::vector<int> forms(nlev, -1);
   for (size_t l = 0; l < nlev && (long)l + 1 < n; l++) {
     const int f = in[l + 1];
-    if (f < -1 || f > 2 || (f == 0 && !(mm_.sdof == 1 && mm_.tdof == 1)))
+    if (f < -1 || f > 4 || f == 3 || ((f == 0 || f == 4) && !(mm_.sdof == 1 && mm_.tdof == 1)))
       throw Error(KIFMM3D_EINVAL, "settings: M2L form " + std::to_string(f) + " not available here");
     forms[l] = f;
   }
@@ -1285,7 +1306,12 @@ void Engine::run_chunk(const double* srcden, double* trgval, int rc, int mode, h
   // reads the FP64 copy
   float* den32 = mixed && user_.kt == 111 ? sden32_.p : nullptr;
   src.den32 = den32;
-  double* den64 = den32 && s2m32_ && near_variant(mixed) == 3 && (rc % 16 == 0 || rc % 16 >= dev::PAD16_MIN)
+  const int nvar = near_variant(mixed);
+  // the FP32 near fields skip the entries flagged for FP64 exclusion tests
+  // (set-up: a pair may be closer than the FP32 offsets resolve); the FP64
+  // LDS-broadcast kernel adds those entries after them
+  const bool near_fix = mixed && (nvar == 3 || nvar == 4) && P.near_fix.n > 0;
+  double* den64 = den32 && s2m32_ && nvar == 3 && (rc % 16 == 0 || rc % 16 >= dev::PAD16_MIN) && !near_fix
                       ? nullptr
                       : sden_.p;
   dev::Targets trg{tx_.p, ty_.p, tz_.p, tval_.p, trs};
@@ -1368,8 +1394,11 @@ void Engine::run_chunk(const double* srcden, double* trgval, int rc, int mode, h
     HIPCHK(hipStreamWaitEvent(side_, fork_ev_, 0));
   }
   timed_on(side ? side_ : s, KIFMM3D_K_P2P, [&] {
-    dev::p2p(side ? side_ : s, user_.kt, P.near.n, I + P.near.tbeg, I + P.near.tnum, I + P.near.eptr,
-             (const int2*)(I + P.near.ent), I + P.near.chk, src, trg, rc, kcu, near_variant(mixed));
+    const int n32 = dev::p2p(side ? side_ : s, user_.kt, P.near.n, I + P.near.tbeg, I + P.near.tnum,
+                             I + P.near.eptr, (const int2*)(I + P.near.ent), I + P.near.chk, src, trg, rc, kcu, nvar);
+    if (near_fix && n32 > 0)
+      dev::p2p(side ? side_ : s, user_.kt, P.near_fix.n, I + P.near_fix.tbeg, I + P.near_fix.tnum,
+               I + P.near_fix.eptr, (const int2*)(I + P.near_fix.ent), nullptr, src, trg, n32, kcu, 0);
   });
   if (side) HIPCHK(hipEventRecord(join_ev_, side_));
   mark(3);
@@ -1379,15 +1408,17 @@ void Engine::run_chunk(const double* srcden, double* trgval, int rc, int mode, h
   for (size_t li = 0; li < P.vlev.size(); li++) {
     const VLevel& vl = P.vlev[li];
     const int form = m2l_form(mixed, li, rc);
-    const bool tiled = form == 0;
-    const int ri = tiled ? 1 : 0, single = mixed && form <= 1 ? 1 : 0;
+    const bool tiled = form == 0 || form == 4;
+    const int ri = tiled ? 1 : 0, single = mixed && (form <= 1 || form == 4) ? 1 : 0;
     cur_lvl = (int)li;
     timed(KIFMM3D_K_FFT_FWD, [&] {
       dev::fft_fwd(s, np_, vl.nsrc, I + vl.src_node, rc, upeden_.p, pUE, sm, D + vl.dscale, c2s_.p, effden_.p, single,
                    ri, (const double2*)tw_.p);
     });
     timed(KIFMM3D_K_M2L, [&] {
-      if (tiled)
+      if (form == 4)
+        dev::m2l_wave(s, O.C, vl.ntile, rc, I + vl.ttile, I + vl.htile, atab32_.p, effden_.p, effval_.p, m2l_depth_);
+      else if (tiled)
         dev::m2l_tiled(s, O.C, vl.ntile, rc, I + vl.ttile, I + vl.htile,
                        single ? (const void*)atab32_.p : (const void*)atab_.p, effden_.p, effval_.p, single);
       else if (form == 3)

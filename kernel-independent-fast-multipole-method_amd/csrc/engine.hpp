@@ -76,6 +76,9 @@ struct VLevel {
 struct Plan {
   ItemList s2m, s2l;
   ChunkList near, far;
+  // the near-field chunks that hold entries flagged 2 (chk), with only those
+  // entries: the FP32 near fields skip them and this FP64 pass adds them
+  ChunkList near_fix;
   std::vector<GemmOp> up, down;
   // Mixed form's up pass: an inner node's equivalent density comes straight
   // from its children's through UC2UE x UE2UC_c (one product instead of M2M
@@ -132,7 +135,8 @@ class Engine {
   // 1 scalar-cache sources, 2 FP64 matrix cores, 3 FP32 matrix cores, 4 FP32
   // LDS broadcast),
   // [1 + l] M2L form of V-list level l (0 tiled matrix-core, 1 parent-grouped
-  // matrix-core, 2 grouped VALU, 3 per pair).  Returns the count (1 + levels).
+  // matrix-core, 2 grouped VALU, 3 per pair, 4 tiled with one wave per tile
+  // and frequency).  Returns the count (1 + levels).
   int settings(int mode, int* out, int n) const;
   // Pins settings for a mode (same layout; -1 entries keep the default), e.g.
   // ones a previous process tuned.  Rejects kernels and forms the problem
@@ -189,6 +193,12 @@ class Engine {
   void run_chunk(const double* srcden, double* trgval, int rc, int mode, hipStream_t s, hipEvent_t* ev);
   void drop_graph();
   int near_variant(bool mixed) const;
+  // The FP32 vector near field (kernel 4) is the mixed form's for the 1/r
+  // Green's functions without normals, Stokes (311) and Navier (511) single
+  // layer.  The Stokes pressure (312, r.f / r^3) and the kernels with normals
+  // keep FP64: their sums cancel more (a 4,000-point cluster set measured
+  // 1.6e-6 from the plain form with the pressure in FP32).
+  bool near32_ok() const { return user_.kt == 311 || user_.kt == 511; }
   int m2l_form(bool mixed, size_t level, int rc) const;
 
   KernelInfo user_, mm_;
@@ -212,6 +222,8 @@ class Engine {
   int gemm_mfma_ = 2;  // uncompensated GEMMs: 2 wide matrix-core tiles, 1 64x64 tiles (KIFMM3D_GEMM=tile64), 0 VALU (=valu)
   bool m2l_tile_ = true;   // tiled matrix-core M2L for scalar kernels and 16-RHS chunks (KIFMM3D_M2L=matrix|group|pair disables)
   bool m2l_matrix_ = true; // parent-grouped matrix-core M2L otherwise (KIFMM3D_M2L=group|pair disables)
+  bool m2l_wave_ = false;  // the tiled M2L with one wave per (tile, frequency) (KIFMM3D_M2L=wave)
+  int m2l_depth_ = 4;       // ... halo parents its spectra loads run ahead (KIFMM3D_M2L_DEPTH)
   hipStream_t stream_ = nullptr;
   hipStream_t side_ = nullptr;  // near field, concurrent with the far-field chain
   hipEvent_t fork_ev_ = nullptr, join_ev_ = nullptr;

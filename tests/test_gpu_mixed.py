@@ -84,16 +84,19 @@ def test_mixed_vector_kernels_one_rhs():
     run(511, 8, 80, c, None, (0.5, 0.5, 0.5), 1, 1, 25)
 
 
-@pytest.mark.parametrize("kt,nrhs", [(111, 16), (511, 4)])
+@pytest.mark.parametrize("kt,nrhs", [(111, 16), (111, 18), (511, 4), (311, 3)])
 def test_mixed_near_coincident_pairs_across_leaves(kt, nrhs):
     """Pairs of points 1e-10 to 2e-9 apart on either side of splitting planes
     (multiples of 1/8, faces of every box of depth <= 3): the reference drops
     such pairs (r < 1e-8, kernel3d.cpp:21, :132) whichever leaves hold them.
     The FP32 offsets of the mixed near field cannot resolve these distances
     (1/8 of the unit cube is ~1e-8 in FP32 ulps), so the set-up flags the leaf
-    pairs where this can happen and those pairs take FP64 displacements.
-    Without that, each pair would add ~1/(4 pi 1e-8) to both targets and the
-    mixed form would differ from the plain form by orders of magnitude."""
+    pairs where this can happen: the FP32 kernels skip them and an FP64 pass
+    adds them (18 right-hand sides: 16 on the FP32 matrix cores, whose flagged
+    entries the FP64 pass adds, and 2 on the FP64 kernel, which takes every
+    entry itself).  Without that, each pair would add ~1/(4 pi 1e-8) to both
+    targets and the mixed form would differ from the plain form by orders of
+    magnitude."""
     p = uniform_cube(60000, 26)
     rng = np.random.default_rng(27)
     k = 48
@@ -106,3 +109,42 @@ def test_mixed_near_coincident_pairs_across_leaves(kt, nrhs):
     b[np.arange(k), axis] = planes + gap / 2
     pts = np.concatenate([p, a, b])
     run(kt, 4, 150, pts, None, (0.5, 0.5, 0.5), 1, nrhs, 26)
+
+
+@pytest.mark.parametrize("depth", [1, 4])
+@pytest.mark.parametrize("geom,nrhs", [("cube", 16), ("clusters", 32)])
+def test_wave_tiled_m2l(geom, nrhs, depth, monkeypatch):
+    """The tiled FP32 M2L with one wave per (tile, frequency) (form 4,
+    spectra loads 1 or 4 halo parents ahead) computes the products of the
+    block-per-tile form (form 0) in another FP32 summation order: the two
+    agree to FP32 rounding on every level, and both stay within the mixed
+    form's bound of the plain form.  The clustered tree has tiles with absent
+    halo parents and target parents without children; 32 right-hand sides
+    take two 16-wide groups."""
+    monkeypatch.setenv("KIFMM3D_M2L", "wave")
+    monkeypatch.setenv("KIFMM3D_M2L_DEPTH", str(depth))
+    p = uniform_cube(20000, 41) if geom == "cube" else clusters(15000, 42)
+    print(f"\n[case] wave M2L {geom} n={len(p)} nrhs={nrhs} depth={depth}", flush=True)
+    f = K.FMM3d("fmm3d_")
+    f.srcpos, f.srcnor, f.trgpos = p, None, p
+    f.ctr, f.rootlvl = (0.5, 0.5, 0.5), 1
+    f.knl = K.Kernel3d(111, COEFS[111])
+    f.setup({"-fmm3d_np": "6", "-fmm3d_let3d_ptsmax": "60", "-fmm3d_let3d_maxlevel": "10"})
+    den = densities(len(p), 1, nrhs, 43)
+    plain = f.eval(den, nrhs=nrhs).reshape(nrhs, -1).copy()
+    nlev = len(f.settings(K.MODE_MIXED)["m2l"])
+    assert f.settings(K.MODE_MIXED)["m2l"] == [4] * nlev  # the switch makes form 4 the default
+    wave = f.eval(den, nrhs=nrhs, mode=K.MODE_MIXED).reshape(nrhs, -1).copy()
+    f.set_settings(K.MODE_MIXED, 3, [0] * nlev)
+    tile = f.eval(den, nrhs=nrhs, mode=K.MODE_MIXED).reshape(nrhs, -1).copy()
+    e_wt = max(relerr(wave[r], tile[r]) for r in range(nrhs))
+    e_wp = max(relerr(wave[r], plain[r]) for r in range(nrhs))
+    e_tp = max(relerr(tile[r], plain[r]) for r in range(nrhs))
+    print(f"  levels {nlev}: wave-vs-tiled {e_wt:.3e}; vs plain: wave {e_wp:.3e}, tiled {e_tp:.3e}", flush=True)
+    assert np.all(np.isfinite(wave))
+    assert e_wt <= 5e-7
+    assert e_wp <= MIXED_VS_PLAIN
+    # pinned per level, mixed with the other forms
+    f.set_settings(K.MODE_MIXED, 3, [4 if l % 2 else 1 for l in range(nlev)])
+    mix = f.eval(den, nrhs=nrhs, mode=K.MODE_MIXED).reshape(nrhs, -1)
+    assert max(relerr(mix[r], plain[r]) for r in range(nrhs)) <= MIXED_VS_PLAIN
