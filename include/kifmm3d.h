@@ -170,7 +170,9 @@ int kifmm3d_direct_device(kifmm3d* h, const double* d_srcden, int nrhs, const in
  * [3] S2M pairs, [4] S2L pairs, [5] up-pass GEMM flops, [6] down-pass GEMM
  * flops, [7] forward FFTs, [8] inverse FFTs, [9] complex spectrum length C,
  * [10] up-pass GEMM flops of the mixed form (inner nodes by the fused
- * UC2UE x UE2UC product). */
+ * UC2UE x UE2UC product), [11] near-field entries the mixed form's FP32 near
+ * field leaves to its FP64 pass (closest pair below 1/128 of the FP32 offsets'
+ * span or below 1e-7; set by the device set-up, 0 before it). */
 int kifmm3d_plan_stats(const kifmm3d* h, double* out, int n);
 
 const char* kifmm3d_last_error(const kifmm3d* h);

@@ -194,10 +194,10 @@ int kifmm3d_plan_stats(const kifmm3d* h, double* out, int n)
   if (!h || !out) return KIFMM3D_EINVAL;
   if (!h->eng->host_ready()) return KIFMM3D_ESTATE;
   const kifmm::Plan& P = h->eng->plan();
-  double v[11] = {(double)P.near_pairs, (double)P.far_pairs, (double)P.v_pairs, (double)P.s2m_pairs,
+  double v[12] = {(double)P.near_pairs, (double)P.far_pairs, (double)P.v_pairs, (double)P.s2m_pairs,
                   (double)P.s2l_pairs, P.up_flops, P.down_flops, (double)P.fft_fwd, (double)P.fft_inv,
-                  (double)h->eng->ops().C, P.up_flops_mixed};
-  for (int i = 0; i < n && i < 11; i++) out[i] = v[i];
+                  (double)h->eng->ops().C, P.up_flops_mixed, (double)h->eng->near_flagged()};
+  for (int i = 0; i < n && i < 12; i++) out[i] = v[i];
   return KIFMM3D_OK;
 }
 

@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cfloat>
 #include <mutex>
 #include <type_traits>
 #include <utility>
@@ -918,6 +919,99 @@ static void rhs_blocks(int nr, F&& fn)
     fn(NR, nb, r);
     r += nb * NR;
   }
+}
+
+// ---------------------------------------------------------------------------
+// Set-up pass of the mixed near field: for every near-field entry e of every
+// chunk c (the chunk's targets against one source leaf), in FP64, the
+// smallest distance between a target and a source that are not the same
+// point (out[e].x, squared; DBL_MAX when there is none) and the largest
+// distance of any of the entry's sources from the chunk's first target, the
+// origin of the FP32 offsets (out[e].y, squared), and the same for the
+// chunk's targets (tout[c], squared).  Block per chunk, one target per
+// thread, sources staged in LDS, block-wide min / max.
+constexpr int NMIN_TILE = 128;
+__global__ __launch_bounds__(P2P_BS) void k_near_extent(const int* __restrict__ tbeg, const int* __restrict__ tnum,
+                                                        const int* __restrict__ eptr, const int2* __restrict__ ent,
+                                                        Points src, Targets trg, double2* __restrict__ out,
+                                                        double* __restrict__ tout)
+{
+  __shared__ double s_p[3][NMIN_TILE];
+  __shared__ double s_red[2][P2P_BS / 64];
+  const int c = blockIdx.x, t = threadIdx.x, w = t >> 6, l = t & 63;
+  const int tb = tbeg[c], tn = tnum[c];
+  const bool act = t < tn;
+  const double ox = trg.x[tb], oy = trg.y[tb], oz = trg.z[tb];
+  double x = ox, y = oy, z = oz;
+  if (act) {
+    x = trg.x[tb + t];
+    y = trg.y[tb + t];
+    z = trg.z[tb + t];
+  }
+  auto wave_min = [](double v) {
+    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o));
+    return v;
+  };
+  auto wave_max = [](double v) {
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+    return v;
+  };
+  {
+    const double dx = x - ox, dy = y - oy, dz = z - oz;
+    const double m = wave_max(dx * dx + dy * dy + dz * dz);
+    if (l == 0) s_red[1][w] = m;
+    __syncthreads();
+    if (t == 0) {
+      double mm = 0.0;
+      for (int q = 0; q < P2P_BS / 64; q++) mm = fmax(mm, s_red[1][q]);
+      tout[c] = mm;
+    }
+  }
+  for (int e = eptr[c]; e < eptr[c + 1]; e++) {
+    const int2 se = ent[e];
+    double rmin = DBL_MAX, omax = 0.0;
+    for (int b0 = 0; b0 < se.y; b0 += NMIN_TILE) {
+      const int cnt = min(NMIN_TILE, se.y - b0), j0 = se.x + b0;
+      __syncthreads();
+      for (int i = t; i < cnt; i += P2P_BS) {
+        s_p[0][i] = src.x[j0 + i];
+        s_p[1][i] = src.y[j0 + i];
+        s_p[2][i] = src.z[j0 + i];
+        const double dx = s_p[0][i] - ox, dy = s_p[1][i] - oy, dz = s_p[2][i] - oz;
+        omax = fmax(omax, dx * dx + dy * dy + dz * dz);
+      }
+      __syncthreads();
+      if (act)
+        for (int jj = 0; jj < cnt; jj++) {
+          const double dx = x - s_p[0][jj], dy = y - s_p[1][jj], dz = z - s_p[2][jj];
+          const double r2 = dx * dx + dy * dy + dz * dz;
+          if (r2 > 0.0) rmin = fmin(rmin, r2);
+        }
+    }
+    rmin = wave_min(rmin);
+    omax = wave_max(omax);
+    __syncthreads();  // the previous entry's reduction has been read
+    if (l == 0) {
+      s_red[0][w] = rmin;
+      s_red[1][w] = omax;
+    }
+    __syncthreads();
+    if (t == 0) {
+      double a = DBL_MAX, m = 0.0;
+      for (int q = 0; q < P2P_BS / 64; q++) {
+        a = fmin(a, s_red[0][q]);
+        m = fmax(m, s_red[1][q]);
+      }
+      out[e] = make_double2(a, m);
+    }
+  }
+}
+
+void near_extent(hipStream_t s, int nchunk, const int* tbeg, const int* tnum, const int* eptr, const int2* ent,
+                 Points src, Targets trg, double2* out, double* tout)
+{
+  if (nchunk <= 0) return;
+  hipLaunchKernelGGL(k_near_extent, dim3(nchunk), dim3(P2P_BS), 0, s, tbeg, tnum, eptr, ent, src, trg, out, tout);
 }
 
 // ---------------------------------------------------------------------------

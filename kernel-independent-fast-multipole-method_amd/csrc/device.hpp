@@ -74,6 +74,14 @@ struct Targets {  // leaf-ordered targets (SoA) and values [rhs][n*tdof]
 int p2p(hipStream_t s, int kt, int nchunk, const int* tbeg, const int* tnum, const int* eptr, const int2* ent,
         const int* echk, Points src, Targets trg, int nr, KC kc, int variant);
 
+// Set-up pass for the mixed near field, per entry e of the near-field chunks:
+// out[e].x = the smallest squared distance between a chunk target and an
+// entry source that are distinct points (DBL_MAX if none), out[e].y = the
+// largest squared distance of an entry source from the chunk's first target;
+// tout[c] = the largest squared distance of chunk c's targets from it.
+void near_extent(hipStream_t s, int nchunk, const int* tbeg, const int* tnum, const int* eptr, const int2* ent,
+                 Points src, Targets trg, double2* out, double* tout);
+
 // FP64 direct sum of the user kernel over all nsrc sources at nchk points
 // (cx, cy, cz): out[r][k*tdof + t]; part holds nr * nseg * nchk * tdof doubles.
 void direct(hipStream_t s, int kt, int tdof, int nchk, const double* cx, const double* cy, const double* cz,

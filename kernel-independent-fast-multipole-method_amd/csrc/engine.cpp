@@ -159,7 +159,6 @@ void Engine::setup(const double* spos, const double* snor, int ns, const double*
   ns_ = ns;
   nt_ = nt;
   tpos_host_.assign(tpos, tpos + 3 * (size_t)nt);
-  mark_edge_leaves(spos, tpos);
   lap("tree");
   build_plan();
   lap("plan");
@@ -204,75 +203,86 @@ void Engine::setup(const double* spos, const double* snor, int ns, const double*
     tiperm_.upload(inv);
   }
   upload_static();
+  near_flags();
   HIPCHK(hipDeviceSynchronize());
   lap("upload");
   gen_++;
   setup_done_ = true;
 }
 
-// Which near-field entries can hold a pair closer than the mixed form's FP32
-// near field resolves -- offsets from the chunk's first target, which carry
-// an error of ~2^-24 of a leaf's width -- or closer than the reference's
-// r < 1e-8 exclusion (kernel3d.cpp:21, :132):
-// * a pair of points of two different leaves closer than d has both points
-//   within d of their own leaf's box boundary (the segment between them
-//   leaves the box), so a cross-leaf entry needs that only when both leaves
-//   hold such a point (sedge_, tedge_);
-// * inside one leaf, equal points have equal FP32 offsets and drop out; two
-//   distinct points closer than d are looked for directly (close_).
-// d = max(4e-8, 1e-6 x the widest leaf): every pair of an entry that is not
-// flagged is at least d apart, more than the FP32 offsets can lose, so its
-// r^2 is positive and above the exclusion radius.
-constexpr double EDGE_TOL = 4e-8;
-
-void Engine::mark_edge_leaves(const double* spos, const double* tpos)
+// Which near-field entries the mixed form's FP32 near fields must leave to
+// the FP64 pass.  Their displacements come from FP32 offsets relative to the
+// chunk's first target, each rounded to 2^-24 of at most span (the largest
+// distance of the entry's points from that origin), so a pair at distance r
+// gets 1/r with a relative error of up to ~2 sqrt(3) 2^-24 span / r.  An
+// entry whose closest pair of distinct points (FP64, measured on the device
+// at set-up) is nearer than span / 128 -- a relative error above 2.6e-5 for
+// that pair, which dominates its target -- or nearer than 1e-7, so that the
+// reference's r < 1e-8 exclusion (kernel3d.cpp:21, :132) cannot be decided
+// from the offsets, is flagged 2: the FP32 kernels skip it and the FP64
+// kernel adds it (near_fix).  Coincident points, which fall in one leaf,
+// give equal offsets and drop out of the own-leaf entry (flag 1) exactly.
+// Uniform clouds flag almost nothing; tight clusters flag the entries of
+// their densest leaves.
+void Engine::near_flags()
 {
-  const Tree& T = tree_;
-  sedge_.assign(T.nnode, 0);
-  tedge_.assign(T.nnode, 0);
-  close_.assign(T.nnode, 0);
-  double wmax = 0.0;
-  for (int g = 0; g < T.nnode; g++)
-    if (T.terminal(g)) wmax = std::max(wmax, 2.0 * T.radius(g));
-  const double tol = std::max(EDGE_TOL, 1e-6 * wmax);
-  auto near_edge = [&](int g, const double* p) {
-    double c[3];
-    T.center(g, c);
-    const double r = T.radius(g);
-    for (int d = 0; d < 3; d++)
-      if (p[d] - (c[d] - r) < tol || (c[d] + r) - p[d] < tol) return true;
-    return false;
-  };
-  std::vector<std::pair<double, int>> xs;
-  for (int g = 0; g < T.nnode; g++) {
-    if (!T.terminal(g)) continue;
-    if (T.tag[g] & NODE_SRC)
-      for (int k = T.sextbeg[g]; k < T.sextbeg[g] + T.sextnum[g] && !sedge_[g]; k++)
-        sedge_[g] = near_edge(g, spos + 3 * (size_t)T.sperm[k]);
-    if (T.tag[g] & NODE_TRG)
-      for (int k = T.textbeg[g]; k < T.textbeg[g] + T.textnum[g] && !tedge_[g]; k++)
-        tedge_[g] = near_edge(g, tpos + 3 * (size_t)T.tperm[k]);
-    if ((T.tag[g] & NODE_SRC) && (T.tag[g] & NODE_TRG)) {
-      // sources sorted by x; each target scans the sources within EDGE_TOL in x
-      xs.clear();
-      for (int k = T.sextbeg[g]; k < T.sextbeg[g] + T.sextnum[g]; k++)
-        xs.emplace_back(spos[3 * (size_t)T.sperm[k]], T.sperm[k]);
-      std::sort(xs.begin(), xs.end());
-      for (int k = T.textbeg[g]; k < T.textbeg[g] + T.textnum[g] && !close_[g]; k++) {
-        const double* q = tpos + 3 * (size_t)T.tperm[k];
-        auto it = std::lower_bound(xs.begin(), xs.end(), std::make_pair(q[0] - tol, -1));
-        for (; it != xs.end() && it->first <= q[0] + tol; ++it) {
-          const double* p = spos + 3 * (size_t)it->second;
-          const double dx = q[0] - p[0], dy = q[1] - p[1], dz = q[2] - p[2];
-          const double r2 = dx * dx + dy * dy + dz * dz;
-          if (r2 > 0.0 && r2 < tol * tol) {
-            close_[g] = 1;
-            break;
-          }
+  Plan& P = plan_;
+  P.near_fix = ChunkList{};
+  fix_.reset();
+  nflag_ = 0;
+  const bool fp32_near = user_.kt == 111 || near32_ok();
+  if (!fp32_near || P.near.n == 0) return;
+  std::vector<int> eptr(P.near.n + 1), tbeg(P.near.n), tnum(P.near.n);
+  HIPCHK(hipMemcpy(eptr.data(), I_.p + P.near.eptr, sizeof(int) * eptr.size(), hipMemcpyDeviceToHost));
+  const int nent = eptr[P.near.n];
+  std::copy(P.I.begin() + P.near.tbeg, P.I.begin() + P.near.tbeg + P.near.n, tbeg.begin());
+  std::copy(P.I.begin() + P.near.tnum, P.I.begin() + P.near.tnum + P.near.n, tnum.begin());
+  DVec<double> ext, text;
+  ext.alloc((size_t)2 * std::max(nent, 1));
+  text.alloc((size_t)P.near.n);
+  dev::Points src{sx_.p, sy_.p, sz_.p, snx_.p, sny_.p, snz_.p, nullptr, 0, 0};
+  dev::Targets trg{tx_.p, ty_.p, tz_.p, nullptr, 0};
+  dev::near_extent(stream_, P.near.n, I_.p + P.near.tbeg, I_.p + P.near.tnum, I_.p + P.near.eptr,
+                   (const int2*)(I_.p + P.near.ent), src, trg, (double2*)ext.p, text.p);
+  HIPCHK(hipGetLastError());
+  std::vector<double> e2((size_t)2 * nent), t2(P.near.n);
+  HIPCHK(hipStreamSynchronize(stream_));
+  if (nent) HIPCHK(hipMemcpy(e2.data(), ext.p, sizeof(double) * e2.size(), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(t2.data(), text.p, sizeof(double) * t2.size(), hipMemcpyDeviceToHost));
+  std::vector<int> chk(P.I.begin() + P.near.chk, P.I.begin() + P.near.chk + nent);
+  const int* ent = P.I.data() + P.near.ent;
+  std::vector<int> F, ftb, ftn, fep(1, 0), fent;
+  for (int c = 0; c < P.near.n; c++) {
+    bool any = false;
+    for (int e = eptr[c]; e < eptr[c + 1]; e++) {
+      const double span2 = std::max(t2[c], e2[2 * (size_t)e + 1]);
+      const double rmin2 = e2[2 * (size_t)e];
+      if (rmin2 < 1e-14 || rmin2 < span2 * (1.0 / 16384.0)) {
+        chk[e] = 2;
+        nflag_++;
+        if (!any) {
+          ftb.push_back(tbeg[c]);
+          ftn.push_back(tnum[c]);
+          any = true;
         }
+        fent.push_back(ent[2 * e]);
+        fent.push_back(ent[2 * e + 1]);
       }
     }
+    if (any) fep.push_back((int)fent.size() / 2);
   }
+  if (std::getenv("KIFMM3D_TRACE"))
+    std::fprintf(stderr, "[kifmm3d] near field: %ld of %d entries left to the FP64 pass of the mixed form\n", nflag_,
+                 nent);
+  if (nent) HIPCHK(hipMemcpy(I_.p + P.near.chk, chk.data(), sizeof(int) * nent, hipMemcpyHostToDevice));
+  std::copy(chk.begin(), chk.end(), P.I.begin() + P.near.chk);
+  if (ftb.empty()) return;
+  P.near_fix.n = (int)ftb.size();
+  P.near_fix.tbeg = pool_i(F, ftb);
+  P.near_fix.tnum = pool_i(F, ftn);
+  P.near_fix.eptr = pool_i(F, fep);
+  P.near_fix.ent = pool_i(F, fent);
+  fix_.upload(F);
 }
 
 void Engine::upload_static()
@@ -474,7 +484,6 @@ void Engine::build_plan()
   // near field, S2L items and L2T/M2T chunks, per target leaf
   {
     std::vector<int> tbeg, tnum, eptr(1, 0), ent, echk;
-    std::vector<int> xtbeg, xtnum, xfeptr(1, 0), xfent;  // flagged near-field entries
     std::vector<int> ftbeg, ftnum, feptr(1, 0), fnode, fkind;
     std::vector<double> fgeom;
     std::vector<int> xout, xeptr(1, 0), xent;
@@ -501,10 +510,10 @@ void Engine::build_plan()
       long npair = 0, fpair = 0;
       auto U = list(LIST_U, g);
       for (auto it = U.first; it != U.second; ++it) {
-        // 2: the pairs' exclusion decided in FP64 (cross-leaf entries whose
-        // leaves both hold points next to their boundaries, the own leaf when
-        // it holds distinct near-coincident points); 1: own leaf otherwise
-        dchk.push_back(*it == g ? (close_[g] ? 2 : 1) : (sedge_[*it] && tedge_[g]) ? 2 : 0);
+        // 1: the target's own leaf (coincident points drop out by the
+        // exclusion test); entries are flagged 2 at the device set-up
+        // (near_flags) where the FP32 offsets cannot resolve a pair
+        dchk.push_back(*it == g ? 1 : 0);
         dent.push_back(T.sextbeg[*it]); dent.push_back(T.sextnum[*it]);
         npair += T.sextnum[*it];
       }
@@ -532,23 +541,12 @@ void Engine::build_plan()
       }
       P.near_pairs += npair * T.textnum[g];
       P.far_pairs += fpair * T.textnum[g];
-      bool flagged = false;
-      for (int v : dchk) flagged = flagged || v == 2;
       for (int off = 0; off < T.textnum[g]; off += 128) {
         int n = std::min(128, T.textnum[g] - off);
         tbeg.push_back(T.textbeg[g] + off); tnum.push_back(n);
         ent.insert(ent.end(), dent.begin(), dent.end());
         echk.insert(echk.end(), dchk.begin(), dchk.end());
         eptr.push_back((int)ent.size() / 2);
-        if (flagged) {  // the flagged entries again, for the FP64 pass of the mixed form
-          xtbeg.push_back(T.textbeg[g] + off); xtnum.push_back(n);
-          for (size_t e = 0; e < dchk.size(); e++)
-            if (dchk[e] == 2) {
-              xfent.push_back(dent[2 * e]);
-              xfent.push_back(dent[2 * e + 1]);
-            }
-          xfeptr.push_back((int)xfent.size() / 2);
-        }
         ftbeg.push_back(T.textbeg[g] + off); ftnum.push_back(n);
         fnode.insert(fnode.end(), sent.begin(), sent.end());
         fkind.insert(fkind.end(), sknd.begin(), sknd.end());
@@ -559,9 +557,6 @@ void Engine::build_plan()
     P.near.n = (int)tbeg.size();
     P.near.tbeg = pool_i(P.I, tbeg); P.near.tnum = pool_i(P.I, tnum);
     P.near.eptr = pool_i(P.I, eptr); P.near.ent = pool_i(P.I, ent); P.near.chk = pool_i(P.I, echk);
-    P.near_fix.n = (int)xtbeg.size();
-    P.near_fix.tbeg = pool_i(P.I, xtbeg); P.near_fix.tnum = pool_i(P.I, xtnum);
-    P.near_fix.eptr = pool_i(P.I, xfeptr); P.near_fix.ent = pool_i(P.I, xfent);
     P.far.n = (int)ftbeg.size();
     P.far.tbeg = pool_i(P.I, ftbeg); P.far.tnum = pool_i(P.I, ftnum);
     P.far.eptr = pool_i(P.I, feptr); P.far.enode = pool_i(P.I, fnode);
@@ -1396,9 +1391,11 @@ void Engine::run_chunk(const double* srcden, double* trgval, int rc, int mode, h
   timed_on(side ? side_ : s, KIFMM3D_K_P2P, [&] {
     const int n32 = dev::p2p(side ? side_ : s, user_.kt, P.near.n, I + P.near.tbeg, I + P.near.tnum,
                              I + P.near.eptr, (const int2*)(I + P.near.ent), I + P.near.chk, src, trg, rc, kcu, nvar);
-    if (near_fix && n32 > 0)
-      dev::p2p(side ? side_ : s, user_.kt, P.near_fix.n, I + P.near_fix.tbeg, I + P.near_fix.tnum,
-               I + P.near_fix.eptr, (const int2*)(I + P.near_fix.ent), nullptr, src, trg, n32, kcu, 0);
+    if (near_fix && n32 > 0) {
+      const int* F = fix_.p;
+      dev::p2p(side ? side_ : s, user_.kt, P.near_fix.n, F + P.near_fix.tbeg, F + P.near_fix.tnum,
+               F + P.near_fix.eptr, (const int2*)(F + P.near_fix.ent), nullptr, src, trg, n32, kcu, 0);
+    }
   });
   if (side) HIPCHK(hipEventRecord(join_ev_, side_));
   mark(3);

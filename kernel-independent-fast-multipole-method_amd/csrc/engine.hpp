@@ -77,7 +77,8 @@ struct Plan {
   ItemList s2m, s2l;
   ChunkList near, far;
   // the near-field chunks that hold entries flagged 2 (chk), with only those
-  // entries: the FP32 near fields skip them and this FP64 pass adds them
+  // entries (offsets into the engine's fix_ pool, set at the device set-up):
+  // the FP32 near fields skip them and this FP64 pass adds them
   ChunkList near_fix;
   std::vector<GemmOp> up, down;
   // Mixed form's up pass: an inner node's equivalent density comes straight
@@ -178,9 +179,10 @@ class Engine {
   // vectors; out is host memory [nrhs][nchk * tdof].
   void direct(const double* srcden, bool on_device, int nrhs, const int* idx, int nchk, double* out);
   const Plan& plan() const { return plan_; }
+  long near_flagged() const { return nflag_; }
 
  private:
-  void mark_edge_leaves(const double* spos, const double* tpos);
+  void near_flags();
   void build_plan();
   void upload_static();
   int choose_rhs_chunk(int nrhs);
@@ -268,10 +270,8 @@ class Engine {
   DVec<double> chk_, part_, dout_;  // direct-sum scratch
   DVec<double> gwork_;              // split-K partials of the GEMMs
   std::vector<double> tpos_host_;  // caller-order target positions (for direct sums)
-  // per node: a source (target) point of the leaf lies within EDGE_TOL of the
-  // leaf's box boundary, so it can be closer than 1e-8 to a point of an
-  // adjacent leaf; the leaf holds a distinct source-target pair that close
-  std::vector<char> sedge_, tedge_, close_;
+  DVec<int> fix_;   // index pool of plan_.near_fix (entries flagged 2 at set-up)
+  long nflag_ = 0;  // near-field entries flagged 2
 };
 
 // Several set-up engines evaluated in one captured graph: problem i runs on

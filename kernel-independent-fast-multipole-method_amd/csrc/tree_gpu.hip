@@ -14,14 +14,20 @@
 //    the order of the host's stable partitions.
 // 4. Targets take steps 1 and 3 against the source tree; per-node target
 //    counts come from the sorted leaf ids.
+// 5. The U/V/W/X lists (Let3d::calgnext, let3d.cpp:203-282): one thread per
+//    target node counts, then writes, its entries (k_lists), and a segmented
+//    radix sort puts every node's entries in ascending node order -- the
+//    arrays of Tree::build_lists.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cstdint>
+#include <cstdlib>
 #include <string>
 #include <vector>
 
 #include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_segmented_radix_sort.hpp>
 
 #include "common.hpp"
 #include "tree.hpp"
@@ -151,6 +157,124 @@ void leaf_order(const DBuf<unsigned long long>& key, int n, const DBuf<int>& dch
   TCHK(hipStreamSynchronize(s));
 }
 
+// ---------------------------------------------------------------------------
+// U/V/W/X lists on the device (Let3d::calgnext, let3d.cpp:203-282; the host
+// form is Tree::build_lists).
+struct NodeView {
+  const int *par, *chd, *depth, *path, *tag;
+  int nnode;
+};
+
+// findgnt (let3d.cpp:325-337): the deepest existing node containing cell
+// (x, y, z) of depth d -- every split creates all eight children, so the
+// walk down from the root by the cell's octant bits finds it.
+__device__ inline int findgnt_dev(const int* __restrict__ chd, int d, int x, int y, int z)
+{
+  int node = 0;
+  for (int l = 0; l < d; l++) {
+    const int c = chd[node];
+    if (c < 0) break;
+    const int s = d - l - 1;
+    node = c + ((x >> s) & 1) * 4 + ((y >> s) & 1) * 2 + ((z >> s) & 1);
+  }
+  return node;
+}
+
+// adjacent (let3d.cpp:339-352): the boxes touch (faces, edges or corners)
+__device__ inline bool adjacent_dev(const NodeView& T, int me, int yo)
+{
+  const int md = max(T.depth[me], T.depth[yo]);
+  const int merad = 1 << (md - T.depth[me]), yorad = 1 << (md - T.depth[yo]);
+  const int r = merad + yorad;
+  int linf = 0;
+  bool ok = true;
+  for (int d = 0; d < 3; d++) {
+    const int dif = abs((2 * T.path[3 * me + d] + 1) * merad - (2 * T.path[3 * yo + d] + 1) * yorad);
+    if (dif > r) ok = false;
+    linf = max(linf, dif);
+  }
+  return ok && linf == r;
+}
+
+constexpr int LIST_STACK = 8 * 17;  // depth-first walk below a same-depth neighbour (7 per level + 1)
+
+// One thread per node.  WRITE = false: cnt[l * nnode + g] = entries of list l
+// of node g; WRITE = true: the entries at out[l] + ptr[l][g], unsorted.  The
+// 6 x 6 x 6 cells at node g's depth around its parent's children are visited
+// as the host does; a coarser node that several cells reach is taken only
+// from the first such cell in that order (the corner of the cells it covers
+// inside the window), so no list holds an entry twice; a node without
+// sources is never entered (tag filter of the host lists).
+template <bool WRITE>
+__global__ __launch_bounds__(256) void k_lists(NodeView T, int* __restrict__ cnt, const int* __restrict__ ptr,
+                                               int* __restrict__ out0, int* __restrict__ out1, int* __restrict__ out2,
+                                               int* __restrict__ out3, int* __restrict__ err)
+{
+  const int g = blockIdx.x * 256 + threadIdx.x;
+  if (g >= T.nnode) return;
+  int n[4] = {0, 0, 0, 0};
+  int* out[4] = {out0, out1, out2, out3};
+  auto emit = [&](int l, int node) {
+    if (!(T.tag[node] & NODE_SRC)) return;
+    if (WRITE) out[l][ptr[l * (T.nnode + 1) + g] + n[l]] = node;
+    n[l]++;
+  };
+  if (T.tag[g] & NODE_TRG) {
+    const bool leaf = T.chd[g] < 0;
+    if (T.par[g] >= 0) {
+      const int pg = T.par[g], dg = T.depth[g], maxidx = 1 << dg;
+      const int p0 = 2 * T.path[3 * pg], p1 = 2 * T.path[3 * pg + 1], p2 = 2 * T.path[3 * pg + 2];
+      const int w0 = max(p0 - 2, 0), w1 = max(p1 - 2, 0), w2 = max(p2 - 2, 0);  // first cell in the window
+      const int g0 = T.path[3 * g], g1 = T.path[3 * g + 1], g2 = T.path[3 * g + 2];
+      for (int i = -2; i < 4; i++)
+        for (int j = -2; j < 4; j++)
+          for (int k = -2; k < 4; k++) {
+            const int t0 = p0 + i, t1 = p1 + j, t2 = p2 + k;
+            if (t0 < 0 || t1 < 0 || t2 < 0 || t0 >= maxidx || t1 >= maxidx || t2 >= maxidx) continue;
+            if (t0 == g0 && t1 == g1 && t2 == g2) continue;
+            const int res = findgnt_dev(T.chd, dg, t0, t1, t2);
+            const int dr = T.depth[res];
+            const bool adj = adjacent_dev(T, res, g);
+            if (dr < dg) {
+              const int sh = dg - dr;
+              const bool first = t0 == max(w0, T.path[3 * res] << sh) && t1 == max(w1, T.path[3 * res + 1] << sh) &&
+                                 t2 == max(w2, T.path[3 * res + 2] << sh);
+              if (!first) continue;
+              if (adj) {
+                if (leaf) emit(LIST_U, res);
+              } else {
+                emit(LIST_X, res);
+              }
+            } else if (!adj) {
+              emit(LIST_V, res);
+            } else if (leaf) {
+              // the descendants of an adjacent same-depth node: W where not
+              // adjacent, U at adjacent leaves (let3d.cpp:251-266)
+              int stk[LIST_STACK];
+              int sp = 0;
+              stk[sp++] = res;
+              while (sp > 0) {
+                const int f = stk[--sp];
+                if (!adjacent_dev(T, f, g)) {
+                  emit(LIST_W, f);
+                } else if (T.chd[f] < 0) {
+                  emit(LIST_U, f);
+                } else if (sp + 8 <= LIST_STACK) {
+                  for (int c = 7; c >= 0; c--) stk[sp++] = T.chd[f] + c;
+                } else {
+                  atomicOr(err, 1);  // deeper than any tree of maxlevel <= 16
+                  sp = 0;
+                }
+              }
+            }
+          }
+    }
+    if (leaf) emit(LIST_U, g);
+  }
+  if (!WRITE)
+    for (int l = 0; l < 4; l++) cnt[l * T.nnode + g] = n[l];
+}
+
 struct Stream {
   hipStream_t s = nullptr;
   Stream() { TCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking)); }
@@ -158,6 +282,64 @@ struct Stream {
     if (s) (void)hipStreamDestroy(s);
   }
 };
+
+// Lists of every node on the device, into the host CSR arrays.
+void lists_device(Tree& t, const DBuf<int>& dchd, hipStream_t s)
+{
+  const int nn = t.nnode;
+  DBuf<int> dpar(nn), ddep(nn), dpath((size_t)3 * nn), dtag(nn), cnt((size_t)4 * nn), err(1);
+  TCHK(hipMemcpyAsync(dpar.p, t.par.data(), sizeof(int) * nn, hipMemcpyHostToDevice, s));
+  TCHK(hipMemcpyAsync(ddep.p, t.depth.data(), sizeof(int) * nn, hipMemcpyHostToDevice, s));
+  TCHK(hipMemcpyAsync(dpath.p, t.path.data(), sizeof(int) * 3 * (size_t)nn, hipMemcpyHostToDevice, s));
+  TCHK(hipMemcpyAsync(dtag.p, t.tag.data(), sizeof(int) * nn, hipMemcpyHostToDevice, s));
+  TCHK(hipMemsetAsync(err.p, 0, sizeof(int), s));
+  const NodeView T{dpar.p, dchd.p, ddep.p, dpath.p, dtag.p, nn};
+  hipLaunchKernelGGL(k_lists<false>, dim3(grid(nn)), dim3(256), 0, s, T, cnt.p, nullptr, nullptr, nullptr, nullptr,
+                     nullptr, err.p);
+  TCHK(hipGetLastError());
+  std::vector<int> c((size_t)4 * nn);
+  TCHK(hipMemcpyAsync(c.data(), cnt.p, sizeof(int) * c.size(), hipMemcpyDeviceToHost, s));
+  TCHK(hipStreamSynchronize(s));
+  // CSR pointers per list (host scan), [4][nnode + 1] on the device
+  std::vector<int> ptr((size_t)4 * (nn + 1));
+  for (int l = 0; l < 4; l++) {
+    int* p = ptr.data() + (size_t)l * (nn + 1);
+    p[0] = 0;
+    for (int g = 0; g < nn; g++) p[g + 1] = p[g] + c[(size_t)l * nn + g];
+  }
+  DBuf<int> dptr(ptr.size());
+  TCHK(hipMemcpyAsync(dptr.p, ptr.data(), sizeof(int) * ptr.size(), hipMemcpyHostToDevice, s));
+  DBuf<int> raw[4], sorted[4];
+  for (int l = 0; l < 4; l++) {
+    const int tot = ptr[(size_t)l * (nn + 1) + nn];
+    raw[l].alloc(tot);
+    sorted[l].alloc(tot);
+  }
+  hipLaunchKernelGGL(k_lists<true>, dim3(grid(nn)), dim3(256), 0, s, T, nullptr, dptr.p, raw[0].p, raw[1].p,
+                     raw[2].p, raw[3].p, err.p);
+  TCHK(hipGetLastError());
+  // ascending node order inside every node's entries
+  int bits = 1;
+  while ((1LL << bits) < nn) bits++;
+  for (int l = 0; l < 4; l++) {
+    t.lptr[l].assign(ptr.begin() + (size_t)l * (nn + 1), ptr.begin() + (size_t)(l + 1) * (nn + 1));
+    const int tot = t.lptr[l][nn];
+    t.lidx[l].resize(tot);
+    if (tot == 0) continue;
+    const int* seg = dptr.p + (size_t)l * (nn + 1);
+    size_t bytes = 0;
+    TCHK(rocprim::segmented_radix_sort_keys(nullptr, bytes, raw[l].p, sorted[l].p, (unsigned)tot, (unsigned)nn, seg,
+                                            seg + 1, 0, bits, s));
+    DBuf<char> tmp(bytes);
+    TCHK(rocprim::segmented_radix_sort_keys((void*)tmp.p, bytes, raw[l].p, sorted[l].p, (unsigned)tot,
+                                            (unsigned)nn, seg, seg + 1, 0, bits, s));
+    TCHK(hipMemcpyAsync(t.lidx[l].data(), sorted[l].p, sizeof(int) * (size_t)tot, hipMemcpyDeviceToHost, s));
+  }
+  int e = 0;
+  TCHK(hipMemcpyAsync(&e, err.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  TCHK(hipStreamSynchronize(s));
+  if (e) throw Error(KIFMM3D_EINVAL, "internal: list construction exceeded its stack");
+}
 
 }  // namespace
 
@@ -227,7 +409,11 @@ void Tree::build_device(const double* spos, int ns, const double* tpos, int nt)
     for (int g = nnode - 1; g > 0; g--) tcnt[par[g]] += tcnt[g];
   }
   index_targets(tcnt);
-  build_lists();
+  const char* lb = std::getenv("KIFMM3D_LISTS");
+  if (lb && std::string(lb) == "host")
+    build_lists();
+  else
+    lists_device(*this, dchd, s);
 }
 
 }  // namespace kifmm

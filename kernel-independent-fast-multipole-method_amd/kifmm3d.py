@@ -368,7 +368,7 @@ class FMM3d:
 
     def plan_stats(self) -> dict:
         keys = ["near_pairs", "far_pairs", "v_pairs", "s2m_pairs", "s2l_pairs", "up_flops", "down_flops",
-                "fft_fwd", "fft_inv", "C", "up_flops_mixed"]
+                "fft_fwd", "fft_inv", "C", "up_flops_mixed", "near_flagged"]
         a = np.zeros(len(keys))
         self._check(lib().kifmm3d_plan_stats(self.h, a.ctypes.data_as(_D), len(keys)))
         return dict(zip(keys, a.tolist()))

@@ -7,8 +7,9 @@ seeds 0x4B1F0000 + workload number, same kernels, np, leaf sizes and boxes):
   form against the oracle on every target (<= 1e-12), the mixed form within
   1e-6 of it, both inside the np 4 band against the direct sum.
 * Navier single layer, 1,048,576 points in 32 Gaussian clusters, np 8,
-  ptsmax 150: the plain form against the oracle on every target (<= 1e-12),
-  the mixed form within 1e-6 of the plain form.
+  ptsmax 150: the plain form against the oracle on every target (<= 2e-12,
+  the oracle's own summation-order floor on this workload; max-abs / max
+  <= 1e-12), the mixed form within 1e-6 of the plain form.
 * Laplace single layer, 4,194,304 uniform points x 16 right-hand sides, np 6,
   ptsmax 256 (the leaf size the benchmark's search picks): mixed within 1e-6
   of plain on every value, both against the GPU direct sum on 1000 targets x
@@ -17,8 +18,8 @@ seeds 0x4B1F0000 + workload number, same kernels, np, leaf sizes and boxes):
 * The 64-problem batch (8 right-hand sides each) in the mixed form in one
   graph: every problem's batch output bitwise equal to its own eager
   evaluate, each problem inside its np band against the direct sum, and the
-  worst Stokes np 4 problem traced to the algorithm (the oracle shows the
-  same error).
+  worst Stokes problem of each np traced to the algorithm (the oracle shows
+  the same error).
 
 The error variant of the reference's check (FMM-norm denominator,
 fmm3d/fmm3d_check.cpp:61-69) is printed beside the direct-sum-norm one.
@@ -36,6 +37,13 @@ from problems import relerr
 pytestmark = pytest.mark.gpu
 
 PARITY = 1e-12
+# Navier clusters, 1,048,576 points, np 8, 9 levels: the oracle itself moves by
+# 1.1e-12 to 1.7e-12 when any one of its summation orders (kernel sums, dense
+# mat-vecs, V-list accumulation, DFT sums) is reversed (measured on this
+# workload; tests/test_oracle_numerics.py::test_rounding_floor_navier_clusters_1m
+# measures the kernel-sum one), so the reference's result is defined only to
+# that level here and the bar is set at the floor
+PARITY_DEEP = 2e-12
 MIXED_VS_PLAIN = 1e-6
 FMM_BAND = {4: 1e-2, 6: 2e-4, 8: 2e-5}
 
@@ -104,7 +112,8 @@ def test_navier_clusters_1m_np8():
     mixed = f.eval(den, mode=K.MODE_MIXED).copy()
     w = f.list_csr(2)[1].size
     x = f.list_csr(3)[1].size
-    print(f"  GPU set-up + evaluates {time.time() - t0:.1f} s; W entries {w}, X entries {x}", flush=True)
+    print(f"  GPU set-up + evaluates {time.time() - t0:.1f} s; W entries {w}, X entries {x}; near-field entries "
+          f"left to the mixed form's FP64 pass {int(f.plan_stats()['near_flagged'])}", flush=True)
     assert w > 0 and x > 0, "the clustered workload must exercise the W and X lists"
     em = relerr(mixed, plain)
     idx, d = check_vs_direct(f, den, 1, c["n"], 3, c["nchk"], seed)
@@ -120,7 +129,7 @@ def test_navier_clusters_1m_np8():
     emax = float(np.max(np.abs(plain - ref)) / np.max(np.abs(ref)))
     print(f"  plain vs oracle on every target {e:.3e} (max-abs/max {emax:.3e}); oracle {time.time() - t0:.1f} s",
           flush=True)
-    assert e <= PARITY and emax <= PARITY
+    assert e <= PARITY_DEEP and emax <= PARITY
     assert em <= MIXED_VS_PLAIN
     assert ep <= FMM_BAND[8] and ex <= FMM_BAND[8]
 
@@ -199,26 +208,31 @@ def test_batch_64_problems_mixed():
         e, _ = errors(v, d)
         key = (pr["kt"], pr["np"])
         worst[key] = max(worst.get(key, (0.0, -1)), (e, i))
-        # Stokes double layer on small spheres at np 4 reaches ~2e-2 (algorithmic: see below)
-        band = 5e-2 if key == (321, 4) else FMM_BAND[pr["np"]]
+        # the Stokes double layer on small spheres reaches ~2e-2 at np 4 and
+        # ~3e-4 at np 6 (the algorithm's error on that geometry: see below)
+        band = {(321, 4): 5e-2, (321, 6): 1e-3}.get(key, FMM_BAND[pr["np"]])
         assert e <= band, f"problem {i} {key}: error {e:.3e} against the direct sum"
     print("  worst error vs direct per (kernel, np): " +
           ", ".join(f"{k}: {v[0]:.3e} (problem {v[1]})" for k, v in sorted(worst.items())), flush=True)
-    # the worst Stokes np 4 problem: the plain GPU form equals the oracle, and
-    # the oracle's own error against the direct sum is the same, so the error
-    # is that of the algorithm on this geometry, not of this implementation
-    e_s, i = worst[(321, 4)]
-    pr, f = probs[i], fmms[i]
-    plain = f.eval(dens[i][0]).copy()
-    o = O.OracleFMM(321, pr["coefs"], pr["np"], 100, 10, pr["ctr"], pr["rootlvl"])
-    o.setup(pr["pos"], pr["nor"], pr["pos"], pinv=(f.operator(0), f.operator(1)))
-    ref = o.eval(dens[i][0])
-    idx = np.sort(np.random.default_rng(pr["seed"] + 2).choice(pr["n"], 200, replace=False)).astype(np.int32)
-    d = f.direct(dens[i][:1], idx, 1)[0]
-    eo, _ = errors(ref.reshape(-1, 3)[idx].ravel(), d)
-    eg, _ = errors(plain.reshape(-1, 3)[idx].ravel(), d)
-    print(f"  problem {i} (Stokes DL, n={pr['n']}, np 4): plain vs oracle {relerr(plain, ref):.3e}; "
-          f"vs direct, first RHS: oracle {eo:.3e}, GPU plain {eg:.3e}", flush=True)
-    assert relerr(plain, ref) <= PARITY
-    assert abs(eo - eg) <= 1e-6 * eo
+    # the worst Stokes problem of each np: the plain GPU form equals the
+    # oracle, and the oracle's own error against the direct sum is the same,
+    # so the error is that of the algorithm on this geometry, not of this
+    # implementation
+    for np_ in (4, 6):
+        if (321, np_) not in worst:
+            continue
+        e_s, i = worst[(321, np_)]
+        pr, f = probs[i], fmms[i]
+        plain = f.eval(dens[i][0]).copy()
+        o = O.OracleFMM(321, pr["coefs"], pr["np"], 100, 10, pr["ctr"], pr["rootlvl"])
+        o.setup(pr["pos"], pr["nor"], pr["pos"], pinv=(f.operator(0), f.operator(1)))
+        ref = o.eval(dens[i][0])
+        idx = np.sort(np.random.default_rng(pr["seed"] + 2).choice(pr["n"], 200, replace=False)).astype(np.int32)
+        d = f.direct(dens[i][:1], idx, 1)[0]
+        eo, _ = errors(ref.reshape(-1, 3)[idx].ravel(), d)
+        eg, _ = errors(plain.reshape(-1, 3)[idx].ravel(), d)
+        print(f"  problem {i} (Stokes DL, n={pr['n']}, np {np_}): plain vs oracle {relerr(plain, ref):.3e}; "
+              f"vs direct, first RHS: oracle {eo:.3e}, GPU plain {eg:.3e}", flush=True)
+        assert relerr(plain, ref) <= PARITY
+        assert abs(eo - eg) <= 1e-6 * eo
     b.destroy()

@@ -49,6 +49,7 @@ def run(kt, np_, ptsmax, spos, snor, ctr, rootlvl, nrhs, seed):
     print(f"  mixed-vs-plain rel L2 {worst:.3e}   mixed fmm-vs-direct {ferr:.3e}", flush=True)
     assert worst <= MIXED_VS_PLAIN
     assert ferr <= FMM_BAND[np_]
+    return f
 
 
 def test_mixed_laplace_16rhs_np6():
@@ -90,13 +91,13 @@ def test_mixed_near_coincident_pairs_across_leaves(kt, nrhs):
     (multiples of 1/8, faces of every box of depth <= 3): the reference drops
     such pairs (r < 1e-8, kernel3d.cpp:21, :132) whichever leaves hold them.
     The FP32 offsets of the mixed near field cannot resolve these distances
-    (1/8 of the unit cube is ~1e-8 in FP32 ulps), so the set-up flags the leaf
-    pairs where this can happen: the FP32 kernels skip them and an FP64 pass
-    adds them (18 right-hand sides: 16 on the FP32 matrix cores, whose flagged
-    entries the FP64 pass adds, and 2 on the FP64 kernel, which takes every
-    entry itself).  Without that, each pair would add ~1/(4 pi 1e-8) to both
-    targets and the mixed form would differ from the plain form by orders of
-    magnitude."""
+    (1/8 of the unit cube is ~1e-8 in FP32 ulps), so the device set-up flags
+    the near-field entries whose closest pair is that close (measured in
+    FP64): the FP32 kernels skip them and an FP64 pass adds them (18
+    right-hand sides: 16 on the FP32 matrix cores, whose flagged entries the
+    FP64 pass adds, and 2 on the FP64 kernel, which takes every entry itself).
+    Without that, each pair would add ~1/(4 pi 1e-8) to both targets and the
+    mixed form would differ from the plain form by orders of magnitude."""
     p = uniform_cube(60000, 26)
     rng = np.random.default_rng(27)
     k = 48
@@ -108,7 +109,9 @@ def test_mixed_near_coincident_pairs_across_leaves(kt, nrhs):
     a[np.arange(k), axis] = planes - gap / 2
     b[np.arange(k), axis] = planes + gap / 2
     pts = np.concatenate([p, a, b])
-    run(kt, 4, 150, pts, None, (0.5, 0.5, 0.5), 1, nrhs, 26)
+    f = run(kt, 4, 150, pts, None, (0.5, 0.5, 0.5), 1, nrhs, 26)
+    # every pair straddles two leaves, each sees the other: flagged entries
+    assert f.plan_stats()["near_flagged"] >= k // 2
 
 
 @pytest.mark.parametrize("depth", [1, 4])

@@ -1,5 +1,6 @@
-"""Octree construction on the GPU (octant keys, radix sorts, walk to the leaves)
-against the host construction, which the CPU suite pins against the oracle's
+"""Octree and list construction on the GPU (octant keys, radix sorts, walk to
+the leaves; U/V/W/X lists by one thread per node and a segmented sort) against
+the host construction, which the CPU suite pins against the oracle's
 restatement of Let3d (fmm3d/let3d.cpp:56-282).  Every exported tree array --
 node table, U/V/W/X lists, source and target leaf permutations -- must be
 identical, including on points that lie exactly on splitting planes or on the
@@ -107,3 +108,18 @@ def test_out_of_box_error_matches(monkeypatch):
         msgs.append((e.value.code, str(e.value)))
     assert msgs[0] == msgs[2] and msgs[1] == msgs[3], msgs
     assert "1234" in msgs[0][1] and "source" in msgs[0][1] and "target" in msgs[1][1]
+
+
+def test_deep_refinement_next_to_big_leaves(monkeypatch):
+    """A tight cluster refined to depth 12 beside a sparse region of large
+    leaves: long W lists (descendants of a large leaf's adjacent neighbours),
+    X lists of the small boxes, and coarser U entries reached from many cells
+    of the window, each of which must appear once."""
+    rng = np.random.default_rng(78)
+    sparse = rng.random((300, 3))
+    tight = 0.52 + 1e-3 * rng.standard_normal((20000, 3))
+    p = np.concatenate([sparse, np.clip(tight, 0.0, 1.0)])
+    d = same_tree(monkeypatch, "cluster beside large leaves", p, p, (0.5, 0.5, 0.5), 1, 8, 14)
+    lens = [np.diff(d.list_csr(l)[0]) for l in range(4)]
+    print("  longest lists U/V/W/X: " + " ".join(str(int(x.max())) for x in lens), flush=True)
+    assert lens[2].max() > 8 and lens[3].max() > 0

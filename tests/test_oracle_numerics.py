@@ -209,3 +209,34 @@ def test_np_error_floor_on_the_4m_uniform_workload():
     # (first density vector: 2.2e-6; the 16 vectors of the workload together: ~1.3e-6)
     assert 1e-6 < err[6] < 5e-6
     assert err[8] < 1e-7
+
+
+@pytest.mark.slow
+def test_rounding_floor_navier_clusters_1m():
+    """The clustered Navier workload (1,048,576 points in 32 Gaussian
+    clusters, np 8, ptsmax 150, 9 levels, seed 0x4B1F0003) through the oracle
+    twice: with the kernel sums over sources in the reference's order and
+    reversed (an equally valid BLAS order), everything else equal and the
+    dense mat-vecs accumulated exactly.  The two differ by ~1.7e-12 relative
+    L2: at this size the reference's own result is defined only to that
+    level, which is the parity bar of the GPU's plain form on this workload.
+    About 2 minutes on 8 host threads."""
+    import bench
+
+    c = bench.CONFIGS["navier-clusters-1m"]
+    seed = 0x4B1F0000 + c["cfg"]
+    pos, nor = bench.make_points(c, seed)
+    den = bench.make_density(c, 3, seed)[0]
+    o = O.OracleFMM(c["kt"], c["coefs"], 8, 150, 10, c["ctr"], c["rootlvl"])
+    o.setup(pos, nor, pos)
+    L = O.lib()
+    L.orc_set_order.argtypes = [ctypes.c_int]
+    try:
+        a = o.eval(den)
+        L.orc_set_order(1)
+        b = o.eval(den)
+    finally:
+        L.orc_set_order(0)
+    floor = relerr(b, a)
+    print(f"kernel-sum order floor, Navier clusters 1M np 8: {floor:.3e}")
+    assert 1e-12 < floor < 5e-12
