@@ -170,8 +170,9 @@ void fft_inv(hipStream_t s, int np, int nslot, const int* slot_node, int Rc, con
 // [C][26 directions x 32][2] complex: entry n*32 + 8k + c holds the operators
 // of (source child k, target child c) and (4 + k, c) in direction n (zero for
 // pairs outside the V list); single selects FP32 spectra and operators.
+// fpart: deal frequency blocks rather than tile ranges to the XCDs.
 void m2l_tiled(hipStream_t s, int C, int ntile, int Rc, const int* ttile, const int* htile, const void* atab,
-               const void* effden, void* effval, int single);
+               const void* effden, void* effval, int single, int fpart = 0);
 // The FP32 products of m2l_tiled with one wave per (tile, frequency): the A
 // operands of all directions in registers, halo spectra loaded straight into
 // the matrix-core operands (atab32: the FP32 table), depth halo parents

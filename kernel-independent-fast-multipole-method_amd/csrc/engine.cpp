@@ -107,6 +107,7 @@ Engine::Engine(int kt, const double* coefs, int ncoefs, int np, int ptsmax, int 
     m2l_wave_ = v == "wave";
   }
   if (const char* d = std::getenv("KIFMM3D_M2L_DEPTH")) m2l_depth_ = std::atoi(d);
+  if (const char* d = std::getenv("KIFMM3D_M2L_FPART")) m2l_fpart_ = std::atoi(d);
   if (const char* gph = std::getenv("KIFMM3D_GRAPH")) use_graph_ = std::atoi(gph) != 0;
   if (const char* ov = std::getenv("KIFMM3D_OVERLAP")) overlap_ = std::atoi(ov) != 0;
   if (const char* v = std::getenv("KIFMM3D_S2M32")) s2m32_ = std::atoi(v) != 0;
@@ -1417,7 +1418,8 @@ void Engine::run_chunk(const double* srcden, double* trgval, int rc, int mode, h
         dev::m2l_wave(s, O.C, vl.ntile, rc, I + vl.ttile, I + vl.htile, atab32_.p, effden_.p, effval_.p, m2l_depth_);
       else if (tiled)
         dev::m2l_tiled(s, O.C, vl.ntile, rc, I + vl.ttile, I + vl.htile,
-                       single ? (const void*)atab32_.p : (const void*)atab_.p, effden_.p, effval_.p, single);
+                       single ? (const void*)atab32_.p : (const void*)atab_.p, effden_.p, effval_.p, single,
+                       m2l_fpart_);
       else if (form == 3)
         dev::m2l(s, sm, tm, O.C, vl.ntrg, rc, I + vl.vptr, I + vl.vsrc, I + vl.vop, (const double2*)m2l_.p,
                  (const double2*)effden_.p, (double2*)effval_.p);

@@ -226,6 +226,7 @@ class Engine {
   bool m2l_matrix_ = true; // parent-grouped matrix-core M2L otherwise (KIFMM3D_M2L=group|pair disables)
   bool m2l_wave_ = false;  // the tiled M2L with one wave per (tile, frequency) (KIFMM3D_M2L=wave)
   int m2l_depth_ = 4;       // ... halo parents its spectra loads run ahead (KIFMM3D_M2L_DEPTH)
+  int m2l_fpart_ = 0;       // tiled M2L: frequency blocks dealt to the XCDs (KIFMM3D_M2L_FPART=1)
   hipStream_t stream_ = nullptr;
   hipStream_t side_ = nullptr;  // near field, concurrent with the far-field chain
   hipEvent_t fork_ev_ = nullptr, join_ev_ = nullptr;
