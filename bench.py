@@ -122,6 +122,8 @@ def cpu_baseline(c):
     lib = O.lib()
     lib.orc_set_accurate.argtypes = [ctypes.c_int]
     lib.orc_set_accurate(0)  # plain FP64 accumulation, as the reference's BLAS does
+    threads_all = O.get_threads()
+    O.set_threads(1)  # the headline: one thread, as the reference is serial
     n = {111: 131072, 321: 32768, 511: 16384}[c["kt"]]
     s = dict(c, n=n, nrhs=1)
     pos, nor = make_points(s, 0x4B1F0000 + c["cfg"])
@@ -134,13 +136,23 @@ def cpu_baseline(c):
         t0 = time.perf_counter()
         o.eval(den)
         times.append(time.perf_counter() - t0)
+    # beside it, labelled: the same evals on every host core (the oracle's
+    # per-node loops on OpenMP threads, same results)
+    O.set_threads(threads_all)
+    tall = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        o.eval(den)
+        tall.append(time.perf_counter() - t0)
     lib.orc_set_accurate(1)
-    dt = float(np.median(times))
+    dt, dta = float(np.median(times)), float(np.median(tall))
     model, ncpu, avail = host_cpu()
     return {"value": n / dt, "unit": "targets/s", "cores": 1, "kind": "port",
             "sample": f"{c['gen']} N={n}, np={c['np']}, ptsmax={c['ptsmax']}, 1 RHS; median of 3 warm evals "
                       f"after one warm-up: {dt:.2f} s ({', '.join(f'{t:.2f}' for t in times)})",
-            "cpu_model": model, "nproc": ncpu, "cpus_available": avail, "threads": 1}
+            "cpu_model": model, "nproc": ncpu, "cpus_available": avail, "threads": 1,
+            "all_cores": {"value": n / dta, "threads": threads_all,
+                          "sample": f"the same, median of 3 on {threads_all} threads: {dta:.2f} s"}}
 
 
 def batch_problems(seed=0x4B1F0005):

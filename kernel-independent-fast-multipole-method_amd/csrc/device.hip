@@ -860,14 +860,20 @@ __global__ __launch_bounds__(P2P_BS) void k_p2p_mfma111_f32(int nchunk, const in
       }
       if (vec) {
         for (int jj = t; jj < cnt4; jj += P2P_BS) {
-          float4 q[4] = {};
+          // named registers (an array here went to scratch memory)
+          float4 q0 = make_float4(0.f, 0.f, 0.f, 0.f), q1 = q0, q2 = q0, q3 = q0;
           if (jj < cnt) {
             const float4* row = reinterpret_cast<const float4*>(den32 + (long)(j0 + jj) * src.den_js);
-#pragma unroll
-            for (int v = 0; v < 4; v++) q[v] = row[v];
+            q0 = row[0];
+            q1 = row[1];
+            q2 = row[2];
+            q3 = row[3];
           }
-#pragma unroll
-          for (int v = 0; v < 4; v++) reinterpret_cast<float4*>(s_den[jj])[v] = q[v];
+          float4* dst = reinterpret_cast<float4*>(s_den[jj]);
+          dst[0] = q0;
+          dst[1] = q1;
+          dst[2] = q2;
+          dst[3] = q3;
         }
       } else {
         for (int i = t; i < cnt4 * NR; i += P2P_BS) {
