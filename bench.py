@@ -40,11 +40,41 @@ CONFIGS = {
 }
 DEFAULT = "laplace-uniform-4m-16rhs"
 
-# Memory traffic (bytes per evaluate, L2 <-> fabric, calibrated against a streaming copy) of a
-# kernel class, measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE; the commands and the
-# calibration are in the named profile.  Keyed by (workload, form, kernel class).
-TRAFFIC = {
-    ("laplace-uniform-4m-16rhs", "mixed", "m2l"): (16.08e9, "profiles/laplace-uniform-4m-16rhs_m2l-tile-f32_traffic.md"),
+# Counter evidence per (workload, form, leaf size) and kernel class, from
+# rocprofv3 runs of the workload with the tuned settings pinned (commands and
+# per-kernel tables in the named profile): the matrix-core busy fraction
+# (SQ_VALU_MFMA_BUSY_CYCLES / (GRBM_GUI_ACTIVE / 8 XCDs x 1024 SIMDs)) and the
+# bytes read and written per evaluate between the L2 and the fabric
+# (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0, the factors of a streaming copy of
+# 1 GiB, bench.py --calibrate).  The JSON line divides the bytes by the live
+# kernel time of the run for the HBM-side rate.
+PROFILED = {
+    ("laplace-uniform-4m-16rhs", "mixed", 256): ("profiles/laplace-uniform-4m-16rhs_mixed.md", {
+        "m2l": {"busy": 0.605, "read": 7.044, "written": 4.831},
+        "p2p": {"busy": 0.402, "read": 3.342, "written": 0.577},
+        "fft_fwd": {"busy": None, "read": 0.728, "written": 4.830},
+        "fft_inv": {"busy": None, "read": 5.560, "written": 0.728},
+        "s2m": {"busy": 0.256, "read": 0.373, "written": 1.246},
+        "s2p": {"busy": 0.200, "read": 1.310, "written": 0.580},
+        "gemm_up": {"busy": 0.427, "read": None, "written": None},
+        "gemm_down": {"busy": 0.427, "read": None, "written": None},
+    }),
+    ("laplace-uniform-4m-16rhs", "plain", 256): ("profiles/laplace-uniform-4m-16rhs_plain.md", {
+        "m2l": {"busy": 0.522, "read": 21.389, "written": 9.663},
+        "p2p": {"busy": 0.352, "read": 8.068, "written": 0.558},
+        "fft_fwd": {"busy": None, "read": 0.730, "written": 9.690},
+        "fft_inv": {"busy": None, "read": 10.391, "written": 0.728},
+        "s2m": {"busy": 0.318, "read": 0.641, "written": 1.245},
+        "s2p": {"busy": 0.260, "read": 1.303, "written": 0.556},
+    }),
+    ("navier-clusters-1m", "mixed", 150): ("profiles/navier-clusters-1m_mixed.md", {
+        "m2l": {"busy": 0.318, "read": 27.667, "written": 3.374},
+        "p2p": {"busy": None, "read": 1.958, "written": 0.026},
+        "fft_fwd": {"busy": None, "read": 0.210, "written": 1.599},
+        "fft_inv": {"busy": None, "read": 1.896, "written": 0.573},
+        "gemm_up": {"busy": 0.355, "read": None, "written": None},
+        "gemm_down": {"busy": 0.355, "read": None, "written": None},
+    }),
 }
 
 # flops per source-target pair for nrhs right-hand sides (FMA = 2, sqrt/div = 1)
@@ -384,13 +414,23 @@ def run_forms(K, f, forms, c, args, d_src, d_trg, nrhs, n, tdof, seed, stats, ca
     return res
 
 
-def kernel_table(rates):
-    """Per kernel class of one form: time, algorithmic rate and its fraction of the class's peak."""
+def kernel_table(rates, prof=None):
+    """Per kernel class of one form: time, algorithmic rate and its fraction of
+    the class's peak; with the workload's profile, the measured matrix-core busy
+    fraction and the L2 <-> fabric bytes per evaluate over the live time."""
     out = {}
     for k, (ms, calls, rate, unit, bound, peak, amt) in rates.items():
         out[k] = {"ms": round(ms, 3), "launch_groups": calls, "bound": bound,
                   "achieved": round(rate, 3), "unit": "TFLOP/s" if unit == "flop" else "GB/s",
                   "peak": peak, "frac": round(rate / peak, 4)}
+        q = (prof or {}).get(k)
+        if q:
+            if q.get("busy") is not None:
+                out[k]["matrix_core_busy_profiled"] = q["busy"]
+            if q.get("read") is not None and ms > 0:
+                tb = (q["read"] + q["written"]) * 1e9
+                out[k]["traffic_profiled"] = {"read": q["read"] * 1e9, "written": q["written"] * 1e9,
+                                              "gbs": round(tb / (ms * 1e-3) / 1e9, 1)}
     return out
 
 
@@ -483,13 +523,17 @@ def main():
     # events on the library's stream during its last timed step
     dom = max(r["rates"], key=lambda k: r["rates"][k][0])
     ms, calls, rate, unit, bound, peak, amt = r["rates"][dom]
-    tr = TRAFFIC.get((args.config, chosen, dom))
+    src, prof = PROFILED.get((args.config, chosen, ptsmax), (None, {}))
+    tr = prof.get(dom, {})
+    traffic = (tr["read"] + tr["written"]) * 1e9 if tr.get("read") is not None else None
     roofline = {"kernel": dom, "bound": bound, "achieved": round(rate, 4), "peak": peak,
                 "unit": "TFLOP/s" if unit == "flop" else "GB/s", "frac": round(rate / peak, 5),
-                "traffic": tr[0] if tr else None, "ms_per_step": round(ms, 3), "launch_groups": calls,
+                "traffic": traffic, "ms_per_step": round(ms, 3), "launch_groups": calls,
                 "algorithmic_per_step": amt}
-    if tr:
-        roofline["traffic_note"] = f"bytes per evaluate over the {calls} launches, from {tr[1]}"
+    if traffic:
+        roofline["traffic_note"] = f"L2 <-> fabric bytes per evaluate over the {calls} launches, from {src}"
+    if tr.get("busy") is not None:
+        roofline["matrix_core_busy_profiled"] = tr["busy"]
 
     # the metric's gate (error <= 1e-6 against the direct sum) is below the
     # truncation error of np 6 on this workload; the same points at the
@@ -516,7 +560,9 @@ def main():
     cfg = {"workload": args.config, "form": "mixed FP32/FP64" if chosen == "mixed" else "plain FP64",
            "kernel": c["kt"], "n_points": n, "nrhs": nrhs, "np": c["np"], "ptsmax": ptsmax,
            "rel_err_vs_direct": r["err"], "rel_err_fmm_norm": r["errf"], "setup_s": round(t_setup, 3),
-           "stage_ms": {k: round(v, 3) for k, v in r["stages"].items()}, "kernels": kernel_table(r["rates"])}
+           "stage_ms": {k: round(v, 3) for k, v in r["stages"].items()}, "kernels": kernel_table(r["rates"], prof)}
+    if src:
+        cfg["profile"] = src
     if tune:
         cfg["ptsmax_tuning_ms"] = tune
     if r["tuned"]:
@@ -526,7 +572,8 @@ def main():
         cfg["plain_fp64"] = {"value": p["value"], "ms_per_step": round(p["ms"], 3), "rel_err_vs_direct": p["err"],
                              "rel_err_fmm_norm": p["errf"],
                              "stage_ms": {k: round(v, 3) for k, v in p["stages"].items()}, "tuned": p["tuned"],
-                             "kernels": kernel_table(p["rates"])}
+                             "kernels": kernel_table(p["rates"],
+                                                     PROFILED.get((args.config, "plain", ptsmax), (None, {}))[1])}
         cfg["speedup_vs_plain"] = round(r["value"] / p["value"], 3)
     if gate:
         cfg["at_error_gate"] = gate

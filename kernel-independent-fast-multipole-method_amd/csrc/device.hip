@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <cfloat>
+#include <cstdlib>
 #include <mutex>
 #include <type_traits>
 #include <utility>
@@ -3138,12 +3139,11 @@ void m2l_tiled(hipStream_t s, int C, int ntile, int Rc, const int* ttile, const 
 // computed and not stored.  FP32 accumulation over the 26 directions, as in
 // k_m2l_tile<float>.
 //
-// Blocks (frequency f, group of 4 TPW tiles): the frequencies are dealt to
+// Blocks (frequency f, group of 4 TPW tiles, TPW = tiles per wave): the frequencies are dealt to
 // the 8 XCDs (block b runs on XCD b % 8) and each XCD sweeps all tile groups
 // of a frequency before the next, so every tile that reads a halo spectrum
 // line (slot, f) runs on the one L2 that holds it, at about the same time.
 constexpr int M2LW_BS = 256;
-constexpr int M2LW_TPW = 2;  // tiles per wave (1 measured slower: 9.25 against 8.98 ms at 4M points, for 2.7x less L2 traffic)
 
 // Direction index (0 .. 25, dz fastest, centre skipped) in which halo parent
 // h (position (h / 16, h / 4 % 4, h % 4)) neighbours target parent p of the
@@ -3164,8 +3164,8 @@ __device__ __forceinline__ void m2lw_each(std::integer_sequence<int, H...>, F&& 
   (f(std::integral_constant<int, H>{}), ...);
 }
 
-template <int D>
-__global__ __launch_bounds__(M2LW_BS, 4) void k_m2l_wave(int C, int Rc, int ntile, int ngrp,
+template <int D, int TPW, int OCC>
+__global__ __launch_bounds__(M2LW_BS, OCC) void k_m2l_wave(int C, int Rc, int ntile, int ngrp,
                                                         const int* __restrict__ ttile, const int* __restrict__ htile,
                                                         const float2* __restrict__ atab,
                                                         const float2* __restrict__ effden, float2* __restrict__ effval)
@@ -3196,8 +3196,8 @@ __global__ __launch_bounds__(M2LW_BS, 4) void k_m2l_wave(int C, int Rc, int ntil
   }
   __syncthreads();
   const float2* eb = effden + r0 + i;
-  for (int tt = 0; tt < M2LW_TPW; tt++) {
-    const int tile = (g * 4 + w) * M2LW_TPW + tt;
+  for (int tt = 0; tt < TPW; tt++) {
+    const int tile = (g * 4 + w) * TPW + tt;
     if (tile >= ntile) break;  // wave-uniform
     int hv[8];
 #pragma unroll
@@ -3267,19 +3267,30 @@ void m2l_wave(hipStream_t s, int C, int ntile, int Rc, const int* ttile, const i
               const void* effden, void* effval, int depth)
 {
   if (ntile <= 0 || Rc < 16) return;
-  const int ngrp = (ntile + 4 * M2LW_TPW - 1) / (4 * M2LW_TPW);
+  // One tile per wave: the resident blocks of an XCD then hold one frequency
+  // at a time, whose spectra fit its L2 (4M points / 16 RHS, levels 4-5:
+  // 6.7 GB read per evaluate, 1.4x the unique spectra, 9.15 ms).  Two tiles
+  // per wave (KIFMM3D_M2L_WAVE=24; =22 the same at 2 waves per SIMD, loads 6
+  // ahead) amortise the A-table staging over 8 tiles a block but keep two
+  // frequencies resident: 18 GB read, 8.92 ms.
+  static const int var = [] {
+    const char* v = std::getenv("KIFMM3D_M2L_WAVE");
+    return v ? std::atoi(v) : 14;
+  }();
+  const int tpw = var == 24 || var == 22 ? 2 : 1;
+  const int ngrp = (ntile + 4 * tpw - 1) / (4 * tpw);
   dim3 grid((unsigned)(8 * ngrp * ((C + 7) / 8)), Rc / 16);
-#define KIFMM_M2LW(DD)                                                                                           \
-  hipLaunchKernelGGL(k_m2l_wave<DD>, grid, dim3(M2LW_BS), 0, s, C, Rc, ntile, ngrp, ttile, htile,               \
+#define KIFMM_M2LW(DD, TT, OO)                                                                                   \
+  hipLaunchKernelGGL((k_m2l_wave<DD, TT, OO>), grid, dim3(M2LW_BS), 0, s, C, Rc, ntile, ngrp, ttile, htile,     \
                      (const float2*)atab32, (const float2*)effden, (float2*)effval)
-  if (depth <= 1)
-    KIFMM_M2LW(1);
-  else if (depth == 2)
-    KIFMM_M2LW(2);
-  else if (depth <= 4)
-    KIFMM_M2LW(4);
+  if (var == 22)
+    KIFMM_M2LW(6, 2, 2);
+  else if (tpw == 2)
+    KIFMM_M2LW(4, 2, 4);
+  else if (depth <= 1)
+    KIFMM_M2LW(1, 1, 4);
   else
-    KIFMM_M2LW(6);
+    KIFMM_M2LW(4, 1, 4);
 #undef KIFMM_M2LW
 }
 
