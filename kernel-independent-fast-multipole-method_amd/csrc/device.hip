@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <cfloat>
+#include <climits>
 #include <cstdlib>
 #include <mutex>
 #include <type_traits>
@@ -3164,7 +3165,12 @@ __device__ __forceinline__ void m2lw_each(std::integer_sequence<int, H...>, F&& 
   (f(std::integral_constant<int, H>{}), ...);
 }
 
-template <int D, int TPW, int OCC>
+// Zero spectra of 16 right-hand sides: absent halo children read these, so
+// every spectrum load is unconditional (no branch around it) and the loads in
+// flight are counted exactly.  Never written.
+__device__ float2 m2lw_zero[16];
+
+template <int D, int TPW, int OCC, int ORD>
 __global__ __launch_bounds__(M2LW_BS, OCC) void k_m2l_wave(int C, int Rc, int ntile, int ngrp,
                                                         const int* __restrict__ ttile, const int* __restrict__ htile,
                                                         const float2* __restrict__ atab,
@@ -3195,13 +3201,20 @@ __global__ __launch_bounds__(M2LW_BS, OCC) void k_m2l_wave(int C, int Rc, int nt
     s_A4[n][ln] = pim ? make_float4(m.y, m.x, m.w, m.z) : make_float4(m.x, -m.y, m.z, -m.w);
   }
   __syncthreads();
-  const float2* eb = effden + r0 + i;
+  // the lane's spectrum of slot s at f is ebf[16 * s * C * Rc / 16]: halo
+  // entries are held as s * C * Rc / 16 (< 2^31, checked by the launcher), so
+  // an address is one shift-add from ebf
+  const float2* ebf = effden + (long)f * Rc + r0 + i;
+  const int cr16 = C * (Rc >> 4);
   for (int tt = 0; tt < TPW; tt++) {
     const int tile = (g * 4 + w) * TPW + tt;
     if (tile >= ntile) break;  // wave-uniform
     int hv[8];
 #pragma unroll
-    for (int q = 0; q < 8; q++) hv[q] = htile[(long)tile * 512 + q * 64 + l];
+    for (int q = 0; q < 8; q++) {
+      const int sl = htile[(long)tile * 512 + q * 64 + l];
+      hv[q] = sl >= 0 ? sl * cr16 : -1;
+    }
     const int tv = ttile[(long)tile * 64 + l];
     // halo parents with a source child: bit h (the entries 8h .. 8h+7 of
     // halo parent h sit in lanes 8 (h % 8) .. 8 (h % 8) + 7 of hv[h / 8])
@@ -3213,24 +3226,37 @@ __global__ __launch_bounds__(M2LW_BS, OCC) void k_m2l_wave(int C, int Rc, int nt
       for (int bb = 0; bb < 8; bb++)
         if ((bal >> (8 * bb)) & 0xffull) present |= 1ull << (8 * q + bb);
     }
-    // spectra of halo parent h, children k and 4 + k, for the lane's right-hand side
-    auto load = [&](int h, float2* x) {
-      const int s0 = __shfl(hv[h >> 3], 8 * (h & 7) + k), s1 = __shfl(hv[h >> 3], 8 * (h & 7) + 4 + k);
-      x[0] = make_float2(0.f, 0.f);
-      x[1] = make_float2(0.f, 0.f);
-      if (s0 >= 0) x[0] = eb[((long)s0 * C + f) * Rc];
-      if (s1 >= 0) x[1] = eb[((long)s1 * C + f) * Rc];
+    // offsets of halo parent h, children k and 4 + k (cross-lane reads)
+    auto offs = [&](int h, int* o) {
+      o[0] = __shfl(hv[h >> 3], 8 * (h & 7) + k);
+      o[1] = __shfl(hv[h >> 3], 8 * (h & 7) + 4 + k);
+    };
+    // their spectra for the lane's right-hand side (zeros for absent children)
+    const float2* zs = m2lw_zero + i;
+    auto load = [&](const int* o, float2* x) {
+      const float2* p0 = o[0] >= 0 ? ebf + ((size_t)(unsigned)o[0] << 4) : zs;
+      const float2* p1 = o[1] >= 0 ? ebf + ((size_t)(unsigned)o[1] << 4) : zs;
+      x[0] = *p0;
+      x[1] = *p1;
     };
     v4f acc[8];
 #pragma unroll
     for (int p = 0; p < 8; p++) acc[p] = v4f{0.f, 0.f, 0.f, 0.f};
     float2 ring[D][2];  // the spectra of the next D halo parents, in flight
+    int on[2];          // the offsets of the halo parent after those
 #pragma unroll
-    for (int d = 0; d < D; d++) load(d, ring[d]);
+    for (int d = 0; d < D; d++) {
+      offs(d, on);
+      load(on, ring[d]);
+    }
+    offs(D, on);
     m2lw_each(std::make_integer_sequence<int, 64>{}, [&](auto hc) {
       constexpr int h = decltype(hc)::value;
       const float2 x0 = ring[h % D][0], x1 = ring[h % D][1];
-      if constexpr (h + D < 64) load(h + D, ring[h % D]);
+      if constexpr (h + D < 64) {
+        load(on, ring[h % D]);
+        if constexpr (h + D + 1 < 64) offs(h + D + 1, on);
+      }
       if ((present >> h) & 1ull) {
         // the target parents that have halo parent h as a neighbour; K
         // slices in turn over them (independent accumulators between
@@ -3241,15 +3267,24 @@ __global__ __launch_bounds__(M2LW_BS, OCC) void k_m2l_wave(int C, int Rc, int nt
           const int n = m2lw_dir(h, p);
           if (n >= 0) av[p] = s_A4[n][l];
         }
+        auto mf = [&](int ks, int p) {
+          const float xb = ks == 0 ? x0.x : ks == 1 ? x0.y : ks == 2 ? x1.x : x1.y;
+          const float ab = ks == 0 ? av[p].x : ks == 1 ? av[p].y : ks == 2 ? av[p].z : av[p].w;
+          acc[p] = __builtin_amdgcn_mfma_f32_16x16x4f32(ab, xb, acc[p], 0, 0, 0);
+        };
+        if constexpr (ORD == 0) {
 #pragma unroll
-        for (int ks = 0; ks < 4; ks++)
+          for (int ks = 0; ks < 4; ks++)
 #pragma unroll
-          for (int p = 0; p < 8; p++) {
-            if (m2lw_dir(h, p) < 0) continue;
-            const float xb = ks == 0 ? x0.x : ks == 1 ? x0.y : ks == 2 ? x1.x : x1.y;
-            const float ab = ks == 0 ? av[p].x : ks == 1 ? av[p].y : ks == 2 ? av[p].z : av[p].w;
-            acc[p] = __builtin_amdgcn_mfma_f32_16x16x4f32(ab, xb, acc[p], 0, 0, 0);
-          }
+            for (int p = 0; p < 8; p++)
+              if (m2lw_dir(h, p) >= 0) mf(ks, p);
+        } else {  // one target parent after the other: its A operand is needed first
+#pragma unroll
+          for (int p = 0; p < 8; p++)
+#pragma unroll
+            for (int ks = 0; ks < 4; ks++)
+              if (m2lw_dir(h, p) >= 0) mf(ks, p);
+        }
       }
       asm volatile("" ::: "memory");  // spectra loads D halo parents ahead, no further
     });
@@ -3263,34 +3298,52 @@ __global__ __launch_bounds__(M2LW_BS, OCC) void k_m2l_wave(int C, int Rc, int nt
   }
 }
 
-void m2l_wave(hipStream_t s, int C, int ntile, int Rc, const int* ttile, const int* htile, const void* atab32,
-              const void* effden, void* effval, int depth)
+void m2l_wave(hipStream_t s, int C, int ntile, int Rc, int nslot, const int* ttile, const int* htile,
+              const void* atab32, const void* effden, void* effval, int depth)
 {
   if (ntile <= 0 || Rc < 16) return;
-  // One tile per wave: the resident blocks of an XCD then hold one frequency
-  // at a time, whose spectra fit its L2 (4M points / 16 RHS, levels 4-5:
-  // 6.7 GB read per evaluate, 1.4x the unique spectra, 9.15 ms).  Two tiles
-  // per wave (KIFMM3D_M2L_WAVE=24; =22 the same at 2 waves per SIMD, loads 6
-  // ahead) amortise the A-table staging over 8 tiles a block but keep two
-  // frequencies resident: 18 GB read, 8.92 ms.
+  if ((long)nslot * C * (Rc / 16) > INT_MAX) {  // halo offsets beyond 32 bits: the block-per-tile kernel
+    m2l_tiled(s, C, ntile, Rc, ttile, htile, atab32, effden, effval, 1);
+    return;
+  }
+  // Measured at 4M points / 16 RHS, levels 4-5 (KIFMM3D_M2L_WAVE = the four
+  // template arguments as digits, for launch-shape sweeps):
+  //  * one tile per wave: the resident blocks of an XCD hold one frequency at
+  //    a time, whose spectra fit its L2: 6.7 GB read per evaluate, 1.4x the
+  //    unique spectra; two tiles per wave (4241) amortise the A-table staging
+  //    over 8 tiles a block but keep two frequencies resident: 18 GB read and
+  //    slower;
+  //  * the products of one target parent after the other (ORD 1) rather than
+  //    K slice by K slice over all of them: 8.7 -> 8.1 ms for the M2L (the A
+  //    operands are read from LDS two parents at a time, 98 instead of 116
+  //    VGPRs); 2 halo parents ahead 8.7 ms, 4, 6 or 8 ahead 8.1-8.3 ms, 5
+  //    waves per SIMD no faster.
   static const int var = [] {
     const char* v = std::getenv("KIFMM3D_M2L_WAVE");
-    return v ? std::atoi(v) : 14;
+    return v ? std::atoi(v) : 4141;
   }();
-  const int tpw = var == 24 || var == 22 ? 2 : 1;
+  const int tpw = (var / 100) % 10 == 2 ? 2 : 1;
   const int ngrp = (ntile + 4 * tpw - 1) / (4 * tpw);
   dim3 grid((unsigned)(8 * ngrp * ((C + 7) / 8)), Rc / 16);
-#define KIFMM_M2LW(DD, TT, OO)                                                                                   \
-  hipLaunchKernelGGL((k_m2l_wave<DD, TT, OO>), grid, dim3(M2LW_BS), 0, s, C, Rc, ntile, ngrp, ttile, htile,     \
+#define KIFMM_M2LW(DD, TT, OO, RR)                                                                               \
+  hipLaunchKernelGGL((k_m2l_wave<DD, TT, OO, RR>), grid, dim3(M2LW_BS), 0, s, C, Rc, ntile, ngrp, ttile, htile, \
                      (const float2*)atab32, (const float2*)effden, (float2*)effval)
-  if (var == 22)
-    KIFMM_M2LW(6, 2, 2);
-  else if (tpw == 2)
-    KIFMM_M2LW(4, 2, 4);
-  else if (depth <= 1)
-    KIFMM_M2LW(1, 1, 4);
+  if (depth <= 1)
+    KIFMM_M2LW(1, 1, 4, 1);
+  else if (var == 4140)
+    KIFMM_M2LW(4, 1, 4, 0);
+  else if (var == 2141)
+    KIFMM_M2LW(2, 1, 4, 1);
+  else if (var == 6141)
+    KIFMM_M2LW(6, 1, 4, 1);
+  else if (var == 8141)
+    KIFMM_M2LW(8, 1, 4, 1);
+  else if (var == 4241)
+    KIFMM_M2LW(4, 2, 4, 1);
+  else if (var == 6151)
+    KIFMM_M2LW(6, 1, 5, 1);
   else
-    KIFMM_M2LW(4, 1, 4);
+    KIFMM_M2LW(4, 1, 4, 1);
 #undef KIFMM_M2LW
 }
 

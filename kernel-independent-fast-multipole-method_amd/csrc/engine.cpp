@@ -1415,7 +1415,8 @@ void Engine::run_chunk(const double* srcden, double* trgval, int rc, int mode, h
     });
     timed(KIFMM3D_K_M2L, [&] {
       if (form == 4)
-        dev::m2l_wave(s, O.C, vl.ntile, rc, I + vl.ttile, I + vl.htile, atab32_.p, effden_.p, effval_.p, m2l_depth_);
+        dev::m2l_wave(s, O.C, vl.ntile, rc, vl.nsrc, I + vl.ttile, I + vl.htile, atab32_.p, effden_.p, effval_.p,
+                      m2l_depth_);
       else if (tiled)
         dev::m2l_tiled(s, O.C, vl.ntile, rc, I + vl.ttile, I + vl.htile,
                        single ? (const void*)atab32_.p : (const void*)atab_.p, effden_.p, effval_.p, single,
