@@ -3165,13 +3165,12 @@ __device__ __forceinline__ void m2lw_each(std::integer_sequence<int, H...>, F&& 
   (f(std::integral_constant<int, H>{}), ...);
 }
 
-// Zero spectra of 16 right-hand sides: absent halo children read these, so
-// every spectrum load is unconditional (no branch around it) and the loads in
-// flight are counted exactly.  Never written.
-__device__ float2 m2lw_zero[16];
-
-template <int D, int TPW, int OCC, int ORD>
-__global__ __launch_bounds__(M2LW_BS, OCC) void k_m2l_wave(int C, int Rc, int ntile, int ngrp,
+// OFF32: the level's spectra and its zero slot span less than 4 GiB, so a
+// spectrum address is a 32-bit byte offset from the frequency's row of slot 0
+// (one VALU instruction per load and the SGPR-based load form); otherwise a
+// 64-bit shift-add.
+template <int D, int TPW, int OCC, int ORD, bool OFF32>
+__global__ __launch_bounds__(M2LW_BS, OCC) void k_m2l_wave(int C, int Rc, int ntile, int ngrp, int zslot,
                                                         const int* __restrict__ ttile, const int* __restrict__ htile,
                                                         const float2* __restrict__ atab,
                                                         const float2* __restrict__ effden, float2* __restrict__ effval)
@@ -3201,43 +3200,46 @@ __global__ __launch_bounds__(M2LW_BS, OCC) void k_m2l_wave(int C, int Rc, int nt
     s_A4[n][ln] = pim ? make_float4(m.y, m.x, m.w, m.z) : make_float4(m.x, -m.y, m.z, -m.w);
   }
   __syncthreads();
-  // the lane's spectrum of slot s at f is ebf[16 * s * C * Rc / 16]: halo
-  // entries are held as s * C * Rc / 16 (< 2^31, checked by the launcher), so
-  // an address is one shift-add from ebf
-  const float2* ebf = effden + (long)f * Rc + r0 + i;
+  // the spectrum of slot s at f for right-hand side r0 + i is eb0[16 o + i]
+  // with o = s * C * Rc / 16 (< 2^31, checked by the launcher); absent halo
+  // children read slot zslot, which holds zeros, so every load is
+  // unconditional and the loads in flight are counted exactly
+  const float2* eb0 = effden + (long)f * Rc + r0;
+  const unsigned i8 = 8u * i;
   const int cr16 = C * (Rc >> 4);
   for (int tt = 0; tt < TPW; tt++) {
     const int tile = (g * 4 + w) * TPW + tt;
     if (tile >= ntile) break;  // wave-uniform
+    // hv: the offsets of the halo children; present: bit h set if halo
+    // parent h has a source child (the entries 8h .. 8h+7 of halo parent h
+    // sit in lanes 8 (h % 8) .. 8 (h % 8) + 7 of hv[h / 8])
     int hv[8];
-#pragma unroll
-    for (int q = 0; q < 8; q++) {
-      const int sl = htile[(long)tile * 512 + q * 64 + l];
-      hv[q] = sl >= 0 ? sl * cr16 : -1;
-    }
-    const int tv = ttile[(long)tile * 64 + l];
-    // halo parents with a source child: bit h (the entries 8h .. 8h+7 of
-    // halo parent h sit in lanes 8 (h % 8) .. 8 (h % 8) + 7 of hv[h / 8])
     unsigned long long present = 0;
 #pragma unroll
     for (int q = 0; q < 8; q++) {
-      const unsigned long long bal = __ballot(hv[q] >= 0);
+      const int sl = htile[(long)tile * 512 + q * 64 + l];
+      hv[q] = (sl >= 0 ? sl : zslot) * cr16;
+      const unsigned long long bal = __ballot(sl >= 0);
 #pragma unroll
       for (int bb = 0; bb < 8; bb++)
         if ((bal >> (8 * bb)) & 0xffull) present |= 1ull << (8 * q + bb);
     }
+    const int tv = ttile[(long)tile * 64 + l];
     // offsets of halo parent h, children k and 4 + k (cross-lane reads)
     auto offs = [&](int h, int* o) {
       o[0] = __shfl(hv[h >> 3], 8 * (h & 7) + k);
       o[1] = __shfl(hv[h >> 3], 8 * (h & 7) + 4 + k);
     };
-    // their spectra for the lane's right-hand side (zeros for absent children)
-    const float2* zs = m2lw_zero + i;
+    // their spectra for the lane's right-hand side
     auto load = [&](const int* o, float2* x) {
-      const float2* p0 = o[0] >= 0 ? ebf + ((size_t)(unsigned)o[0] << 4) : zs;
-      const float2* p1 = o[1] >= 0 ? ebf + ((size_t)(unsigned)o[1] << 4) : zs;
-      x[0] = *p0;
-      x[1] = *p1;
+      if constexpr (OFF32) {
+        const char* e8 = reinterpret_cast<const char*>(eb0);
+        x[0] = *reinterpret_cast<const float2*>(e8 + (((unsigned)o[0] << 7) + i8));
+        x[1] = *reinterpret_cast<const float2*>(e8 + (((unsigned)o[1] << 7) + i8));
+      } else {
+        x[0] = eb0[((size_t)(unsigned)o[0] << 4) + i];
+        x[1] = eb0[((size_t)(unsigned)o[1] << 4) + i];
+      }
     };
     v4f acc[8];
 #pragma unroll
@@ -3302,7 +3304,7 @@ void m2l_wave(hipStream_t s, int C, int ntile, int Rc, int nslot, const int* tti
               const void* atab32, const void* effden, void* effval, int depth)
 {
   if (ntile <= 0 || Rc < 16) return;
-  if ((long)nslot * C * (Rc / 16) > INT_MAX) {  // halo offsets beyond 32 bits: the block-per-tile kernel
+  if ((long)(nslot + 1) * C * (Rc / 16) > INT_MAX) {  // halo offsets beyond 32 bits: the block-per-tile kernel
     m2l_tiled(s, C, ntile, Rc, ttile, htile, atab32, effden, effval, 1);
     return;
   }
@@ -3325,23 +3327,23 @@ void m2l_wave(hipStream_t s, int C, int ntile, int Rc, int nslot, const int* tti
   const int tpw = (var / 100) % 10 == 2 ? 2 : 1;
   const int ngrp = (ntile + 4 * tpw - 1) / (4 * tpw);
   dim3 grid((unsigned)(8 * ngrp * ((C + 7) / 8)), Rc / 16);
-#define KIFMM_M2LW(DD, TT, OO, RR)                                                                               \
-  hipLaunchKernelGGL((k_m2l_wave<DD, TT, OO, RR>), grid, dim3(M2LW_BS), 0, s, C, Rc, ntile, ngrp, ttile, htile, \
-                     (const float2*)atab32, (const float2*)effden, (float2*)effval)
+  // slot nslot holds zeros (the caller's); byte offsets below 4 GiB: 32-bit
+  const bool off32 = (unsigned long long)(nslot + 1) * C * Rc * sizeof(float2) <= 0xffffffffull;
+#define KIFMM_M2LW(DD, TT, OO, RR)                                                                             \
+  do {                                                                                                         \
+    if (off32)                                                                                                 \
+      hipLaunchKernelGGL((k_m2l_wave<DD, TT, OO, RR, true>), grid, dim3(M2LW_BS), 0, s, C, Rc, ntile, ngrp,   \
+                         nslot, ttile, htile, (const float2*)atab32, (const float2*)effden, (float2*)effval);  \
+    else                                                                                                       \
+      hipLaunchKernelGGL((k_m2l_wave<DD, TT, OO, RR, false>), grid, dim3(M2LW_BS), 0, s, C, Rc, ntile, ngrp,  \
+                         nslot, ttile, htile, (const float2*)atab32, (const float2*)effden, (float2*)effval);  \
+  } while (0)
   if (depth <= 1)
     KIFMM_M2LW(1, 1, 4, 1);
   else if (var == 4140)
     KIFMM_M2LW(4, 1, 4, 0);
-  else if (var == 2141)
-    KIFMM_M2LW(2, 1, 4, 1);
-  else if (var == 6141)
-    KIFMM_M2LW(6, 1, 4, 1);
-  else if (var == 8141)
-    KIFMM_M2LW(8, 1, 4, 1);
   else if (var == 4241)
     KIFMM_M2LW(4, 2, 4, 1);
-  else if (var == 6151)
-    KIFMM_M2LW(6, 1, 5, 1);
   else
     KIFMM_M2LW(4, 1, 4, 1);
 #undef KIFMM_M2LW

@@ -176,7 +176,8 @@ void m2l_tiled(hipStream_t s, int C, int ntile, int Rc, const int* ttile, const 
 // The FP32 products of m2l_tiled with one wave per (tile, frequency): the A
 // operands of all directions in registers, halo spectra loaded straight into
 // the matrix-core operands (atab32: the FP32 table), depth halo parents
-// ahead (1 or 4); nslot: the spectra slots of the level.
+// ahead (1 or 4); nslot: the spectra slots of the level, followed by one
+// slot of zeros (the caller's, effden[nslot]), which absent halo children read.
 void m2l_wave(hipStream_t s, int C, int ntile, int Rc, int nslot, const int* ttile, const int* htile,
               const void* atab32, const void* effden, void* effval, int depth);
 

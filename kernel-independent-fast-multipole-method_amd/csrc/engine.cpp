@@ -886,7 +886,7 @@ void Engine::work_counts(int rc, size_t cnt[NWORK]) const
   cnt[4] = (size_t)rc * T.sndecnt * O.pln[SURF_UE];
   cnt[5] = (size_t)rc * T.tndecnt * O.pln[SURF_DC];
   cnt[6] = (size_t)rc * T.tndecnt * O.pln[SURF_DE];
-  cnt[7] = (size_t)2 * rc * plan_.max_vsrc * mm_.sdof * O.C;
+  cnt[7] = (size_t)2 * rc * (plan_.max_vsrc + 1) * mm_.sdof * O.C;  // + the zero slot of the wave-per-tile M2L
   cnt[8] = (size_t)2 * rc * plan_.max_vtrg * mm_.tdof * O.C;
   // split-K partials of the matrix-core GEMMs with few columns, for every
   // chunk size up to rc (the last chunk of an evaluate may be smaller)
@@ -1414,10 +1414,13 @@ void Engine::run_chunk(const double* srcden, double* trgval, int rc, int mode, h
                    ri, (const double2*)tw_.p);
     });
     timed(KIFMM3D_K_M2L, [&] {
-      if (form == 4)
+      if (form == 4) {
+        // FP32 spectra: slot vl.nsrc (C x rc float2, the size of rc doubles
+        // per frequency) is the zero slot that absent halo children read
+        dev::zero(s, effden_.p + (size_t)vl.nsrc * O.C * rc, (long)O.C * rc);
         dev::m2l_wave(s, O.C, vl.ntile, rc, vl.nsrc, I + vl.ttile, I + vl.htile, atab32_.p, effden_.p, effval_.p,
                       m2l_depth_);
-      else if (tiled)
+      } else if (tiled)
         dev::m2l_tiled(s, O.C, vl.ntile, rc, I + vl.ttile, I + vl.htile,
                        single ? (const void*)atab32_.p : (const void*)atab_.p, effden_.p, effval_.p, single,
                        m2l_fpart_);

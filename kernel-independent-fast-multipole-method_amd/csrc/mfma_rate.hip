@@ -4,6 +4,13 @@
 // chains from registers, no memory traffic inside the loop.  The rates are
 // the practical ceilings quoted beside the published peaks in the roofline
 // notes (the published peaks assume the 2.4 GHz peak clock).
+//
+// Second table: whether vector (VALU) instructions overlap FP32 MFMAs on a
+// SIMD.  Each round of 8 chains is interleaved with NV independent FP32 FMAs
+// per MFMA (one of them a v_rsq_f32 in the last rows), and the same FMAs are
+// timed alone; the time of one MFMA slot is printed as SIMD cycles at the
+// 2.4 GHz clock.  If the units overlapped, MFMA + FMAs would cost the larger
+// of the two, not their sum.
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -77,6 +84,34 @@ __global__ __launch_bounds__(BS) void k_f16(int iter, float* out)
   out[blockIdx.x * BS + threadIdx.x] = s;
 }
 
+template <int NV, int MF, int RSQ>
+__global__ __launch_bounds__(BS) void k_mix(int iter, float* out)
+{
+  v4f acc[NCHAIN];
+  for (int i = 0; i < NCHAIN; i++) acc[i] = v4f{0, 0, 0, 0};
+  float f[8];
+  for (int j = 0; j < 8; j++) f[j] = threadIdx.x * 1e-3f + j;
+  const float a = threadIdx.x * 1e-3f, b = blockIdx.x * 1e-3f;
+  for (int it = 0; it < iter; it++) {
+#pragma unroll
+    for (int i = 0; i < NCHAIN; i++) {
+      if (MF) acc[i] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[i], 0, 0, 0);
+#pragma unroll
+      for (int v = 0; v < NV; v++) {
+        const int j = (i * NV + v) & 7;
+        if (RSQ && v == 0)
+          f[j] = __builtin_amdgcn_rsqf(f[j] + 1.0f);
+        else
+          f[j] = __builtin_fmaf(f[j], 0.999f, 0.5f);
+      }
+    }
+  }
+  float s = 0;
+  for (int i = 0; i < NCHAIN; i++) s += acc[i][0] + acc[i][1] + acc[i][2] + acc[i][3];
+  for (int j = 0; j < 8; j++) s += f[j];
+  out[blockIdx.x * BS + threadIdx.x] = s;
+}
+
 bool ok(hipError_t e, const char* what)
 {
   if (e == hipSuccess) return true;
@@ -118,6 +153,32 @@ int main()
         std::printf("%-14s %8.3f ms  %7.1f TFLOP/s  (%.0f %% of the published %.1f)\n", sh[k].name, ms, tf,
                     100.0 * tf / sh[k].peak, sh[k].peak);
     }
+  struct Mix {
+    const char* name;
+    void (*k)(int, float*);
+  } mix[] = {{"MFMA", k_mix<0, 1, 0>},
+             {"MFMA + 4 FMA", k_mix<4, 1, 0>},
+             {"MFMA + 8 FMA", k_mix<8, 1, 0>},
+             {"MFMA + 7 FMA + 1 rsq", k_mix<8, 1, 1>},
+             {"4 FMA", k_mix<4, 0, 0>},
+             {"8 FMA", k_mix<8, 0, 0>},
+             {"7 FMA + 1 rsq", k_mix<8, 0, 1>}};
+  std::printf("\nFP32 16x16x4 MFMA beside independent FP32 vector instructions (cycles per MFMA slot per SIMD, 2.4 GHz):\n");
+  for (const Mix& m : mix) {
+    float best = 1e30f;
+    for (int rep = 0; rep < 3; rep++) {
+      if (!ok(hipEventRecord(a), "hipEventRecord")) return 1;
+      hipLaunchKernelGGL(m.k, dim3(nb), dim3(BS), 0, 0, iter / 2, out);
+      if (!ok(hipGetLastError(), "launch") || !ok(hipEventRecord(b), "hipEventRecord") ||
+          !ok(hipEventSynchronize(b), "hipEventSynchronize"))
+        return 1;
+      float ms = 0;
+      if (!ok(hipEventElapsedTime(&ms, a, b), "hipEventElapsedTime")) return 1;
+      best = ms < best ? ms : best;
+    }
+    const double slots = (double)nb * (BS / 64) * (iter / 2) * NCHAIN / 1024.0;  // per SIMD
+    std::printf("%-22s %8.3f ms  %6.1f cycles\n", m.name, best, best * 1e-3 * 2.4e9 / slots);
+  }
   (void)hipFree(out);
   return 0;
 }
