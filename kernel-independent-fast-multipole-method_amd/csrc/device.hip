@@ -2010,7 +2010,7 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(GemmArgs p)
 // nS(n) x dof is a multiple of 8, so a step never crosses a segment.  The A
 // tile of a step (RT*16 x 8, with the power-of-two B scale of its K rows
 // folded in) is double-buffered in LDS; each lane loads the two B values it
-// feeds straight from its column (contiguous in K) one step ahead.  Lane
+// feeds straight from its column (contiguous in K) PB steps ahead.  Lane
 // (k, col) feeds B[k0 + 2k + s][col] to MFMA slice s, so slice s reads A
 // rows at K index k0 + 2k + s.  One barrier per step.
 //
@@ -2018,7 +2018,7 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(GemmArgs p)
 // split), each writing its raw partial product to work[split][column][row];
 // k_gemm_reduce then sums the partials in split order, so the result does not
 // depend on scheduling.
-template <int RT, int G>
+template <int RT, int G, int PB = 1>
 __global__ __launch_bounds__(256) void k_gemm_mx(GemmArgs p, int ks)
 {
   constexpr int MR = RT * 16, NAT = (8 * MR + 255) / 256;
@@ -2074,7 +2074,9 @@ __global__ __launch_bounds__(256) void k_gemm_mx(GemmArgs p, int ks)
       if (e < 8 * MR) sA[buf][kk][row] = ar[i];
     }
   };
-  double2 bc[G], bn2[G];
+  // B values of steps st .. st + PB - 1 in flight (bq[0] is step st's);
+  // the column pointers follow the step being loaded
+  double2 bq[PB][G];
   auto load_b = [&](int st, double2* dst) {
     const int kin = (st - (st / sps) * sps) * 8;
 #pragma unroll
@@ -2090,16 +2092,25 @@ __global__ __launch_bounds__(256) void k_gemm_mx(GemmArgs p, int ks)
     for (int g = 0; g < G; g++) acc[rt][g] = v4d{0.0, 0.0, 0.0, 0.0};
   seg_ptrs(st0 / sps);
   load_a(st0);
-  load_b(st0, bc);
+#pragma unroll
+  for (int d = 0; d < PB; d++) {
+    const int ls = st0 + d;
+    if (ls < st1) {
+      if (d > 0 && ls % sps == 0) seg_ptrs(ls / sps);
+      load_b(ls, bq[d]);
+    }
+  }
   store_a(0);
   __syncthreads();
   for (int st = st0; st < st1; st++) {
     const int buf = (st - st0) & 1;
     const bool more = st + 1 < st1;
-    if (more) {
-      if ((st + 1) % sps == 0) seg_ptrs((st + 1) / sps);
-      load_a(st + 1);
-      load_b(st + 1, bn2);
+    if (more) load_a(st + 1);
+    double2 bn[G];
+    const int ls = st + PB;  // the step whose B values are loaded now
+    if (ls < st1) {
+      if (ls % sps == 0) seg_ptrs(ls / sps);
+      load_b(ls, bn);
     }
 #pragma unroll
     for (int sl = 0; sl < 2; sl++)
@@ -2108,13 +2119,16 @@ __global__ __launch_bounds__(256) void k_gemm_mx(GemmArgs p, int ks)
         const double a = sA[buf][2 * lk + sl][rt * 16 + li];
 #pragma unroll
         for (int g = 0; g < G; g++)
-          acc[rt][g] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, sl ? bc[g].y : bc[g].x, acc[rt][g], 0, 0, 0);
+          acc[rt][g] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, sl ? bq[0][g].y : bq[0][g].x, acc[rt][g], 0, 0, 0);
       }
-    if (more) {
-      store_a(buf ^ 1);
+    if (more) store_a(buf ^ 1);
 #pragma unroll
-      for (int g = 0; g < G; g++) bc[g] = bn2[g];
-    }
+    for (int d = 0; d + 1 < PB; d++)
+#pragma unroll
+      for (int g = 0; g < G; g++) bq[d][g] = bq[d + 1][g];
+    if (ls < st1)
+#pragma unroll
+      for (int g = 0; g < G; g++) bq[PB - 1][g] = bn[g];
     __syncthreads();
   }
 #pragma unroll
@@ -2196,7 +2210,18 @@ void gemm(hipStream_t s, const GemmArgs& a)
         hipLaunchKernelGGL((k_gemm_mx<4, 4>), grid, dim3(256), 0, s, a, ks);
       } else {
         dim3 grid((unsigned)((cols + 64 * GMX_G - 1) / (64 * GMX_G)), (a.M + 16 * GMX_RT - 1) / (16 * GMX_RT), a.ngroup * ks);
-        hipLaunchKernelGGL((k_gemm_mx<GMX_RT, GMX_G>), grid, dim3(256), 0, s, a, ks);
+        // B values 2 steps ahead (4M points / 16 RHS, mixed: up 2.35 -> 2.2 ms,
+        // down 1.89 -> 1.76 ms; 3 ahead: 3.3 and 2.6 ms); KIFMM3D_GEMM_PB = 1, 2, 3
+        static const int pb = [] {
+          const char* v = std::getenv("KIFMM3D_GEMM_PB");
+          return v ? std::atoi(v) : 2;
+        }();
+        if (pb == 2)
+          hipLaunchKernelGGL((k_gemm_mx<GMX_RT, GMX_G, 2>), grid, dim3(256), 0, s, a, ks);
+        else if (pb == 3)
+          hipLaunchKernelGGL((k_gemm_mx<GMX_RT, GMX_G, 3>), grid, dim3(256), 0, s, a, ks);
+        else
+          hipLaunchKernelGGL((k_gemm_mx<GMX_RT, GMX_G>), grid, dim3(256), 0, s, a, ks);
       }
       if (ks > 1) {
         const long n = a.ctot * a.M;
