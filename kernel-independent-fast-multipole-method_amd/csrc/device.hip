@@ -2299,6 +2299,44 @@ __device__ inline void load_twiddles_t(A2 (*s_wt)[2 * NP], const double2* tw, in
 }
 
 
+// Half-length DFT stages.  The twiddle of input i (placed at i + NP/2 on the
+// grid of N = 2 NP points) and output m + NP is the one of output m times
+// (-1)^(i + NP/2), since w(t + N/2) = -w(t).  So the N outputs of NP inputs are
+// P + Q at m and P - Q at m + NP, with P (Q) the sum over the inputs of even
+// (odd) i + NP/2 of the twiddled values at m: NP^2 complex products instead of
+// 2 NP^2.  The adjoint stage folds its N inputs the same way first.
+template <class A2>
+__device__ __forceinline__ void fft_half_mac(A2 x, A2 w, bool odd, A2& sp, A2& sn)
+{
+  const A2 v = cmul2(x, w);
+  if (odd) {
+    sn.x += v.x;
+    sn.y += v.y;
+  } else {
+    sp.x += v.x;
+    sp.y += v.y;
+  }
+}
+template <class A2, class LT>
+__device__ __forceinline__ void fft_half_out(A2 sp, A2 sn, LT& lo, LT& hi)
+{
+  A2 a, b;
+  a.x = sp.x + sn.x;
+  a.y = sp.y + sn.y;
+  b.x = sp.x - sn.x;
+  b.y = sp.y - sn.y;
+  cset(lo, a);
+  cset(hi, b);
+}
+template <class A2>
+__device__ __forceinline__ void fft_half_in(A2 lo, A2 hi, A2& s, A2& d)
+{
+  s.x = lo.x + hi.x;
+  s.y = lo.y + hi.y;
+  d.x = lo.x - hi.x;
+  d.y = lo.y - hi.y;
+}
+
 template <int NP, class CT>
 __global__ __launch_bounds__(256) void k_fft_fwd(const int* __restrict__ slot_node, int Rc,
                                                  const double* __restrict__ upeden, int pln, int sm,
@@ -2505,15 +2543,11 @@ __global__ __launch_bounds__(64 * R) void k_fft_fwd_ri(int nslot, const int* __r
 #pragma unroll
     for (int j = 0; j < NP; j++) x[j] = cget<A2>(X1[rr][(k * NP + j) * H + kx]);
 #pragma unroll
-    for (int ky = 0; ky < N; ky++) {
-      A2 acc = cz<A2>();
+    for (int ky = 0; ky < NP; ky++) {  // outputs ky and ky + NP together
+      A2 sp = cz<A2>(), sn = cz<A2>();
 #pragma unroll
-      for (int j = 0; j < NP; j++) {
-        const A2 v = cmul2(x[j], s_wt[j][ky]);
-        acc.x += v.x;
-        acc.y += v.y;
-      }
-      cset(X2[rr][(k * N + ky) * H + kx], acc);
+      for (int j = 0; j < NP; j++) fft_half_mac(x[j], s_wt[j][ky], ((j + NP / 2) & 1) != 0, sp, sn);
+      fft_half_out(sp, sn, X2[rr][(k * N + ky) * H + kx], X2[rr][(k * N + ky + NP) * H + kx]);
     }
   }
   __syncthreads();
@@ -2525,15 +2559,11 @@ __global__ __launch_bounds__(64 * R) void k_fft_fwd_ri(int nslot, const int* __r
 #pragma unroll
     for (int k = 0; k < NP; k++) x[k] = cget<A2>(X2[rr][k * N * H + col]);
 #pragma unroll
-    for (int kz = 0; kz < N; kz++) {
-      A2 acc = cz<A2>();
+    for (int kz = 0; kz < NP; kz++) {  // outputs kz and kz + NP together
+      A2 sp = cz<A2>(), sn = cz<A2>();
 #pragma unroll
-      for (int k = 0; k < NP; k++) {
-        const A2 v = cmul2(x[k], s_wt[k][kz]);
-        acc.x += v.x;
-        acc.y += v.y;
-      }
-      cset(out[(long)(kz * N * H + col) * Rc + rr], acc);
+      for (int k = 0; k < NP; k++) fft_half_mac(x[k], s_wt[k][kz], ((k + NP / 2) & 1) != 0, sp, sn);
+      fft_half_out(sp, sn, out[(long)(kz * N * H + col) * Rc + rr], out[(long)((kz + NP) * N * H + col) * Rc + rr]);
     }
   }
 }
@@ -2560,14 +2590,17 @@ __global__ __launch_bounds__(64 * R) void k_fft_inv_ri(int nslot, const int* __r
   for (int e = t; e < R * N * H; e += BS) {  // along z: line (ky, kx), spectra read once from HBM
     asm volatile("" ::: "memory");  // keep the twiddle loads inside the loop (registers)
     const int rr = e % R, col = e / R;
-    A2 x[N];
+    A2 xs[NP], xd[NP];  // x[kz] + x[kz + NP], x[kz] - x[kz + NP]
 #pragma unroll
-    for (int kz = 0; kz < N; kz++) x[kz] = cget<A2>(in[(long)(kz * N * H + col) * Rc + rr]);
+    for (int kz = 0; kz < NP; kz++)
+      fft_half_in(cget<A2>(in[(long)(kz * N * H + col) * Rc + rr]),
+                  cget<A2>(in[(long)((kz + NP) * N * H + col) * Rc + rr]), xs[kz], xd[kz]);
 #pragma unroll
     for (int k = 0; k < NP; k++) {
+      const A2* x = ((k + NP / 2) & 1) ? xd : xs;
       A2 acc = cz<A2>();
 #pragma unroll
-      for (int kz = 0; kz < N; kz++) {
+      for (int kz = 0; kz < NP; kz++) {
         const A2 v = cmulc(x[kz], s_wt[k][kz]);
         acc.x += v.x;
         acc.y += v.y;
@@ -2579,14 +2612,17 @@ __global__ __launch_bounds__(64 * R) void k_fft_inv_ri(int nslot, const int* __r
   for (int e = t; e < R * NP * H; e += BS) {  // along y: line (k, kx)
     asm volatile("" ::: "memory");  // keep the twiddle loads inside the loop (registers)
     const int rr = e % R, col = e / R, k = col / H, kx = col - k * H;
-    A2 y[N];
+    A2 ys[NP], yd[NP];  // y[ky] + y[ky + NP], y[ky] - y[ky + NP]
 #pragma unroll
-    for (int ky = 0; ky < N; ky++) y[ky] = cget<A2>(Y1[rr][(k * N + ky) * H + kx]);
+    for (int ky = 0; ky < NP; ky++)
+      fft_half_in(cget<A2>(Y1[rr][(k * N + ky) * H + kx]), cget<A2>(Y1[rr][(k * N + ky + NP) * H + kx]), ys[ky],
+                  yd[ky]);
 #pragma unroll
     for (int j = 0; j < NP; j++) {
+      const A2* y = ((j + NP / 2) & 1) ? yd : ys;
       A2 acc = cz<A2>();
 #pragma unroll
-      for (int ky = 0; ky < N; ky++) {
+      for (int ky = 0; ky < NP; ky++) {
         const A2 v = cmulc(y[ky], s_wt[j][ky]);
         acc.x += v.x;
         acc.y += v.y;
