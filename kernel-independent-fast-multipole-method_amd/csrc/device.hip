@@ -1406,24 +1406,29 @@ __global__ __launch_bounds__(P2P_BS) void k_p2s_mfma111(int nitem, int npc, cons
 // cores, coordinates relative to the item's box centre rounded to FP32 (the
 // surface points exactly, c + r u - c = r u), FP32 products of one source
 // tile summed in FP64.  Check surfaces never touch the item's sources, so
-// 1/r is taken of max(r^2, 1e-16) without the exclusion test.
-__global__ __launch_bounds__(P2P_BS) void k_p2s_mfma111_f32(int nitem, int npc, const int* __restrict__ out_node,
-                                                            const double4* __restrict__ geom,
-                                                            const int* __restrict__ eptr, const int2* __restrict__ ent,
-                                                            const double* __restrict__ unit, int npt, Points src,
-                                                            double* __restrict__ out, int Rc, int pln, int roff, KC kc,
-                                                            int accum, int nrv)
+// 1/r is taken of max(r^2, 1e-16) without the exclusion test.  A block of NW
+// waves takes 64 NW surface points of an item (all of them for np <= 8 when
+// NW = ceil(npt / 64)), so each source tile is staged once per item; FP32
+// density rows of 16 right-hand sides are staged as four 16-byte loads.
+template <int NW>
+__global__ __launch_bounds__(64 * NW) void k_p2s_mfma111_f32(int nitem, int npc, const int* __restrict__ out_node,
+                                                             const double4* __restrict__ geom,
+                                                             const int* __restrict__ eptr,
+                                                             const int2* __restrict__ ent,
+                                                             const double* __restrict__ unit, int npt, Points src,
+                                                             double* __restrict__ out, int Rc, int pln, int roff,
+                                                             KC kc, int accum, int nrv)
 {
-  constexpr int NR = 16;
+  constexpr int NR = 16, BS = 64 * NW;
   __shared__ float s_pos[3][TILE32];
-  __shared__ float s_den[TILE32][NR];
+  __shared__ __align__(16) float s_den[TILE32][NR];
   const int nb = nitem * npc, b = xcd_item(blockIdx.x, nb);
   if (b >= nb) return;  // block-uniform
   const int it = b / npc, pc = b - it * npc;
   const int t = threadIdx.x, w = t >> 6, l = t & 63, col = l & 15, kk = l >> 4;
   const int r0 = roff + blockIdx.y * NR;
-  const int pb = pc * P2P_BS, tn = min(P2P_BS, npt - pb);  // surface points of this block
-  const bool wact = 64 * w < tn;                           // wave-uniform
+  const int pb = pc * BS, tn = min(BS, npt - pb);  // surface points of this block
+  const bool wact = 64 * w < tn;                    // wave-uniform
   const int ng = wact ? min(4, (tn - 64 * w + 15) >> 4) : 0;
   const double4 g = geom[it];
   float tx[4], ty[4], tz[4];
@@ -1442,22 +1447,43 @@ __global__ __launch_bounds__(P2P_BS) void k_p2s_mfma111_f32(int nitem, int npc, 
     for (int v = 0; v < 4; v++) acc[i][v] = 0.0;
   const double* den = src.den + (long)r0 * src.den_rs;
   const float* den32 = src.den32 ? src.den32 + (long)r0 * src.den_rs : nullptr;
+  const bool vec = den32 && nrv == NR && src.den_rs == 1 && src.den_js % 4 == 0 &&
+                   ((unsigned long)den32 & 15) == 0;
   for (int e = eptr[it]; e < eptr[it + 1]; e++) {
     const int2 se = ent[e];
     for (int b0 = 0; b0 < se.y; b0 += TILE32) {
       const int cnt = min(TILE32, se.y - b0), cnt4 = (cnt + 3) & ~3;
       const int j0 = se.x + b0;
       __syncthreads();
-      for (int i = t; i < cnt4; i += P2P_BS) {
+      for (int i = t; i < cnt4; i += BS) {
         const bool in = i < cnt;
         s_pos[0][i] = in ? (float)(src.x[j0 + i] - g.x) : 0.0f;
         s_pos[1][i] = in ? (float)(src.y[j0 + i] - g.y) : 0.0f;
         s_pos[2][i] = in ? (float)(src.z[j0 + i] - g.z) : 0.0f;
       }
-      for (int i = t; i < cnt4 * NR; i += P2P_BS) {
-        const int jj = i / NR, q = i - jj * NR;
-        const long at = (long)(j0 + jj) * src.den_js + (long)q * src.den_rs;
-        s_den[jj][q] = jj < cnt && q < nrv ? (den32 ? den32[at] : (float)den[at]) : 0.0f;
+      if (vec) {
+        for (int jj = t; jj < cnt4; jj += BS) {
+          // named registers (an array here goes to scratch memory)
+          float4 q0 = make_float4(0.f, 0.f, 0.f, 0.f), q1 = q0, q2 = q0, q3 = q0;
+          if (jj < cnt) {
+            const float4* row = reinterpret_cast<const float4*>(den32 + (long)(j0 + jj) * src.den_js);
+            q0 = row[0];
+            q1 = row[1];
+            q2 = row[2];
+            q3 = row[3];
+          }
+          float4* dst = reinterpret_cast<float4*>(s_den[jj]);
+          dst[0] = q0;
+          dst[1] = q1;
+          dst[2] = q2;
+          dst[3] = q3;
+        }
+      } else {
+        for (int i = t; i < cnt4 * NR; i += BS) {
+          const int jj = i / NR, q = i - jj * NR;
+          const long at = (long)(j0 + jj) * src.den_js + (long)q * src.den_rs;
+          s_den[jj][q] = jj < cnt && q < nrv ? (den32 ? den32[at] : (float)den[at]) : 0.0f;
+        }
       }
       __syncthreads();
       if (ng == 4)
@@ -1483,6 +1509,31 @@ __global__ __launch_bounds__(P2P_BS) void k_p2s_mfma111_f32(int nitem, int npc, 
   }
 }
 
+// Launch of k_p2s_mfma111_f32 with NW = ceil(npt / 64) waves per block (at
+// most 8; more points than 512 take several blocks per item).
+static void p2s_f32_launch(hipStream_t s, int nb, int nitem, const int* out_node, const double4* geom,
+                           const int* eptr, const int2* ent, const double* unit, int npt, Points src, double* out,
+                           int Rc, int pln, int roff, KC kc, int accum, int nrv)
+{
+  const int nw = std::min(8, std::max(2, (npt + 63) / 64));
+#define KIFMM_P2S32(NW)                                                                                         \
+  do {                                                                                                          \
+    const int npc = (npt + 64 * NW - 1) / (64 * NW);                                                            \
+    hipLaunchKernelGGL(k_p2s_mfma111_f32<NW>, dim3(xcd_grid(nitem * npc), nb), dim3(64 * NW), 0, s, nitem, npc, \
+                       out_node, geom, eptr, ent, unit, npt, src, out, Rc, pln, roff, kc, accum, nrv);          \
+  } while (0)
+  switch (nw) {
+  case 2: KIFMM_P2S32(2); break;
+  case 3: KIFMM_P2S32(3); break;
+  case 4: KIFMM_P2S32(4); break;
+  case 5: KIFMM_P2S32(5); break;
+  case 6: KIFMM_P2S32(6); break;
+  case 7: KIFMM_P2S32(7); break;
+  default: KIFMM_P2S32(8); break;
+  }
+#undef KIFMM_P2S32
+}
+
 template <int KT>
 static void p2s_kt(hipStream_t s, int nitem, const int* out_node, const double4* geom, const int* eptr,
                    const int2* ent, const double* unit, int npt, Points src, double* out, int Rc, int pln, int nr,
@@ -1490,10 +1541,8 @@ static void p2s_kt(hipStream_t s, int nitem, const int* out_node, const double4*
 {
   int r0 = 0;
   if (KT == 111 && variant == 3) {
-    const int npc = (npt + P2P_BS - 1) / P2P_BS;
     r0 = nr - rhs_blocks16(nr, [&](int nb, int roff, int nrv) {
-      hipLaunchKernelGGL(k_p2s_mfma111_f32, dim3(xcd_grid(nitem * npc), nb), dim3(P2P_BS), 0, s, nitem, npc,
-                         out_node, geom, eptr, ent, unit, npt, src, out, Rc, pln, roff, kc, accum, nrv);
+      p2s_f32_launch(s, nb, nitem, out_node, geom, eptr, ent, unit, npt, src, out, Rc, pln, roff, kc, accum, nrv);
     });
   }
   rhs_blocks(nr - r0, [&](int NR, int nb, int roff0) {
