@@ -696,7 +696,10 @@ __device__ __forceinline__ void p2p_tile64(int cnt4, int kk, int col, const doub
   }
 }
 
-constexpr int TILE32 = 128;  // sources per staged tile of the FP32 kernel (measured 2 % faster than 64)
+// sources per staged tile of the FP32 kernels: leaves of up to 160 points in
+// one tile (4M points / 16 RHS, against 128: near field 7.69 -> 7.55 ms, S2M
+// 0.92 -> 0.85 ms; 128 was 2 % faster than 64)
+constexpr int TILE32 = 160;
 
 // The reference's r < 1e-8 rule (kernel3d.cpp:21, :132) by entry, as the
 // set-up flagged it (echk):
@@ -709,8 +712,8 @@ constexpr int TILE32 = 128;  // sources per staged tile of the FP32 kernel (meas
 // (Flag 2 -- a pair can be closer than the FP32 offsets resolve: leaves with
 // points next to a shared boundary, or a leaf holding two distinct
 // near-coincident points -- is left to an FP64 pass over those entries.)
-template <int NG, int CHK>
-__device__ __forceinline__ void p2p_tile32(int cnt4, int kk, int col, const float (*s_pos)[TILE32],
+template <int NG, int CHK, int TW = TILE32>
+__device__ __forceinline__ void p2p_tile32(int cnt4, int kk, int col, const float (*s_pos)[TW],
                                            const float (*s_den)[16], const float* tx, const float* ty,
                                            const float* tz, double (*acc)[4])
 {
@@ -1577,6 +1580,8 @@ void p2s(hipStream_t s, int kt, int nitem, const int* out_node, const double4* g
   }
 }
 
+constexpr int S2P_TILE32 = 160;
+
 // L2T + M2T for the Laplace mm kernel and 16-RHS blocks on the matrix cores:
 // the wave layout of k_p2p_mfma111 (FP64) and k_p2p_mfma111_f32 (FP32, mixed
 // form) with the surface points of each entry as the sources -- positions
@@ -1594,7 +1599,8 @@ __global__ __launch_bounds__(P2P_BS) void k_s2p_mfma111(int nchunk, const int* _
                                                         Targets trg, int roff, KC kc, int nrv)
 {
   typedef typename std::conditional<F32, float, double>::type R;
-  constexpr int NR = 16, TL = F32 ? TILE32 : TILE;
+  // FP32: 160 surface points per staged tile, all of an np 6 surface (152) in one
+  constexpr int NR = 16, TL = F32 ? S2P_TILE32 : TILE;
   __shared__ R s_pos[3][TL];
   __shared__ R s_den[TL][NR];
   const int c = xcd_item(blockIdx.x, nchunk), t = threadIdx.x, w = t >> 6, l = t & 63;
@@ -1645,16 +1651,16 @@ __global__ __launch_bounds__(P2P_BS) void k_s2p_mfma111(int nchunk, const int* _
       __syncthreads();
       if constexpr (F32) {
         if (ng == 4)
-          p2p_tile32<4, 1>(cnt4, kk, col, (const float(*)[TILE32])s_pos, (const float(*)[16])s_den, (const float*)tx,
+          p2p_tile32<4, 1, TL>(cnt4, kk, col, (const float(*)[TL])s_pos, (const float(*)[16])s_den, (const float*)tx,
                         (const float*)ty, (const float*)tz, acc64);
         else if (ng == 3)
-          p2p_tile32<3, 1>(cnt4, kk, col, (const float(*)[TILE32])s_pos, (const float(*)[16])s_den, (const float*)tx,
+          p2p_tile32<3, 1, TL>(cnt4, kk, col, (const float(*)[TL])s_pos, (const float(*)[16])s_den, (const float*)tx,
                         (const float*)ty, (const float*)tz, acc64);
         else if (ng == 2)
-          p2p_tile32<2, 1>(cnt4, kk, col, (const float(*)[TILE32])s_pos, (const float(*)[16])s_den, (const float*)tx,
+          p2p_tile32<2, 1, TL>(cnt4, kk, col, (const float(*)[TL])s_pos, (const float(*)[16])s_den, (const float*)tx,
                         (const float*)ty, (const float*)tz, acc64);
         else if (ng == 1)
-          p2p_tile32<1, 1>(cnt4, kk, col, (const float(*)[TILE32])s_pos, (const float(*)[16])s_den, (const float*)tx,
+          p2p_tile32<1, 1, TL>(cnt4, kk, col, (const float(*)[TL])s_pos, (const float(*)[16])s_den, (const float*)tx,
                         (const float*)ty, (const float*)tz, acc64);
       } else {
         if (ng == 4)
