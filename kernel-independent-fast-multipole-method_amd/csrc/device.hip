@@ -463,9 +463,10 @@ __device__ inline int xcd_item(int b, int n)
 static inline unsigned xcd_grid(int n) { return 8u * (unsigned)((n + 7) / 8); }
 
 // Densities of sources j0 .. j0+cnt for NR right-hand sides into LDS,
-// s_den[jj][q*S + d]; den points at right-hand side r0 of the block.
-template <int S, int NR, int BS, class E>
-__device__ inline void stage_den(E (*s_den)[NR * S], const double* den, const Points& src, int j0, int cnt, int t)
+// s_den[jj][q*S + d]; den points at right-hand side r0 of the block (FP64,
+// or the FP32 copy).
+template <int S, int NR, int BS, class E, class D>
+__device__ inline void stage_den(E (*s_den)[NR * S], const D* den, const Points& src, int j0, int cnt, int t)
 {
   constexpr int ND = NR * S;
   for (int i = t; i < cnt * ND; i += BS) {
@@ -500,7 +501,10 @@ __global__ __launch_bounds__(P2P_BS) void k_p2p(const int* __restrict__ tbeg, co
   for (int q = 0; q < NR; q++)
 #pragma unroll
     for (int d = 0; d < K::T; d++) u[q][d] = 0.0;
-  const double* den = src.den + (long)r0 * src.den_rs;
+  // FP64 densities, or (src.den null: the mixed form's pass over the entries
+  // its FP32 kernels skip) the FP32 copy
+  const double* den = src.den ? src.den + (long)r0 * src.den_rs : nullptr;
+  const float* den32 = src.den ? nullptr : src.den32 + (long)r0 * src.den_rs;
   for (int e = eptr[c]; e < eptr[c + 1]; e++) {
     const int2 se = ent[e];
     for (int b0 = 0; b0 < se.y; b0 += TILE) {
@@ -517,7 +521,10 @@ __global__ __launch_bounds__(P2P_BS) void k_p2p(const int* __restrict__ tbeg, co
           s_pos[i][NPOS - 1] = src.nz[j0 + i];
         }
       }
-      stage_den<K::S, NR, P2P_BS>(s_den, den, src, j0, cnt, t);
+      if (den)
+        stage_den<K::S, NR, P2P_BS>(s_den, den, src, j0, cnt, t);
+      else
+        stage_den<K::S, NR, P2P_BS>(s_den, den32, src, j0, cnt, t);
       __syncthreads();
       if (act)
         for (int jj = 0; jj < cnt; jj++)

@@ -1307,9 +1307,9 @@ void Engine::run_chunk(const double* srcden, double* trgval, int rc, int mode, h
   // (set-up: a pair may be closer than the FP32 offsets resolve); the FP64
   // LDS-broadcast kernel adds those entries after them
   const bool near_fix = mixed && (nvar == 3 || nvar == 4) && P.near_fix.n > 0;
-  double* den64 = den32 && s2m32_ && nvar == 3 && (rc % 16 == 0 || rc % 16 >= dev::PAD16_MIN) && !near_fix
-                      ? nullptr
-                      : sden_.p;
+  // (the FP64 pass over the flagged entries reads the FP32 copy when there is
+  // no FP64 one: its positions and sums are FP64)
+  double* den64 = den32 && s2m32_ && nvar == 3 && (rc % 16 == 0 || rc % 16 >= dev::PAD16_MIN) ? nullptr : sden_.p;
   dev::Targets trg{tx_.p, ty_.p, tz_.p, tval_.p, trs};
 
   auto run_gemm = [&](const GemmOp& op) {
@@ -1394,8 +1394,10 @@ void Engine::run_chunk(const double* srcden, double* trgval, int rc, int mode, h
                              I + P.near.eptr, (const int2*)(I + P.near.ent), I + P.near.chk, src, trg, rc, kcu, nvar);
     if (near_fix && n32 > 0) {
       const int* F = fix_.p;
+      dev::Points sfix = src;
+      sfix.den = den64;  // null when only the FP32 copy was gathered
       dev::p2p(side ? side_ : s, user_.kt, P.near_fix.n, F + P.near_fix.tbeg, F + P.near_fix.tnum,
-               F + P.near_fix.eptr, (const int2*)(F + P.near_fix.ent), nullptr, src, trg, n32, kcu, 0);
+               F + P.near_fix.eptr, (const int2*)(F + P.near_fix.ent), nullptr, sfix, trg, n32, kcu, 0);
     }
   });
   if (side) HIPCHK(hipEventRecord(join_ev_, side_));
