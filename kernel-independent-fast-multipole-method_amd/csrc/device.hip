@@ -3288,19 +3288,34 @@ __device__ __forceinline__ void m2lw_each(std::integer_sequence<int, H...>, F&& 
   (f(std::integral_constant<int, H>{}), ...);
 }
 
-// OFF32: the level's spectra and its zero slot span less than 4 GiB, so a
-// spectrum address is a 32-bit byte offset from the frequency's row of slot 0
-// (one VALU instruction per load and the SGPR-based load form); otherwise a
-// 64-bit shift-add.
-template <int D, int TPW, int OCC, int ORD, bool OFF32>
+// R: float (the mixed form; FP32 accumulation over the 26 directions) or
+// double (the plain form, FP64 matrix cores).  OFF32: the level's spectra and
+// its zero slot span less than 4 GiB, so a spectrum address is a 32-bit byte
+// offset from the frequency's row of slot 0 (one VALU instruction per load
+// and the SGPR-based load form); otherwise a 64-bit shift-add.
+template <class R> struct Mw;
+template <> struct Mw<float> {
+  typedef float4 A4;
+  __device__ static inline A4 make(float a, float b, float c, float d) { return make_float4(a, b, c, d); }
+};
+template <> struct Mw<double> {
+  typedef double4 A4;
+  __device__ static inline A4 make(double a, double b, double c, double d) { return make_double4(a, b, c, d); }
+};
+
+template <class R, int D, int TPW, int OCC, int ORD, bool OFF32>
 __global__ __launch_bounds__(M2LW_BS, OCC) void k_m2l_wave(int C, int Rc, int ntile, int ngrp, int zslot,
                                                         const int* __restrict__ ttile, const int* __restrict__ htile,
-                                                        const float2* __restrict__ atab,
-                                                        const float2* __restrict__ effden, float2* __restrict__ effval)
+                                                        const typename Mt<R>::C2* __restrict__ atab,
+                                                        const typename Mt<R>::C2* __restrict__ effden,
+                                                        typename Mt<R>::C2* __restrict__ effval)
 {
+  using C2 = typename Mt<R>::C2;
+  using V4 = typename Mt<R>::V4;
+  using A4 = typename Mw<R>::A4;
   // the A operands of frequency f, [direction][lane] as the four values each
-  // lane feeds: one 16-byte LDS read per (halo parent, target parent)
-  __shared__ float4 s_A4[26][64];
+  // lane feeds: one LDS read per (halo parent, target parent)
+  __shared__ A4 s_A4[26][64];
   // XCD b % 8 takes the frequencies f = b % 8 (mod 8) one after another and,
   // for each, every tile group of the level: the blocks resident on one XCD
   // work on one or two frequencies, whose spectra (every slot of the level at
@@ -3312,23 +3327,26 @@ __global__ __launch_bounds__(M2LW_BS, OCC) void k_m2l_wave(int C, int Rc, int nt
   const int t = threadIdx.x, w = t >> 6, l = t & 63;
   const int i = l & 15, k = l >> 4;
   // Lane (i, k) feeds row i of A, which holds output (child 2 kr + vr / 2,
-  // part vr % 2) (the FP32 accumulator item v of lane (i, k) is row 4k + v),
-  // in K slices 0 .. 3: with (m.x, m.y) the operator of source child k and
-  // (m.z, m.w) that of source child 4 + k, slice 0 meets the real part of
-  // child k's spectrum, 1 its imaginary part, 2 and 3 those of child 4 + k.
+  // part vr % 2) (accumulator item v of lane (i, k) is row 4k + v for the FP32
+  // MFMA, k + 4v for the FP64 one), in K slices 0 .. 3: with (m.x, m.y) the
+  // operator of source child k and (m.z, m.w) that of source child 4 + k,
+  // slice 0 meets the real part of child k's spectrum, 1 its imaginary part,
+  // 2 and 3 those of child 4 + k.
   for (int e = t; e < 26 * 64; e += M2LW_BS) {
-    const int n = e >> 6, ln = e & 63, ii = ln & 15, kk = ln >> 4, kr = ii >> 2, vr = ii & 3;
+    const int n = e >> 6, ln = e & 63, ii = ln & 15, kk = ln >> 4;
+    const int kr = Mt<R>::KV ? ii & 3 : ii >> 2, vr = Mt<R>::KV ? ii >> 2 : ii & 3;
     const int ca = 2 * kr + (vr >> 1), pim = vr & 1;
-    const float4 m = *reinterpret_cast<const float4*>(atab + ((long)f * NAENT + n * 32 + kk * 8 + ca) * 2);
-    s_A4[n][ln] = pim ? make_float4(m.y, m.x, m.w, m.z) : make_float4(m.x, -m.y, m.z, -m.w);
+    const A4 m = *reinterpret_cast<const A4*>(atab + ((long)f * NAENT + n * 32 + kk * 8 + ca) * 2);
+    s_A4[n][ln] = pim ? Mw<R>::make(m.y, m.x, m.w, m.z) : Mw<R>::make(m.x, -m.y, m.z, -m.w);
   }
   __syncthreads();
   // the spectrum of slot s at f for right-hand side r0 + i is eb0[16 o + i]
   // with o = s * C * Rc / 16 (< 2^31, checked by the launcher); absent halo
   // children read slot zslot, which holds zeros, so every load is
   // unconditional and the loads in flight are counted exactly
-  const float2* eb0 = effden + (long)f * Rc + r0;
-  const unsigned i8 = 8u * i;
+  const C2* eb0 = effden + (long)f * Rc + r0;
+  constexpr int SH = sizeof(C2) == 8 ? 7 : 8;  // bytes of 16 spectra, log2
+  const unsigned ib = (unsigned)sizeof(C2) * i;
   const int cr16 = C * (Rc >> 4);
   for (int tt = 0; tt < TPW; tt++) {
     const int tile = (g * 4 + w) * TPW + tt;
@@ -3354,20 +3372,20 @@ __global__ __launch_bounds__(M2LW_BS, OCC) void k_m2l_wave(int C, int Rc, int nt
       o[1] = __shfl(hv[h >> 3], 8 * (h & 7) + 4 + k);
     };
     // their spectra for the lane's right-hand side
-    auto load = [&](const int* o, float2* x) {
+    auto load = [&](const int* o, C2* x) {
       if constexpr (OFF32) {
         const char* e8 = reinterpret_cast<const char*>(eb0);
-        x[0] = *reinterpret_cast<const float2*>(e8 + (((unsigned)o[0] << 7) + i8));
-        x[1] = *reinterpret_cast<const float2*>(e8 + (((unsigned)o[1] << 7) + i8));
+        x[0] = *reinterpret_cast<const C2*>(e8 + (((unsigned)o[0] << SH) + ib));
+        x[1] = *reinterpret_cast<const C2*>(e8 + (((unsigned)o[1] << SH) + ib));
       } else {
         x[0] = eb0[((size_t)(unsigned)o[0] << 4) + i];
         x[1] = eb0[((size_t)(unsigned)o[1] << 4) + i];
       }
     };
-    v4f acc[8];
+    V4 acc[8];
 #pragma unroll
-    for (int p = 0; p < 8; p++) acc[p] = v4f{0.f, 0.f, 0.f, 0.f};
-    float2 ring[D][2];  // the spectra of the next D halo parents, in flight
+    for (int p = 0; p < 8; p++) acc[p] = V4{0, 0, 0, 0};
+    C2 ring[D][2];  // the spectra of the next D halo parents, in flight
     int on[2];          // the offsets of the halo parent after those
 #pragma unroll
     for (int d = 0; d < D; d++) {
@@ -3377,7 +3395,7 @@ __global__ __launch_bounds__(M2LW_BS, OCC) void k_m2l_wave(int C, int Rc, int nt
     offs(D, on);
     m2lw_each(std::make_integer_sequence<int, 64>{}, [&](auto hc) {
       constexpr int h = decltype(hc)::value;
-      const float2 x0 = ring[h % D][0], x1 = ring[h % D][1];
+      const C2 x0 = ring[h % D][0], x1 = ring[h % D][1];
       if constexpr (h + D < 64) {
         load(on, ring[h % D]);
         if constexpr (h + D + 1 < 64) offs(h + D + 1, on);
@@ -3386,16 +3404,16 @@ __global__ __launch_bounds__(M2LW_BS, OCC) void k_m2l_wave(int C, int Rc, int nt
         // the target parents that have halo parent h as a neighbour; K
         // slices in turn over them (independent accumulators between
         // dependent products)
-        float4 av[8];
+        A4 av[8];
 #pragma unroll
         for (int p = 0; p < 8; p++) {
           const int n = m2lw_dir(h, p);
           if (n >= 0) av[p] = s_A4[n][l];
         }
         auto mf = [&](int ks, int p) {
-          const float xb = ks == 0 ? x0.x : ks == 1 ? x0.y : ks == 2 ? x1.x : x1.y;
-          const float ab = ks == 0 ? av[p].x : ks == 1 ? av[p].y : ks == 2 ? av[p].z : av[p].w;
-          acc[p] = __builtin_amdgcn_mfma_f32_16x16x4f32(ab, xb, acc[p], 0, 0, 0);
+          const R xb = ks == 0 ? x0.x : ks == 1 ? x0.y : ks == 2 ? x1.x : x1.y;
+          const R ab = ks == 0 ? av[p].x : ks == 1 ? av[p].y : ks == 2 ? av[p].z : av[p].w;
+          acc[p] = Mt<R>::mfma(ab, xb, acc[p]);
         };
         if constexpr (ORD == 0) {
 #pragma unroll
@@ -3418,17 +3436,22 @@ __global__ __launch_bounds__(M2LW_BS, OCC) void k_m2l_wave(int C, int Rc, int nt
 #pragma unroll
       for (int h2 = 0; h2 < 2; h2++) {
         const int sl = __shfl(tv, p * 8 + 2 * k + h2);
-        if (sl >= 0) effval[((long)sl * C + f) * Rc + r0 + i] = make_float2(acc[p][2 * h2], acc[p][2 * h2 + 1]);
+        if (sl >= 0) {
+          C2 v;
+          v.x = acc[p][2 * h2];
+          v.y = acc[p][2 * h2 + 1];
+          effval[((long)sl * C + f) * Rc + r0 + i] = v;
+        }
       }
   }
 }
 
 void m2l_wave(hipStream_t s, int C, int ntile, int Rc, int nslot, const int* ttile, const int* htile,
-              const void* atab32, const void* effden, void* effval, int depth)
+              const void* atab, const void* effden, void* effval, int depth, int single)
 {
   if (ntile <= 0 || Rc < 16) return;
   if ((long)(nslot + 1) * C * (Rc / 16) > INT_MAX) {  // halo offsets beyond 32 bits: the block-per-tile kernel
-    m2l_tiled(s, C, ntile, Rc, ttile, htile, atab32, effden, effval, 1);
+    m2l_tiled(s, C, ntile, Rc, ttile, htile, atab, effden, effval, single);
     return;
   }
   // Measured at 4M points / 16 RHS, levels 4-5 (KIFMM3D_M2L_WAVE = the four
@@ -3451,16 +3474,37 @@ void m2l_wave(hipStream_t s, int C, int ntile, int Rc, int nslot, const int* tti
   const int ngrp = (ntile + 4 * tpw - 1) / (4 * tpw);
   dim3 grid((unsigned)(8 * ngrp * ((C + 7) / 8)), Rc / 16);
   // slot nslot holds zeros (the caller's); byte offsets below 4 GiB: 32-bit
-  const bool off32 = (unsigned long long)(nslot + 1) * C * Rc * sizeof(float2) <= 0xffffffffull;
-#define KIFMM_M2LW(DD, TT, OO, RR)                                                                             \
-  do {                                                                                                         \
-    if (off32)                                                                                                 \
-      hipLaunchKernelGGL((k_m2l_wave<DD, TT, OO, RR, true>), grid, dim3(M2LW_BS), 0, s, C, Rc, ntile, ngrp,   \
-                         nslot, ttile, htile, (const float2*)atab32, (const float2*)effden, (float2*)effval);  \
-    else                                                                                                       \
-      hipLaunchKernelGGL((k_m2l_wave<DD, TT, OO, RR, false>), grid, dim3(M2LW_BS), 0, s, C, Rc, ntile, ngrp,  \
-                         nslot, ttile, htile, (const float2*)atab32, (const float2*)effden, (float2*)effval);  \
+  const size_t esz = single ? sizeof(float2) : sizeof(double2);
+  const bool off32 = (unsigned long long)(nslot + 1) * C * Rc * esz <= 0xffffffffull;
+#define KIFMM_M2LW_R(R, C2, DD, TT, OO, RR)                                                                     \
+  do {                                                                                                          \
+    if (off32)                                                                                                  \
+      hipLaunchKernelGGL((k_m2l_wave<R, DD, TT, OO, RR, true>), grid, dim3(M2LW_BS), 0, s, C, Rc, ntile, ngrp,  \
+                         nslot, ttile, htile, (const C2*)atab, (const C2*)effden, (C2*)effval);                 \
+    else                                                                                                        \
+      hipLaunchKernelGGL((k_m2l_wave<R, DD, TT, OO, RR, false>), grid, dim3(M2LW_BS), 0, s, C, Rc, ntile, ngrp, \
+                         nslot, ttile, htile, (const C2*)atab, (const C2*)effden, (C2*)effval);                 \
   } while (0)
+#define KIFMM_M2LW(DD, TT, OO, RR) KIFMM_M2LW_R(float, float2, DD, TT, OO, RR)
+  // FP64 (the plain form): 53 KB of A operands per block, 3 blocks (waves per
+  // SIMD) per CU.  4M points / 16 RHS, levels 4-5: M2L 21.1 -> 13.9 ms against
+  // the block-per-tile kernel; 2 blocks per CU (KIFMM3D_M2L_WAVE64=2) 14.5 ms,
+  // 3 halo parents ahead (=3) the same as 4.
+  if (!single) {
+    static const int v64 = [] {
+      const char* v = std::getenv("KIFMM3D_M2L_WAVE64");
+      return v ? std::atoi(v) : 0;
+    }();
+    if (depth <= 1)
+      KIFMM_M2LW_R(double, double2, 1, 1, 3, 1);
+    else if (v64 == 2)
+      KIFMM_M2LW_R(double, double2, 4, 1, 2, 1);
+    else if (v64 == 3)
+      KIFMM_M2LW_R(double, double2, 3, 1, 3, 1);
+    else
+      KIFMM_M2LW_R(double, double2, 4, 1, 3, 1);
+    return;
+  }
   if (depth <= 1)
     KIFMM_M2LW(1, 1, 4, 1);
   else if (var == 4140)
@@ -3470,6 +3514,7 @@ void m2l_wave(hipStream_t s, int C, int ntile, int Rc, int nslot, const int* tti
   else
     KIFMM_M2LW(4, 1, 4, 1);
 #undef KIFMM_M2LW
+#undef KIFMM_M2LW_R
 }
 
 // ---------------------------------------------------------------------------

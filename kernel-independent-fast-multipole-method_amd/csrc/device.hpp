@@ -173,13 +173,14 @@ void fft_inv(hipStream_t s, int np, int nslot, const int* slot_node, int Rc, con
 // fpart: deal frequency blocks rather than tile ranges to the XCDs.
 void m2l_tiled(hipStream_t s, int C, int ntile, int Rc, const int* ttile, const int* htile, const void* atab,
                const void* effden, void* effval, int single, int fpart = 0);
-// The FP32 products of m2l_tiled with one wave per (tile, frequency): the A
-// operands of all directions in registers, halo spectra loaded straight into
-// the matrix-core operands (atab32: the FP32 table), depth halo parents
+// The products of m2l_tiled with one wave per (tile, frequency): the A
+// operands of all directions in LDS, halo spectra loaded straight into the
+// matrix-core operands (atab: the table of the precision), depth halo parents
 // ahead (1 or 4); nslot: the spectra slots of the level, followed by one
-// slot of zeros (the caller's, effden[nslot]), which absent halo children read.
+// slot of zeros (the caller's, effden[nslot]), which absent halo children
+// read.  single: FP32 spectra and operators, otherwise FP64.
 void m2l_wave(hipStream_t s, int C, int ntile, int Rc, int nslot, const int* ttile, const int* htile,
-              const void* atab32, const void* effden, void* effval, int depth);
+              const void* atab, const void* effden, void* effval, int depth, int single);
 
 }  // namespace dev
 }  // namespace kifmm

@@ -1166,8 +1166,8 @@ int Engine::m2l_form(bool mixed, size_t level, int rc) const
   const bool tile_ok = mm_.sdof == 1 && mm_.tdof == 1 && rc % 16 == 0;
   const std::vector<int>& tf = m2l_form_[mixed ? 1 : 0];
   if (level < tf.size() && tf[level] >= 0 && ((tf[level] != 0 && tf[level] != 4) || tile_ok))
-    return tf[level] == 4 && !mixed ? 0 : tf[level];  // form 4 is FP32: the mixed form's
-  return m2l_tile_ && tile_ok ? (m2l_wave_ && mixed ? 4 : 0) : m2l_matrix_ ? 1 : 2;
+    return tf[level];
+  return m2l_tile_ && tile_ok ? (m2l_wave_ ? 4 : 0) : m2l_matrix_ ? 1 : 2;
 }
 
 int Engine::tune(int nrhs, int mode)
@@ -1192,7 +1192,7 @@ int Engine::tune(int nrhs, int mode)
   if (!plan_.m2l_pairs) {
     if (mm_.sdof == 1 && mm_.tdof == 1 && rc % 16 == 0) {
       mf.push_back(0);
-      if (mixed) mf.push_back(4);
+      mf.push_back(4);
     }
     mf.push_back(1);
     mf.push_back(2);
@@ -1417,11 +1417,13 @@ void Engine::run_chunk(const double* srcden, double* trgval, int rc, int mode, h
     });
     timed(KIFMM3D_K_M2L, [&] {
       if (form == 4) {
-        // FP32 spectra: slot vl.nsrc (C x rc float2, the size of rc doubles
-        // per frequency) is the zero slot that absent halo children read
-        dev::zero(s, effden_.p + (size_t)vl.nsrc * O.C * rc, (long)O.C * rc);
-        dev::m2l_wave(s, O.C, vl.ntile, rc, vl.nsrc, I + vl.ttile, I + vl.htile, atab32_.p, effden_.p, effval_.p,
-                      m2l_depth_);
+        // slot vl.nsrc (C x rc float2 or double2: rc or 2 rc doubles per
+        // frequency) is the zero slot that absent halo children read
+        const long zs = (long)O.C * rc * (single ? 1 : 2);
+        dev::zero(s, effden_.p + (size_t)vl.nsrc * zs, zs);
+        dev::m2l_wave(s, O.C, vl.ntile, rc, vl.nsrc, I + vl.ttile, I + vl.htile,
+                      single ? (const void*)atab32_.p : (const void*)atab_.p, effden_.p, effval_.p, m2l_depth_,
+                      single);
       } else if (tiled)
         dev::m2l_tiled(s, O.C, vl.ntile, rc, I + vl.ttile, I + vl.htile,
                        single ? (const void*)atab32_.p : (const void*)atab_.p, effden_.p, effval_.p, single,

@@ -191,6 +191,19 @@ def test_m2l_forms(kt, form, monkeypatch):
     run_case(f"kt{kt}-m2l-{form}", kt, 4, 30, p, nrm, p, ctr, lvl, nrhs=5)
 
 
+@pytest.mark.parametrize("geom,nrhs", [("cube", 16), ("clusters", 32)])
+def test_wave_m2l_fp64(geom, nrhs, monkeypatch):
+    """The wave-per-tile M2L on the FP64 matrix cores (form 4 of the plain
+    form) against the oracle at the parity bar: a uniform tree whose tiles
+    have every halo parent, and a clustered one with absent halo children
+    (read from the zero slot) and target parents without children; 32
+    right-hand sides take two groups of 16."""
+    monkeypatch.setenv("KIFMM3D_M2L", "wave")
+    p = uniform_cube(12000, 64) if geom == "cube" else clusters(8000, 65)
+    f, _, _ = run_case(f"wave-fp64-{geom}", 111, 4, 30, p, None, p, (0.5, 0.5, 0.5), 1, nrhs=nrhs)
+    assert set(f.settings(K.MODE_PLAIN)["m2l"]) == {4}
+
+
 @pytest.mark.parametrize("kt", [121, 321, 322, 521])
 def test_double_layer_near_coincident_pairs(kt):
     """Targets 1e-7 to 1e-5 from a source along the sphere's tangent plane:

@@ -65,7 +65,7 @@ def test_tune_plain_keeps_parity(kt, nrhs):
     print(f"  tuned: {st}", flush=True)
     assert st["evaluates"] >= 3
     assert st["near"] in ((0, 1, 2) if kt == 111 else (0, 1))
-    assert len(st["m2l"]) >= 1 and all(v in ((0, 1, 2) if kt == 111 else (1, 2)) for v in st["m2l"])
+    assert len(st["m2l"]) >= 1 and all(v in ((0, 1, 2, 4) if kt == 111 else (1, 2)) for v in st["m2l"])
     assert f.settings(K.MODE_PLAIN) == {"near": st["near"], "m2l": st["m2l"]}
     ref = oracle_vals(f, kt, 4, 40, p, nrm, ctr, lvl, den, (0, nrhs - 1))
     check_parity(f"tuned kt{kt}", f, den, nrhs, ref)
@@ -83,7 +83,7 @@ def test_pinned_settings_every_kernel_and_mixed_levels(kt):
     nlev = len(f.settings(K.MODE_PLAIN)["m2l"])
     assert nlev >= 2, "the case must have V lists on two levels at least"
     ref = oracle_vals(f, kt, 4, 40, p, nrm, ctr, lvl, den, (0, nrhs - 1))
-    forms = (0, 1, 2) if kt == 111 else (1, 2)
+    forms = (0, 1, 2, 4) if kt == 111 else (1, 2)
     nears = (0, 1, 2) if kt == 111 else (0, 1)
     for k, near in enumerate(nears):
         m2l = [forms[(k + l) % len(forms)] for l in range(nlev)]
