@@ -50,22 +50,22 @@ DEFAULT = "laplace-uniform-4m-16rhs"
 # kernel time of the run for the HBM-side rate.
 PROFILED = {
     ("laplace-uniform-4m-16rhs", "mixed", 256): ("profiles/laplace-uniform-4m-16rhs_mixed.md", {
-        "m2l": {"busy": 0.768, "read": 7.403, "written": 4.831},
-        "p2p": {"busy": 0.402, "read": 3.373, "written": 0.577},
-        "fft_fwd": {"busy": None, "read": 0.728, "written": 4.868},
+        "m2l": {"busy": 0.768, "read": 7.396, "written": 4.831},
+        "p2p": {"busy": 0.410, "read": 3.406, "written": 0.577},
+        "fft_fwd": {"busy": None, "read": 0.728, "written": 4.869},
         "fft_inv": {"busy": None, "read": 5.560, "written": 0.728},
-        "s2m": {"busy": 0.255, "read": 0.373, "written": 1.246},
-        "s2p": {"busy": 0.202, "read": 1.311, "written": 0.579},
-        "gemm_up": {"busy": 0.462, "read": None, "written": None},
-        "gemm_down": {"busy": 0.462, "read": None, "written": None},
+        "s2m": {"busy": 0.318, "read": 0.372, "written": 1.242},
+        "s2p": {"busy": 0.210, "read": 1.285, "written": 0.575},
+        "gemm_up": {"busy": 0.459, "read": None, "written": None},
+        "gemm_down": {"busy": 0.459, "read": None, "written": None},
     }),
     ("laplace-uniform-4m-16rhs", "plain", 256): ("profiles/laplace-uniform-4m-16rhs_plain.md", {
-        "m2l": {"busy": 0.525, "read": 21.733, "written": 9.663},
-        "p2p": {"busy": 0.352, "read": 8.061, "written": 0.558},
-        "fft_fwd": {"busy": None, "read": 0.729, "written": 9.693},
-        "fft_inv": {"busy": None, "read": 10.391, "written": 0.728},
+        "m2l": {"busy": 0.823, "read": 19.448, "written": 10.323},
+        "p2p": {"busy": 0.354, "read": 8.106, "written": 0.558},
+        "fft_fwd": {"busy": None, "read": 0.729, "written": 9.691},
+        "fft_inv": {"busy": None, "read": 10.390, "written": 0.728},
         "s2m": {"busy": 0.318, "read": 0.641, "written": 1.245},
-        "s2p": {"busy": 0.259, "read": 1.302, "written": 0.556},
+        "s2p": {"busy": 0.260, "read": 1.303, "written": 0.556},
     }),
     ("navier-clusters-1m", "mixed", 150): ("profiles/navier-clusters-1m_mixed.md", {
         "m2l": {"busy": 0.318, "read": 27.667, "written": 3.374},
