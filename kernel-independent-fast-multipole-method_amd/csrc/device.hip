@@ -686,7 +686,10 @@ typedef float v4f __attribute__((ext_vector_type(4)));
 // One staged source tile against the wave's first NG 16-target column blocks
 // (NG = the blocks that hold targets, so ragged leaves do not pay for empty
 // blocks).
-template <int NG>
+// GUARD: the r < 1e-8 exclusion; an entry whose closest pair of distinct
+// points is farther apart than 1e-7 (flag 0 of the set-up's near-field
+// pass: two different leaves, so no equal points either) goes without it.
+template <int NG, bool GUARD = true>
 __device__ __forceinline__ void p2p_tile64(int cnt4, int kk, int col, const double (*s_pos)[TILE],
                                            const double (*s_den)[16], const double* tx, const double* ty,
                                            const double* tz, v4d* acc)
@@ -697,7 +700,8 @@ __device__ __forceinline__ void p2p_tile64(int cnt4, int kk, int col, const doub
 #pragma unroll
     for (int i = 0; i < NG; i++) {
       const double dx = tx[i] - sx, dy = ty[i] - sy, dz = tz[i] - sz;
-      const double gv = rinv(dx * dx + dy * dy + dz * dz);
+      const double r2 = dx * dx + dy * dy + dz * dz;
+      const double gv = GUARD ? rinv(r2) : rsqrt_nr(r2);
       acc[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(f, gv, acc[i], 0, 0, 0);
     }
   }
@@ -746,8 +750,8 @@ __device__ __forceinline__ void p2p_tile32(int cnt4, int kk, int col, const floa
 
 __global__ __launch_bounds__(P2P_BS) void k_p2p_mfma111(int nchunk, const int* __restrict__ tbeg,
                                                         const int* __restrict__ tnum, const int* __restrict__ eptr,
-                                                        const int2* __restrict__ ent, Points src, Targets trg, int roff,
-                                                        KC kc)
+                                                        const int2* __restrict__ ent, const int* __restrict__ echk,
+                                                        Points src, Targets trg, int roff, KC kc)
 {
   constexpr int NR = 16;
   __shared__ double s_pos[3][TILE];
@@ -774,6 +778,7 @@ __global__ __launch_bounds__(P2P_BS) void k_p2p_mfma111(int nchunk, const int* _
   const double* den = src.den + (long)r0 * src.den_rs;
   for (int e = eptr[c]; e < eptr[c + 1]; e++) {
     const int2 se = ent[e];
+    const bool guard = !echk || echk[e] != 0;  // block-uniform
     for (int b0 = 0; b0 < se.y; b0 += TILE) {
       const int cnt = min(TILE, se.y - b0), cnt4 = (cnt + 3) & ~3;
       const int j0 = se.x + b0;
@@ -787,7 +792,16 @@ __global__ __launch_bounds__(P2P_BS) void k_p2p_mfma111(int nchunk, const int* _
       stage_den<1, NR, P2P_BS>(s_den, den, src, j0, cnt, t);
       for (int i = cnt * NR + t; i < cnt4 * NR; i += P2P_BS) s_den[i / NR][i % NR] = 0.0;
       __syncthreads();
-      if (ng == 4)
+      if (!guard) {
+        if (ng == 4)
+          p2p_tile64<4, false>(cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc);
+        else if (ng == 3)
+          p2p_tile64<3, false>(cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc);
+        else if (ng == 2)
+          p2p_tile64<2, false>(cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc);
+        else if (ng == 1)
+          p2p_tile64<1, false>(cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc);
+      } else if (ng == 4)
         p2p_tile64<4>(cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc);
       else if (ng == 3)
         p2p_tile64<3>(cnt4, kk, col, s_pos, s_den, tx, ty, tz, acc);
@@ -1288,7 +1302,7 @@ static void p2p_launch(hipStream_t s, int variant, int nchunk, int nb, const int
 {
   if (variant >= 2 && KT == 111 && NR == 16 && src.den_rs == 1)
     hipLaunchKernelGGL(k_p2p_mfma111, dim3(xcd_grid(nchunk), nb), dim3(P2P_BS), 0, s, nchunk, tbeg, tnum, eptr, ent,
-                       src, trg, roff, kc);
+                       echk, src, trg, roff, kc);
   else if (variant == 1)
     hipLaunchKernelGGL((k_p2p_s<KT, NR>), dim3(2 * nchunk, nb), dim3(64), 0, s, tbeg, tnum, eptr, ent, src, trg, roff,
                        kc);

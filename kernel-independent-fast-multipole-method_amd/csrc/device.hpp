@@ -64,11 +64,11 @@ struct Targets {  // leaf-ordered targets (SoA) and values [rhs][n*tdof]
 // variant 3: the same on the FP32 matrix cores (mixed precision).  Variants
 // 2 and 3 fall back to variant 0 for other kernels and smaller blocks.
 // variant 4: the kernels without normals in FP32 (mixed precision).
-// echk[e] (variants 3 and 4; without it variant 3 takes the FP64 matrix-core
-// kernel): 0 no pair of entry e can be closer than 1e-8 or than its FP32
-// offsets resolve, 1 the target's own leaf (exclusion tested in FP32), 2 a
-// pair may be closer than the FP32 offsets resolve: the FP32 kernels skip such
-// entries.  Returns n: right-hand sides [0, n) skipped the entries flagged 2,
+// echk[e] (variants 2, 3 and 4; without it variant 3 takes the FP64
+// matrix-core kernel): 0 no pair of entry e can be closer than 1e-8 or than
+// its FP32 offsets resolve (the FP64 matrix-core kernel skips the exclusion
+// test there), 1 the target's own leaf (exclusion tested), 2 a pair may be
+// closer than the FP32 offsets resolve: the FP32 kernels skip such entries.  Returns n: right-hand sides [0, n) skipped the entries flagged 2,
 // which a second call with variant 0 over those entries alone (and nr = n)
 // then adds in FP64.
 int p2p(hipStream_t s, int kt, int nchunk, const int* tbeg, const int* tnum, const int* eptr, const int2* ent,
