@@ -208,8 +208,9 @@ int kifmm3d_set_graph(kifmm3d* h, int on);
  * settings (may be NULL) receives up to nsettings entries: [0] near-field
  * kernel (0 LDS broadcast, 1 scalar-cache sources, 2 FP64 matrix cores,
  * 3 FP32 matrix cores, 4 FP32 LDS broadcast), [1 + l] M2L form of V-list level l, top level first
- * (0 tiled matrix-core, 1 parent-grouped matrix-core, 2 grouped VALU,
- * 3 per pair); nsettings = 17 always suffices.  *nrun (may be NULL) receives
+ * (0 tiled matrix-core, block per tile, 1 parent-grouped matrix-core,
+ * 2 grouped VALU, 3 per pair, 4 tiled matrix-core, wave per tile and
+ * frequency); nsettings = 17 always suffices.  *nrun (may be NULL) receives
  * the number of evaluates run.  Requires setup; allocates scratch densities
  * and values of nrhs vectors for its duration. */
 int kifmm3d_tune(kifmm3d* h, int nrhs, int mode, int* settings, int nsettings, int* nrun);
