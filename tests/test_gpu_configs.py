@@ -145,6 +145,19 @@ def test_laplace_uniform_4m_16rhs():
     plain = f.eval(den, nrhs=nrhs).reshape(nrhs, n).copy()
     mixed = f.eval(den, nrhs=nrhs, mode=K.MODE_MIXED).reshape(nrhs, n).copy()
     em = max(relerr(mixed[r], plain[r]) for r in range(nrhs))
+    # the wave-per-tile M2L on every level (at level 5 the FP64 spectra span
+    # 8.5 GB, so its halo addresses take the 64-bit path) against the
+    # block-per-tile default: the same sums in another order
+    nlev = len(f.settings(K.MODE_PLAIN)["m2l"])
+    f.set_settings(K.MODE_PLAIN, -1, [4] * nlev)
+    f.set_settings(K.MODE_MIXED, -1, [4] * nlev)
+    pw = f.eval(den, nrhs=nrhs).reshape(nrhs, n).copy()
+    mw = f.eval(den, nrhs=nrhs, mode=K.MODE_MIXED).reshape(nrhs, n).copy()
+    ewp = max(relerr(pw[r], plain[r]) for r in range(nrhs))
+    ewm = max(relerr(mw[r], mixed[r]) for r in range(nrhs))
+    print(f"  wave-per-tile M2L against block-per-tile: plain {ewp:.3e}, mixed {ewm:.3e}", flush=True)
+    assert ewp <= 1e-13 and ewm <= 5e-7
+    del pw, mw
     idx, d = check_vs_direct(f, den, nrhs, n, 1, c["nchk"], seed)
     ep, epf = errors(plain[:, idx], d)
     ex, exf = errors(mixed[:, idx], d)
