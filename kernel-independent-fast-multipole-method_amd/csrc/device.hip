@@ -2644,13 +2644,93 @@ __global__ __launch_bounds__(64 * R) void k_fft_fwd_ri(int nslot, const int* __r
   }
 }
 
+// The forward transform with the x and y stages fused: a thread takes the
+// line (k, kx) of one right-hand side, sums the x direction of the np rows j
+// of plane k straight into registers (cube points off the surface read a zero
+// kept at index NSP of the density row) and transforms them along y at once,
+// so the x-stage values never pass through LDS: one barrier fewer, and at np 6
+// with 8 FP32 right-hand sides 38 KB of LDS per block instead of 54 KB.
 template <int NP, class CT, int R>
-__global__ __launch_bounds__(64 * R) void k_fft_inv_ri(int nslot, const int* __restrict__ slot_node, int Rc,
+__global__ __launch_bounds__(64 * R) void k_fft_fwd_xy(int nslot, const int* __restrict__ slot_node, int Rc,
+                                                             const double* __restrict__ upeden, int pln,
+                                                             const double* __restrict__ dscale,
+                                                             const short* __restrict__ c2s, CT* __restrict__ effden,
+                                                             const double2* __restrict__ tw)
+{
+  constexpr int N = 2 * NP, H = NP + 1, C = N * N * H, NC = NP * NP * NP, BS = 64 * R;
+  constexpr int NSP = NP * NP * NP - (NP - 2) * (NP - 2) * (NP - 2);
+  typedef CT LT;
+  using A = typename FftMath<CT>::A;
+  using A2 = typename FftMath<CT>::A2;
+  __shared__ A s_den[R][NSP + 1];
+  __shared__ short s_c2s[NC];
+  __shared__ A2 s_wt[NP][N];
+  __shared__ LT X2[R][NP * N * H + 1];  // [k][ky][kx]
+  int slot, r0;
+  if (!fft_block(nslot, Rc / R, R, slot, r0)) return;  // block-uniform
+  const int t = threadIdx.x;
+  for (int i = t; i < NC; i += BS) {
+    const short v = c2s[i];
+    s_c2s[i] = v >= 0 ? v : (short)NSP;
+  }
+  load_twiddles_t<NP>(s_wt, tw, t, BS);
+  const double ds = dscale[0];
+  const double* den = upeden + ((long)slot_node[slot] * Rc + r0) * pln;
+  for (int e = t; e < R * NSP; e += BS) {
+    const int rr = e / NSP, i = e - rr * NSP;
+    s_den[rr][i] = (A)(den[(long)rr * pln + i] * ds);
+  }
+  if (t < R) s_den[t][NSP] = 0;
+  __syncthreads();
+  for (int e = t; e < R * NP * H; e += BS) {  // along x and y: line (k, kx)
+    asm volatile("" ::: "memory");  // keep the twiddle loads inside the loop (registers)
+    const int rr = e % R, col = e / R, k = col / H, kx = col - k * H;
+    A2 x[NP];
+#pragma unroll
+    for (int j = 0; j < NP; j++) {
+      A2 a = cz<A2>();
+#pragma unroll
+      for (int i = 0; i < NP; i++) {
+        const A v = s_den[rr][s_c2s[(i * NP + j) * NP + k]];
+        a.x += v * s_wt[i][kx].x;
+        a.y += v * s_wt[i][kx].y;
+      }
+      x[j] = a;
+    }
+#pragma unroll
+    for (int ky = 0; ky < NP; ky++) {  // outputs ky and ky + NP together
+      A2 sp = cz<A2>(), sn = cz<A2>();
+#pragma unroll
+      for (int j = 0; j < NP; j++) fft_half_mac(x[j], s_wt[j][ky], ((j + NP / 2) & 1) != 0, sp, sn);
+      fft_half_out(sp, sn, X2[rr][(k * N + ky) * H + kx], X2[rr][(k * N + ky + NP) * H + kx]);
+    }
+  }
+  __syncthreads();
+  CT* out = effden + (long)slot * C * Rc + r0;
+  for (int e = t; e < R * N * H; e += BS) {  // along z: line (ky, kx), right-hand sides innermost
+    asm volatile("" ::: "memory");  // keep the twiddle loads inside the loop (registers)
+    const int rr = e % R, col = e / R;
+    A2 x[NP];
+#pragma unroll
+    for (int k = 0; k < NP; k++) x[k] = cget<A2>(X2[rr][k * N * H + col]);
+#pragma unroll
+    for (int kz = 0; kz < NP; kz++) {  // outputs kz and kz + NP together
+      A2 sp = cz<A2>(), sn = cz<A2>();
+#pragma unroll
+      for (int k = 0; k < NP; k++) fft_half_mac(x[k], s_wt[k][kz], ((k + NP / 2) & 1) != 0, sp, sn);
+      fft_half_out(sp, sn, out[(long)(kz * N * H + col) * Rc + rr], out[(long)((kz + NP) * N * H + col) * Rc + rr]);
+    }
+  }
+}
+
+// BS threads per block (64 R by default).
+template <int NP, class CT, int R, int BS = 64 * R>
+__global__ __launch_bounds__(BS) void k_fft_inv_ri(int nslot, const int* __restrict__ slot_node, int Rc,
                                                        const CT* __restrict__ effval, const int* __restrict__ s2c,
                                                        int nsp, double* __restrict__ dncval, int pln,
                                                        const double2* __restrict__ tw)
 {
-  constexpr int N = 2 * NP, H = NP + 1, C = N * N * H, BS = 64 * R;
+  constexpr int N = 2 * NP, H = NP + 1, C = N * N * H;
   typedef CT LT;
   using A = typename FftMath<CT>::A;
   using A2 = typename FftMath<CT>::A2;
@@ -2737,6 +2817,16 @@ static void fft_fwd_t(hipStream_t s, int np, int nslot, const int* slot_node, in
   if (nslot <= 0) return;
   if (ri && sm == 1) {
     constexpr int RW = sizeof(CT) == 8 ? FFT_RW32_FWD : 4;
+    // np 6, FP32: the x and y stages fused (4M points / 16 RHS: 1.71 ->
+    // 1.59 ms a forward pass; the fused form of FP64 spectra measured 3.75 ms
+    // against 3.19, and 384 or 256 threads per block of 8 FP32 right-hand
+    // sides 1.95 and 1.74)
+    if constexpr (sizeof(CT) == 8)
+      if (np == 6) {
+        hipLaunchKernelGGL((k_fft_fwd_xy<6, CT, RW>), dim3(xcd_grid(nslot * (Rc / RW))), dim3(64 * RW), 0, s, nslot,
+                           slot_node, Rc, upeden, pln, dscale, cube2surf, effden, tw);
+        return;
+      }
     switch (np) {
     case 4: hipLaunchKernelGGL((k_fft_fwd_ri<4, CT, RW>), dim3(xcd_grid(nslot * (Rc / RW))), dim3(64 * RW), 0, s, nslot, slot_node, Rc, upeden, pln, dscale, cube2surf, effden, tw); break;
     case 6: hipLaunchKernelGGL((k_fft_fwd_ri<6, CT, RW>), dim3(xcd_grid(nslot * (Rc / RW))), dim3(64 * RW), 0, s, nslot, slot_node, Rc, upeden, pln, dscale, cube2surf, effden, tw); break;
@@ -2759,6 +2849,16 @@ static void fft_inv_t(hipStream_t s, int np, int nslot, const int* slot_node, in
   if (nslot <= 0) return;
   if (ri && tm == 1) {
     constexpr int RW = sizeof(CT) == 8 ? FFT_RW32_INV : 4;
+    // np 6, FP64: 6 waves for the 4 right-hand sides of a block, so the z
+    // stage's 336 lines take one pass and 3 blocks fit a CU (4M points /
+    // 16 RHS: 2.94 -> 2.53 ms an inverse pass; for 8 FP32 right-hand sides
+    // 384 and 256 threads measured 1.54 and 1.35 ms against 1.39 at 512)
+    if constexpr (sizeof(CT) == 16)
+      if (np == 6) {
+        hipLaunchKernelGGL((k_fft_inv_ri<6, CT, RW, 384>), dim3(xcd_grid(nslot * (Rc / RW))), dim3(384), 0, s, nslot,
+                           slot_node, Rc, effval, surf2cube, nsp, dncval, pln, tw);
+        return;
+      }
     switch (np) {
     case 4: hipLaunchKernelGGL((k_fft_inv_ri<4, CT, RW>), dim3(xcd_grid(nslot * (Rc / RW))), dim3(64 * RW), 0, s, nslot, slot_node, Rc, effval, surf2cube, nsp, dncval, pln, tw); break;
     case 6: hipLaunchKernelGGL((k_fft_inv_ri<6, CT, RW>), dim3(xcd_grid(nslot * (Rc / RW))), dim3(64 * RW), 0, s, nslot, slot_node, Rc, effval, surf2cube, nsp, dncval, pln, tw); break;
@@ -3692,6 +3792,130 @@ __global__ __launch_bounds__(M2LV_BS) void k_m2l_vec(int C, int Rc, long ncol, i
     }
 }
 
+// The same products with the loads of a direction issued before its
+// barriers: the operator-table entries and source slots of direction n + 1
+// are loaded while direction n is applied, and the operator values and the
+// source spectra of a direction are fetched together, so a direction waits on
+// one memory round trip instead of four.  It holds more registers (FP32 3x3:
+// 170 against 116, two waves per SIMD instead of four), and measured (1M
+// Navier clusters, 262k Stokes sphere, M2L per evaluate): FP64 3x3 29.0 ->
+// 26.7 ms, FP64 4x3 5.4 -> 4.7 ms, FP32 4x3 3.5 -> 3.0 ms, FP32 3x3 19.9 ->
+// 21.0 ms (the FP32 3x3 shape keeps k_m2l_vec).
+template <class R, int SM, int TM, int FB>
+__global__ __launch_bounds__(M2LV_BS) void k_m2l_vec_ahead(int C, int Rc, long ncol, int ncb,
+                                                           const int* __restrict__ tslot,
+                                                           const int* __restrict__ sslot,
+                                                           const int* __restrict__ pmask,
+                                                           const int* __restrict__ aidx,
+                                                           const typename Mv<R>::C2* __restrict__ opsT,
+                                                           const typename Mv<R>::C2* __restrict__ effden,
+                                                           typename Mv<R>::C2* __restrict__ effval)
+{
+  typedef typename Mv<R>::C2 C2;
+  typedef typename Mv<R>::V4 V4;
+  constexpr int NKS = 4 * SM, NH = NKS / 2, ST = SM * TM, NA = TM * NKS * 64;
+  __shared__ R s_A[FB][NA];
+  // items (group of 8 frequencies, column block, FB-frequency run in the
+  // group): the runs of one 128-byte line of spectra run side by side on one XCD
+  constexpr int RUNS = 8 / FB;
+  const int ngrp = (C + 7) / 8, nitem = ngrp * ncb * RUNS;
+  const int item = xcd_item(blockIdx.x, nitem);
+  const int fg = item / (RUNS * ncb), rem = item - fg * RUNS * ncb, cb = rem / RUNS;
+  const int f0 = fg * 8 + (rem - cb * RUNS) * FB;
+  if (item >= nitem || f0 >= C) return;  // block-uniform
+  const int nf = min(FB, C - f0);
+  const int t = threadIdx.x, w = t >> 6, l = t & 63, li = l & 15, lk = l >> 4;
+  const long col = ((long)cb * 4 + w) * 16 + li;
+  const int q = col < ncol ? (int)(col / Rc) : -1;
+  const int r = col < ncol ? (int)(col - (long)(col / Rc) * Rc) : 0;
+  const int msk = q >= 0 ? pmask[q] : 0;
+  const int* ss = sslot + (long)(q < 0 ? 0 : q) * 26 * 8;
+  V4 acc[FB][TM];
+#pragma unroll
+  for (int fb = 0; fb < FB; fb++)
+#pragma unroll
+    for (int rt = 0; rt < TM; rt++) acc[fb][rt] = V4{0, 0, 0, 0};
+  // operator-table entries and source slots of the next direction, loaded one
+  // direction ahead; the operator values and the source spectra of a
+  // direction are then fetched together, before its barriers
+  int ai[ST], sl[NH];
+  auto fetch_idx = [&](int n) {
+#pragma unroll
+    for (int i = 0; i < ST; i++) ai[i] = aidx[(long)n * NA + t + M2LV_BS * i];
+#pragma unroll
+    for (int kh = 0; kh < NH; kh++) sl[kh] = q >= 0 ? ss[n * 8 + (kh * 4 + lk) / SM] : -1;
+  };
+  fetch_idx(0);
+  for (int n = 0; n < 26; n++) {
+    // the direction-n blocks [[Mr,-Mi],[Mi,Mr]] of the FB frequencies: entry
+    // e is aidx[n][e] = 2 x (index of the real value among one frequency's
+    // 343 sm tm complex operators) + 1 if negated, or -1 (adjacent pair)
+    R av[FB][ST];
+#pragma unroll
+    for (int fb = 0; fb < FB; fb++) {
+      const R* Of = (const R*)(opsT + (long)(f0 + (fb < nf ? fb : 0)) * 343 * ST);
+#pragma unroll
+      for (int i = 0; i < ST; i++) av[fb][i] = Of[ai[i] < 0 ? 0 : ai[i] >> 1];
+    }
+    // this lane's NH source spectra for the FB frequencies (contiguous in the
+    // spectra layout), clamped to slot 0 / the run and zeroed below
+    const bool use = __any((msk >> n) & 1);  // wave-uniform: a parent of the tile has direction n
+    C2 x[NH][FB];
+    bool xz[NH];
+    if (use) {
+#pragma unroll
+      for (int kh = 0; kh < NH; kh++) {
+        const int j = (kh * 4 + lk) % SM;
+        xz[kh] = sl[kh] < 0;
+        const C2* xp = effden + (((long)(xz[kh] ? 0 : sl[kh]) * Rc + r) * SM + j) * C + f0;
+#pragma unroll
+        for (int fb = 0; fb < FB; fb++) x[kh][fb] = xp[fb < nf ? fb : 0];
+      }
+    }
+    int an[ST];
+#pragma unroll
+    for (int i = 0; i < ST; i++) an[i] = ai[i];
+    if (n + 1 < 26) fetch_idx(n + 1);
+    __syncthreads();  // the previous direction is done with s_A
+#pragma unroll
+    for (int fb = 0; fb < FB; fb++)
+#pragma unroll
+      for (int i = 0; i < ST; i++)
+        s_A[fb][t + M2LV_BS * i] = an[i] < 0 ? (R)0 : ((an[i] & 1) ? -av[fb][i] : av[fb][i]);
+    __syncthreads();
+    if (!use) continue;
+#pragma unroll
+    for (int kh = 0; kh < NH; kh++)
+#pragma unroll
+      for (int fb = 0; fb < FB; fb++) {
+        const R xr = xz[kh] ? (R)0 : x[kh][fb].x, xi = xz[kh] ? (R)0 : x[kh][fb].y;
+#pragma unroll
+        for (int rt = 0; rt < TM; rt++) {
+          acc[fb][rt] = Mv<R>::mfma(s_A[fb][(rt * NKS + kh) * 64 + l], xr, acc[fb][rt]);
+          acc[fb][rt] = Mv<R>::mfma(s_A[fb][(rt * NKS + kh + NH) * 64 + l], xi, acc[fb][rt]);
+        }
+      }
+  }
+  if (q < 0) return;
+#pragma unroll
+  for (int rt = 0; rt < TM; rt++)
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      const int qo = rt * 8 + Mv<R>::item_out(lk, h), c = qo / TM, ti = qo - c * TM;
+      const int tl = tslot[(long)q * 8 + c];
+      if (tl < 0) continue;
+      C2* out = effval + (((long)tl * Rc + r) * TM + ti) * C + f0;
+#pragma unroll
+      for (int fb = 0; fb < FB; fb++)
+        if (fb < nf) {
+          C2 v;
+          v.x = acc[fb][rt][Mv<R>::item_re(h)];
+          v.y = acc[fb][rt][Mv<R>::item_im(h)];
+          out[fb] = v;
+        }
+    }
+}
+
 template <class R, int SM, int TM, int FB>
 static void m2l_vec_launch(hipStream_t s, int C, int npar, int Rc, const int* tslot, const int* sslot,
                            const int* pmask, const int* aidx, const void* opsT, const void* effden, void* effval)
@@ -3699,8 +3923,13 @@ static void m2l_vec_launch(hipStream_t s, int C, int npar, int Rc, const int* ts
   typedef typename Mv<R>::C2 C2;
   const long ncol = (long)npar * Rc;
   const int ncb = (int)((ncol + 63) / 64);
-  hipLaunchKernelGGL((k_m2l_vec<R, SM, TM, FB>), dim3(xcd_grid((C + 7) / 8 * ncb * (8 / FB))), dim3(M2LV_BS), 0, s, C,
-                     Rc, ncol, ncb, tslot, sslot, pmask, aidx, (const C2*)opsT, (const C2*)effden, (C2*)effval);
+  const dim3 grid(xcd_grid((C + 7) / 8 * ncb * (8 / FB)));
+  if constexpr (sizeof(R) == 4 && SM == 3 && TM == 3)
+    hipLaunchKernelGGL((k_m2l_vec<R, SM, TM, FB>), grid, dim3(M2LV_BS), 0, s, C, Rc, ncol, ncb, tslot, sslot, pmask,
+                       aidx, (const C2*)opsT, (const C2*)effden, (C2*)effval);
+  else
+    hipLaunchKernelGGL((k_m2l_vec_ahead<R, SM, TM, FB>), grid, dim3(M2LV_BS), 0, s, C, Rc, ncol, ncb, tslot, sslot,
+                       pmask, aidx, (const C2*)opsT, (const C2*)effden, (C2*)effval);
 }
 
 template <class R>

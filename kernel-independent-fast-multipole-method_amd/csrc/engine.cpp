@@ -715,14 +715,29 @@ void Engine::build_plan()
         }
         if (checked != npair) throw Error(KIFMM3D_EINVAL, "internal: grouped M2L plan misses V pairs");
         vl.npar = (int)(tsl.size() / 8);
-        vl.tslot = pool_i(P.I, tsl);
-        vl.sslot = pool_i(P.I, ssl);
         std::vector<int> pm(vl.npar, 0);  // bit n: the parent has a source child in direction n
         for (int q = 0; q < vl.npar; q++)
           for (int n = 0; n < 26; n++)
             for (int sc = 0; sc < 8; sc++)
               if (ssl[((size_t)q * 26 + n) * 8 + sc] >= 0) pm[q] |= 1 << n;
-        vl.pmask = pool_i(P.I, pm);
+        // the parent-grouped form's columns: parents in decreasing order of
+        // their direction masks (stable), so that the 16 parents of a
+        // matrix-core tile share their directions and a tile skips more of
+        // them (1M Navier clusters: tiles 51 -> 60 % occupied, M2L 20.4 ->
+        // 19.9 ms); the tiled form below keeps the tree order
+        std::vector<int> ord(vl.npar);
+        for (int q = 0; q < vl.npar; q++) ord[q] = q;
+        std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return pm[a] > pm[b]; });
+        std::vector<int> gtsl(tsl.size()), gssl(ssl.size()), gpm(pm.size());
+        for (int q = 0; q < vl.npar; q++) {
+          const int o = ord[q];
+          std::copy(tsl.begin() + (size_t)o * 8, tsl.begin() + (size_t)o * 8 + 8, gtsl.begin() + (size_t)q * 8);
+          std::copy(ssl.begin() + (size_t)o * 208, ssl.begin() + (size_t)o * 208 + 208, gssl.begin() + (size_t)q * 208);
+          gpm[q] = pm[o];
+        }
+        vl.tslot = pool_i(P.I, gtsl);
+        vl.sslot = pool_i(P.I, gssl);
+        vl.pmask = pool_i(P.I, gpm);
         if (std::getenv("KIFMM3D_TRACE")) {
           // occupancy of the matrix-core M2L tiles (16 parents x 26 directions x
           // 8 x 8 child pairs) by V pairs, for one right-hand side
@@ -731,10 +746,10 @@ void Engine::build_plan()
             for (int n = 0; n < 26; n++) {
               bool any = false;
               for (int q = q0; q < std::min(q0 + 16, vl.npar); q++) {
-                any = any || ((pm[q] >> n) & 1);
+                any = any || ((gpm[q] >> n) & 1);
                 for (int sc = 0; sc < 8; sc++)
                   for (int c = 0; c < 8; c++)
-                    used += ssl[((size_t)q * 26 + n) * 8 + sc] >= 0 && tsl[(size_t)q * 8 + c] >= 0 &&
+                    used += gssl[((size_t)q * 26 + n) * 8 + sc] >= 0 && gtsl[(size_t)q * 8 + c] >= 0 &&
                             optab[(n * 8 + sc) * 8 + c] >= 0;
               }
               run += any ? 16 * 64 : 0;
