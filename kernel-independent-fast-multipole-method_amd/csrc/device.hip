@@ -3661,6 +3661,19 @@ void m2l_wave(hipStream_t s, int C, int ntile, int Rc, int nslot, const int* tti
 // spectra (8 frequencies) is fetched into its L2 once for all 8.
 constexpr int M2LV_BS = 256;
 
+// Raw buffer loads (gfx9 resource word 3; offsets past the resource's size
+// read zeros).
+constexpr int BUF_DW3 = 0x00020000;
+template <class R> __device__ inline R buf_ld(__amdgpu_buffer_rsrc_t r, unsigned off, int soff);
+template <> __device__ inline float buf_ld<float>(__amdgpu_buffer_rsrc_t r, unsigned off, int soff)
+{
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, soff, 0));
+}
+template <> __device__ inline double buf_ld<double>(__amdgpu_buffer_rsrc_t r, unsigned off, int soff)
+{
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, soff, 0));
+}
+
 // Precision traits of the parent-grouped M2L.  FP64: row r16 of a row tile is
 // output r16 % 8, part r16 / 8, so accumulator items (h, h + 2) of lane k are
 // (re, im) of output k + 4h.  FP32 (output item v of lane l at row
@@ -3691,6 +3704,15 @@ struct Mv<float> {
   __device__ static inline int item_im(int h) { return 2 * h + 1; }
 };
 
+// The loads of a direction are issued before its barriers: the operator-
+// table entries and source slots of direction n + 1 are loaded while
+// direction n is applied, and the operator values (through a buffer resource:
+// one 32-bit offset per entry for both frequencies) and the source spectra of
+// a direction are fetched together, so a direction waits on one memory round
+// trip instead of four.  Measured against the direction-by-direction order
+// (1M Navier clusters, 262k Stokes sphere, M2L per evaluate): FP64 3x3 29.0 ->
+// 26.7 ms, FP64 4x3 5.4 -> 4.7 ms, FP32 4x3 3.5 -> 2.9 ms, FP32 3x3 19.8 ->
+// 19.9 ms.
 template <class R, int SM, int TM, int FB>
 __global__ __launch_bounds__(M2LV_BS) void k_m2l_vec(int C, int Rc, long ncol, int ncb, const int* __restrict__ tslot,
                                                      const int* __restrict__ sslot, const int* __restrict__ pmask,
@@ -3698,118 +3720,6 @@ __global__ __launch_bounds__(M2LV_BS) void k_m2l_vec(int C, int Rc, long ncol, i
                                                      const typename Mv<R>::C2* __restrict__ opsT,
                                                      const typename Mv<R>::C2* __restrict__ effden,
                                                      typename Mv<R>::C2* __restrict__ effval)
-{
-  typedef typename Mv<R>::C2 C2;
-  typedef typename Mv<R>::V4 V4;
-  constexpr int NKS = 4 * SM, NH = NKS / 2, ST = SM * TM, NA = TM * NKS * 64;
-  __shared__ R s_A[FB][NA];
-  // items (group of 8 frequencies, column block, FB-frequency run in the
-  // group): the runs of one 128-byte line of spectra run side by side on one XCD
-  constexpr int RUNS = 8 / FB;
-  const int ngrp = (C + 7) / 8, nitem = ngrp * ncb * RUNS;
-  const int item = xcd_item(blockIdx.x, nitem);
-  const int fg = item / (RUNS * ncb), rem = item - fg * RUNS * ncb, cb = rem / RUNS;
-  const int f0 = fg * 8 + (rem - cb * RUNS) * FB;
-  if (item >= nitem || f0 >= C) return;  // block-uniform
-  const int nf = min(FB, C - f0);
-  const int t = threadIdx.x, w = t >> 6, l = t & 63, li = l & 15, lk = l >> 4;
-  const long col = ((long)cb * 4 + w) * 16 + li;
-  const int q = col < ncol ? (int)(col / Rc) : -1;
-  const int r = col < ncol ? (int)(col - (long)(col / Rc) * Rc) : 0;
-  const int msk = q >= 0 ? pmask[q] : 0;
-  V4 acc[FB][TM];
-#pragma unroll
-  for (int fb = 0; fb < FB; fb++)
-#pragma unroll
-    for (int rt = 0; rt < TM; rt++) acc[fb][rt] = V4{0, 0, 0, 0};
-  for (int n = 0; n < 26; n++) {
-    __syncthreads();  // the previous direction is done with s_A
-    // the direction-n blocks [[Mr,-Mi],[Mi,Mr]] of the FB frequencies: entry
-    // e is aidx[n][e] = 2 x (index of the real value among one frequency's
-    // 343 sm tm complex operators) + 1 if negated, or -1 (adjacent pair)
-    {
-      int ai[ST];
-      R av[FB][ST];
-#pragma unroll
-      for (int i = 0; i < ST; i++) ai[i] = aidx[(long)n * NA + t + M2LV_BS * i];
-#pragma unroll
-      for (int fb = 0; fb < FB; fb++) {
-        const R* Of = (const R*)(opsT + (long)(f0 + (fb < nf ? fb : 0)) * 343 * ST);
-#pragma unroll
-        for (int i = 0; i < ST; i++) av[fb][i] = Of[ai[i] < 0 ? 0 : ai[i] >> 1];
-      }
-#pragma unroll
-      for (int fb = 0; fb < FB; fb++)
-#pragma unroll
-        for (int i = 0; i < ST; i++)
-          s_A[fb][t + M2LV_BS * i] = ai[i] < 0 ? (R)0 : ((ai[i] & 1) ? -av[fb][i] : av[fb][i]);
-    }
-    __syncthreads();
-    if (!__any((msk >> n) & 1)) continue;  // wave-uniform: no parent of the tile has direction n
-    // this lane's NH source spectra for the FB frequencies (contiguous in
-    // the spectra layout), loads batched and clamped to slot 0 / the run
-    const int* ss = sslot + ((long)(q < 0 ? 0 : q) * 26 + n) * 8;
-    int sl[NH];
-#pragma unroll
-    for (int kh = 0; kh < NH; kh++) sl[kh] = q >= 0 ? ss[(kh * 4 + lk) / SM] : -1;
-    C2 x[NH][FB];
-#pragma unroll
-    for (int kh = 0; kh < NH; kh++) {
-      const int j = (kh * 4 + lk) % SM;
-      const C2* xp = effden + (((long)(sl[kh] < 0 ? 0 : sl[kh]) * Rc + r) * SM + j) * C + f0;
-#pragma unroll
-      for (int fb = 0; fb < FB; fb++) x[kh][fb] = xp[fb < nf ? fb : 0];
-    }
-#pragma unroll
-    for (int kh = 0; kh < NH; kh++)
-#pragma unroll
-      for (int fb = 0; fb < FB; fb++) {
-        const R xr = sl[kh] < 0 ? (R)0 : x[kh][fb].x, xi = sl[kh] < 0 ? (R)0 : x[kh][fb].y;
-#pragma unroll
-        for (int rt = 0; rt < TM; rt++) {
-          acc[fb][rt] = Mv<R>::mfma(s_A[fb][(rt * NKS + kh) * 64 + l], xr, acc[fb][rt]);
-          acc[fb][rt] = Mv<R>::mfma(s_A[fb][(rt * NKS + kh + NH) * 64 + l], xi, acc[fb][rt]);
-        }
-      }
-  }
-  if (q < 0) return;
-#pragma unroll
-  for (int rt = 0; rt < TM; rt++)
-#pragma unroll
-    for (int h = 0; h < 2; h++) {
-      const int qo = rt * 8 + Mv<R>::item_out(lk, h), c = qo / TM, ti = qo - c * TM;
-      const int tl = tslot[(long)q * 8 + c];
-      if (tl < 0) continue;
-      C2* out = effval + (((long)tl * Rc + r) * TM + ti) * C + f0;
-#pragma unroll
-      for (int fb = 0; fb < FB; fb++)
-        if (fb < nf) {
-          C2 v;
-          v.x = acc[fb][rt][Mv<R>::item_re(h)];
-          v.y = acc[fb][rt][Mv<R>::item_im(h)];
-          out[fb] = v;
-        }
-    }
-}
-
-// The same products with the loads of a direction issued before its
-// barriers: the operator-table entries and source slots of direction n + 1
-// are loaded while direction n is applied, and the operator values and the
-// source spectra of a direction are fetched together, so a direction waits on
-// one memory round trip instead of four.  It holds more registers (FP32 3x3:
-// 170 against 116, two waves per SIMD instead of four), and measured (1M
-// Navier clusters, 262k Stokes sphere, M2L per evaluate): FP64 3x3 29.0 ->
-// 26.7 ms, FP64 4x3 5.4 -> 4.7 ms, FP32 4x3 3.5 -> 3.0 ms, FP32 3x3 19.9 ->
-// 21.0 ms (the FP32 3x3 shape keeps k_m2l_vec).
-template <class R, int SM, int TM, int FB>
-__global__ __launch_bounds__(M2LV_BS) void k_m2l_vec_ahead(int C, int Rc, long ncol, int ncb,
-                                                           const int* __restrict__ tslot,
-                                                           const int* __restrict__ sslot,
-                                                           const int* __restrict__ pmask,
-                                                           const int* __restrict__ aidx,
-                                                           const typename Mv<R>::C2* __restrict__ opsT,
-                                                           const typename Mv<R>::C2* __restrict__ effden,
-                                                           typename Mv<R>::C2* __restrict__ effval)
 {
   typedef typename Mv<R>::C2 C2;
   typedef typename Mv<R>::V4 V4;
@@ -3846,17 +3756,24 @@ __global__ __launch_bounds__(M2LV_BS) void k_m2l_vec_ahead(int C, int Rc, long n
     for (int kh = 0; kh < NH; kh++) sl[kh] = q >= 0 ? ss[n * 8 + (kh * 4 + lk) / SM] : -1;
   };
   fetch_idx(0);
+  // the operators of the block's frequencies through a buffer resource based
+  // at frequency f0: one 32-bit byte offset per entry serves all FB
+  // frequencies, which lie a scalar FSTR bytes apart
+  constexpr unsigned FSTR = 343u * ST * sizeof(C2);
+  const __amdgpu_buffer_rsrc_t orsrc =
+      __builtin_amdgcn_make_buffer_rsrc((void*)(opsT + (long)f0 * 343 * ST), (short)0, (int)(FSTR * nf), BUF_DW3);
   for (int n = 0; n < 26; n++) {
     // the direction-n blocks [[Mr,-Mi],[Mi,Mr]] of the FB frequencies: entry
     // e is aidx[n][e] = 2 x (index of the real value among one frequency's
     // 343 sm tm complex operators) + 1 if negated, or -1 (adjacent pair)
     R av[FB][ST];
+    unsigned ao[ST];
 #pragma unroll
-    for (int fb = 0; fb < FB; fb++) {
-      const R* Of = (const R*)(opsT + (long)(f0 + (fb < nf ? fb : 0)) * 343 * ST);
+    for (int i = 0; i < ST; i++) ao[i] = (unsigned)(ai[i] < 0 ? 0 : ai[i] >> 1) * (unsigned)sizeof(R);
 #pragma unroll
-      for (int i = 0; i < ST; i++) av[fb][i] = Of[ai[i] < 0 ? 0 : ai[i] >> 1];
-    }
+    for (int fb = 0; fb < FB; fb++)
+#pragma unroll
+      for (int i = 0; i < ST; i++) av[fb][i] = buf_ld<R>(orsrc, ao[i], fb < nf ? fb * (int)FSTR : 0);
     // this lane's NH source spectra for the FB frequencies (contiguous in the
     // spectra layout), clamped to slot 0 / the run and zeroed below
     const bool use = __any((msk >> n) & 1);  // wave-uniform: a parent of the tile has direction n
@@ -3872,16 +3789,18 @@ __global__ __launch_bounds__(M2LV_BS) void k_m2l_vec_ahead(int C, int Rc, long n
         for (int fb = 0; fb < FB; fb++) x[kh][fb] = xp[fb < nf ? fb : 0];
       }
     }
-    int an[ST];
+    // the entries' signs (waits for the operator values only: the spectra
+    // loads stay in flight), then the next direction's indices
 #pragma unroll
-    for (int i = 0; i < ST; i++) an[i] = ai[i];
+    for (int fb = 0; fb < FB; fb++)
+#pragma unroll
+      for (int i = 0; i < ST; i++) av[fb][i] = ai[i] < 0 ? (R)0 : ((ai[i] & 1) ? -av[fb][i] : av[fb][i]);
     if (n + 1 < 26) fetch_idx(n + 1);
     __syncthreads();  // the previous direction is done with s_A
 #pragma unroll
     for (int fb = 0; fb < FB; fb++)
 #pragma unroll
-      for (int i = 0; i < ST; i++)
-        s_A[fb][t + M2LV_BS * i] = an[i] < 0 ? (R)0 : ((an[i] & 1) ? -av[fb][i] : av[fb][i]);
+      for (int i = 0; i < ST; i++) s_A[fb][t + M2LV_BS * i] = av[fb][i];
     __syncthreads();
     if (!use) continue;
 #pragma unroll
@@ -3923,13 +3842,8 @@ static void m2l_vec_launch(hipStream_t s, int C, int npar, int Rc, const int* ts
   typedef typename Mv<R>::C2 C2;
   const long ncol = (long)npar * Rc;
   const int ncb = (int)((ncol + 63) / 64);
-  const dim3 grid(xcd_grid((C + 7) / 8 * ncb * (8 / FB)));
-  if constexpr (sizeof(R) == 4 && SM == 3 && TM == 3)
-    hipLaunchKernelGGL((k_m2l_vec<R, SM, TM, FB>), grid, dim3(M2LV_BS), 0, s, C, Rc, ncol, ncb, tslot, sslot, pmask,
-                       aidx, (const C2*)opsT, (const C2*)effden, (C2*)effval);
-  else
-    hipLaunchKernelGGL((k_m2l_vec_ahead<R, SM, TM, FB>), grid, dim3(M2LV_BS), 0, s, C, Rc, ncol, ncb, tslot, sslot,
-                       pmask, aidx, (const C2*)opsT, (const C2*)effden, (C2*)effval);
+  hipLaunchKernelGGL((k_m2l_vec<R, SM, TM, FB>), dim3(xcd_grid((C + 7) / 8 * ncb * (8 / FB))), dim3(M2LV_BS), 0, s, C,
+                     Rc, ncol, ncb, tslot, sslot, pmask, aidx, (const C2*)opsT, (const C2*)effden, (C2*)effval);
 }
 
 template <class R>
