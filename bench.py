@@ -52,8 +52,8 @@ PROFILED = {
     ("laplace-uniform-4m-16rhs", "mixed", 256): ("profiles/laplace-uniform-4m-16rhs_mixed.md", {
         "m2l": {"busy": 0.768, "read": 7.396, "written": 4.831},
         "p2p": {"busy": 0.410, "read": 3.406, "written": 0.577},
-        "fft_fwd": {"busy": None, "read": 0.728, "written": 4.869},
-        "fft_inv": {"busy": None, "read": 5.560, "written": 0.728},
+        "fft_fwd": {"busy": None, "read": 0.728, "written": 4.872},
+        "fft_inv": {"busy": None, "read": 5.550, "written": 0.727},
         "s2m": {"busy": 0.318, "read": 0.372, "written": 1.242},
         "s2p": {"busy": 0.210, "read": 1.285, "written": 0.575},
         "gemm_up": {"busy": 0.459, "read": None, "written": None},
@@ -63,17 +63,17 @@ PROFILED = {
         "m2l": {"busy": 0.823, "read": 19.448, "written": 10.323},
         "p2p": {"busy": 0.354, "read": 8.106, "written": 0.558},
         "fft_fwd": {"busy": None, "read": 0.729, "written": 9.691},
-        "fft_inv": {"busy": None, "read": 10.390, "written": 0.728},
+        "fft_inv": {"busy": None, "read": 10.390, "written": 0.727},
         "s2m": {"busy": 0.318, "read": 0.641, "written": 1.245},
         "s2p": {"busy": 0.260, "read": 1.303, "written": 0.556},
     }),
     ("navier-clusters-1m", "mixed", 150): ("profiles/navier-clusters-1m_mixed.md", {
-        "m2l": {"busy": 0.318, "read": 27.667, "written": 3.374},
+        "m2l": {"busy": 0.280, "read": 15.775, "written": 3.200},
         "p2p": {"busy": None, "read": 1.958, "written": 0.026},
         "fft_fwd": {"busy": None, "read": 0.210, "written": 1.599},
         "fft_inv": {"busy": None, "read": 1.896, "written": 0.573},
-        "gemm_up": {"busy": 0.355, "read": None, "written": None},
-        "gemm_down": {"busy": 0.355, "read": None, "written": None},
+        "gemm_up": {"busy": 0.369, "read": None, "written": None},
+        "gemm_down": {"busy": 0.369, "read": None, "written": None},
     }),
 }
 
