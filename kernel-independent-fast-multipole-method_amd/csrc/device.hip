@@ -3713,11 +3713,17 @@ struct Mv<float> {
 // (1M Navier clusters, 262k Stokes sphere, M2L per evaluate): FP64 3x3 29.0 ->
 // 26.7 ms, FP64 4x3 5.4 -> 4.7 ms, FP32 4x3 3.5 -> 2.9 ms, FP32 3x3 19.8 ->
 // 19.9 ms.
-template <class R, int SM, int TM, int FB>
+//
+// XA: the A operands come from the table expanded at set-up (atabv, [C][26]
+// [NA] in the block's entry order, signs and zeros applied), one coalesced
+// run per (frequency, direction), instead of gathers through aidx from the
+// operators (1M Navier clusters, FP32: M2L 19.9 -> 16.8 ms).
+template <class R, int SM, int TM, int FB, bool XA>
 __global__ __launch_bounds__(M2LV_BS) void k_m2l_vec(int C, int Rc, long ncol, int ncb, const int* __restrict__ tslot,
                                                      const int* __restrict__ sslot, const int* __restrict__ pmask,
                                                      const int* __restrict__ aidx,
                                                      const typename Mv<R>::C2* __restrict__ opsT,
+                                                     const R* __restrict__ atabv,
                                                      const typename Mv<R>::C2* __restrict__ effden,
                                                      typename Mv<R>::C2* __restrict__ effval)
 {
@@ -3750,8 +3756,9 @@ __global__ __launch_bounds__(M2LV_BS) void k_m2l_vec(int C, int Rc, long ncol, i
   // direction are then fetched together, before its barriers
   int ai[ST], sl[NH];
   auto fetch_idx = [&](int n) {
+    if constexpr (!XA)
 #pragma unroll
-    for (int i = 0; i < ST; i++) ai[i] = aidx[(long)n * NA + t + M2LV_BS * i];
+      for (int i = 0; i < ST; i++) ai[i] = aidx[(long)n * NA + t + M2LV_BS * i];
 #pragma unroll
     for (int kh = 0; kh < NH; kh++) sl[kh] = q >= 0 ? ss[n * 8 + (kh * 4 + lk) / SM] : -1;
   };
@@ -3767,13 +3774,22 @@ __global__ __launch_bounds__(M2LV_BS) void k_m2l_vec(int C, int Rc, long ncol, i
     // e is aidx[n][e] = 2 x (index of the real value among one frequency's
     // 343 sm tm complex operators) + 1 if negated, or -1 (adjacent pair)
     R av[FB][ST];
-    unsigned ao[ST];
+    if constexpr (XA) {
 #pragma unroll
-    for (int i = 0; i < ST; i++) ao[i] = (unsigned)(ai[i] < 0 ? 0 : ai[i] >> 1) * (unsigned)sizeof(R);
+      for (int fb = 0; fb < FB; fb++) {
+        const R* ap = atabv + ((long)(f0 + (fb < nf ? fb : 0)) * 26 + n) * NA + t;
 #pragma unroll
-    for (int fb = 0; fb < FB; fb++)
+        for (int i = 0; i < ST; i++) av[fb][i] = ap[M2LV_BS * i];
+      }
+    } else {
+      unsigned ao[ST];
 #pragma unroll
-      for (int i = 0; i < ST; i++) av[fb][i] = buf_ld<R>(orsrc, ao[i], fb < nf ? fb * (int)FSTR : 0);
+      for (int i = 0; i < ST; i++) ao[i] = (unsigned)(ai[i] < 0 ? 0 : ai[i] >> 1) * (unsigned)sizeof(R);
+#pragma unroll
+      for (int fb = 0; fb < FB; fb++)
+#pragma unroll
+        for (int i = 0; i < ST; i++) av[fb][i] = buf_ld<R>(orsrc, ao[i], fb < nf ? fb * (int)FSTR : 0);
+    }
     // this lane's NH source spectra for the FB frequencies (contiguous in the
     // spectra layout), clamped to slot 0 / the run and zeroed below
     const bool use = __any((msk >> n) & 1);  // wave-uniform: a parent of the tile has direction n
@@ -3791,10 +3807,11 @@ __global__ __launch_bounds__(M2LV_BS) void k_m2l_vec(int C, int Rc, long ncol, i
     }
     // the entries' signs (waits for the operator values only: the spectra
     // loads stay in flight), then the next direction's indices
+    if constexpr (!XA)
 #pragma unroll
-    for (int fb = 0; fb < FB; fb++)
+      for (int fb = 0; fb < FB; fb++)
 #pragma unroll
-      for (int i = 0; i < ST; i++) av[fb][i] = ai[i] < 0 ? (R)0 : ((ai[i] & 1) ? -av[fb][i] : av[fb][i]);
+        for (int i = 0; i < ST; i++) av[fb][i] = ai[i] < 0 ? (R)0 : ((ai[i] & 1) ? -av[fb][i] : av[fb][i]);
     if (n + 1 < 26) fetch_idx(n + 1);
     __syncthreads();  // the previous direction is done with s_A
 #pragma unroll
@@ -3837,37 +3854,74 @@ __global__ __launch_bounds__(M2LV_BS) void k_m2l_vec(int C, int Rc, long ncol, i
 
 template <class R, int SM, int TM, int FB>
 static void m2l_vec_launch(hipStream_t s, int C, int npar, int Rc, const int* tslot, const int* sslot,
-                           const int* pmask, const int* aidx, const void* opsT, const void* effden, void* effval)
+                           const int* pmask, const int* aidx, const void* opsT, const void* atabv, const void* effden,
+                           void* effval)
 {
   typedef typename Mv<R>::C2 C2;
   const long ncol = (long)npar * Rc;
   const int ncb = (int)((ncol + 63) / 64);
-  hipLaunchKernelGGL((k_m2l_vec<R, SM, TM, FB>), dim3(xcd_grid((C + 7) / 8 * ncb * (8 / FB))), dim3(M2LV_BS), 0, s, C,
-                     Rc, ncol, ncb, tslot, sslot, pmask, aidx, (const C2*)opsT, (const C2*)effden, (C2*)effval);
+  const dim3 grid(xcd_grid((C + 7) / 8 * ncb * (8 / FB)));
+  if constexpr (sizeof(R) == 4)
+    if (atabv) {
+      hipLaunchKernelGGL((k_m2l_vec<R, SM, TM, FB, true>), grid, dim3(M2LV_BS), 0, s, C, Rc, ncol, ncb, tslot, sslot,
+                         pmask, aidx, (const C2*)opsT, (const R*)atabv, (const C2*)effden, (C2*)effval);
+      return;
+    }
+  hipLaunchKernelGGL((k_m2l_vec<R, SM, TM, FB, false>), grid, dim3(M2LV_BS), 0, s, C, Rc, ncol, ncb, tslot, sslot,
+                     pmask, aidx, (const C2*)opsT, (const R*)nullptr, (const C2*)effden, (C2*)effval);
+}
+
+// atabv[(f * 26 + n) * na + e]: entry e of aidx[n] resolved against the
+// operators of frequency f (FP32, signs applied, zeros for -1)
+__global__ void k_expand_m2l_vec(int C, int na, int st, const int* __restrict__ aidx, const float* __restrict__ opsT,
+                                 float* __restrict__ atabv)
+{
+  const long n = (long)C * 26 * na;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const long fn = i / na;
+    const int e = (int)(i - fn * na), f = (int)(fn / 26), d = (int)(fn - (long)f * 26);
+    const int ai = aidx[(long)d * na + e];
+    float v = 0.0f;
+    if (ai >= 0) {
+      v = opsT[(long)f * 343 * st * 2 + (ai >> 1)];
+      if (ai & 1) v = -v;
+    }
+    atabv[i] = v;
+  }
+}
+
+void expand_m2l_vec(hipStream_t s, int C, int sm, int tm, const int* aidx32, const float* opsT32, float* atabv)
+{
+  const int na = tm * 4 * sm * 64;
+  const long n = (long)C * 26 * na;
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_expand_m2l_vec, dim3(grid_for(n, 256)), dim3(256), 0, s, C, na, sm * tm, aidx32, opsT32, atabv);
 }
 
 template <class R>
 static void m2l_matrix_t(hipStream_t s, int sm, int tm, int C, int npar, int Rc, const int* tslot, const int* sslot,
-                         const int* pmask, const int* aidx, const void* opsT, const void* effden, void* effval)
+                         const int* pmask, const int* aidx, const void* opsT, const void* atabv, const void* effden,
+                         void* effval)
 {
   // frequencies per block: 2 measured best in both precisions (of 1, 2, 4, 8)
   constexpr int FB = 2;
   if (sm == 1 && tm == 1)
-    m2l_vec_launch<R, 1, 1, FB>(s, C, npar, Rc, tslot, sslot, pmask, aidx, opsT, effden, effval);
+    m2l_vec_launch<R, 1, 1, FB>(s, C, npar, Rc, tslot, sslot, pmask, aidx, opsT, atabv, effden, effval);
   else if (sm == 4 && tm == 3)
-    m2l_vec_launch<R, 4, 3, FB>(s, C, npar, Rc, tslot, sslot, pmask, aidx, opsT, effden, effval);
+    m2l_vec_launch<R, 4, 3, FB>(s, C, npar, Rc, tslot, sslot, pmask, aidx, opsT, atabv, effden, effval);
   else if (sm == 3 && tm == 3)
-    m2l_vec_launch<R, 3, 3, FB>(s, C, npar, Rc, tslot, sslot, pmask, aidx, opsT, effden, effval);
+    m2l_vec_launch<R, 3, 3, FB>(s, C, npar, Rc, tslot, sslot, pmask, aidx, opsT, atabv, effden, effval);
 }
 
 void m2l_matrix(hipStream_t s, int sm, int tm, int C, int npar, int Rc, const int* tslot, const int* sslot,
-                const int* pmask, const int* aidx, const void* opsT, const void* effden, void* effval, int single)
+                const int* pmask, const int* aidx, const void* opsT, const void* atabv, const void* effden,
+                void* effval, int single)
 {
   if (npar <= 0) return;
   if (single)
-    m2l_matrix_t<float>(s, sm, tm, C, npar, Rc, tslot, sslot, pmask, aidx, opsT, effden, effval);
+    m2l_matrix_t<float>(s, sm, tm, C, npar, Rc, tslot, sslot, pmask, aidx, opsT, atabv, effden, effval);
   else
-    m2l_matrix_t<double>(s, sm, tm, C, npar, Rc, tslot, sslot, pmask, aidx, opsT, effden, effval);
+    m2l_matrix_t<double>(s, sm, tm, C, npar, Rc, tslot, sslot, pmask, aidx, opsT, nullptr, effden, effval);
 }
 
 }  // namespace dev

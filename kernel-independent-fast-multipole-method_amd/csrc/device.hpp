@@ -155,9 +155,15 @@ void m2l_grouped(hipStream_t s, int sm, int tm, int C, int npar, int Rc, int rb,
 // [26][tm][4 sm][64] the A-operand table of the precision's row layout
 // (2 x index of the real value among one frequency's operators + 1 if
 // negated, -1 for zero).  single selects FP32 spectra and operators (float2,
-// all directions accumulated in FP32), otherwise FP64 (double2).
+// all directions accumulated in FP32), otherwise FP64 (double2).  atabv: the
+// FP32 A operands expanded by expand_m2l_vec, used by the FP32 form when not
+// null (the FP64 form resolves aidx against opsT).
 void m2l_matrix(hipStream_t s, int sm, int tm, int C, int npar, int Rc, const int* tslot, const int* sslot,
-                const int* pmask, const int* aidx, const void* opsT, const void* effden, void* effval, int single);
+                const int* pmask, const int* aidx, const void* opsT, const void* atabv, const void* effden,
+                void* effval, int single);
+// atabv [C][26][tm 4 sm 64] = the FP32 A-operand table aidx32 resolved
+// against the FP32 operators opsT32 of every frequency
+void expand_m2l_vec(hipStream_t s, int C, int sm, int tm, const int* aidx32, const float* opsT32, float* atabv);
 // c2r and gather onto the DC surface: dncval[(node*Rc + r)*pln + p*tm + f] += ...
 // (effval in the layout and precision described for fft_fwd)
 void fft_inv(hipStream_t s, int np, int nslot, const int* slot_node, int Rc, const void* effval, int single,

@@ -336,6 +336,11 @@ void Engine::upload_static()
   tw_.upload(tw);
   I_.upload(plan_.I);
   D_.upload(plan_.D);
+  {  // FP32 A operands of the parent-grouped M2L, expanded per (frequency, direction)
+    const size_t na = (size_t)mm_.tdof * 4 * mm_.sdof * 64;
+    avec32_.alloc((size_t)O.C * 26 * na);
+    dev::expand_m2l_vec(stream_, O.C, mm_.sdof, mm_.tdof, I_.p + plan_.aidx[1], opsT32_.p, avec32_.p);
+  }
   {  // fused operator of the mixed up pass: m2me_ = UC2UE x [UE2UC_0 .. UE2UC_7],
      // compensated (Dot2) like the pseudoinverse applies
     const int pUE = O.pln[SURF_UE], pUC = O.pln[SURF_UC];
@@ -1448,7 +1453,8 @@ void Engine::run_chunk(const double* srcden, double* trgval, int rc, int mode, h
                  (const double2*)effden_.p, (double2*)effval_.p);
       else if (form == 1)
         dev::m2l_matrix(s, sm, tm, O.C, vl.npar, rc, I + vl.tslot, I + vl.sslot, I + vl.pmask, I + P.aidx[single],
-                        single ? (const void*)opsT32_.p : (const void*)opsT_.p, effden_.p, effval_.p, single);
+                        single ? (const void*)opsT32_.p : (const void*)opsT_.p, avec32_.p, effden_.p, effval_.p,
+                        single);
       else
         dev::m2l_grouped(s, sm, tm, O.C, vl.npar, rc, m2l_rb_, m2l_bs_, I + vl.tslot, I + vl.sslot, I + P.optab,
                          (const double2*)m2l_.p, (const double2*)effden_.p, (double2*)effval_.p);

@@ -262,6 +262,7 @@ class Engine {
   DVec<double> atab_;  // tiled M2L A-operand tables [C][26 x 32][2] complex (scalar kernels)
   DVec<float> atab32_;  // the same in FP32
   DVec<float> opsT32_; // the same in FP32
+  DVec<float> avec32_; // FP32 A operands of the parent-grouped M2L, [C][26][tdof 4 sdof 64]
   DVec<double> tw_;
   DVec<int> I_;
   DVec<double> D_;
