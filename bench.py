@@ -50,7 +50,7 @@ DEFAULT = "laplace-uniform-4m-16rhs"
 # kernel time of the run for the HBM-side rate.
 PROFILED = {
     ("laplace-uniform-4m-16rhs", "mixed", 256): ("profiles/laplace-uniform-4m-16rhs_mixed.md", {
-        "m2l": {"busy": 0.768, "read": 7.396, "written": 4.831},
+        "m2l": {"busy": 0.770, "read": 7.428, "written": 4.831},
         "p2p": {"busy": 0.410, "read": 3.406, "written": 0.577},
         "fft_fwd": {"busy": None, "read": 0.728, "written": 4.872},
         "fft_inv": {"busy": None, "read": 5.550, "written": 0.727},
@@ -68,7 +68,7 @@ PROFILED = {
         "s2p": {"busy": 0.260, "read": 1.303, "written": 0.556},
     }),
     ("navier-clusters-1m", "mixed", 150): ("profiles/navier-clusters-1m_mixed.md", {
-        "m2l": {"busy": 0.280, "read": 15.775, "written": 3.200},
+        "m2l": {"busy": 0.322, "read": 61.435, "written": 3.235},
         "p2p": {"busy": None, "read": 1.958, "written": 0.026},
         "fft_fwd": {"busy": None, "read": 0.210, "written": 1.599},
         "fft_inv": {"busy": None, "read": 1.896, "written": 0.573},
