@@ -446,6 +446,8 @@ def main():
                     help="both: time the plain FP64 form and the mixed form; value is the mixed form's")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-check", action="store_true", help="skip the direct-sum error check")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="skip the graph-replay leg (the chosen form's pass captured once and relaunched)")
     ap.add_argument("--no-gate", action="store_true",
                     help="skip the higher-np leg timed beside the workload's np (for the 1e-6 error gate)")
     ap.add_argument("--pin", default=None,
@@ -535,6 +537,17 @@ def main():
     if tr.get("busy") is not None:
         roofline["matrix_core_busy_profiled"] = tr["busy"]
 
+    # repeated evaluation through the library's graph replay (kifmm3d_set_graph):
+    # the chosen form's whole pass captured once, then relaunched each step;
+    # reported beside `value`, which times the ordinary evaluate
+    graph = None
+    if not args.no_graph:
+        f.set_graph(True)
+        ms_g, _ = timed_steps(f, K, d_src, d_trg, nrhs, chosen_mode, args.steps, args.warmup, "graph replay")
+        f.set_graph(False)
+        graph = {"ms_per_step": round(ms_g, 3), "value": n * nrhs / (ms_g / 1000.0)}
+        log(f"graph replay: {ms_g:.3f} ms/step ({graph['value']:.4e} targets/s)")
+
     # the metric's gate (error <= 1e-6 against the direct sum) is below the
     # truncation error of np 6 on this workload; the same points at the
     # higher np are timed beside it, both forms, with the same checks
@@ -575,6 +588,8 @@ def main():
                              "kernels": kernel_table(p["rates"],
                                                      PROFILED.get((args.config, "plain", ptsmax), (None, {}))[1])}
         cfg["speedup_vs_plain"] = round(r["value"] / p["value"], 3)
+    if graph:
+        cfg["graph_replay"] = graph
     if gate:
         cfg["at_error_gate"] = gate
     out = {
