@@ -69,7 +69,7 @@ PROFILED = {
     }),
     ("navier-clusters-1m", "mixed", 150): ("profiles/navier-clusters-1m_mixed.md", {
         "m2l": {"busy": 0.322, "read": 61.435, "written": 3.235},
-        "p2p": {"busy": None, "read": 1.958, "written": 0.026},
+        "p2p": {"busy": None, "read": 1.976, "written": 0.026},
         "fft_fwd": {"busy": None, "read": 0.210, "written": 1.599},
         "fft_inv": {"busy": None, "read": 1.896, "written": 0.573},
         "gemm_up": {"busy": 0.369, "read": None, "written": None},
