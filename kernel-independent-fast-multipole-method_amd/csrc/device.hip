@@ -410,6 +410,22 @@ void zero(hipStream_t s, double* p, long n)
   hipLaunchKernelGGL(k_zero, dim3(grid_for(n, 256)), dim3(256), 0, s, n, p);
 }
 
+__global__ void k_zero_rows(long rowlen, const int* __restrict__ rows, int nrows, double* __restrict__ p)
+{
+  const long n = (long)nrows * rowlen;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const long q = i / rowlen;
+    p[(long)rows[q] * rowlen + (i - q * rowlen)] = 0.0;
+  }
+}
+
+void zero_rows(hipStream_t s, double* p, const int* rows, int nrows, long rowlen)
+{
+  const long n = (long)nrows * rowlen;
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_zero_rows, dim3(grid_for(n, 256)), dim3(256), 0, s, rowlen, rows, nrows, p);
+}
+
 // Streaming copy of n doubles (reads n*8 bytes, writes n*8 bytes): the
 // reference kernel for scaling memory-traffic counters.
 __global__ void k_copy(long n, const double* __restrict__ src, double* __restrict__ dst)
@@ -2576,7 +2592,7 @@ __global__ __launch_bounds__(256) void k_fft_inv(const int* __restrict__ slot_no
         const A2 w = s_w[(xi * kx) % N];
         v += (A)2 * (y[kx].x * w.x + y[kx].y * w.y);
       }
-      out[p * tm + f] += v;
+      out[p * tm + f] = v;
     }
   }
 }
@@ -2841,7 +2857,7 @@ __global__ __launch_bounds__(BS) void k_fft_inv_ri(int nslot, const int* __restr
       const A2 w = s_wt[i][kx], yk = cget<A2>(y[kx]);
       v += (A)2 * (yk.x * w.x + yk.y * w.y);
     }
-    out[(long)rr * pln + p] += v;
+    out[(long)rr * pln + p] = v;
   }
 }
 

@@ -21,6 +21,8 @@ hipError_t scratch_ptr(void** out);
 
 // p[0 .. n) = 0
 void zero(hipStream_t s, double* p, long n);
+// p[rows[i] * rowlen .. + rowlen) = 0 for i < nrows
+void zero_rows(hipStream_t s, double* p, const int* rows, int nrows, long rowlen);
 // dst[0 .. n) = src[0 .. n)
 void copy(hipStream_t s, const double* src, double* dst, long n);
 // Source-major gather through the inverse permutation (iperm[perm[k]] = k):

@@ -806,6 +806,17 @@ void Engine::build_plan()
       P.vlev.push_back(vl);
     }
   }
+  {  // the inverse FFT of a level assigns the DC rows of its V-list targets
+     // (each node is a target on its own level only); the other rows are zeroed
+    std::vector<char> covered(T.tndecnt, 0);
+    for (const VLevel& vl : P.vlev)
+      for (int i = 0; i < vl.ntrg; i++) covered[P.I[vl.trg_node + i]] = 1;
+    std::vector<int> rows;
+    for (int n = 0; n < T.tndecnt; n++)
+      if (!covered[n]) rows.push_back(n);
+    P.n_dc_zero = (int)rows.size();
+    P.dc_zero = pool_i(P.I, rows);
+  }
   // down pass, top level first (fmm3d_eval.cpp:195-209)
   const bool fuse_down = pDC == pDE && pDC % 8 == 0 && gemm_mfma_ == 2;
   for (int L = 2; L < T.nlevel; L++) {
@@ -1383,9 +1394,10 @@ void Engine::run_chunk(const double* srcden, double* trgval, int rc, int mode, h
 
   mark(0);
   // zero the accumulated buffers (fmm3d_eval.cpp:33-42); upward check values
-  // at depth >= 2 are overwritten by S2M (leaves) and M2M (inner nodes)
+  // at depth >= 2 are overwritten by S2M (leaves) and M2M (inner nodes), the
+  // downward check values of V-list targets by the inverse FFT of their level
   dev::zero(s, tval_.p, (long)rc * trs);
-  dev::zero(s, dncval_.p, (long)rc * T.tndecnt * pDC);
+  dev::zero_rows(s, dncval_.p, I + P.dc_zero, P.n_dc_zero, (long)rc * pDC);
   if (P.nlow_s) dev::zero(s, upcval_.p, (long)rc * P.nlow_s * pUC);
   if (P.nlow_s) dev::zero(s, upeden_.p, (long)rc * P.nlow_s * pUE);
   if (P.nlow_t) dev::zero(s, dneden_.p, (long)rc * P.nlow_t * pDE);

@@ -93,6 +93,8 @@ struct Plan {
   std::vector<GemmOp> down_mixed;
   size_t fdn_b = 0, fdn_c = 0, fdn_g = 0;  // index arrays that build the fused operator
   std::vector<VLevel> vlev;
+  size_t dc_zero = 0;          // target nodes (tndeidx) in no V list: their DC rows start at zero
+  int n_dc_zero = 0;
   int max_vsrc = 0, max_vtrg = 0;
   bool m2l_pairs = false;      // per-pair M2L kernel instead of the parent-grouped one
   size_t optab = 0;            // [26 directions][8 source children][8 target children] -> operator or -1
