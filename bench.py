@@ -53,7 +53,7 @@ PROFILED = {
         "m2l": {"busy": 0.770, "read": 7.428, "written": 4.831},
         "p2p": {"busy": 0.410, "read": 3.406, "written": 0.577},
         "fft_fwd": {"busy": None, "read": 0.728, "written": 4.872},
-        "fft_inv": {"busy": None, "read": 5.550, "written": 0.727},
+        "fft_inv": {"busy": None, "read": 4.833, "written": 0.728},
         "s2m": {"busy": 0.318, "read": 0.372, "written": 1.242},
         "s2p": {"busy": 0.210, "read": 1.285, "written": 0.575},
         "gemm_up": {"busy": 0.459, "read": None, "written": None},
@@ -63,7 +63,7 @@ PROFILED = {
         "m2l": {"busy": 0.823, "read": 19.448, "written": 10.323},
         "p2p": {"busy": 0.354, "read": 8.106, "written": 0.558},
         "fft_fwd": {"busy": None, "read": 0.729, "written": 9.691},
-        "fft_inv": {"busy": None, "read": 10.390, "written": 0.727},
+        "fft_inv": {"busy": None, "read": 9.662, "written": 0.728},
         "s2m": {"busy": 0.318, "read": 0.641, "written": 1.245},
         "s2p": {"busy": 0.260, "read": 1.303, "written": 0.556},
     }),
