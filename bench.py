@@ -52,7 +52,7 @@ PROFILED = {
     ("laplace-uniform-4m-16rhs", "mixed", 256): ("profiles/laplace-uniform-4m-16rhs_mixed.md", {
         "m2l": {"busy": 0.770, "read": 7.428, "written": 4.831},
         "p2p": {"busy": 0.410, "read": 3.406, "written": 0.577},
-        "fft_fwd": {"busy": None, "read": 0.728, "written": 4.872},
+        "fft_fwd": {"busy": None, "read": 0.728, "written": 4.822},
         "fft_inv": {"busy": None, "read": 4.833, "written": 0.728},
         "s2m": {"busy": 0.318, "read": 0.372, "written": 1.242},
         "s2p": {"busy": 0.210, "read": 1.285, "written": 0.575},

@@ -2705,7 +2705,7 @@ __global__ __launch_bounds__(64 * R) void k_fft_fwd_ri(int nslot, const int* __r
 // of plane k straight into registers (cube points off the surface read a zero
 // kept at index NSP of the density row) and transforms them along y at once,
 // so the x-stage values never pass through LDS: one barrier fewer, and at np 6
-// with 8 FP32 right-hand sides 38 KB of LDS per block instead of 54 KB.
+// 38 KB of LDS per 8 FP32 right-hand sides instead of 54 KB.
 template <int NP, class CT, int R>
 __global__ __launch_bounds__(64 * R) void k_fft_fwd_xy(int nslot, const int* __restrict__ slot_node, int Rc,
                                                              const double* __restrict__ upeden, int pln,
@@ -2873,13 +2873,15 @@ static void fft_fwd_t(hipStream_t s, int np, int nslot, const int* slot_node, in
   if (nslot <= 0) return;
   if (ri && sm == 1) {
     constexpr int RW = sizeof(CT) == 8 ? FFT_RW32_FWD : 4;
-    // np 6, FP32: the x and y stages fused (4M points / 16 RHS: 1.71 ->
-    // 1.59 ms a forward pass; the fused form of FP64 spectra measured 3.75 ms
-    // against 3.19, and 384 or 256 threads per block of 8 FP32 right-hand
-    // sides 1.95 and 1.74)
+    // np 6, FP32: the x and y stages fused, 16 right-hand sides (1024
+    // threads) per block so that a block writes whole 128-byte lines of
+    // spectra (4M points / 16 RHS: 1.71 -> 1.58 ms a forward pass fused with
+    // 8 right-hand sides a block, 1.54 ms with 16; the fused form of FP64
+    // spectra measured 3.75 ms against 3.19, and 384 or 256 threads per block
+    // of 8 FP32 right-hand sides 1.95 and 1.74)
     if constexpr (sizeof(CT) == 8)
-      if (np == 6) {
-        hipLaunchKernelGGL((k_fft_fwd_xy<6, CT, RW>), dim3(xcd_grid(nslot * (Rc / RW))), dim3(64 * RW), 0, s, nslot,
+      if (np == 6 && Rc % 16 == 0) {
+        hipLaunchKernelGGL((k_fft_fwd_xy<6, CT, 16>), dim3(xcd_grid(nslot * (Rc / 16))), dim3(1024), 0, s, nslot,
                            slot_node, Rc, upeden, pln, dscale, cube2surf, effden, tw);
         return;
       }
