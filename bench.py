@@ -62,7 +62,7 @@ PROFILED = {
     ("laplace-uniform-4m-16rhs", "plain", 256): ("profiles/laplace-uniform-4m-16rhs_plain.md", {
         "m2l": {"busy": 0.823, "read": 19.448, "written": 10.323},
         "p2p": {"busy": 0.354, "read": 8.106, "written": 0.558},
-        "fft_fwd": {"busy": None, "read": 0.729, "written": 9.691},
+        "fft_fwd": {"busy": None, "read": 0.729, "written": 9.662},
         "fft_inv": {"busy": None, "read": 9.662, "written": 0.728},
         "s2m": {"busy": 0.318, "read": 0.641, "written": 1.245},
         "s2p": {"busy": 0.260, "read": 1.303, "written": 0.556},

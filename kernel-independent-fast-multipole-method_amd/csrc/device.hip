@@ -2873,18 +2873,18 @@ static void fft_fwd_t(hipStream_t s, int np, int nslot, const int* slot_node, in
   if (nslot <= 0) return;
   if (ri && sm == 1) {
     constexpr int RW = sizeof(CT) == 8 ? FFT_RW32_FWD : 4;
-    // np 6, FP32: the x and y stages fused, 16 right-hand sides (1024
-    // threads) per block so that a block writes whole 128-byte lines of
-    // spectra (4M points / 16 RHS: 1.71 -> 1.58 ms a forward pass fused with
-    // 8 right-hand sides a block, 1.54 ms with 16; the fused form of FP64
-    // spectra measured 3.75 ms against 3.19, and 384 or 256 threads per block
-    // of 8 FP32 right-hand sides 1.95 and 1.74)
-    if constexpr (sizeof(CT) == 8)
-      if (np == 6 && Rc % 16 == 0) {
-        hipLaunchKernelGGL((k_fft_fwd_xy<6, CT, 16>), dim3(xcd_grid(nslot * (Rc / 16))), dim3(1024), 0, s, nslot,
-                           slot_node, Rc, upeden, pln, dscale, cube2surf, effden, tw);
-        return;
-      }
+    // np 6: the x and y stages fused, a block writing whole 128-byte lines
+    // of spectra: 16 FP32 right-hand sides (1024 threads) or 8 FP64 ones
+    // (512).  4M points / 16 RHS, a forward pass: FP32 1.71 -> 1.58 ms fused
+    // with 8 right-hand sides a block, 1.54 ms with 16; FP64 3.20 ms (three
+    // stages, 4 a block) -> 3.75 ms fused with 4, 2.44 ms fused with 8; 384
+    // or 256 threads per block of 8 FP32 right-hand sides 1.95 and 1.74 ms
+    constexpr int RX = sizeof(CT) == 8 ? 16 : 8;
+    if (np == 6 && Rc % RX == 0) {
+      hipLaunchKernelGGL((k_fft_fwd_xy<6, CT, RX>), dim3(xcd_grid(nslot * (Rc / RX))), dim3(64 * RX), 0, s, nslot,
+                         slot_node, Rc, upeden, pln, dscale, cube2surf, effden, tw);
+      return;
+    }
     switch (np) {
     case 4: hipLaunchKernelGGL((k_fft_fwd_ri<4, CT, RW>), dim3(xcd_grid(nslot * (Rc / RW))), dim3(64 * RW), 0, s, nslot, slot_node, Rc, upeden, pln, dscale, cube2surf, effden, tw); break;
     case 6: hipLaunchKernelGGL((k_fft_fwd_ri<6, CT, RW>), dim3(xcd_grid(nslot * (Rc / RW))), dim3(64 * RW), 0, s, nslot, slot_node, Rc, upeden, pln, dscale, cube2surf, effden, tw); break;
