@@ -166,8 +166,9 @@ void m2l_matrix(hipStream_t s, int sm, int tm, int C, int npar, int Rc, const in
 // atabv [C][26][tm 4 sm 64] = the FP32 A-operand table aidx32 resolved
 // against the FP32 operators opsT32 of every frequency
 void expand_m2l_vec(hipStream_t s, int C, int sm, int tm, const int* aidx32, const float* opsT32, float* atabv);
-// c2r and gather onto the DC surface: dncval[(node*Rc + r)*pln + p*tm + f] += ...
-// (effval in the layout and precision described for fft_fwd)
+// c2r and gather onto the DC surface: dncval[(node*Rc + r)*pln + p*tm + f] = ...
+// (assigned: every DC value of the listed nodes; effval in the layout and
+// precision described for fft_fwd)
 void fft_inv(hipStream_t s, int np, int nslot, const int* slot_node, int Rc, const void* effval, int single,
              int rhs_inner, int tm, const int* surf2cube, int nsp, double* dncval, int pln, const double2* tw);
 // Tiled M2L on the matrix cores for scalar kernels, spectra with the
